@@ -1,0 +1,702 @@
+// solver.hip -- device-resident elastic CG-SEM solver and its C ABI.
+//
+// Replaces the reference's CudaDeviceState and per-step wrappers
+// (forward/share/include/gf/cuda_step.hpp:20-129, forward/share/src/cuda_step.cu)
+// with one handle that owns a HIP stream, FP32 SoA device arrays and the
+// launch schedule of one step:
+//   element kernel per colour -> node kernel -> (snapshot) record kernel.
+// The host-side setup (colouring, node renumbering, FP64 -> FP32 packing)
+// happens once in gfx_solver_create.
+#include <cstring>
+#include <memory>
+#include <numeric>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+#define GFX_HIP(x)                                                                         \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess)                                                              \
+            throw ::gfx::Error(GFX_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+namespace gfx {
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) return;
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            throw Error(GFX_ERR_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) +
+                                           " bytes failed: " + hipGetErrorString(e));
+        }
+        n = count;
+    }
+    void upload(const std::vector<T>& h) {
+        alloc(h.size());
+        if (!h.empty()) GFX_HIP(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    }
+    void zero(hipStream_t st) {
+        if (p) GFX_HIP(hipMemsetAsync(p, 0, n * sizeof(T), st));
+    }
+};
+
+namespace {
+struct FieldSource {
+    int direction = -1;
+    std::vector<int32_t> cells;  // caller element indices
+    std::vector<float> weights;  // [n*125]
+    std::vector<float> stf;
+};
+}  // namespace
+
+}  // namespace gfx
+
+struct gfx_solver {
+    int E = 0, G = 0, nf = 1, Ppml = 0, n_rec = 0, n_pml_elem = 0, G_pml = 0;
+    int64_t fstride = 0;
+    int assembly = GFX_ASSEMBLY_COLORED, epb = 2;
+    double dt = 0;
+    int stride = 1;
+    bool has_cpml = false;
+    hipStream_t stream = nullptr;
+    std::vector<int32_t> int_of_elem;        // caller element -> solver element
+    std::vector<int32_t> node_new_of_old;    // caller node -> solver node
+    std::vector<std::pair<int, int>> ranges;  // solver element ranges launched together
+    gfx::DevBuf<float> G9, lam, mu, minv, damp, palpha, pabar, pmem, pnew;
+    gfx::DevBuf<int32_t> ibool, pml_slot, src_off;
+    gfx::DevBuf<float> U0, U1, V, A, R;
+    bool ucur_is_u0 = true;
+    std::vector<gfx::FieldSource> sources;
+    gfx::DevBuf<int4> src_ent;
+    gfx::DevBuf<float> src_w, stf;
+    int nstf = 0;
+    // recording
+    std::vector<int32_t> rec_cells;
+    gfx::DevBuf<int32_t> rec_ibool;
+    gfx::DevBuf<float> rec_dd, rec_strain, rec_uva;
+    int rec_cap = 0, n_snap = 0;
+    bool rec_uva_on = false;
+    float D[25];
+    int step = 0;
+    // timing
+    bool timing = false;
+    double t_elem = 0, t_node = 0, t_rec = 0, t_total = 0;
+    int n_elem_launch = 0, n_node_launch = 0, n_rec_launch = 0, n_total_launch = 0;
+
+    ~gfx_solver() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    float* ucur() { return ucur_is_u0 ? U0.p : U1.p; }
+    float* unext() { return ucur_is_u0 ? U1.p : U0.p; }
+    const float* ucur() const { return ucur_is_u0 ? U0.p : U1.p; }
+};
+
+namespace gfx {
+namespace {
+
+constexpr int NP = dev::NP;
+
+void build_sources(gfx_solver& S) {
+    std::vector<std::vector<int4>> per_elem(S.E);
+    std::vector<float> w;
+    int nstf = 0;
+    for (int f = 0; f < S.nf; ++f) {
+        const FieldSource& fs = S.sources[f];
+        if (fs.direction < 0) continue;
+        nstf = std::max(nstf, int(fs.stf.size()));
+        for (size_t c = 0; c < fs.cells.size(); ++c) {
+            const int e = fs.cells[c];
+            if (e < 0) continue;  // solver.cpp:585-587
+            const int ie = S.int_of_elem[e];
+            const int off = int(w.size());
+            w.insert(w.end(), fs.weights.begin() + c * NP, fs.weights.begin() + (c + 1) * NP);
+            per_elem[ie].push_back(make_int4(f, fs.direction, off, 0));
+        }
+    }
+    std::vector<int32_t> off(S.E + 1, 0);
+    std::vector<int4> ent;
+    for (int e = 0; e < S.E; ++e) {
+        off[e] = int32_t(ent.size());
+        for (const int4& q : per_elem[e]) ent.push_back(q);
+    }
+    off[S.E] = int32_t(ent.size());
+    std::vector<float> stf(size_t(std::max(nstf, 1)) * S.nf, 0.f);
+    for (int f = 0; f < S.nf; ++f)
+        for (size_t t = 0; t < S.sources[f].stf.size(); ++t)
+            stf[size_t(f) * std::max(nstf, 1) + t] = S.sources[f].stf[t];
+    if (ent.empty()) ent.push_back(make_int4(-1, 0, 0, 0));
+    if (w.empty()) w.push_back(0.f);
+    S.src_off.upload(off);
+    S.src_ent.upload(ent);
+    S.src_w.upload(w);
+    S.stf.upload(stf);
+    S.nstf = nstf;
+}
+
+void init_state(gfx_solver& S) {
+    S.U0.zero(S.stream);
+    S.U1.zero(S.stream);
+    S.V.zero(S.stream);
+    S.A.zero(S.stream);
+    S.R.zero(S.stream);
+    S.pmem.zero(S.stream);
+    S.pnew.zero(S.stream);
+    GFX_HIP(hipStreamSynchronize(S.stream));
+    S.ucur_is_u0 = true;
+    S.step = 0;
+    S.n_snap = 0;
+}
+
+template <int EPB>
+void launch_element_t(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
+    constexpr int NT = ((EPB * NP + 63) / 64) * 64;
+    const int grid = (n + EPB - 1) / EPB;
+    if (grid == 0) return;
+    if (S.assembly == GFX_ASSEMBLY_ATOMIC)
+        dev::k_element<EPB, true><<<grid, NT, 0, S.stream>>>(a, b, n);
+    else
+        dev::k_element<EPB, false><<<grid, NT, 0, S.stream>>>(a, b, n);
+}
+
+void launch_element(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
+    switch (S.epb) {
+        case 1: launch_element_t<1>(S, a, b, n); break;
+        case 2: launch_element_t<2>(S, a, b, n); break;
+        case 4: launch_element_t<4>(S, a, b, n); break;
+        default: launch_element_t<8>(S, a, b, n); break;
+    }
+}
+
+void run_steps(gfx_solver& S, int nsteps) {
+    dev::ElemArgs ea{};
+    ea.G = S.G9.p;
+    ea.lam = S.lam.p;
+    ea.mu = S.mu.p;
+    ea.ibool = S.ibool.p;
+    ea.P = int64_t(S.E) * NP;
+    ea.nf = S.nf;
+    ea.fstride = S.fstride;
+    ea.v = S.V.p;
+    ea.a = S.A.p;
+    ea.r = S.R.p;
+    ea.pml_slot = S.pml_slot.p;
+    ea.Ppml = S.Ppml;
+    ea.palpha = S.palpha.p;
+    ea.pabar = S.pabar.p;
+    ea.pmem = S.pmem.p;
+    ea.pnew = S.pnew.p;
+    const double theta = 1.0 / 8.0;  // pml.cpp:35, coefficients pml.cpp:63-64
+    ea.c1 = float((1.0 - 2.0 * theta) * 0.5 * S.dt);
+    ea.c2 = float((1.0 - theta) * 0.5 * S.dt * S.dt);
+    ea.src_off = S.src_off.p;
+    ea.src_ent = S.src_ent.p;
+    ea.src_w = S.src_w.p;
+    ea.stf = S.stf.p;
+    ea.nstf = S.nstf;
+    std::memcpy(ea.D, S.D, sizeof(S.D));
+
+    dev::NodeArgs na{};
+    na.G = S.G;
+    na.nf = S.nf;
+    na.fstride = S.fstride;
+    na.minv = S.minv.p;
+    na.v = S.V.p;
+    na.a = S.A.p;
+    na.r = S.R.p;
+    na.damp = S.has_cpml ? nullptr : S.damp.p;
+    na.dt = float(S.dt);
+    const int node_grid = int((int64_t(S.G) + 255) / 256);
+
+    dev::RecArgs ra{};
+    ra.n_rec = S.n_rec;
+    ra.ibool = S.rec_ibool.p;
+    ra.dd = S.rec_dd.p;
+    ra.nf = S.nf;
+    ra.fstride = S.fstride;
+    ra.v = S.V.p;
+    ra.a = S.A.p;
+    std::memcpy(ra.D, S.D, sizeof(S.D));
+    constexpr int REPB = 2;
+    const int rec_grid = (S.n_rec + REPB - 1) / REPB;
+
+    std::vector<hipEvent_t> ev;
+    if (S.timing) {
+        ev.resize(size_t(nsteps) * 4 + 1);
+        for (auto& e : ev) GFX_HIP(hipEventCreate(&e));
+        GFX_HIP(hipEventRecord(ev[0], S.stream));
+    }
+    std::vector<char> did_rec(nsteps, 0);
+    for (int t = 0; t < nsteps; ++t) {
+        ea.ut = S.unext();
+        ea.u = S.ucur();
+        ea.step = S.step;
+        for (const auto& rg : S.ranges) launch_element(S, ea, rg.first, rg.second - rg.first);
+        GFX_HIP(hipGetLastError());
+        if (S.timing) GFX_HIP(hipEventRecord(ev[size_t(t) * 4 + 1], S.stream));
+        na.ucur = S.ucur();
+        na.unext = S.unext();
+        dev::k_node<<<node_grid, 256, 0, S.stream>>>(na);
+        GFX_HIP(hipGetLastError());
+        if (S.timing) GFX_HIP(hipEventRecord(ev[size_t(t) * 4 + 2], S.stream));
+        S.ucur_is_u0 = !S.ucur_is_u0;  // u_{n+1} now current
+        if (S.stride > 0 && S.step % S.stride == 0 && S.rec_cap > 0 && S.n_rec > 0) {
+            const int slot = S.n_snap % S.rec_cap;
+            const int64_t nrp = int64_t(S.n_rec) * NP;
+            ra.u = S.ucur();
+            ra.strain = S.rec_strain.p + int64_t(slot) * S.nf * nrp * 6;
+            ra.uva = S.rec_uva_on ? S.rec_uva.p + int64_t(slot) * S.nf * nrp * 9 : nullptr;
+            dev::k_record<REPB><<<rec_grid, ((REPB * NP + 63) / 64) * 64, 0, S.stream>>>(ra);
+            GFX_HIP(hipGetLastError());
+            did_rec[t] = 1;
+        }
+        if (S.stride > 0 && S.step % S.stride == 0) ++S.n_snap;
+        if (S.timing) GFX_HIP(hipEventRecord(ev[size_t(t) * 4 + 3], S.stream));
+        ++S.step;
+    }
+    GFX_HIP(hipStreamSynchronize(S.stream));
+    if (S.timing) {
+        S.t_elem = S.t_node = S.t_rec = S.t_total = 0;
+        S.n_elem_launch = S.n_node_launch = S.n_rec_launch = 0;
+        for (int t = 0; t < nsteps; ++t) {
+            float a = 0, b = 0, c = 0;
+            hipEvent_t e0 = ev[size_t(t) * 4], e1 = ev[size_t(t) * 4 + 1],
+                       e2 = ev[size_t(t) * 4 + 2], e3 = ev[size_t(t) * 4 + 3];
+            GFX_HIP(hipEventElapsedTime(&a, e0, e1));
+            GFX_HIP(hipEventElapsedTime(&b, e1, e2));
+            GFX_HIP(hipEventElapsedTime(&c, e2, e3));
+            S.t_elem += a;
+            S.t_node += b;
+            S.t_rec += c;
+            S.n_elem_launch += int(S.ranges.size());
+            S.n_node_launch += 1;
+            S.n_rec_launch += did_rec[t];
+        }
+        float tot = 0;
+        GFX_HIP(hipEventElapsedTime(&tot, ev[0], ev[size_t(nsteps) * 4]));
+        S.t_total = tot;
+        S.n_total_launch = nsteps;
+        for (auto& e : ev) (void)hipEventDestroy(e);
+    }
+}
+
+// Greedy element colouring: no two elements of one colour share a node.
+// Elements are visited in caller order, so the result is deterministic.
+std::vector<int> colour_elements(int E, const int32_t* ibool, int G) {
+    std::vector<uint64_t> used(G, 0);
+    std::vector<int> col(E, 0);
+    for (int e = 0; e < E; ++e) {
+        uint64_t m = 0;
+        for (int p = 0; p < NP; ++p) m |= used[ibool[int64_t(e) * NP + p]];
+        int c = 0;
+        while (c < 64 && (m >> c) & 1ULL) ++c;
+        require(c < 64, "element colouring needs more than 64 colours");
+        col[e] = c;
+        for (int p = 0; p < NP; ++p) used[ibool[int64_t(e) * NP + p]] |= (1ULL << c);
+    }
+    return col;
+}
+
+void create(const gfx_solver_desc& d, gfx_solver& S) {
+    require(d.ngll == 5, "the device solver supports NGLL = 5 (polynomial order 4) only");
+    require(d.n_elem > 0 && d.n_rank_node > 0, "empty mesh");
+    require(d.dxi_dx && d.jacobian && d.lambda_ && d.mu_ && d.mass && d.ibool,
+            "missing geometry/material arrays");
+    require(d.n_fields >= 1 && d.n_fields <= 4096, "n_fields must be in [1, 4096]");
+    require(d.dt > 0, "dt must be > 0");
+    require(d.assembly == GFX_ASSEMBLY_COLORED || d.assembly == GFX_ASSEMBLY_ATOMIC,
+            "unknown assembly mode");
+    const int epb = d.elems_per_block ? d.elems_per_block : 2;
+    require(epb == 1 || epb == 2 || epb == 4 || epb == 8, "elems_per_block must be 1, 2, 4 or 8");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        throw Error(GFX_ERR_DEVICE, "no HIP device available");
+    }
+    const int E = d.n_elem, G = d.n_rank_node;
+    const int64_t P = int64_t(E) * NP;
+    for (int64_t p = 0; p < P; ++p)
+        require(d.ibool[p] >= 0 && d.ibool[p] < G, "ibool entry out of range");
+    S.E = E;
+    S.G = G;
+    S.nf = d.n_fields;
+    S.assembly = d.assembly;
+    S.epb = epb;
+    S.dt = d.dt;
+    S.stride = d.snapshot_stride;
+    S.has_cpml = d.has_cpml != 0;
+    GFX_HIP(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
+
+    const GLL gll(4);
+    for (int q = 0; q < 25; ++q) S.D[q] = float(gll.D[q]);
+
+    // solver element order: by colour (coloured assembly) or caller order
+    std::vector<int32_t> elem_of_int(E);
+    std::iota(elem_of_int.begin(), elem_of_int.end(), 0);
+    if (S.assembly == GFX_ASSEMBLY_COLORED) {
+        const std::vector<int> col = colour_elements(E, d.ibool, G);
+        std::stable_sort(elem_of_int.begin(), elem_of_int.end(),
+                         [&](int a, int b) { return col[a] < col[b]; });
+        int b = 0;
+        for (int i = 1; i <= E; ++i)
+            if (i == E || col[elem_of_int[i]] != col[elem_of_int[b]]) {
+                S.ranges.push_back({b, i});
+                b = i;
+            }
+    } else {
+        S.ranges.push_back({0, E});
+    }
+    S.int_of_elem.assign(E, 0);
+    for (int i = 0; i < E; ++i) S.int_of_elem[elem_of_int[i]] = i;
+
+    // node numbering by first touch in solver element order
+    S.node_new_of_old.assign(G, -1);
+    int next = 0;
+    for (int i = 0; i < E; ++i) {
+        const int64_t e = elem_of_int[i];
+        for (int p = 0; p < NP; ++p) {
+            const int g = d.ibool[e * NP + p];
+            if (S.node_new_of_old[g] < 0) S.node_new_of_old[g] = next++;
+        }
+    }
+    for (int g = 0; g < G; ++g)
+        if (S.node_new_of_old[g] < 0) S.node_new_of_old[g] = next++;  // nodes in no element
+
+    // geometry / material, FP64 -> FP32 SoA
+    std::vector<float> hG(size_t(9) * P), hlam(P), hmu(P);
+    std::vector<int32_t> hib(P);
+    bool neg_jac = false;
+    parallel_for(E, [&](int64_t ie) {
+        const int64_t e = elem_of_int[ie];
+        for (int p = 0; p < NP; ++p) {
+            const int64_t src = e * NP + p, dst = ie * NP + p;
+            const int pi = p / 25, pj = (p / 5) % 5, pk = p % 5;
+            const double jw = d.jacobian[src] * gll.w[pi] * gll.w[pj] * gll.w[pk];
+            if (jw < 0) neg_jac = true;
+            const double sq = std::sqrt(std::max(jw, 0.0));
+            for (int c = 0; c < 9; ++c) hG[c * P + dst] = float(d.dxi_dx[src * 9 + c] * sq);
+            const bool solid = d.mu_[src] > 0.0;  // element_cpu.cpp:42-43
+            hlam[dst] = solid ? float(d.lambda_[src]) : 0.f;
+            hmu[dst] = solid ? float(d.mu_[src]) : 0.f;
+            hib[dst] = S.node_new_of_old[d.ibool[src]];
+        }
+    });
+    if (neg_jac)
+        throw Error(GFX_ERR_UNSUPPORTED, "negative jacobian*weight (inverted element)");
+    S.G9.upload(hG);
+    S.lam.upload(hlam);
+    S.mu.upload(hmu);
+    S.ibool.upload(hib);
+    std::vector<float>().swap(hG);
+
+    // nodal mass (solver.cpp:193-202, accumulated in caller order, FP64)
+    std::vector<double> mg(G, 0.0), dg(G, 0.0);
+    for (int64_t p = 0; p < P; ++p) {
+        mg[d.ibool[p]] += d.mass[p];
+        if (d.damping) dg[d.ibool[p]] = d.damping[p];  // last writer wins
+    }
+    std::vector<float> hminv(G), hdamp(G);
+    for (int g = 0; g < G; ++g) {
+        const int n = S.node_new_of_old[g];
+        hminv[n] = mg[g] > 0.0 ? float(1.0 / mg[g]) : 0.f;
+        hdamp[n] = float(dg[g]);
+    }
+    S.minv.upload(hminv);
+    S.damp.upload(hdamp);
+
+    // C-PML, compact over PML elements in solver order
+    std::vector<int32_t> hslot(E, -1);
+    int npml = 0;
+    if (S.has_cpml) {
+        require(d.pml_region && d.coef_alpha && d.coef_abar && d.density,
+                "has_cpml needs pml_region, coef_alpha, coef_abar and density");
+        for (int ie = 0; ie < E; ++ie)
+            if (d.pml_region[elem_of_int[ie]] != 0) hslot[ie] = npml++;
+    }
+    S.n_pml_elem = npml;
+    S.Ppml = npml * NP;
+    {
+        const int64_t Q = std::max<int64_t>(S.Ppml, 1);
+        std::vector<float> hal(size_t(9) * Q, 0.f), hab(size_t(5) * Q, 0.f);
+        std::vector<char> pml_node(G, 0);
+        for (int ie = 0; ie < E; ++ie) {
+            if (hslot[ie] < 0) continue;
+            const int64_t e = elem_of_int[ie];
+            for (int p = 0; p < NP; ++p) {
+                const int64_t src = e * NP + p, q = int64_t(hslot[ie]) * NP + p;
+                const int pi = p / 25, pj = (p / 5) % 5, pk = p % 5;
+                const double wg = gll.w[pi] * gll.w[pj] * gll.w[pk];
+                const double rho = d.density[src];
+                const double scale = wg * (rho > 0.0 ? 1.0 / rho : 0.0) * d.jacobian[src];
+                for (int c = 0; c < 9; ++c) hal[c * Q + q] = float(d.coef_alpha[src * 9 + c]);
+                for (int c = 0; c < 5; ++c) hab[c * Q + q] = float(scale * d.coef_abar[src * 5 + c]);
+                pml_node[d.ibool[src]] = 1;
+            }
+        }
+        S.G_pml = int(std::count(pml_node.begin(), pml_node.end(), 1));
+        S.palpha.upload(hal);
+        S.pabar.upload(hab);
+        S.pmem.alloc(size_t(S.nf) * 9 * Q);
+        S.pnew.alloc(size_t(S.nf) * 3 * Q);
+    }
+    S.pml_slot.upload(hslot);
+
+    // nodal state: [nf][fstride], fstride padded to 64 floats
+    S.fstride = ((int64_t(G) * 3 + 63) / 64) * 64;
+    const size_t fs = size_t(S.fstride) * S.nf;
+    S.U0.alloc(fs);
+    S.U1.alloc(fs);
+    S.V.alloc(fs);
+    S.A.alloc(fs);
+    S.R.alloc(fs);
+
+    // recording cells
+    S.rec_cells.assign(d.rec_cell, d.rec_cell + std::max(d.n_rec_cell, 0));
+    S.n_rec = int(S.rec_cells.size());
+    S.rec_cap = std::max(d.record_capacity, 0);
+    S.rec_uva_on = d.record_uva != 0;
+    if (S.n_rec > 0) {
+        const int64_t nrp = int64_t(S.n_rec) * NP;
+        std::vector<int32_t> rib(nrp);
+        std::vector<float> rdd(size_t(9) * nrp);
+        for (int c = 0; c < S.n_rec; ++c) {
+            const int64_t e = S.rec_cells[c];
+            require(e >= 0 && e < E, "recording cell out of range");
+            for (int p = 0; p < NP; ++p) {
+                rib[int64_t(c) * NP + p] = S.node_new_of_old[d.ibool[e * NP + p]];
+                for (int q = 0; q < 9; ++q)
+                    rdd[q * nrp + int64_t(c) * NP + p] = float(d.dxi_dx[(e * NP + p) * 9 + q]);
+            }
+        }
+        S.rec_ibool.upload(rib);
+        S.rec_dd.upload(rdd);
+        if (S.rec_cap > 0) {
+            S.rec_strain.alloc(size_t(S.rec_cap) * S.nf * nrp * 6);
+            if (S.rec_uva_on) S.rec_uva.alloc(size_t(S.rec_cap) * S.nf * nrp * 9);
+        }
+    }
+
+    S.sources.assign(S.nf, FieldSource{});
+    build_sources(S);
+    init_state(S);
+}
+
+void check_field(const gfx_solver* s, int field) {
+    require(s != nullptr, "null solver");
+    require(field >= 0 && field < s->nf, "field index out of range");
+}
+
+}  // namespace
+}  // namespace gfx
+
+using namespace gfx;
+
+extern "C" int gfx_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+extern "C" gfx_status gfx_solver_create(const gfx_solver_desc* desc, gfx_solver** out) {
+    return guarded([&] {
+        require(desc && out, "null argument");
+        *out = nullptr;
+        auto S = std::make_unique<gfx_solver>();
+        create(*desc, *S);
+        *out = S.release();
+    });
+}
+
+extern "C" gfx_status gfx_solver_destroy(gfx_solver* s) {
+    return guarded([&] {
+        if (s) {
+            if (s->stream) (void)hipStreamSynchronize(s->stream);
+            delete s;
+        }
+    });
+}
+
+extern "C" gfx_status gfx_solver_set_source(gfx_solver* s, int field, int direction,
+                                            int n_src_cell, const int32_t* src_elem,
+                                            const double* src_weights, int n_stf,
+                                            const double* stf) {
+    return guarded([&] {
+        check_field(s, field);
+        require(direction >= 0 && direction <= 2, "direction must be 0, 1 or 2");
+        require(n_src_cell >= 0 && n_stf >= 0, "negative count");
+        require(n_src_cell == 0 || (src_elem && src_weights), "missing source arrays");
+        require(n_stf == 0 || stf, "missing stf");
+        FieldSource fs;
+        fs.direction = direction;
+        for (int c = 0; c < n_src_cell; ++c) {
+            require(src_elem[c] < s->E, "source cell out of range");
+            fs.cells.push_back(src_elem[c]);
+            for (int p = 0; p < NP; ++p) fs.weights.push_back(float(src_weights[int64_t(c) * NP + p]));
+        }
+        for (int t = 0; t < n_stf; ++t) fs.stf.push_back(float(stf[t]));
+        s->sources[field] = std::move(fs);
+        build_sources(*s);
+    });
+}
+
+extern "C" gfx_status gfx_solver_reset(gfx_solver* s) {
+    return guarded([&] {
+        require(s != nullptr, "null solver");
+        init_state(*s);
+    });
+}
+
+extern "C" gfx_status gfx_solver_run(gfx_solver* s, int n_steps) {
+    return guarded([&] {
+        require(s != nullptr, "null solver");
+        require(n_steps >= 0, "n_steps must be >= 0");
+        if (n_steps > 0) run_steps(*s, n_steps);
+    });
+}
+
+extern "C" int gfx_solver_step_index(const gfx_solver* s) { return s ? s->step : -1; }
+extern "C" int gfx_solver_n_snapshots(const gfx_solver* s) { return s ? s->n_snap : -1; }
+
+extern "C" gfx_status gfx_solver_get_state(const gfx_solver* s, int field, double* u, double* v,
+                                           double* a) {
+    return guarded([&] {
+        check_field(s, field);
+        std::vector<float> h(size_t(s->G) * 3);
+        const float* srcs[3] = {s->ucur(), s->V.p, s->A.p};
+        double* dsts[3] = {u, v, a};
+        GFX_HIP(hipStreamSynchronize(s->stream));
+        for (int q = 0; q < 3; ++q) {
+            if (!dsts[q]) continue;
+            GFX_HIP(hipMemcpy(h.data(), srcs[q] + int64_t(field) * s->fstride,
+                              h.size() * sizeof(float), hipMemcpyDeviceToHost));
+            for (int g = 0; g < s->G; ++g) {
+                const int64_t n = s->node_new_of_old[g];
+                for (int c = 0; c < 3; ++c) dsts[q][int64_t(g) * 3 + c] = h[n * 3 + c];
+            }
+        }
+    });
+}
+
+extern "C" gfx_status gfx_solver_set_state(gfx_solver* s, int field, const double* u,
+                                           const double* v, const double* a) {
+    return guarded([&] {
+        check_field(s, field);
+        require(u && v && a, "u, v and a are required");
+        const size_t n3 = size_t(s->G) * 3;
+        std::vector<float> hu(n3), hv(n3), ha(n3), ht(n3);
+        const double dt = s->dt;
+        for (int g = 0; g < s->G; ++g) {
+            const int64_t n = s->node_new_of_old[g];
+            for (int c = 0; c < 3; ++c) {
+                const int64_t o = int64_t(g) * 3 + c;
+                hu[n * 3 + c] = float(u[o]);
+                hv[n * 3 + c] = float(v[o]);
+                ha[n * 3 + c] = float(a[o]);
+                ht[n * 3 + c] = float(u[o] + dt * v[o] + 0.5 * dt * dt * a[o]);  // predictor
+            }
+        }
+        GFX_HIP(hipStreamSynchronize(s->stream));
+        const int64_t fo = int64_t(field) * s->fstride;
+        GFX_HIP(hipMemcpy(s->ucur() + fo, hu.data(), n3 * 4, hipMemcpyHostToDevice));
+        GFX_HIP(hipMemcpy(s->unext() + fo, ht.data(), n3 * 4, hipMemcpyHostToDevice));
+        GFX_HIP(hipMemcpy(s->V.p + fo, hv.data(), n3 * 4, hipMemcpyHostToDevice));
+        GFX_HIP(hipMemcpy(s->A.p + fo, ha.data(), n3 * 4, hipMemcpyHostToDevice));
+    });
+}
+
+extern "C" gfx_status gfx_solver_get_record(const gfx_solver* s, int field, int snap,
+                                            double* strain, double* disp, double* vel,
+                                            double* acc) {
+    return guarded([&] {
+        check_field(s, field);
+        require(s->rec_cap > 0 && s->n_rec > 0, "recording is off");
+        require(snap >= 0 && snap < s->n_snap && snap >= s->n_snap - s->rec_cap,
+                "snapshot not held in the record ring");
+        require(!(disp || vel || acc) || s->rec_uva_on, "u, v, a records are off");
+        const int slot = snap % s->rec_cap;
+        const int64_t nrp = int64_t(s->n_rec) * NP;
+        GFX_HIP(hipStreamSynchronize(s->stream));
+        if (strain) {
+            std::vector<float> h(nrp * 6);
+            GFX_HIP(hipMemcpy(h.data(), s->rec_strain.p + (int64_t(slot) * s->nf + field) * nrp * 6,
+                              h.size() * 4, hipMemcpyDeviceToHost));
+            for (size_t q = 0; q < h.size(); ++q) strain[q] = h[q];
+        }
+        if (disp || vel || acc) {
+            std::vector<float> h(nrp * 9);
+            GFX_HIP(hipMemcpy(h.data(), s->rec_uva.p + (int64_t(slot) * s->nf + field) * nrp * 9,
+                              h.size() * 4, hipMemcpyDeviceToHost));
+            for (int64_t q = 0; q < nrp; ++q)
+                for (int c = 0; c < 3; ++c) {
+                    if (disp) disp[q * 3 + c] = h[q * 9 + c];
+                    if (vel) vel[q * 3 + c] = h[q * 9 + 3 + c];
+                    if (acc) acc[q * 3 + c] = h[q * 9 + 6 + c];
+                }
+        }
+    });
+}
+
+extern "C" gfx_status gfx_solver_set_timing(gfx_solver* s, int enable) {
+    return guarded([&] {
+        require(s != nullptr, "null solver");
+        s->timing = enable != 0;
+    });
+}
+
+extern "C" gfx_status gfx_solver_kernel_time(const gfx_solver* s, const char* name, double* ms,
+                                             int* launches) {
+    return guarded([&] {
+        require(s && name && ms && launches, "null argument");
+        const std::string n(name);
+        if (n == "element") {
+            *ms = s->t_elem;
+            *launches = s->n_elem_launch;
+        } else if (n == "node") {
+            *ms = s->t_node;
+            *launches = s->n_node_launch;
+        } else if (n == "record") {
+            *ms = s->t_rec;
+            *launches = s->n_rec_launch;
+        } else if (n == "total") {
+            *ms = s->t_total;
+            *launches = s->n_total_launch;
+        } else {
+            throw Error(GFX_ERR_INVALID, "unknown kernel name: " + n);
+        }
+    });
+}
+
+extern "C" double gfx_solver_algorithmic_bytes(const gfx_solver* s, int which) {
+    if (!s) return 0.0;
+    const double P = double(s->E) * NP, G = s->G, F = s->nf, Q = s->Ppml, Gp = s->G_pml;
+    // element stage: geometry 9+1+1 floats + ibool per point; per field the
+    // predicted displacement read and residual write (12 B each per node);
+    // C-PML: alpha 9 + scaled abar 5 floats per PML point, per field the
+    // memory (9) and "new" (3) read and written, and u, v, a read at PML nodes.
+    const double elem = 48.0 * P + F * 24.0 * G + 56.0 * Q + F * (96.0 * Q + 36.0 * Gp);
+    // node stage: 1/m per node; per field r, u_{n+1}, v, a read and
+    // v, a, predicted u, zeroed r written (12 B each).
+    const double node = 4.0 * G + F * 96.0 * G;
+    if (which == 1) return elem;
+    if (which == 2) return node;
+    return elem + node;
+}

@@ -1,0 +1,396 @@
+"""ctypes bindings of libgfx.so (the MI355X elastic CG-SEM solver C ABI).
+
+The library is built in-tree by ``make -C gf-only-sem_amd``; importing this
+module fails loudly when it is missing (there is no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libgfx.so")
+
+GFX_ASSEMBLY_COLORED = 0
+GFX_ASSEMBLY_ATOMIC = 1
+MAX_LAYERS = 8
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class GfxError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"gfx status {status}: {msg}")
+        self.status = status
+
+
+class BoxSpec(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double),
+        ("order", C.c_int),
+        ("n_layers", C.c_int),
+        ("layer_zbot", C.c_double * MAX_LAYERS),
+        ("vp", C.c_double * MAX_LAYERS),
+        ("vs", C.c_double * MAX_LAYERS),
+        ("rho", C.c_double * MAX_LAYERS),
+        ("rf_amp", C.c_double), ("rf_corr_len", C.c_double),
+        ("rf_modes", C.c_int),
+        ("rf_seed", C.c_uint64),
+        ("pml_xmin", C.c_int), ("pml_xmax", C.c_int), ("pml_ymin", C.c_int),
+        ("pml_ymax", C.c_int), ("pml_zmax", C.c_int),
+        ("f0_pml", C.c_double),
+        ("jitter", C.c_double),
+        ("jitter_seed", C.c_uint64),
+        ("cfl_safety", C.c_double), ("output_dt", C.c_double), ("duration", C.c_double),
+        ("record_depth_max", C.c_double),
+        ("want_coords", C.c_int),
+        ("want_beta", C.c_int),
+    ]
+
+
+class _Mesh(C.Structure):
+    _fields_ = [
+        ("n_elem", C.c_int), ("ngll", C.c_int), ("n_node_glob", C.c_int),
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("xmin", C.c_double), ("xmax", C.c_double), ("ymin", C.c_double),
+        ("ymax", C.c_double), ("zmin", C.c_double), ("zmax", C.c_double),
+        ("coords", _dp), ("jacobian", _dp), ("dxi_dx", _dp), ("mass", _dp),
+        ("vp", _dp), ("vs", _dp), ("density", _dp), ("lambda_", _dp), ("mu_", _dp),
+        ("damping", _dp), ("ibool", _ip), ("node_xyz", _dp),
+        ("pml_region", _ip), ("coef_alpha", _dp), ("coef_beta", _dp), ("coef_abar", _dp),
+        ("n_pml_elem", C.c_int),
+        ("h_min", C.c_double), ("vp_max", C.c_double), ("cfl_dt", C.c_double), ("dt", C.c_double),
+        ("snapshot_stride", C.c_int), ("nsteps", C.c_int),
+        ("record_depth_actual", C.c_double),
+        ("n_rec_cell", C.c_int),
+        ("rec_cell", _ip),
+    ]
+
+
+class SolverDesc(C.Structure):
+    _fields_ = [
+        ("n_elem", C.c_int), ("ngll", C.c_int), ("n_rank_node", C.c_int),
+        ("dxi_dx", _dp), ("jacobian", _dp), ("lambda_", _dp), ("mu_", _dp),
+        ("mass", _dp), ("density", _dp), ("damping", _dp), ("ibool", _ip),
+        ("has_cpml", C.c_int), ("pml_region", _ip), ("coef_alpha", _dp), ("coef_abar", _dp),
+        ("dt", C.c_double), ("snapshot_stride", C.c_int),
+        ("n_rec_cell", C.c_int), ("rec_cell", _ip),
+        ("n_fields", C.c_int), ("assembly", C.c_int), ("elems_per_block", C.c_int),
+        ("record_capacity", C.c_int), ("record_uva", C.c_int),
+    ]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run `make -C gf-only-sem_amd` (or __graft_entry__.build())")
+    lib = C.CDLL(LIB_PATH)
+    st = C.c_int
+    vp = C.c_void_p
+    sig = {
+        "gfx_last_error": (C.c_char_p, []),
+        "gfx_version": (C.c_char_p, []),
+        "gfx_device_count": (C.c_int, []),
+        "gfx_compute_element_residual": (st, [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp, _dp]),
+        "gfx_compute_element_residual_f32": (st, [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp, _dp]),
+        "gfx_solver_create": (st, [C.POINTER(SolverDesc), C.POINTER(vp)]),
+        "gfx_solver_destroy": (st, [vp]),
+        "gfx_solver_set_source": (st, [vp, C.c_int, C.c_int, C.c_int, _ip, _dp, C.c_int, _dp]),
+        "gfx_solver_reset": (st, [vp]),
+        "gfx_solver_run": (st, [vp, C.c_int]),
+        "gfx_solver_step_index": (C.c_int, [vp]),
+        "gfx_solver_get_state": (st, [vp, C.c_int, _dp, _dp, _dp]),
+        "gfx_solver_set_state": (st, [vp, C.c_int, _dp, _dp, _dp]),
+        "gfx_solver_n_snapshots": (C.c_int, [vp]),
+        "gfx_solver_get_record": (st, [vp, C.c_int, C.c_int, _dp, _dp, _dp, _dp]),
+        "gfx_solver_kernel_time": (st, [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+        "gfx_solver_set_timing": (st, [vp, C.c_int]),
+        "gfx_solver_algorithmic_bytes": (C.c_double, [vp, C.c_int]),
+        "gfx_box_spec_default": (st, [C.POINTER(BoxSpec)]),
+        "gfx_mesh_build_box": (st, [C.POINTER(BoxSpec), C.POINTER(C.POINTER(_Mesh))]),
+        "gfx_mesh_free": (st, [C.POINTER(_Mesh)]),
+        "gfx_mesh_locate_source": (st, [C.POINTER(_Mesh), C.c_double, C.c_double, C.c_double,
+                                        C.c_int, _ip, _dp, C.POINTER(C.c_int)]),
+        "gfx_ricker": (st, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, _dp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _check(status: int):
+    if status != 0:
+        raise GfxError(status, lib.gfx_last_error().decode())
+
+
+def _ptr(a: np.ndarray | None, ct=C.c_double):
+    if a is None:
+        return C.cast(None, C.POINTER(ct))
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def device_count() -> int:
+    return int(lib.gfx_device_count())
+
+
+def element_residual(dxi_dx, jacobian, lambda_, mu_, D, weights, ngll, u, r, fp32=False):
+    """gf::compute_element_residual<BackendHIP>: r (float64, in/out) -= K u."""
+    arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (dxi_dx, jacobian, lambda_, mu_, D, weights, u)]
+    assert r.dtype == np.float64 and r.flags.c_contiguous
+    n_elem = arrs[1].size // (ngll ** 3)
+    fn = lib.gfx_compute_element_residual_f32 if fp32 else lib.gfx_compute_element_residual
+    _check(fn(n_elem, *[_ptr(a) for a in arrs[:6]], ngll, _ptr(arrs[6]), _ptr(r)))
+
+
+def ricker(f0, t0, amplitude, dt, n):
+    out = np.zeros(n, dtype=np.float64)
+    _check(lib.gfx_ricker(f0, t0, amplitude, dt, n, _ptr(out)))
+    return out
+
+
+@dataclass
+class Mesh:
+    """Structured-box inputs in the reference partition layout (numpy copies)."""
+    n_elem: int
+    ngll: int
+    n_node: int
+    nx: int
+    ny: int
+    nz: int
+    bounds: tuple
+    arrays: dict
+    dt: float
+    snapshot_stride: int
+    nsteps: int
+    h_min: float
+    vp_max: float
+    cfl_dt: float
+    n_pml_elem: int
+    record_depth_actual: float
+    rec_cell: np.ndarray
+    _handle: object = field(default=None, repr=False)
+
+    def __getattr__(self, k):
+        arrays = self.__dict__.get("arrays", {})
+        if k in arrays:
+            return arrays[k]
+        raise AttributeError(k)
+
+    def locate_source(self, x, y, z, max_cells=8):
+        nn = self.ngll ** 3
+        cells = np.zeros(max_cells, dtype=np.int32)
+        w = np.zeros(max_cells * nn, dtype=np.float64)
+        n = C.c_int(0)
+        _check(lib.gfx_mesh_locate_source(self._handle, x, y, z, max_cells, _ptr(cells, C.c_int32),
+                                          _ptr(w), C.byref(n)))
+        return cells[: n.value].copy(), w[: n.value * nn].copy()
+
+    def close(self):
+        if self._handle is not None:
+            lib.gfx_mesh_free(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def default_spec() -> BoxSpec:
+    s = BoxSpec()
+    _check(lib.gfx_box_spec_default(C.byref(s)))
+    return s
+
+
+def make_spec(**kw) -> BoxSpec:
+    """Box spec from defaults; `layers` = [(zbot, vp, vs, rho), ...] (last zbot ignored)."""
+    s = default_spec()
+    layers = kw.pop("layers", None)
+    for k, v in kw.items():
+        if not hasattr(s, k):
+            raise KeyError(k)
+        setattr(s, k, v)
+    if layers is not None:
+        s.n_layers = len(layers)
+        for l, (zb, vp, vs, rho) in enumerate(layers):
+            s.layer_zbot[l] = zb
+            s.vp[l] = vp
+            s.vs[l] = vs
+            s.rho[l] = rho
+    return s
+
+
+def build_box(spec: BoxSpec | None = None, keep_handle=True, **kw) -> Mesh:
+    if spec is None:
+        spec = make_spec(**kw)
+    h = C.POINTER(_Mesh)()
+    _check(lib.gfx_mesh_build_box(C.byref(spec), C.byref(h)))
+    m = h.contents
+    E, ng = m.n_elem, m.ngll
+    nn = ng ** 3
+    P = E * nn
+    G = m.n_node_glob
+
+    def arr(p, n, dt=np.float64):
+        if not p:
+            return None
+        return np.ctypeslib.as_array(p, shape=(n,)).astype(dt, copy=True)
+
+    arrays = {
+        "coords": arr(m.coords, P * 3),
+        "jacobian": arr(m.jacobian, P),
+        "dxi_dx": arr(m.dxi_dx, P * 9),
+        "mass": arr(m.mass, P),
+        "vp": arr(m.vp, P),
+        "vs": arr(m.vs, P),
+        "density": arr(m.density, P),
+        "lambda_": arr(m.lambda_, P),
+        "mu_": arr(m.mu_, P),
+        "damping": arr(m.damping, P),
+        "ibool": arr(m.ibool, P, np.int32),
+        "node_xyz": arr(m.node_xyz, G * 3),
+        "pml_region": arr(m.pml_region, E, np.int32),
+        "coef_alpha": arr(m.coef_alpha, P * 9),
+        "coef_beta": arr(m.coef_beta, P * 9),
+        "coef_abar": arr(m.coef_abar, P * 5),
+    }
+    rec = arr(m.rec_cell, m.n_rec_cell, np.int32)
+    mesh = Mesh(n_elem=E, ngll=ng, n_node=G, nx=m.nx, ny=m.ny, nz=m.nz,
+                bounds=(m.xmin, m.xmax, m.ymin, m.ymax, m.zmin, m.zmax), arrays=arrays,
+                dt=m.dt, snapshot_stride=m.snapshot_stride, nsteps=m.nsteps, h_min=m.h_min,
+                vp_max=m.vp_max, cfl_dt=m.cfl_dt, n_pml_elem=m.n_pml_elem,
+                record_depth_actual=m.record_depth_actual,
+                rec_cell=rec if rec is not None else np.zeros(0, np.int32), _handle=h)
+    if not keep_handle:
+        mesh.close()
+    return mesh
+
+
+class Solver:
+    """Device-resident solver over one mesh and n_fields right-hand sides."""
+
+    def __init__(self, mesh: Mesh, n_fields=1, assembly=GFX_ASSEMBLY_COLORED, elems_per_block=0,
+                 record_capacity=None, record_uva=False, has_cpml=True, dt=None,
+                 snapshot_stride=None, rec_cell=None):
+        a = mesh.arrays
+        self._keep = []
+
+        def keep(x, dt=np.float64):
+            if x is None:
+                return None
+            x = np.ascontiguousarray(x, dtype=dt)
+            self._keep.append(x)
+            return x
+
+        rec = mesh.rec_cell if rec_cell is None else np.asarray(rec_cell, np.int32)
+        d = SolverDesc()
+        d.n_elem = mesh.n_elem
+        d.ngll = mesh.ngll
+        d.n_rank_node = mesh.n_node
+        d.dxi_dx = _ptr(keep(a["dxi_dx"]))
+        d.jacobian = _ptr(keep(a["jacobian"]))
+        d.lambda_ = _ptr(keep(a["lambda_"]))
+        d.mu_ = _ptr(keep(a["mu_"]))
+        d.mass = _ptr(keep(a["mass"]))
+        d.density = _ptr(keep(a["density"]))
+        d.damping = _ptr(keep(a["damping"]))
+        d.ibool = _ptr(keep(a["ibool"], np.int32), C.c_int32)
+        d.has_cpml = 1 if has_cpml else 0
+        d.pml_region = _ptr(keep(a["pml_region"], np.int32), C.c_int32)
+        d.coef_alpha = _ptr(keep(a["coef_alpha"]))
+        d.coef_abar = _ptr(keep(a["coef_abar"]))
+        d.dt = mesh.dt if dt is None else dt
+        d.snapshot_stride = mesh.snapshot_stride if snapshot_stride is None else snapshot_stride
+        rk = keep(rec, np.int32)
+        d.n_rec_cell = rk.size
+        d.rec_cell = _ptr(rk, C.c_int32)
+        d.n_fields = n_fields
+        d.assembly = assembly
+        d.elems_per_block = elems_per_block
+        if record_capacity is None:
+            record_capacity = 0 if rk.size == 0 else (mesh.nsteps // max(d.snapshot_stride, 1) + 1)
+        d.record_capacity = record_capacity
+        d.record_uva = 1 if record_uva else 0
+        h = C.c_void_p()
+        _check(lib.gfx_solver_create(C.byref(d), C.byref(h)))
+        self._keep = []
+        self.h = h
+        self.mesh = mesh
+        self.n_fields = n_fields
+        self.n_rec = int(rk.size)
+        self.ngll = mesh.ngll
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.gfx_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_source(self, field, direction, cells, weights, stf):
+        cells = np.ascontiguousarray(cells, np.int32)
+        weights = np.ascontiguousarray(weights, np.float64)
+        stf = np.ascontiguousarray(stf, np.float64)
+        _check(lib.gfx_solver_set_source(self.h, field, direction, cells.size, _ptr(cells, C.c_int32),
+                                         _ptr(weights), stf.size, _ptr(stf)))
+
+    def reset(self):
+        _check(lib.gfx_solver_reset(self.h))
+
+    def run(self, n_steps):
+        _check(lib.gfx_solver_run(self.h, int(n_steps)))
+
+    @property
+    def step(self):
+        return int(lib.gfx_solver_step_index(self.h))
+
+    @property
+    def n_snapshots(self):
+        return int(lib.gfx_solver_n_snapshots(self.h))
+
+    def get_state(self, field=0):
+        n = self.mesh.n_node * 3
+        u, v, a = (np.zeros(n) for _ in range(3))
+        _check(lib.gfx_solver_get_state(self.h, field, _ptr(u), _ptr(v), _ptr(a)))
+        return u, v, a
+
+    def set_state(self, field, u, v, a):
+        u, v, a = (np.ascontiguousarray(x, np.float64) for x in (u, v, a))
+        _check(lib.gfx_solver_set_state(self.h, field, _ptr(u), _ptr(v), _ptr(a)))
+
+    def get_record(self, field, snap, uva=False):
+        nn = self.ngll ** 3
+        s = np.zeros(self.n_rec * nn * 6)
+        if uva:
+            u, v, a = (np.zeros(self.n_rec * nn * 3) for _ in range(3))
+            _check(lib.gfx_solver_get_record(self.h, field, snap, _ptr(s), _ptr(u), _ptr(v), _ptr(a)))
+            return s, u, v, a
+        _check(lib.gfx_solver_get_record(self.h, field, snap, _ptr(s), _ptr(None), _ptr(None), _ptr(None)))
+        return s
+
+    def set_timing(self, on=True):
+        _check(lib.gfx_solver_set_timing(self.h, 1 if on else 0))
+
+    def kernel_time(self, name):
+        ms = C.c_double(0)
+        n = C.c_int(0)
+        _check(lib.gfx_solver_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def algorithmic_bytes(self, which=0):
+        return float(lib.gfx_solver_algorithmic_bytes(self.h, which))

@@ -1,0 +1,238 @@
+/*
+ * gfx.h -- C ABI of the MI355X-native elastic CG-SEM forward solver.
+ *
+ * Plain C: pointers and sizes only, no C++ or torch types.  Every entry point
+ * returns a gfx_status; on failure gfx_last_error() gives a message (thread
+ * local).  Nothing in this library aborts the process.
+ *
+ * Array layouts on the host side are the reference's own
+ * (forward/share/include/gf/types.hpp:27-95): element-major, per element
+ * point index n = (i*NGLL + j)*NGLL + k, dxi_dx[e][n][m*3+p] = d(xi_p)/d(x_m),
+ * vectors interleaved xyz, node ids 0-based (local_cell2global_node).
+ */
+#ifndef GFX_H
+#define GFX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    GFX_OK = 0,
+    GFX_ERR_INVALID = 1,     /* bad argument or shape */
+    GFX_ERR_DEVICE = 2,      /* HIP runtime error */
+    GFX_ERR_UNSUPPORTED = 3, /* valid request this build does not implement */
+    GFX_ERR_NOMEM = 4,       /* host or device allocation failed */
+    GFX_ERR_IO = 5           /* file input/output (HDF5) failed */
+} gfx_status;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* gfx_last_error(void);
+/* Library version string. */
+const char* gfx_version(void);
+/* Number of HIP devices visible (0 when no GPU); never fails. */
+int gfx_device_count(void);
+
+/* -------------------------------------------------------------------------
+ * Element plugin.
+ * Replaces gf::compute_element_residual<Backend>
+ * (forward/share/include/gf/element.hpp:39-42; CUDA specialisation
+ * forward/elastic/src/element_cuda.cu:195-252).  Host FP64 pointers,
+ * element-major AoS, r ACCUMULATES the internal force (r -= K u), as the CPU
+ * specialisation does (forward/elastic/src/element_cpu.cpp:116).  The device
+ * computes in FP64, so results match the CPU backend to rounding.
+ * Any NGLL in [2, 8] is accepted.
+ * ---------------------------------------------------------------------- */
+gfx_status gfx_compute_element_residual(int n_elem, const double* dxi_dx, const double* jacobian,
+                                        const double* lambda_, const double* mu_,
+                                        const double* D, const double* weights, int ngll,
+                                        const double* u, double* r);
+
+/* Same contract, FP32 device arithmetic (the solver's arithmetic); inputs are
+ * still the reference's FP64 host arrays. NGLL must be 5. */
+gfx_status gfx_compute_element_residual_f32(int n_elem, const double* dxi_dx,
+                                            const double* jacobian, const double* lambda_,
+                                            const double* mu_, const double* D,
+                                            const double* weights, int ngll, const double* u,
+                                            double* r);
+
+/* -------------------------------------------------------------------------
+ * Device-resident solver state.
+ * Replaces CudaDeviceState + cuda_allocate_state/cuda_free_state and the
+ * per-step host wrappers (forward/share/include/gf/cuda_step.hpp:20-129),
+ * and adds what the CUDA path lacks: the C-PML memory update of the CPU path
+ * (forward/share/src/pml.cpp:55-216), device strain recording
+ * (forward/share/src/solver.cpp:282-335,640-675) and many right-hand sides
+ * ("fields") advanced by one launch.
+ * ---------------------------------------------------------------------- */
+typedef struct gfx_solver gfx_solver;
+
+typedef enum {
+    GFX_ASSEMBLY_COLORED = 0, /* deterministic: element colouring, plain read-modify-write */
+    GFX_ASSEMBLY_ATOMIC = 1   /* float atomics; order not deterministic */
+} gfx_assembly;
+
+typedef struct {
+    int n_elem;              /* local cells */
+    int ngll;                /* must be 5 (polynomial order 4) */
+    int n_rank_node;         /* unique nodes, ids in [0, n_rank_node) */
+    /* [n_elem*ngll^3*...] reference layout, FP64 */
+    const double* dxi_dx;    /* x9 */
+    const double* jacobian;
+    const double* lambda_;
+    const double* mu_;
+    const double* mass;      /* element-local lumped mass, already times density */
+    const double* density;   /* used by C-PML only */
+    const double* damping;   /* legacy ramp; used only when has_cpml == 0 */
+    const int32_t* ibool;    /* local_cell2global_node */
+    /* C-PML (pml_coef_abar non-empty in the partition file => has_cpml) */
+    int has_cpml;
+    const int32_t* pml_region; /* [n_elem] 0 = interior */
+    const double* coef_alpha;  /* x9 */
+    const double* coef_abar;   /* x5 */
+    /* time stepping */
+    double dt;
+    int snapshot_stride;
+    /* recording cells (local element indices) */
+    int n_rec_cell;
+    const int32_t* rec_cell;
+    /* batching */
+    int n_fields;            /* right-hand sides advanced together (>= 1) */
+    int assembly;            /* gfx_assembly */
+    int elems_per_block;     /* 1, 2, 4 or 8; 0 = library default */
+    /* device record ring: snapshots kept (0 = no recording); u,v,a copies on/off */
+    int record_capacity;
+    int record_uva;
+} gfx_solver_desc;
+
+gfx_status gfx_solver_create(const gfx_solver_desc* desc, gfx_solver** out);
+gfx_status gfx_solver_destroy(gfx_solver* s);
+
+/* Point-force source of one field: the cells containing the point with their
+ * normalised Lagrange weights [n_src_cell*ngll^3] (config.h5 /source/cells),
+ * the force direction (0=x,1=y,2=z) and stf[n_stf] = f(step*dt) (as
+ * forward/share/src/solver.cpp:576-605). */
+gfx_status gfx_solver_set_source(gfx_solver* s, int field, int direction, int n_src_cell,
+                                 const int32_t* src_elem, const double* src_weights, int n_stf,
+                                 const double* stf);
+
+/* Zero u, v, a, C-PML memory, step counter and record buffers. */
+gfx_status gfx_solver_reset(gfx_solver* s);
+
+/* Advance n_steps steps.  At every step with step % snapshot_stride == 0 the
+ * strain (and u, v, a) of the recording cells is written to the device
+ * record buffer for that snapshot. */
+gfx_status gfx_solver_run(gfx_solver* s, int n_steps);
+
+/* Current step counter (number of steps taken). */
+int gfx_solver_step_index(const gfx_solver* s);
+
+/* Copy a field's nodal state to host FP64 [n_rank_node*3]; NULL skips
+ * (cuda_copy_state_to_host, forward/share/src/cuda_step.cu:260-279). */
+gfx_status gfx_solver_get_state(const gfx_solver* s, int field, double* u, double* v, double* a);
+/* Load a field's nodal state from host FP64 [n_rank_node*3] (for restart). */
+gfx_status gfx_solver_set_state(gfx_solver* s, int field, const double* u, const double* v,
+                                const double* a);
+
+/* Number of snapshots held so far. */
+int gfx_solver_n_snapshots(const gfx_solver* s);
+/* Copy snapshot `snap` of `field`: strain [n_rec_cell][ngll^3][6], and u, v, a
+ * [n_rec_cell][ngll^3][3] (any may be NULL), FP64 like the reference's write
+ * buffers (forward/share/src/solver.cpp:651-675). */
+gfx_status gfx_solver_get_record(const gfx_solver* s, int field, int snap, double* strain,
+                                 double* disp, double* vel, double* acc);
+
+/* Per-kernel device time of the last gfx_solver_run (HIP events on the
+ * solver's stream).  name is "element", "node", "record" or "total";
+ * returns milliseconds summed over the run and the launch count. */
+gfx_status gfx_solver_kernel_time(const gfx_solver* s, const char* name, double* ms,
+                                  int* launches);
+/* Enable/disable per-kernel event timing (default off). */
+gfx_status gfx_solver_set_timing(gfx_solver* s, int enable);
+
+/* Algorithmic HBM bytes per step for the current problem (each array counted
+ * once per step).  which: 0 = whole step, 1 = element kernel, 2 = node kernel. */
+double gfx_solver_algorithmic_bytes(const gfx_solver* s, int which);
+
+/* -------------------------------------------------------------------------
+ * Structured-box inputs (restates preprocess/ geometry, material, CFL,
+ * C-PML, source-location and recording-map rules for regular hex boxes;
+ * the reference preprocessor itself is Python and out of scope).
+ * ---------------------------------------------------------------------- */
+#define GFX_MAX_LAYERS 8
+
+typedef struct {
+    int nx, ny, nz;          /* elements per axis */
+    double lx, ly, lz;       /* box size (m); x,y in [0,l], z in [0,lz] downward, free surface z=0 */
+    int order;               /* polynomial order N (ngll = N+1) */
+    /* material: n_layers >= 1 layers; layer l holds z < ztop[l+1] (last is halfspace) */
+    int n_layers;
+    double layer_zbot[GFX_MAX_LAYERS]; /* bottom depth of layer l (ignored for the last) */
+    double vp[GFX_MAX_LAYERS], vs[GFX_MAX_LAYERS], rho[GFX_MAX_LAYERS];
+    /* random velocity perturbation: vp,vs *= (1 + rf_amp * f), f ~ Gaussian field */
+    double rf_amp, rf_corr_len;
+    int rf_modes;
+    uint64_t rf_seed;
+    /* C-PML thickness in elements (zmin is the free surface and has none) */
+    int pml_xmin, pml_xmax, pml_ymin, pml_ymax, pml_zmax;
+    double f0_pml;           /* Hz */
+    /* interior vertex jitter as a fraction of the element size (0 = regular box) */
+    double jitter;
+    uint64_t jitter_seed;
+    /* time stepping (cfl_validator.py): dt = output_dt/stride <= cfl*h_min/vp_max */
+    double cfl_safety, output_dt, duration;
+    double record_depth_max;  /* m; 0 = no recording */
+    int want_coords;          /* produce coords/vp/vs (x3 memory) */
+    int want_beta;            /* produce pml_coef_beta */
+} gfx_box_spec;
+
+typedef struct {
+    int n_elem, ngll, n_node_glob;
+    int nx, ny, nz;
+    double xmin, xmax, ymin, ymax, zmin, zmax;
+    double* coords;     /* [E,n,3] or NULL */
+    double* jacobian;   /* [E,n]   */
+    double* dxi_dx;     /* [E,n,9] */
+    double* mass;       /* [E,n]   */
+    double* vp;         /* [E,n] or NULL */
+    double* vs;         /* [E,n] or NULL */
+    double* density;    /* [E,n]   */
+    double* lambda_;    /* [E,n]   */
+    double* mu_;        /* [E,n]   */
+    double* damping;    /* [E,n]   */
+    int32_t* ibool;     /* [E*n]   */
+    double* node_xyz;   /* [G,3] global node coordinates */
+    int32_t* pml_region;  /* [E] */
+    double* coef_alpha;   /* [E,n,9] */
+    double* coef_beta;    /* [E,n,9] or NULL */
+    double* coef_abar;    /* [E,n,5] */
+    int n_pml_elem;
+    double h_min, vp_max, cfl_dt, dt;
+    int snapshot_stride, nsteps;
+    double record_depth_actual;
+    int n_rec_cell;
+    int32_t* rec_cell;  /* [n_rec_cell] */
+} gfx_mesh;
+
+gfx_status gfx_box_spec_default(gfx_box_spec* spec);
+gfx_status gfx_mesh_build_box(const gfx_box_spec* spec, gfx_mesh** out);
+gfx_status gfx_mesh_free(gfx_mesh* m);
+
+/* Locate a point force (preprocess/source_locator.py:244-375): surface mode
+ * when z == zmin (free-surface cells), else buried mode (non-PML cells).
+ * Fills up to max_cells cells and weights [n_src*ngll^3], normalised to sum 1
+ * over all containing cells.  *n_src receives the count. */
+gfx_status gfx_mesh_locate_source(const gfx_mesh* m, double x, double y, double z, int max_cells,
+                                  int32_t* cells, double* weights, int* n_src);
+
+/* Ricker wavelet A (1 - 2a^2) exp(-a^2), a = pi f0 (t - t0), sampled at
+ * t = i*dt for i < n (preprocess/stf_evaluator.py:19-45). */
+gfx_status gfx_ricker(double f0, double t0, double amplitude, double dt, int n, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

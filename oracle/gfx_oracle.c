@@ -1,0 +1,560 @@
+/*
+ * gfx_oracle.c -- CPU FP64 restatement of the reference's elastic CG-SEM
+ * forward step.  TEST INFRASTRUCTURE ONLY (parity checker); see gfx_oracle.h
+ * for which checks pin each function.
+ *
+ * Every function cites the reference file:line whose behaviour it restates,
+ * including the quirks the parity target depends on:
+ *   - the source at loop index `step` uses f(step*dt), i.e. f(t_n)
+ *     (forward/share/src/solver.cpp:579-582);
+ *   - the C-PML acceleration term is ADDED to the residual using u_n, v_n
+ *     (forward/share/src/pml.cpp:197-212);
+ *   - pml_displ_old mixes step indices: previous "new" plus c2*a_n
+ *     (forward/share/src/pml.cpp:66-109);
+ *   - a GLL point with mu <= 0 contributes nothing
+ *     (forward/elastic/src/element_cpu.cpp:42-43);
+ *   - a node with mass <= 0 gets a = 0 and keeps u, v
+ *     (forward/share/src/solver.cpp:57-61).
+ */
+#include "gfx_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------ */
+/* GLL quadrature: forward/share/include/gf/gll.hpp                    */
+/* ------------------------------------------------------------------ */
+
+/* Bonnet recurrence for P_n(x) (gll.hpp:14-26). */
+static double legendre(int n, double x) {
+    if (n == 0) return 1.0;
+    if (n == 1) return x;
+    double p0 = 1.0, p1 = x, pn = 0.0;
+    for (int k = 1; k < n; ++k) {
+        pn = ((2.0 * k + 1.0) * x * p1 - k * p0) / (k + 1.0);
+        p0 = p1;
+        p1 = pn;
+    }
+    return pn;
+}
+
+/* P'_n(x) from n (x P_n - P_{n-1}) / (x^2 - 1), endpoint limits (gll.hpp:29-41). */
+static double legendre_prime(int n, double x) {
+    if (n == 0) return 0.0;
+    if (fabs(x - 1.0) < 1e-14) return 0.5 * n * (n + 1.0);
+    if (fabs(x + 1.0) < 1e-14) return ((n % 2 == 0) ? -1.0 : 1.0) * 0.5 * n * (n + 1.0);
+    return n * (x * legendre(n, x) - legendre(n - 1, x)) / (x * x - 1.0);
+}
+
+/* Interior nodes: Newton on P'_N from -cos(pi i/N) (gll.hpp:48-81). */
+int gfo_gll_nodes(int N, double* nodes) {
+    if (N < 1) return -1;
+    nodes[0] = -1.0;
+    nodes[N] = 1.0;
+    for (int i = 1; i < N; ++i) {
+        double xi = -cos(M_PI * i / N);
+        for (int it = 0; it < 100; ++it) {
+            double f = legendre_prime(N, xi);
+            double df = (2.0 * xi * f - N * (N + 1.0) * legendre(N, xi)) / (1.0 - xi * xi);
+            if (fabs(df) < 1.0e-14) return -2;
+            double dx = -f / df;
+            xi += dx;
+            if (fabs(dx) < 1.0e-15 * (1.0 + fabs(xi))) break;
+        }
+        nodes[i] = xi;
+    }
+    return 0;
+}
+
+/* w_i = 2 / (N (N+1) P_N(x_i)^2) (gll.hpp:87-97). */
+int gfo_gll_weights(int N, const double* nodes, double* w) {
+    if (N < 1) return -1;
+    for (int i = 0; i <= N; ++i) {
+        double pn = legendre(N, nodes[i]);
+        w[i] = 2.0 / (N * (N + 1.0) * pn * pn);
+    }
+    return 0;
+}
+
+/* D[i][j] = l_j'(x_i), row-major (gll.hpp:103-131). */
+int gfo_gll_derivative_matrix(int N, const double* nodes, double* D) {
+    if (N < 1) return -1;
+    const int n = N + 1;
+    double PN[64];
+    if (n > 64) return -1;
+    for (int i = 0; i < n; ++i) PN[i] = legendre(N, nodes[i]);
+    for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < n; ++j) {
+            if (i == j) {
+                if (i == 0)
+                    D[i * n + j] = -0.25 * N * (N + 1.0);
+                else if (i == N)
+                    D[i * n + j] = 0.25 * N * (N + 1.0);
+                else
+                    D[i * n + j] = 0.0;
+            } else {
+                D[i * n + j] = PN[i] / (PN[j] * (nodes[i] - nodes[j]));
+            }
+        }
+    }
+    return 0;
+}
+
+/* Lagrange basis values at xi (gll.hpp:136-147). */
+void gfo_lagrange_basis(double xi, int ngll, const double* nodes, double* ell) {
+    for (int j = 0; j < ngll; ++j) {
+        ell[j] = 1.0;
+        for (int k = 0; k < ngll; ++k) {
+            if (k == j) continue;
+            ell[j] *= (xi - nodes[k]) / (nodes[j] - nodes[k]);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Element internal force: forward/elastic/src/element_cpu.cpp:19-162  */
+/* ------------------------------------------------------------------ */
+
+static inline int nidx(int i, int j, int k, int n) { return (i * n + j) * n + k; }
+
+void gfo_element_residual(int n_elem, const double* dxi_dx, const double* jacobian,
+                          const double* lambda_, const double* mu_, const double* D,
+                          const double* weights, int ngll, const double* u, double* r) {
+    const int n_node = ngll * ngll * ngll;
+    for (long e = 0; e < n_elem; ++e) {
+        const double* edd = dxi_dx + e * n_node * 9;
+        const double* ejac = jacobian + e * n_node;
+        const double* elam = lambda_ + e * n_node;
+        const double* emu = mu_ + e * n_node;
+        const double* eu = u + e * n_node * 3;
+        double* er = r + e * n_node * 3;
+        for (int i = 0; i < ngll; ++i)
+            for (int j = 0; j < ngll; ++j)
+                for (int k = 0; k < ngll; ++k) {
+                    const int n = nidx(i, j, k, ngll);
+                    const double lam = elam[n], mu = emu[n];
+                    if (mu <= 0.0) continue; /* element_cpu.cpp:42-43 */
+                    const double* dd = edd + 9 * n;
+                    double gx[3] = {0, 0, 0}, ge[3] = {0, 0, 0}, gz[3] = {0, 0, 0};
+                    for (int s = 0; s < ngll; ++s) {
+                        const double di = D[i * ngll + s], dj = D[j * ngll + s],
+                                     dk = D[k * ngll + s];
+                        const int a = nidx(s, j, k, ngll), b = nidx(i, s, k, ngll),
+                                  c = nidx(i, j, s, ngll);
+                        for (int d = 0; d < 3; ++d) {
+                            gx[d] += di * eu[3 * a + d];
+                            ge[d] += dj * eu[3 * b + d];
+                            gz[d] += dk * eu[3 * c + d];
+                        }
+                    }
+                    double g[3][3];
+                    for (int c = 0; c < 3; ++c) {
+                        g[c][0] = gx[c] * dd[0] + ge[c] * dd[1] + gz[c] * dd[2];
+                        g[c][1] = gx[c] * dd[3] + ge[c] * dd[4] + gz[c] * dd[5];
+                        g[c][2] = gx[c] * dd[6] + ge[c] * dd[7] + gz[c] * dd[8];
+                    }
+                    double eps[3][3], sig[3][3];
+                    for (int l = 0; l < 3; ++l)
+                        for (int m = 0; m < 3; ++m) eps[l][m] = 0.5 * (g[l][m] + g[m][l]);
+                    const double tr = eps[0][0] + eps[1][1] + eps[2][2];
+                    for (int l = 0; l < 3; ++l) {
+                        for (int m = 0; m < 3; ++m) sig[l][m] = 2.0 * mu * eps[l][m];
+                        sig[l][l] += lam * tr;
+                    }
+                    const double fac = ejac[n] * weights[i] * weights[j] * weights[k];
+                    /* xi-, eta-, zeta-direction test-function derivatives */
+                    for (int dir = 0; dir < 3; ++dir) {
+                        for (int s = 0; s < ngll; ++s) {
+                            double Dq;
+                            int ns;
+                            if (dir == 0) {
+                                Dq = D[i * ngll + s];
+                                ns = nidx(s, j, k, ngll);
+                            } else if (dir == 1) {
+                                Dq = D[j * ngll + s];
+                                ns = nidx(i, s, k, ngll);
+                            } else {
+                                Dq = D[k * ngll + s];
+                                ns = nidx(i, j, s, ngll);
+                            }
+                            const double gn[3] = {Dq * dd[dir], Dq * dd[3 + dir], Dq * dd[6 + dir]};
+                            for (int c = 0; c < 3; ++c)
+                                er[3 * ns + c] -=
+                                    (sig[c][0] * gn[0] + sig[c][1] * gn[1] + sig[c][2] * gn[2]) *
+                                    fac;
+                        }
+                    }
+                }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Assembly: forward/share/src/assembly.cpp:10-41                      */
+/* ------------------------------------------------------------------ */
+
+void gfo_scatter_to_rank(const double* loc, const int32_t* ibool, int n_elem, int n_node,
+                         int n_rank_node, double* glob) {
+    memset(glob, 0, sizeof(double) * 3 * (size_t)n_rank_node); /* zero-then-add, :15 */
+    for (long e = 0; e < n_elem; ++e)
+        for (int n = 0; n < n_node; ++n) {
+            const long g = ibool[e * n_node + n];
+            for (int d = 0; d < 3; ++d) glob[g * 3 + d] += loc[(e * n_node + n) * 3 + d];
+        }
+}
+
+void gfo_gather_from_rank(const double* glob, const int32_t* ibool, int n_elem, int n_node,
+                          double* loc) {
+    for (long e = 0; e < n_elem; ++e)
+        for (int n = 0; n < n_node; ++n) {
+            const long g = ibool[e * n_node + n];
+            for (int d = 0; d < 3; ++d) loc[(e * n_node + n) * 3 + d] = glob[g * 3 + d];
+        }
+}
+
+/* ------------------------------------------------------------------ */
+/* Newmark: forward/share/src/solver.cpp:40-69                         */
+/* ------------------------------------------------------------------ */
+
+void gfo_newmark_predict(int n_dof, double dt, double beta, const double* u, const double* v,
+                         const double* a, double* ut) {
+    const double c = 0.5 * dt * dt * (1.0 - 2.0 * beta);
+    for (long i = 0; i < n_dof; ++i) ut[i] = u[i] + dt * v[i] + c * a[i];
+}
+
+void gfo_newmark_correct(int n_dof, double dt, double beta, double gamma, const double* mass,
+                         double* u, double* v, double* a, const double* r) {
+    for (long i = 0; i < n_dof; ++i) {
+        const double m = mass[i / 3];
+        if (m <= 0.0) {
+            a[i] = 0.0;
+            continue;
+        }
+        const double a_old = a[i];
+        const double a_new = r[i] / m;
+        u[i] += dt * v[i] + dt * dt * ((0.5 - beta) * a_old + beta * a_new);
+        v[i] += dt * ((1.0 - gamma) * a_old + gamma * a_new);
+        a[i] = a_new;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* C-PML: forward/share/src/pml.cpp                                    */
+/* ------------------------------------------------------------------ */
+
+#define GFO_THETA (1.0 / 8.0) /* pml.cpp:35 */
+
+static inline int region_of(const gfo_cpml* p, int e) {
+    return p->pml_region ? p->pml_region[e] : 0;
+}
+
+/* pml.cpp:55-110: swap(old,new); new = u + c1 v; old += c2 a. */
+void gfo_cpml_update_displ_fields(gfo_cpml* p, const double* u, const double* v, const double* a,
+                                  double dt) {
+    const int n_node = p->ngll * p->ngll * p->ngll;
+    const double c1 = (1.0 - 2.0 * GFO_THETA) * 0.5 * dt;
+    const double c2 = (1.0 - GFO_THETA) * 0.5 * dt * dt;
+    double* t = p->displ_old;
+    p->displ_old = p->displ_new;
+    p->displ_new = t;
+    for (long e = 0; e < p->n_elem; ++e) {
+        if (region_of(p, (int)e) == 0) continue;
+        for (int n = 0; n < n_node; ++n) {
+            const long g = p->ibool[e * n_node + n];
+            for (int d = 0; d < 3; ++d)
+                p->displ_new[(e * n_node + n) * 3 + d] = u[g * 3 + d] + c1 * v[g * 3 + d];
+        }
+    }
+    for (long e = 0; e < p->n_elem; ++e) {
+        if (region_of(p, (int)e) == 0) continue;
+        for (int n = 0; n < n_node; ++n) {
+            const long g = p->ibool[e * n_node + n];
+            for (int d = 0; d < 3; ++d) p->displ_old[(e * n_node + n) * 3 + d] += c2 * a[g * 3 + d];
+        }
+    }
+}
+
+/* pml.cpp:112-149: m[c*3+d] = a0_d m + a1_d new_c + a2_d old_c. */
+void gfo_cpml_update_displ_memory(gfo_cpml* p) {
+    const int n_node = p->ngll * p->ngll * p->ngll;
+    for (long e = 0; e < p->n_elem; ++e) {
+        if (region_of(p, (int)e) == 0) continue;
+        for (int n = 0; n < n_node; ++n) {
+            const long pt = e * n_node + n;
+            for (int c = 0; c < 3; ++c) {
+                const double nv = p->displ_new[pt * 3 + c];
+                const double ov = p->displ_old[pt * 3 + c];
+                for (int d = 0; d < 3; ++d) {
+                    const double* al = p->coef_alpha + pt * 9 + d * 3;
+                    double* m = p->rmemory + pt * 9 + c * 3 + d;
+                    *m = al[0] * (*m) + al[1] * nv + al[2] * ov;
+                }
+            }
+        }
+    }
+}
+
+/* pml.cpp:151-216: r_loc += w J/rho (A1 v + A2 u + A3 mx + A4 my + A5 mz). */
+void gfo_cpml_accel_contribution(const gfo_cpml* p, const double* u, const double* v,
+                                 const double* gll_w, double* r_loc) {
+    const int ngll = p->ngll;
+    const int n_node = ngll * ngll * ngll;
+    for (long e = 0; e < p->n_elem; ++e) {
+        if (region_of(p, (int)e) == 0) continue;
+        for (int n = 0; n < n_node; ++n) {
+            const int nk = n % ngll, nj = (n / ngll) % ngll, ni = n / (ngll * ngll);
+            const double wg = gll_w[ni] * gll_w[nj] * gll_w[nk];
+            const long pt = e * n_node + n;
+            const long g = p->ibool[pt];
+            const double rho = p->density[pt];
+            const double rho_inv = (rho > 0.0) ? 1.0 / rho : 0.0;
+            const double* A = p->coef_abar + pt * 5;
+            const double scale = wg * rho_inv * p->jacobian[pt];
+            for (int c = 0; c < 3; ++c) {
+                const double* m = p->rmemory + pt * 9 + c * 3;
+                const double acc = scale * (A[0] * v[g * 3 + c] + A[1] * u[g * 3 + c] +
+                                            A[2] * m[0] + A[3] * m[1] + A[4] * m[2]);
+                r_loc[pt * 3 + c] += acc;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Strain: forward/share/src/solver.cpp:282-335 (recording cells only) */
+/* ------------------------------------------------------------------ */
+
+void gfo_full_strain(int n_cell, const int32_t* cells, const double* dxi_dx, const double* D,
+                     int ngll, const double* u_loc_all, double* strain) {
+    const int n_node = ngll * ngll * ngll;
+    for (long ci = 0; ci < n_cell; ++ci) {
+        const long e = cells ? cells[ci] : ci;
+        const double* eu = u_loc_all + e * n_node * 3;
+        for (int i = 0; i < ngll; ++i)
+            for (int j = 0; j < ngll; ++j)
+                for (int k = 0; k < ngll; ++k) {
+                    const int n = nidx(i, j, k, ngll);
+                    const double* dd = dxi_dx + (e * n_node + n) * 9;
+                    double gx[3] = {0, 0, 0}, ge[3] = {0, 0, 0}, gz[3] = {0, 0, 0};
+                    for (int s = 0; s < ngll; ++s) {
+                        const double di = D[i * ngll + s], dj = D[j * ngll + s],
+                                     dk = D[k * ngll + s];
+                        const int a = nidx(s, j, k, ngll), b = nidx(i, s, k, ngll),
+                                  c = nidx(i, j, s, ngll);
+                        for (int d = 0; d < 3; ++d) {
+                            gx[d] += di * eu[3 * a + d];
+                            ge[d] += dj * eu[3 * b + d];
+                            gz[d] += dk * eu[3 * c + d];
+                        }
+                    }
+                    double g[3][3];
+                    for (int c = 0; c < 3; ++c) {
+                        g[c][0] = gx[c] * dd[0] + ge[c] * dd[1] + gz[c] * dd[2];
+                        g[c][1] = gx[c] * dd[3] + ge[c] * dd[4] + gz[c] * dd[5];
+                        g[c][2] = gx[c] * dd[6] + ge[c] * dd[7] + gz[c] * dd[8];
+                    }
+                    double* s6 = strain + (ci * n_node + n) * 6;
+                    s6[0] = g[0][0];
+                    s6[1] = g[1][1];
+                    s6[2] = g[2][2];
+                    s6[3] = 0.5 * (g[0][1] + g[1][0]);
+                    s6[4] = 0.5 * (g[0][2] + g[2][0]);
+                    s6[5] = 0.5 * (g[1][2] + g[2][1]);
+                }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Whole run: CPU branch of forward/share/src/solver.cpp:168-688       */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    double *u, *v, *a, *r, *ut, *uloc, *rloc, *mass_g, *damp_g;
+    double *pold, *pnew, *rmem;
+    double gll_x[16], gll_w[16], D[256];
+    gfo_cpml pml;
+} run_ws;
+
+static void ws_free(run_ws* w) {
+    free(w->u); free(w->v); free(w->a); free(w->r); free(w->ut);
+    free(w->uloc); free(w->rloc); free(w->mass_g); free(w->damp_g);
+    free(w->pold); free(w->pnew); free(w->rmem);
+}
+
+static int ws_init(run_ws* w, const gfo_problem* pb) {
+    memset(w, 0, sizeof(*w));
+    const int ngll = pb->ngll, N = ngll - 1;
+    if (ngll < 2 || ngll > 16) return -1;
+    const long n_node = (long)ngll * ngll * ngll;
+    const long P = (long)pb->n_elem * n_node;
+    const long G = pb->n_rank_node;
+    gfo_gll_nodes(N, w->gll_x);
+    gfo_gll_weights(N, w->gll_x, w->gll_w);
+    gfo_gll_derivative_matrix(N, w->gll_x, w->D);
+    w->u = calloc(G * 3, sizeof(double));
+    w->v = calloc(G * 3, sizeof(double));
+    w->a = calloc(G * 3, sizeof(double));
+    w->r = calloc(G * 3, sizeof(double));
+    w->ut = calloc(G * 3, sizeof(double));
+    w->uloc = calloc(P * 3, sizeof(double));
+    w->rloc = calloc(P * 3, sizeof(double));
+    w->mass_g = calloc(G, sizeof(double));
+    w->damp_g = calloc(G, sizeof(double));
+    if (!w->u || !w->v || !w->a || !w->r || !w->ut || !w->uloc || !w->rloc || !w->mass_g ||
+        !w->damp_g)
+        return -2;
+    /* global mass sums, damping "last writer wins" (solver.cpp:193-202) */
+    for (long p = 0; p < P; ++p) {
+        const long g = pb->ibool[p];
+        w->mass_g[g] += pb->mass[p];
+        if (pb->damping) w->damp_g[g] = pb->damping[p];
+    }
+    if (pb->has_cpml) { /* cpml_initialize, pml.cpp:41-53 */
+        w->pold = calloc(P * 3, sizeof(double));
+        w->pnew = calloc(P * 3, sizeof(double));
+        w->rmem = calloc(P * 9, sizeof(double));
+        if (!w->pold || !w->pnew || !w->rmem) return -2;
+        w->pml.n_elem = pb->n_elem;
+        w->pml.ngll = ngll;
+        w->pml.pml_region = pb->pml_region;
+        w->pml.coef_alpha = pb->coef_alpha;
+        w->pml.coef_abar = pb->coef_abar;
+        w->pml.density = pb->density;
+        w->pml.jacobian = pb->jacobian;
+        w->pml.ibool = pb->ibool;
+        w->pml.displ_old = w->pold;
+        w->pml.displ_new = w->pnew;
+        w->pml.rmemory = w->rmem;
+    }
+    return 0;
+}
+
+/* One solver step (solver.cpp:510-622). */
+static void run_one_step(run_ws* w, const gfo_problem* pb, int step) {
+    const int ngll = pb->ngll;
+    const int n_node = ngll * ngll * ngll;
+    const long P = (long)pb->n_elem * n_node;
+    const int n_dof = pb->n_rank_node * 3;
+    const double dt = pb->dt, beta = 0.0, gamma = 0.5;
+    /* 1. predictor */
+    gfo_newmark_predict(n_dof, dt, beta, w->u, w->v, w->a, w->ut);
+    /* 3. gather predicted displacement (the reference calls it twice; idempotent) */
+    gfo_gather_from_rank(w->ut, pb->ibool, pb->n_elem, n_node, w->uloc);
+    /* 3a. C-PML displacement fields from u_n, v_n, a_n */
+    if (pb->has_cpml) gfo_cpml_update_displ_fields(&w->pml, w->u, w->v, w->a, dt);
+    /* 3b. zero + element kernel */
+    memset(w->rloc, 0, sizeof(double) * 3 * P);
+    gfo_element_residual(pb->n_elem, pb->dxi_dx, pb->jacobian, pb->lambda_, pb->mu_, w->D,
+                         w->gll_w, ngll, w->uloc, w->rloc);
+    /* 4. C-PML acceleration contribution, else legacy damping on v (solver.cpp:557-574) */
+    if (pb->has_cpml) {
+        gfo_cpml_accel_contribution(&w->pml, w->u, w->v, w->gll_w, w->rloc);
+    } else {
+        for (long g = 0; g < pb->n_rank_node; ++g) {
+            const double d = w->damp_g[g];
+            if (d > 0.0)
+                for (int c = 0; c < 3; ++c) w->v[g * 3 + c] -= d * w->v[g * 3 + c];
+        }
+    }
+    /* 5. source: stf_values[step] = f(t_n) (solver.cpp:576-605) */
+    {
+        double sv = 0.0;
+        if (step < pb->n_stf) sv = pb->stf[step];
+        if (sv != 0.0) {
+            for (int si = 0; si < pb->n_src_cell; ++si) {
+                const int e = pb->src_elem[si];
+                if (e < 0) continue;
+                for (int k = 0; k < ngll; ++k)
+                    for (int j = 0; j < ngll; ++j)
+                        for (int i = 0; i < ngll; ++i) {
+                            const int no = nidx(i, j, k, ngll);
+                            const double wt = pb->src_weights[(long)si * n_node + no];
+                            if (wt == 0.0) continue;
+                            w->rloc[((long)e * n_node + no) * 3 + pb->direction] += sv * wt;
+                        }
+            }
+        }
+    }
+    /* 6. scatter (zero + add) */
+    gfo_scatter_to_rank(w->rloc, pb->ibool, pb->n_elem, n_node, pb->n_rank_node, w->r);
+    /* 7. corrector */
+    gfo_newmark_correct(n_dof, dt, beta, gamma, w->mass_g, w->u, w->v, w->a, w->r);
+    /* 8. C-PML memory */
+    if (pb->has_cpml) gfo_cpml_update_displ_memory(&w->pml);
+}
+
+int gfo_run(const gfo_problem* pb, double* rec_strain, double* rec_disp, double* rec_vel,
+            double* rec_acc, double* u_out, double* v_out, double* a_out) {
+    run_ws w;
+    if (!pb || pb->direction < 0 || pb->direction > 2) return -1;
+    if (ws_init(&w, pb) != 0) {
+        ws_free(&w);
+        return -1;
+    }
+    const int ngll = pb->ngll;
+    const int n_node = ngll * ngll * ngll;
+    const long nrec = pb->n_rec_cell;
+    int snap = 0;
+    double* strain_tmp = NULL;
+    if (rec_strain && nrec > 0) strain_tmp = malloc(sizeof(double) * nrec * n_node * 6);
+    for (int step = 0; step < pb->nsteps; ++step) {
+        run_one_step(&w, pb, step);
+        if (pb->snapshot_stride > 0 && step % pb->snapshot_stride == 0) {
+            /* solver.cpp:640-675: strain from gathered u after the corrector */
+            if (nrec > 0) {
+                gfo_gather_from_rank(w.u, pb->ibool, pb->n_elem, n_node, w.uloc);
+                if (rec_strain) {
+                    gfo_full_strain((int)nrec, pb->rec_cell, pb->dxi_dx, w.D, ngll, w.uloc,
+                                    strain_tmp);
+                    memcpy(rec_strain + (long)snap * nrec * n_node * 6, strain_tmp,
+                           sizeof(double) * nrec * n_node * 6);
+                }
+                for (long ci = 0; ci < nrec; ++ci) {
+                    const long e = pb->rec_cell[ci];
+                    for (int n = 0; n < n_node; ++n) {
+                        const long g = pb->ibool[e * n_node + n];
+                        const long o = (((long)snap * nrec + ci) * n_node + n) * 3;
+                        for (int d = 0; d < 3; ++d) {
+                            if (rec_disp) rec_disp[o + d] = w.u[g * 3 + d];
+                            if (rec_vel) rec_vel[o + d] = w.v[g * 3 + d];
+                            if (rec_acc) rec_acc[o + d] = w.a[g * 3 + d];
+                        }
+                    }
+                }
+            }
+            ++snap;
+        }
+    }
+    const long nd = (long)pb->n_rank_node * 3;
+    if (u_out) memcpy(u_out, w.u, sizeof(double) * nd);
+    if (v_out) memcpy(v_out, w.v, sizeof(double) * nd);
+    if (a_out) memcpy(a_out, w.a, sizeof(double) * nd);
+    free(strain_tmp);
+    ws_free(&w);
+    return snap;
+}
+
+int gfo_run_state(const gfo_problem* pb, double* u, double* v, double* a) {
+    run_ws w;
+    if (!pb || pb->direction < 0 || pb->direction > 2) return -1;
+    if (ws_init(&w, pb) != 0) {
+        ws_free(&w);
+        return -1;
+    }
+    const long nd = (long)pb->n_rank_node * 3;
+    memcpy(w.u, u, sizeof(double) * nd);
+    memcpy(w.v, v, sizeof(double) * nd);
+    memcpy(w.a, a, sizeof(double) * nd);
+    for (int step = 0; step < pb->nsteps; ++step) run_one_step(&w, pb, step);
+    memcpy(u, w.u, sizeof(double) * nd);
+    memcpy(v, w.v, sizeof(double) * nd);
+    memcpy(a, w.a, sizeof(double) * nd);
+    ws_free(&w);
+    return 0;
+}
+
+/* tests/test_element_cuda.cu:27-32 */
+void gfo_random_fill(double* v, int n, unsigned seed) {
+    srand(seed);
+    for (int i = 0; i < n; ++i) v[i] = 2.0 * (double)rand() / RAND_MAX - 1.0;
+}

@@ -1,0 +1,134 @@
+/*
+ * gfx_oracle.h -- CPU FP64 restatement of the reference's elastic CG-SEM
+ * forward step (CPU branch of forward/share/src/solver.cpp).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * solver in gf-only-sem_amd/.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product never links it and
+ * never falls back to it.
+ *
+ * Layouts follow the reference exactly (forward/share/include/gf/types.hpp:27-95):
+ *   per element point arrays are element-major, node index n = (i*NGLL+j)*NGLL+k,
+ *   dxi_dx[e][n][9] with [m*3+p] = d(xi_p)/d(x_m), vectors are xyz-interleaved.
+ *
+ * Pinning (what checks this restatement, by function):
+ *   gfo_gll_*             reference tests/test_gll.cpp known answers + closed
+ *                         forms (N=4 nodes/weights), tests/test_oracle_gll.py
+ *   gfo_element_residual  tests/test_element.cpp invariants (rigid translation,
+ *                         rotation, force balance), brute-force weak form by an
+ *                         independent numpy quadrature, symmetry of K,
+ *                         tests/test_oracle_element.py
+ *   gfo_scatter/gather    tests/test_assembly.cpp exact sums
+ *   gfo_newmark_*         tests/test_newmark.cpp closed forms, discrete
+ *                         eigenmode cos(n*theta) of the whole step
+ *   gfo_source            momentum balance sum(m*a) = F(t_n) per step
+ *   gfo_cpml_*            single-point closed-form recurrences with distinct
+ *                         per-direction coefficients (index layout), the
+ *                         old/new step-index mixing over hand-derived 3-step
+ *                         sequences; no reference test exists for C-PML
+ *                         (parity partially pinned: formula-level only)
+ *   gfo_full_strain       exact strain of affine displacement fields
+ */
+#ifndef GFX_ORACLE_H
+#define GFX_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- GLL quadrature (forward/share/include/gf/gll.hpp:14-147) ---- */
+int gfo_gll_nodes(int N, double* nodes);
+int gfo_gll_weights(int N, const double* nodes, double* w);
+int gfo_gll_derivative_matrix(int N, const double* nodes, double* D);
+void gfo_lagrange_basis(double xi, int ngll, const double* nodes, double* ell);
+
+/* ---- element internal force (forward/elastic/src/element_cpu.cpp:19-162) ---- */
+void gfo_element_residual(int n_elem, const double* dxi_dx, const double* jacobian,
+                          const double* lambda_, const double* mu_, const double* D,
+                          const double* weights, int ngll, const double* u, double* r);
+
+/* ---- assembly (forward/share/src/assembly.cpp:10-41) ---- */
+void gfo_scatter_to_rank(const double* loc, const int32_t* ibool, int n_elem, int n_node,
+                         int n_rank_node, double* glob);
+void gfo_gather_from_rank(const double* glob, const int32_t* ibool, int n_elem, int n_node,
+                          double* loc);
+
+/* ---- Newmark (forward/share/src/solver.cpp:40-69) ---- */
+void gfo_newmark_predict(int n_dof, double dt, double beta, const double* u, const double* v,
+                         const double* a, double* ut);
+void gfo_newmark_correct(int n_dof, double dt, double beta, double gamma, const double* mass,
+                         double* u, double* v, double* a, const double* r);
+
+/* ---- C-PML (forward/share/src/pml.cpp:55-216) ---- */
+typedef struct {
+    int n_elem, ngll;
+    const int32_t* pml_region;   /* [n_elem]            */
+    const double* coef_alpha;    /* [n_elem*n_node*9]   */
+    const double* coef_abar;     /* [n_elem*n_node*5]   */
+    const double* density;       /* [n_elem*n_node]     */
+    const double* jacobian;      /* [n_elem*n_node]     */
+    const int32_t* ibool;        /* [n_elem*n_node]     */
+    double* displ_old;           /* [n_elem*n_node*3]   state */
+    double* displ_new;           /* [n_elem*n_node*3]   state */
+    double* rmemory;             /* [n_elem*n_node*9]   state */
+} gfo_cpml;
+void gfo_cpml_update_displ_fields(gfo_cpml* p, const double* u, const double* v, const double* a,
+                                  double dt);
+void gfo_cpml_update_displ_memory(gfo_cpml* p);
+void gfo_cpml_accel_contribution(const gfo_cpml* p, const double* u, const double* v,
+                                 const double* gll_w, double* r_loc);
+
+/* ---- strain (forward/share/src/solver.cpp:282-335) ---- */
+void gfo_full_strain(int n_cell, const int32_t* cells, const double* dxi_dx, const double* D,
+                     int ngll, const double* u_loc_all, double* strain);
+
+/* ---- whole forward run (CPU branch of solver.cpp:389-688) ---- */
+typedef struct {
+    int n_elem, ngll, n_rank_node;
+    const double* dxi_dx;       /* [E*n_node*9] */
+    const double* jacobian;     /* [E*n_node]   */
+    const double* lambda_;      /* [E*n_node]   */
+    const double* mu_;          /* [E*n_node]   */
+    const double* mass;         /* [E*n_node]  element-local lumped mass (already * rho) */
+    const double* density;      /* [E*n_node]   */
+    const double* damping;      /* [E*n_node]  legacy ramp (used only without C-PML) */
+    const int32_t* ibool;       /* [E*n_node]  local_cell2global_node */
+    int has_cpml;
+    const int32_t* pml_region;  /* [E]            */
+    const double* coef_alpha;   /* [E*n_node*9]   */
+    const double* coef_abar;    /* [E*n_node*5]   */
+    int n_src_cell;
+    const int32_t* src_elem;    /* [n_src_cell] local element index, -1 = absent */
+    const double* src_weights;  /* [n_src_cell*n_node] */
+    int direction;              /* 0,1,2 */
+    int nsteps;
+    double dt;
+    int n_stf;
+    const double* stf;          /* [n_stf] f(step*dt) */
+    int snapshot_stride;
+    int n_rec_cell;
+    const int32_t* rec_cell;    /* [n_rec_cell] local element index */
+} gfo_problem;
+
+/* Runs nsteps steps.  Outputs (any may be NULL):
+ *   rec_strain [n_snap][n_rec_cell][n_node][6], rec_disp/vel/acc [n_snap][n_rec_cell][n_node][3]
+ *   with n_snap = number of steps with step % snapshot_stride == 0.
+ *   u_out/v_out/a_out [n_rank_node*3]: final state.
+ * Returns the number of snapshots written, or -1 on bad input. */
+int gfo_run(const gfo_problem* pb, double* rec_strain, double* rec_disp, double* rec_vel,
+            double* rec_acc, double* u_out, double* v_out, double* a_out);
+
+/* Runs nsteps starting from a given state (u,v,a in/out, C-PML state zero at
+ * start); no recording.  Used by step-level parity tests. */
+int gfo_run_state(const gfo_problem* pb, double* u, double* v, double* a);
+
+/* glibc rand()-based fill in [-1,1], as in the reference's
+ * tests/test_element_cuda.cu:27-32 (random_fill). */
+void gfo_random_fill(double* v, int n, unsigned seed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,178 @@
+"""ctypes binding of oracle/liboracle.so -- TEST INFRASTRUCTURE ONLY.
+
+The FP64 CPU restatement of the reference solver is the parity checker.  Only
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class Problem(C.Structure):
+    _fields_ = [
+        ("n_elem", C.c_int), ("ngll", C.c_int), ("n_rank_node", C.c_int),
+        ("dxi_dx", _dp), ("jacobian", _dp), ("lambda_", _dp), ("mu_", _dp), ("mass", _dp),
+        ("density", _dp), ("damping", _dp), ("ibool", _ip),
+        ("has_cpml", C.c_int), ("pml_region", _ip), ("coef_alpha", _dp), ("coef_abar", _dp),
+        ("n_src_cell", C.c_int), ("src_elem", _ip), ("src_weights", _dp),
+        ("direction", C.c_int), ("nsteps", C.c_int), ("dt", C.c_double),
+        ("n_stf", C.c_int), ("stf", _dp), ("snapshot_stride", C.c_int),
+        ("n_rec_cell", C.c_int), ("rec_cell", _ip),
+    ]
+
+
+class Cpml(C.Structure):
+    _fields_ = [
+        ("n_elem", C.c_int), ("ngll", C.c_int),
+        ("pml_region", _ip), ("coef_alpha", _dp), ("coef_abar", _dp), ("density", _dp),
+        ("jacobian", _dp), ("ibool", _ip),
+        ("displ_old", _dp), ("displ_new", _dp), ("rmemory", _dp),
+    ]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run `make -C oracle`")
+    lib = C.CDLL(LIB_PATH)
+    sig = {
+        "gfo_gll_nodes": (C.c_int, [C.c_int, _dp]),
+        "gfo_gll_weights": (C.c_int, [C.c_int, _dp, _dp]),
+        "gfo_gll_derivative_matrix": (C.c_int, [C.c_int, _dp, _dp]),
+        "gfo_lagrange_basis": (None, [C.c_double, C.c_int, _dp, _dp]),
+        "gfo_element_residual": (None, [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp, _dp]),
+        "gfo_scatter_to_rank": (None, [_dp, _ip, C.c_int, C.c_int, C.c_int, _dp]),
+        "gfo_gather_from_rank": (None, [_dp, _ip, C.c_int, C.c_int, _dp]),
+        "gfo_newmark_predict": (None, [C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp]),
+        "gfo_newmark_correct": (None, [C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp]),
+        "gfo_cpml_update_displ_fields": (None, [C.POINTER(Cpml), _dp, _dp, _dp, C.c_double]),
+        "gfo_cpml_update_displ_memory": (None, [C.POINTER(Cpml)]),
+        "gfo_cpml_accel_contribution": (None, [C.POINTER(Cpml), _dp, _dp, _dp, _dp]),
+        "gfo_full_strain": (None, [C.c_int, _ip, _dp, _dp, C.c_int, _dp, _dp]),
+        "gfo_run": (C.c_int, [C.POINTER(Problem), _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "gfo_run_state": (C.c_int, [C.POINTER(Problem), _dp, _dp, _dp]),
+        "gfo_random_fill": (None, [_dp, C.c_int, C.c_uint]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def P(a, ct=C.c_double):
+    if a is None:
+        return C.cast(None, C.POINTER(ct))
+    assert a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def gll(N):
+    x = np.zeros(N + 1)
+    w = np.zeros(N + 1)
+    D = np.zeros((N + 1) * (N + 1))
+    assert lib.gfo_gll_nodes(N, P(x)) == 0
+    assert lib.gfo_gll_weights(N, P(x), P(w)) == 0
+    assert lib.gfo_gll_derivative_matrix(N, P(x), P(D)) == 0
+    return x, w, D
+
+
+def lagrange(xi, nodes):
+    ell = np.zeros(len(nodes))
+    lib.gfo_lagrange_basis(xi, len(nodes), P(np.ascontiguousarray(nodes, np.float64)), P(ell))
+    return ell
+
+
+def element_residual(n_elem, dxi_dx, jac, lam, mu, D, w, ngll, u, r):
+    lib.gfo_element_residual(n_elem, P(dxi_dx), P(jac), P(lam), P(mu), P(D), P(w), ngll, P(u), P(r))
+
+
+def scatter(loc, ibool, n_elem, n_node, n_rank_node):
+    glob = np.zeros(n_rank_node * 3)
+    lib.gfo_scatter_to_rank(P(loc), P(ibool, C.c_int32), n_elem, n_node, n_rank_node, P(glob))
+    return glob
+
+
+def gather(glob, ibool, n_elem, n_node):
+    loc = np.zeros(n_elem * n_node * 3)
+    lib.gfo_gather_from_rank(P(glob), P(ibool, C.c_int32), n_elem, n_node, P(loc))
+    return loc
+
+
+def random_fill(n, seed):
+    v = np.zeros(n)
+    lib.gfo_random_fill(P(v), n, seed)
+    return v
+
+
+class Run:
+    """One forward run of the oracle on mesh arrays (reference layout)."""
+
+    def __init__(self, arrays, n_elem, ngll, n_node, dt, nsteps, snapshot_stride, direction,
+                 src_cells, src_weights, stf, rec_cell, has_cpml=True):
+        k = self._k = {}
+
+        def keep(name, x, dt_=np.float64):
+            if x is None:
+                k[name] = None
+                return None
+            k[name] = np.ascontiguousarray(x, dtype=dt_)
+            return k[name]
+
+        pb = Problem()
+        pb.n_elem, pb.ngll, pb.n_rank_node = n_elem, ngll, n_node
+        for name in ("dxi_dx", "jacobian", "lambda_", "mu_", "mass", "density", "damping",
+                     "coef_alpha", "coef_abar"):
+            setattr(pb, name, P(keep(name, arrays[name])))
+        pb.ibool = P(keep("ibool", arrays["ibool"], np.int32), C.c_int32)
+        pb.has_cpml = 1 if has_cpml else 0
+        pb.pml_region = P(keep("pml_region", arrays["pml_region"], np.int32), C.c_int32)
+        sc = keep("src_cells", src_cells, np.int32)
+        pb.n_src_cell = 0 if sc is None else sc.size
+        pb.src_elem = P(sc, C.c_int32)
+        pb.src_weights = P(keep("src_weights", src_weights))
+        pb.direction = direction
+        pb.nsteps = nsteps
+        pb.dt = dt
+        st = keep("stf", stf)
+        pb.n_stf = 0 if st is None else st.size
+        pb.stf = P(st)
+        pb.snapshot_stride = snapshot_stride
+        rc = keep("rec", rec_cell, np.int32)
+        pb.n_rec_cell = 0 if rc is None else rc.size
+        pb.rec_cell = P(rc, C.c_int32)
+        self.pb = pb
+        self.nn = ngll ** 3
+        self.n_node = n_node
+        self.n_snap = sum(1 for s in range(nsteps) if snapshot_stride > 0 and s % snapshot_stride == 0)
+
+    def run(self, uva=False):
+        nrec = self.pb.n_rec_cell
+        strain = np.zeros(self.n_snap * nrec * self.nn * 6)
+        d = v = a = None
+        if uva:
+            d, v, a = (np.zeros(self.n_snap * nrec * self.nn * 3) for _ in range(3))
+        uo, vo, ao = (np.zeros(self.n_node * 3) for _ in range(3))
+        ns = lib.gfo_run(C.byref(self.pb), P(strain), P(d), P(v), P(a), P(uo), P(vo), P(ao))
+        assert ns == self.n_snap, (ns, self.n_snap)
+        out = {"strain": strain.reshape(self.n_snap, nrec, self.nn, 6), "u": uo, "v": vo, "a": ao}
+        if uva:
+            for name, x in (("disp", d), ("vel", v), ("acc", a)):
+                out[name] = x.reshape(self.n_snap, nrec, self.nn, 3)
+        return out
+
+    def run_state(self, u, v, a):
+        u, v, a = (np.ascontiguousarray(x, np.float64).copy() for x in (u, v, a))
+        assert lib.gfo_run_state(C.byref(self.pb), P(u), P(v), P(a)) == 0
+        return u, v, a

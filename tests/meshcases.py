@@ -1,0 +1,65 @@
+"""Small, seeded box problems shared by the CPU (oracle) and GPU parity tests."""
+from __future__ import annotations
+
+import numpy as np
+
+import pygfx
+
+HALFSPACE = [(0.0, 6000.0, 3464.0, 2700.0)]
+# three-layer model (layer bottoms scaled to the small boxes)
+LAYERED = [(1800.0, 5000.0, 2900.0, 2500.0), (3600.0, 6200.0, 3600.0, 2800.0),
+           (0.0, 8000.0, 4600.0, 3300.0)]
+
+
+def small_halfspace(nx=8, ny=8, nz=6, pml=2, duration=2.5, **kw):
+    """Homogeneous halfspace of 1 km hexes, C-PML on five sides, order 4."""
+    spec = dict(nx=nx, ny=ny, nz=nz, lx=1000.0 * nx, ly=1000.0 * ny, lz=1000.0 * nz,
+                layers=HALFSPACE, pml_xmin=pml, pml_xmax=pml, pml_ymin=pml, pml_ymax=pml,
+                pml_zmax=pml, f0_pml=1.0, duration=duration, record_depth_max=2000.0)
+    spec.update(kw)
+    return pygfx.build_box(**spec)
+
+
+def small_layered(nx=8, ny=7, nz=6, pml=2, duration=2.0, jitter=0.12, **kw):
+    """Three layers + 5 % random velocity field on a jittered (non-affine) mesh."""
+    spec = dict(nx=nx, ny=ny, nz=nz, lx=900.0 * nx, ly=900.0 * ny, lz=900.0 * nz,
+                layers=LAYERED, rf_amp=0.05, rf_corr_len=2000.0, rf_modes=64, rf_seed=11,
+                jitter=jitter, jitter_seed=5, pml_xmin=pml, pml_xmax=pml, pml_ymin=pml,
+                pml_ymax=pml, pml_zmax=pml, f0_pml=1.5, duration=duration,
+                record_depth_max=1800.0)
+    spec.update(kw)
+    return pygfx.build_box(**spec)
+
+
+def surface_centre_source(mesh, f0=1.0, t0=1.5, amp=1e20):
+    x = 0.5 * (mesh.bounds[0] + mesh.bounds[1])
+    y = 0.5 * (mesh.bounds[2] + mesh.bounds[3])
+    cells, w = mesh.locate_source(x, y, 0.0)
+    stf = pygfx.ricker(f0, t0, amp, mesh.dt, mesh.nsteps)
+    return cells, w, stf
+
+
+def buried_source(mesh, frac=(0.47, 0.53, 0.0), depth=None, f0=1.5, t0=1.0, amp=1e20):
+    x = mesh.bounds[0] + frac[0] * (mesh.bounds[1] - mesh.bounds[0])
+    y = mesh.bounds[2] + frac[1] * (mesh.bounds[3] - mesh.bounds[2])
+    z = depth if depth is not None else 0.3 * (mesh.bounds[5] - mesh.bounds[4])
+    cells, w = mesh.locate_source(x, y, z)
+    stf = pygfx.ricker(f0, t0, amp, mesh.dt, mesh.nsteps)
+    return cells, w, stf
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a))
+
+
+def per_trace_rel_l2(s, ref, floor=1e-3):
+    """Per (cell, point, component) trace rel L2 over time; traces whose norm is
+    below floor * max trace norm are skipped (symmetry zeros)."""
+    s = np.asarray(s).reshape(ref.shape[0], -1)
+    r = np.asarray(ref).reshape(ref.shape[0], -1)
+    nr = np.linalg.norm(r, axis=0)
+    keep = nr >= floor * nr.max()
+    return np.linalg.norm(s[:, keep] - r[:, keep], axis=0) / nr[keep]

@@ -1,0 +1,185 @@
+"""GPU parity: the HIP solver (through the C ABI) against the FP64 CPU oracle.
+
+Tolerances:
+  * element plugin, FP64 device arithmetic: max rel diff < 1e-12, as the
+    reference's own CUDA-vs-CPU test (tests/test_element_cuda.cu:119-135);
+  * FP32 solver: aggregate rel L2 of recorded strain <= 1e-4 against the FP64
+    oracle on the same inputs (the parity bar), final u, v, a <= 1e-4.
+"""
+import numpy as np
+import pytest
+
+import meshcases as mc
+import pygfx
+import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if pygfx.device_count() < 1:
+        pytest.fail("no HIP device visible; GPU tests must run on an MI355X box")
+
+
+def _random_element(N, seed):
+    """tests/test_element_cuda.cu:35-95 RandomElement, same glibc rand() seeds."""
+    ng = N + 1
+    nn = ng ** 3
+    x, w, D = pyoracle.gll(N)
+    dd = pyoracle.random_fill(nn * 9, seed)
+    dd[0::9] += 3.0
+    dd[4::9] += 3.0
+    dd[8::9] += 3.0
+    jac = np.abs(pyoracle.random_fill(nn, seed + 1)) + 0.01
+    vp = np.abs(pyoracle.random_fill(nn, seed + 2)) + 1000.0
+    vs = np.abs(pyoracle.random_fill(nn, seed + 3)) + 500.0
+    rho = np.abs(pyoracle.random_fill(nn, seed + 4)) + 1000.0
+    mu = rho * vs * vs
+    lam = rho * (vp * vp - 2.0 * vs * vs)
+    return dict(dd=dd, jac=jac, lam=lam, mu=mu, D=D, w=w, ng=ng, nn=nn)
+
+
+@pytest.mark.parametrize("N,seed,useed", [(3, 42, 99), (5, 123, 456)])
+def test_element_plugin_fp64_matches_cpu(N, seed, useed):
+    el = _random_element(N, seed)
+    u = pyoracle.random_fill(el["nn"] * 3, useed)
+    r_cpu = np.zeros(el["nn"] * 3)
+    pyoracle.element_residual(1, el["dd"], el["jac"], el["lam"], el["mu"], el["D"], el["w"], el["ng"], u, r_cpu)
+    r_gpu = np.zeros(el["nn"] * 3)
+    pygfx.element_residual(el["dd"], el["jac"], el["lam"], el["mu"], el["D"], el["w"], el["ng"], u, r_gpu)
+    rel = np.abs(r_cpu - r_gpu) / np.maximum(np.abs(r_cpu), 1e-14)
+    assert rel.max() < 1e-12
+
+
+def test_element_plugin_accumulates_and_rigid_translation():
+    el = _random_element(3, 77)
+    u = np.tile([1.0, 2.0, 3.0], el["nn"])
+    r = np.full(el["nn"] * 3, 5.0)
+    pygfx.element_residual(el["dd"], el["jac"], el["lam"], el["mu"], el["D"], el["w"], el["ng"], u, r)
+    assert np.abs(r - 5.0).max() < 1e-5  # tests/test_element_cuda.cu:168-187, r accumulates
+
+
+def test_element_plugin_fp32_many_elements():
+    mesh = mc.small_layered(nx=4, ny=4, nz=3, pml=1)
+    E, nn = mesh.n_elem, 125
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal(E * nn * 3)
+    x, w, D = pyoracle.gll(4)
+    ref = np.zeros(E * nn * 3)
+    pyoracle.element_residual(E, mesh.dxi_dx, mesh.jacobian, mesh.lambda_, mesh.mu_, D, w, 5, u, ref)
+    out = np.zeros(E * nn * 3)
+    pygfx.element_residual(mesh.dxi_dx, mesh.jacobian, mesh.lambda_, mesh.mu_, D, w, 5, u, out, fp32=True)
+    assert mc.rel_l2(out, ref) < 1e-5
+    out64 = np.zeros(E * nn * 3)
+    pygfx.element_residual(mesh.dxi_dx, mesh.jacobian, mesh.lambda_, mesh.mu_, D, w, 5, u, out64)
+    assert mc.rel_l2(out64, ref) < 1e-13
+
+
+def _oracle(mesh, cells, w, stf, direction, nsteps=None, has_cpml=True):
+    nsteps = mesh.nsteps if nsteps is None else nsteps
+    run = pyoracle.Run(mesh.arrays, mesh.n_elem, mesh.ngll, mesh.n_node, mesh.dt, nsteps,
+                       mesh.snapshot_stride, direction, cells, w, stf, mesh.rec_cell,
+                       has_cpml=has_cpml)
+    return run.run(uva=True)
+
+
+def _gpu(mesh, sources, nsteps=None, assembly=pygfx.GFX_ASSEMBLY_COLORED, epb=0, has_cpml=True):
+    nsteps = mesh.nsteps if nsteps is None else nsteps
+    s = pygfx.Solver(mesh, n_fields=len(sources), assembly=assembly, elems_per_block=epb,
+                     record_uva=True, has_cpml=has_cpml)
+    for f, (cells, w, stf, d) in enumerate(sources):
+        s.set_source(f, d, cells, w, stf)
+    s.run(nsteps)
+    return s
+
+
+def _records(s, field, n_snap):
+    nn = 125
+    st = np.stack([s.get_record(field, k) for k in range(n_snap)]).reshape(n_snap, s.n_rec, nn, 6)
+    return st
+
+
+@pytest.mark.parametrize("assembly", [pygfx.GFX_ASSEMBLY_COLORED, pygfx.GFX_ASSEMBLY_ATOMIC])
+def test_solver_matches_oracle_halfspace(assembly):
+    mesh = mc.small_halfspace()
+    cells, w, stf = mc.surface_centre_source(mesh)
+    ref = _oracle(mesh, cells, w, stf, 2)
+    s = _gpu(mesh, [(cells, w, stf, 2)], assembly=assembly)
+    n_snap = ref["strain"].shape[0]
+    assert s.n_snapshots == n_snap
+    st = _records(s, 0, n_snap)
+    err = mc.rel_l2(st, ref["strain"])
+    print(f"halfspace assembly={assembly} strain rel L2 = {err:.3e}")
+    assert err <= TOL
+    u, v, a = s.get_state(0)
+    for got, want in ((u, ref["u"]), (v, ref["v"]), (a, ref["a"])):
+        assert mc.rel_l2(got, want) <= TOL
+    _, d, vel, acc = s.get_record(0, n_snap - 1, uva=True)
+    assert mc.rel_l2(d, ref["disp"][-1]) <= TOL
+    assert mc.rel_l2(vel, ref["vel"][-1]) <= TOL
+    assert mc.rel_l2(acc, ref["acc"][-1]) <= TOL
+
+
+@pytest.mark.parametrize("direction", [0, 1])
+def test_solver_matches_oracle_layered_jittered(direction):
+    mesh = mc.small_layered()
+    cells, w, stf = mc.buried_source(mesh)
+    ref = _oracle(mesh, cells, w, stf, direction)
+    s = _gpu(mesh, [(cells, w, stf, direction)])
+    st = _records(s, 0, ref["strain"].shape[0])
+    err = mc.rel_l2(st, ref["strain"])
+    print(f"layered dir={direction} strain rel L2 = {err:.3e}")
+    assert err <= TOL
+
+
+def test_solver_legacy_damping_path():
+    mesh = mc.small_halfspace(duration=1.5)
+    cells, w, stf = mc.surface_centre_source(mesh)
+    ref = _oracle(mesh, cells, w, stf, 2, has_cpml=False)
+    s = _gpu(mesh, [(cells, w, stf, 2)], has_cpml=False)
+    st = _records(s, 0, ref["strain"].shape[0])
+    assert mc.rel_l2(st, ref["strain"]) <= TOL
+    u, v, a = s.get_state(0)
+    assert mc.rel_l2(u, ref["u"]) <= TOL
+
+
+def test_coloured_assembly_is_deterministic_and_epb_invariant():
+    mesh = mc.small_layered(duration=0.6)
+    cells, w, stf = mc.buried_source(mesh)
+    runs = []
+    for epb in (1, 2, 4, 8, 2):
+        s = _gpu(mesh, [(cells, w, stf, 2)], epb=epb)
+        runs.append(s.get_state(0))
+    for r in runs[1:]:
+        for x, y in zip(r, runs[0]):
+            assert np.array_equal(x, y)
+
+
+def test_batched_fields_equal_single_runs():
+    mesh = mc.small_halfspace(duration=1.0)
+    cells, w, stf = mc.surface_centre_source(mesh)
+    c2, w2, stf2 = mc.buried_source(mesh, frac=(0.45, 0.4, 0.0), depth=1500.0, f0=1.2)
+    sources = [(cells, w, stf, 0), (cells, w, stf, 1), (cells, w, stf, 2), (c2, w2, stf2, 2)]
+    sb = _gpu(mesh, sources)
+    for f, src in enumerate(sources):
+        s1 = _gpu(mesh, [src])
+        for x, y in zip(sb.get_state(f), s1.get_state(0)):
+            assert np.array_equal(x, y)
+        n_snap = s1.n_snapshots
+        assert np.array_equal(_records(sb, f, n_snap), _records(s1, 0, n_snap))
+
+
+def test_reset_and_restart_state_roundtrip():
+    mesh = mc.small_halfspace(duration=1.0)
+    cells, w, stf = mc.surface_centre_source(mesh)
+    s = _gpu(mesh, [(cells, w, stf, 2)], nsteps=30)
+    u1, v1, a1 = s.get_state(0)
+    s.reset()
+    assert s.step == 0 and s.n_snapshots == 0
+    assert np.abs(s.get_state(0)[0]).max() == 0.0
+    s.run(30)
+    for x, y in zip(s.get_state(0), (u1, v1, a1)):
+        assert np.array_equal(x, y)
