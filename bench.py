@@ -1,0 +1,217 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the MI355X elastic CG-SEM time step.
+
+Workload: the 500k-element layered mesh (100 x 100 x 50 hexes of 0.6 km,
+polynomial order 4, 3 layers + 5 % Gaussian random velocity field, C-PML 5
+elements thick on the sides and bottom, free surface on top, Ricker f0 = 1.5 Hz,
+strain recorded at the cells down to 2.4 km every snapshot stride).
+  * plain:   one z-direction point force at the surface centre (1 field);
+  * batched: x, y and z point forces at 8 seeded surface stations in the
+             central 40 x 40 km (24 fields) advanced by one launch per colour.
+Metric: GLL-point updates/s = elements * 125 * fields * steps / time-loop wall
+time (inputs resident in HBM, device recording included).  `value` is the
+batched aggregate; the plain figure and the batched/plain ratio sit beside it.
+
+A step is one pass of the whole hot path (element kernels for all colours,
+C-PML memory, sources, Newmark node kernel, and the recording kernel on
+snapshot steps).  The timed region is bracketed by device synchronisations.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gf-only-sem_amd"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "GLL-point updates/s, batched 3×N sources vs plain, at ≤1e-4 rel L2 strain"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cfg2_spec(pygfx, nx=100, ny=100, nz=50, h=600.0, duration=30.0):
+    return pygfx.make_spec(
+        nx=nx, ny=ny, nz=nz, lx=h * nx, ly=h * ny, lz=h * nz,
+        layers=[(3000.0, 5000.0, 2900.0, 2500.0), (20000.0, 6200.0, 3600.0, 2800.0),
+                (0.0, 8000.0, 4600.0, 3300.0)],
+        rf_amp=0.05, rf_corr_len=2000.0, rf_modes=256, rf_seed=2,
+        pml_xmin=5, pml_xmax=5, pml_ymin=5, pml_ymax=5, pml_zmax=5, f0_pml=1.5,
+        cfl_safety=0.5, output_dt=0.05, duration=duration, record_depth_max=2000.0,
+        want_coords=0, want_beta=0)
+
+
+def stations(mesh, n=8, seed=3):
+    rng = np.random.default_rng(seed)
+    cx = 0.5 * (mesh.bounds[0] + mesh.bounds[1])
+    cy = 0.5 * (mesh.bounds[2] + mesh.bounds[3])
+    half = 20000.0
+    return [(cx + rng.uniform(-half, half), cy + rng.uniform(-half, half)) for _ in range(n)]
+
+
+def time_run(pygfx, solver, steps, warmup):
+    solver.run(warmup)
+    solver.set_timing(True)
+    t0 = time.perf_counter()
+    solver.run(steps)  # returns after hipStreamSynchronize on the solver stream
+    t1 = time.perf_counter()
+    solver.set_timing(False)
+    ms_el, n_el = solver.kernel_time("element")
+    ms_nd, n_nd = solver.kernel_time("node")
+    ms_rc, n_rc = solver.kernel_time("record")
+    ms_tot, _ = solver.kernel_time("total")
+    return dict(wall_s=t1 - t0, ms_elem=ms_el, n_elem_launch=n_el, ms_node=ms_nd, n_node=n_nd,
+                ms_rec=ms_rc, n_rec=n_rc, ms_events=ms_tot)
+
+
+def cpu_baseline(seconds=15.0):
+    """FP64 CPU restatement of the reference solver (oracle/), one core, on a
+    20 x 20 x 10 sub-box of the same 0.6 km layered discretisation."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pygfx
+    import pyoracle
+
+    mesh = pygfx.build_box(cfg2_spec(pygfx, nx=20, ny=20, nz=10, duration=30.0))
+    cells, w = mesh.locate_source(6000.0, 6000.0, 0.0)
+    stf = pygfx.ricker(1.5, 1.0, 1e20, mesh.dt, mesh.nsteps)
+    # calibrate: 5 steps, then as many as fit the time budget
+    probe = pyoracle.Run(mesh.arrays, mesh.n_elem, 5, mesh.n_node, mesh.dt, 5, mesh.snapshot_stride,
+                         2, cells, w, stf, mesh.rec_cell)
+    t0 = time.perf_counter()
+    probe.run()
+    per = (time.perf_counter() - t0) / 5
+    n = max(5, min(mesh.nsteps, int(seconds / per)))
+    run = pyoracle.Run(mesh.arrays, mesh.n_elem, 5, mesh.n_node, mesh.dt, n, mesh.snapshot_stride,
+                       2, cells, w, stf, mesh.rec_cell)
+    t0 = time.perf_counter()
+    run.run()
+    dt = time.perf_counter() - t0
+    return {"value": mesh.n_elem * 125 * n / dt, "unit": "GLL-point updates/s", "cores": 1,
+            "kind": "port",
+            "sample": f"oracle (FP64 C restatement of the reference CPU step), 1 thread, "
+                      f"{mesh.n_elem}-element sub-box of the 0.6 km layered mesh, z force, "
+                      f"{n} steps with C-PML and recording, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--fields", type=int, default=24)
+    ap.add_argument("--assembly", choices=["colored", "atomic"], default="colored")
+    ap.add_argument("--epb", type=int, default=0)
+    ap.add_argument("--small", action="store_true", help="40x40x20 mesh instead of 100x100x50 (smoke)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would run "
+              "its own fields or meshes with no communication); run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import pygfx
+
+    if pygfx.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    assembly = pygfx.GFX_ASSEMBLY_ATOMIC if args.assembly == "atomic" else pygfx.GFX_ASSEMBLY_COLORED
+
+    t0 = time.perf_counter()
+    dims = (40, 40, 20) if args.small else (100, 100, 50)
+    mesh = pygfx.build_box(cfg2_spec(pygfx, *dims), copy=False)
+    log(f"mesh: {mesh.n_elem} elements, {mesh.n_node} nodes, {mesh.n_pml_elem} PML elements, "
+        f"{len(mesh.rec_cell)} recording cells, dt={mesh.dt:.5g} s, stride={mesh.snapshot_stride}, "
+        f"nsteps={mesh.nsteps} ({time.perf_counter() - t0:.1f} s)")
+    E = mesh.n_elem
+    G = mesh.n_node
+    cx = 0.5 * (mesh.bounds[0] + mesh.bounds[1])
+    cy = 0.5 * (mesh.bounds[2] + mesh.bounds[3])
+    stf = pygfx.ricker(1.5, 1.0, 1e20, mesh.dt, mesh.nsteps)
+
+    result = {}
+    plain = None
+    if not args.no_plain:
+        t0 = time.perf_counter()
+        s1 = pygfx.Solver(mesh, n_fields=1, assembly=assembly, elems_per_block=args.epb,
+                          record_capacity=2)
+        cells, w = mesh.locate_source(cx, cy, 0.0)
+        s1.set_source(0, 2, cells, w, stf)
+        log(f"plain solver ready ({time.perf_counter() - t0:.1f} s)")
+        tp = time_run(pygfx, s1, args.steps, args.warmup)
+        plain = dict(value=E * 125 * 1 * args.steps / tp["wall_s"],
+                     ms_per_step=1e3 * tp["wall_s"] / args.steps,
+                     ms_element_per_step=tp["ms_elem"] / args.steps,
+                     ms_node_per_step=tp["ms_node"] / args.steps,
+                     node_updates_per_s=G * args.steps / tp["wall_s"],
+                     bytes_per_step=s1.algorithmic_bytes(0))
+        plain["hbm_gbs_whole_step"] = plain["bytes_per_step"] / (plain["ms_per_step"] * 1e-3) / 1e9
+        log("plain:", json.dumps(plain))
+        s1.close()
+        del s1
+
+    F = args.fields
+    t0 = time.perf_counter()
+    sb = pygfx.Solver(mesh, n_fields=F, assembly=assembly, elems_per_block=args.epb, record_capacity=2)
+    st = stations(mesh, n=max(1, (F + 2) // 3))
+    for f in range(F):
+        x, y = st[f // 3]
+        cells, w = mesh.locate_source(x, y, 0.0)
+        sb.set_source(f, f % 3, cells, w, stf)
+    log(f"batched solver ready, {F} fields ({time.perf_counter() - t0:.1f} s)")
+    tb = time_run(pygfx, sb, args.steps, args.warmup)
+    value = E * 125 * F * args.steps / tb["wall_s"]
+    ms_step = 1e3 * tb["wall_s"] / args.steps
+
+    # roofline of the dominant kernel (element stage), HIP events on the solver stream
+    bytes_elem_step = sb.algorithmic_bytes(1)
+    n_launch_per_step = tb["n_elem_launch"] / args.steps
+    ms_elem_launch = tb["ms_elem"] / tb["n_elem_launch"]
+    achieved = (bytes_elem_step / n_launch_per_step) / (ms_elem_launch * 1e-3) / 1e9
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "GLL-point updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"layered {dims[0]}x{dims[1]}x{dims[2]} hexes of 0.6 km, order 4, "
+                               f"C-PML 5, x/y/z forces at {len(st)} surface stations = {F} fields",
+                   "elements": E, "nodes": G, "fields": F, "pml_elements": mesh.n_pml_elem,
+                   "recording_cells": int(len(mesh.rec_cell)), "snapshot_stride": mesh.snapshot_stride,
+                   "dt_s": mesh.dt, "assembly": args.assembly, "parallelism": "none (1 GPU)"},
+        "node_updates_per_s": G * F * args.steps / tb["wall_s"],
+        "plain": plain,
+        "batched_vs_plain": (value / plain["value"]) if plain else None,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_element", "algorithmic_bytes_per_launch": bytes_elem_step / n_launch_per_step,
+                     "ms_per_launch": ms_elem_launch, "launches_per_step": n_launch_per_step},
+        "whole_step_hbm_gbs": sb.algorithmic_bytes(0) / (ms_step * 1e-3) / 1e9,
+        "ms_element_per_step": tb["ms_elem"] / args.steps,
+        "ms_node_per_step": tb["ms_node"] / args.steps,
+        "ms_record_per_snapshot": (tb["ms_rec"] / tb["n_rec"]) if tb["n_rec"] else None,
+    }
+    sb.close()
+    del sb
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        result["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -231,7 +231,9 @@ def make_spec(**kw) -> BoxSpec:
     return s
 
 
-def build_box(spec: BoxSpec | None = None, keep_handle=True, **kw) -> Mesh:
+def build_box(spec: BoxSpec | None = None, keep_handle=True, copy=True, **kw) -> Mesh:
+    """Build a box mesh.  copy=False returns zero-copy views that are valid only
+    while the returned Mesh is alive (use for very large meshes)."""
     if spec is None:
         spec = make_spec(**kw)
     h = C.POINTER(_Mesh)()
@@ -243,9 +245,11 @@ def build_box(spec: BoxSpec | None = None, keep_handle=True, **kw) -> Mesh:
     G = m.n_node_glob
 
     def arr(p, n, dt=np.float64):
-        if not p:
+        # views over the library's buffers (valid until Mesh.close())
+        if not p or n == 0:
             return None
-        return np.ctypeslib.as_array(p, shape=(n,)).astype(dt, copy=True)
+        a = np.ctypeslib.as_array(p, shape=(n,))
+        return a.copy() if copy else a
 
     arrays = {
         "coords": arr(m.coords, P * 3),
@@ -266,6 +270,7 @@ def build_box(spec: BoxSpec | None = None, keep_handle=True, **kw) -> Mesh:
         "coef_abar": arr(m.coef_abar, P * 5),
     }
     rec = arr(m.rec_cell, m.n_rec_cell, np.int32)
+    rec = None if rec is None else rec.copy()
     mesh = Mesh(n_elem=E, ngll=ng, n_node=G, nx=m.nx, ny=m.ny, nz=m.nz,
                 bounds=(m.xmin, m.xmax, m.ymin, m.ymax, m.zmin, m.zmax), arrays=arrays,
                 dt=m.dt, snapshot_stride=m.snapshot_stride, nsteps=m.nsteps, h_min=m.h_min,
@@ -273,6 +278,8 @@ def build_box(spec: BoxSpec | None = None, keep_handle=True, **kw) -> Mesh:
                 record_depth_actual=m.record_depth_actual,
                 rec_cell=rec if rec is not None else np.zeros(0, np.int32), _handle=h)
     if not keep_handle:
+        if not copy:
+            mesh.arrays = {k: (None if v is None else v.copy()) for k, v in arrays.items()}
         mesh.close()
     return mesh
 
