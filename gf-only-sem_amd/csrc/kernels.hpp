@@ -1,24 +1,23 @@
 // kernels.hpp -- CDNA4 (gfx950) device kernels of the elastic CG-SEM step.
 //
-// Per step (FP32 throughout):
-//   element kernel, once per element colour (or once with atomics):
-//     gather predicted displacement of the element's 125 nodes into LDS,
-//     reference-space gradients (GLL derivative contractions over LDS),
-//     scaled physical gradient -> strain -> isotropic stress -> fluxes (LDS),
-//     weighted divergence, plus the C-PML acceleration term and the point
-//     source, then scatter-add into the assembled residual;
-//   node kernel: diagonal-mass Newmark corrector fused with the next
-//     predictor (beta = 0, gamma = 1/2), residual reset;
-//   record kernel (snapshot steps): strain of the recording cells.
+// Nodal state per field (FP32): U = predicted displacement u~_n
+// (= u_{n+1}, beta = 0), W = v_n + dt/2 a_n, R = assembled residual, and a_n
+// only at "special" nodes (C-PML, legacy-damped, recording, massless nodes,
+// numbered first: ids [0, Ga)).  The reference's Newmark pair
+// (forward/share/src/solver.cpp:40-69, beta = 0, gamma = 1/2) is carried
+// algebraically unchanged:
+//   a_{n+1} = r / m,  w_{n+1} = w_n + dt a_{n+1},  u~_{n+1} = u~_n + dt w_{n+1},
+//   u_n = u~_n - dt w_n,  v_n = w_n - dt/2 a_n.
 //
-// Numerics follow the reference CPU path:
-//   element force   forward/elastic/src/element_cpu.cpp:19-162
-//   C-PML           forward/share/src/pml.cpp:55-216 (fused: the memory
-//                   update only needs u_n, v_n, a_n and the previous "new",
-//                   all known when the element kernel runs)
-//   source          forward/share/src/solver.cpp:576-605
-//   Newmark         forward/share/src/solver.cpp:40-69
-//   strain          forward/share/src/solver.cpp:282-335
+// Per step:
+//   element kernel per colour: gather u~ into LDS, GLL contractions, scaled
+//     gradient -> strain -> isotropic stress -> fluxes (LDS) -> divergence,
+//     point sources, then store/add into R (first touch of a node stores);
+//   special-node kernel: C-PML (forward/share/src/pml.cpp:55-216, assembled
+//     per node), legacy damping, massless rule, corrector/predictor;
+//   regular-node kernel: streaming corrector/predictor, 4 nodes per lane;
+//   record kernel (snapshot steps): strain of the recording cells
+//     (forward/share/src/solver.cpp:282-335).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -29,62 +28,68 @@ namespace dev {
 
 constexpr int NG = 5;    // GLL points per axis (polynomial order 4)
 constexpr int NP = 125;  // points per element
+constexpr uint32_t FIRST_TOUCH = 0x80000000u;
 
 struct ElemArgs {
     // geometry, SoA over element points in the solver's element order
-    const float* __restrict__ G;      // [9][P]: dxi_dx[m*3+a] * sqrt(J w_i w_j w_k)
-    const float* __restrict__ lam;    // [P]   (0 where mu <= 0)
-    const float* __restrict__ mu;     // [P]   (0 where mu <= 0)
-    const int32_t* __restrict__ ibool;  // [P]  solver node numbering
+    const float* __restrict__ G;        // [9][P]: dxi_dx[m*3+a] * sqrt(J w_i w_j w_k)
+    const float* __restrict__ lam;      // [P]   (0 where mu <= 0)
+    const float* __restrict__ mu;       // [P]   (0 where mu <= 0)
+    const uint32_t* __restrict__ ibool; // [P]  node id | FIRST_TOUCH
     int64_t P;
-    // nodal fields, field-major [nf][fstride], xyz interleaved per node
     int nf;
     int64_t fstride;
-    const float* __restrict__ ut;     // predicted displacement (u_{n+1})
-    const float* __restrict__ u;      // u_n  (C-PML only)
-    const float* __restrict__ v;      // v_n  (C-PML only)
-    const float* __restrict__ a;      // a_n  (C-PML only)
-    float* __restrict__ r;            // assembled residual
-    // C-PML, compact over PML elements
+    const float* __restrict__ U;        // [nf][fstride] predicted displacement
+    float* __restrict__ R;              // [nf][fstride] assembled residual
+    // element-local C-PML (only when the nodal form is not applicable)
+    const float* __restrict__ W;
+    const float* __restrict__ Aacc;     // [nf][astride] a_n at special nodes
+    int64_t astride;
     const int32_t* __restrict__ pml_slot;  // [E] slot or -1
     int64_t Ppml;
-    const float* __restrict__ palpha;  // [9][Ppml] coef_alpha (dir*3 + {0,1,2})
+    const float* __restrict__ palpha;  // [9][Ppml]
     const float* __restrict__ pabar;   // [5][Ppml] w J / rho * A_k
-    float* __restrict__ pmem;          // [nf][9][Ppml] rmemory (comp*3 + dir)
-    float* __restrict__ pnew;          // [nf][3][Ppml] pml_displ_new
-    float c1, c2;
+    float* __restrict__ pmem;          // [nf][9][Ppml]
+    float* __restrict__ pnew;          // [nf][3][Ppml]
+    float c1, c2, dt;
     // point sources
     const int32_t* __restrict__ src_off;  // [E+1]
     const int4* __restrict__ src_ent;     // (field, dir, weight offset, 0)
-    const float* __restrict__ src_w;      // weights
+    const float* __restrict__ src_w;
     const float* __restrict__ stf;        // [nf][nstf]
     int nstf, step;
     float D[25];
 };
 
 struct NodeArgs {
-    int64_t G;
+    int64_t G, Ga, Gpml, G4;
     int nf;
-    int64_t fstride;
-    const float* __restrict__ minv;  // [G] 1/m (0 where m <= 0)
-    float* __restrict__ ucur;        // in: u_n,     out: predicted u_{n+2}
-    float* __restrict__ unext;       // in: u_{n+1} (predicted); out: u_{n+1}
-    float* __restrict__ v;
-    float* __restrict__ a;
-    float* __restrict__ r;           // in: residual; out: 0
-    const float* __restrict__ damp;  // [G] legacy velocity damping or nullptr
-    float dt;
+    int64_t fstride, astride;
+    const float* __restrict__ minv;   // [G4] 1/m (0 where m <= 0 or padding)
+    const float* __restrict__ damp;   // [Ga] legacy ramp or nullptr
+    float* __restrict__ U;
+    float* __restrict__ W;
+    float* __restrict__ R;
+    float* __restrict__ Aacc;         // [nf][astride]
+    // nodal C-PML
+    int pml_nodal;
+    const float* __restrict__ pc;      // [5][Gpml] sum over PML elements of w J / rho * A_k
+    const float* __restrict__ palpha;  // [9][Gpml] coef_alpha (dir*3 + {0,1,2})
+    float* __restrict__ pmem;          // [nf][9][Gpml] rmemory (comp*3 + dir)
+    float* __restrict__ pnew;          // [nf][3][Gpml] pml_displ_new
+    float c1, c2, dt;
 };
 
 struct RecArgs {
     int n_rec;
-    const int32_t* __restrict__ ibool;  // [n_rec*125] solver node ids of recording cells
+    const int32_t* __restrict__ ibool;  // [n_rec*125] special node ids of recording cells
     const float* __restrict__ dd;       // [9][n_rec*125] unscaled dxi_dx
     int nf;
-    int64_t fstride;
-    const float* __restrict__ u;
-    const float* __restrict__ v;
-    const float* __restrict__ a;
+    int64_t fstride, astride;
+    const float* __restrict__ U;
+    const float* __restrict__ W;
+    const float* __restrict__ Aacc;
+    float dt;
     float* __restrict__ strain;  // [nf][n_rec*125][6]
     float* __restrict__ uva;     // [nf][n_rec*125][9] or nullptr
     float D[25];
@@ -99,7 +104,7 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return (x < r) ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
 }
 
-template <int EPB, bool ATOMIC>
+template <int EPB, bool ATOMIC, bool PML_ELEM>
 __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
     k_element(ElemArgs A, int e0, int ne) {
     __shared__ float sD[25];
@@ -118,23 +123,33 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
 
     float g[9];
     float lam = 0.f, mu = 0.f;
-    int node = 0;
+    uint32_t nodef = 0;
     const int64_t pt = int64_t(e) * NP + p;
     if (valid) {
 #pragma unroll
         for (int c = 0; c < 9; ++c) g[c] = __builtin_nontemporal_load(&A.G[c * A.P + pt]);
         lam = __builtin_nontemporal_load(&A.lam[pt]);
         mu = __builtin_nontemporal_load(&A.mu[pt]);
-        node = __builtin_nontemporal_load(&A.ibool[pt]);
+        nodef = __builtin_nontemporal_load(&A.ibool[pt]);
     } else {
 #pragma unroll
         for (int c = 0; c < 9; ++c) g[c] = 0.f;
     }
-    const int pslot = valid ? A.pml_slot[e] : -1;
+    const int64_t node = int64_t(nodef & ~FIRST_TOUCH);
+    const bool first = (nodef & FIRST_TOUCH) != 0;
+    const int pslot = (PML_ELEM && valid) ? A.pml_slot[e] : -1;
     int sb = 0, se = 0;
     if (valid) {
         sb = A.src_off[e];
         se = A.src_off[e + 1];
+    }
+    // displacement of field 0 (later fields are prefetched one iteration ahead)
+    float un[3] = {0.f, 0.f, 0.f};
+    if (valid) {
+        const float* ut = A.U + node * 3;
+        un[0] = ut[0];
+        un[1] = ut[1];
+        un[2] = ut[2];
     }
     __syncthreads();
     float Di[5], Dj[5], Dk[5], Dti[5], Dtj[5], Dtk[5];
@@ -148,13 +163,29 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
         Dtk[s] = sD[s * 5 + k];
     }
 
+    // Two barriers per field: (1) after the displacement is staged, (2) after
+    // the fluxes are staged.  Staging the next field's displacement happens after
+    // (2), when every lane has finished reading su; writing the next fluxes
+    // happens after the next (1), when every lane has finished reading sf.
     for (int f = 0; f < A.nf; ++f) {
         const int64_t fo = int64_t(f) * A.fstride;
         if (valid) {
-            const float* ut = A.ut + fo + int64_t(node) * 3;
-            su[le][0][p] = ut[0];
-            su[le][1][p] = ut[1];
-            su[le][2][p] = ut[2];
+            su[le][0][p] = un[0];
+            su[le][1][p] = un[1];
+            su[le][2][p] = un[2];
+        }
+        if (valid && f + 1 < A.nf) {
+            const float* ut = A.U + fo + A.fstride + node * 3;
+            un[0] = ut[0];
+            un[1] = ut[1];
+            un[2] = ut[2];
+        }
+        float rold[3] = {0.f, 0.f, 0.f};
+        if (!ATOMIC && valid && !first) {
+            const float* r = A.R + fo + node * 3;
+            rold[0] = r[0];
+            rold[1] = r[1];
+            rold[2] = r[2];
         }
         __syncthreads();
         if (valid) {
@@ -172,7 +203,7 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
                 du[c][1] = a1;
                 du[c][2] = a2;
             }
-            // scaled physical gradient: gr[c][m] = sum_a du[c][a] * dxi_a/dx_m * sqrt(Jw)
+            // scaled physical gradient gr[c][m] = sum_a du[c][a] dxi_a/dx_m sqrt(Jw)
             float gr[3][3];
 #pragma unroll
             for (int c = 0; c < 3; ++c)
@@ -180,19 +211,16 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
                 for (int m = 0; m < 3; ++m)
                     gr[c][m] = du[c][0] * g[m * 3 + 0] + du[c][1] * g[m * 3 + 1] +
                                du[c][2] * g[m * 3 + 2];
-            const float exy = 0.5f * (gr[0][1] + gr[1][0]);
-            const float exz = 0.5f * (gr[0][2] + gr[2][0]);
-            const float eyz = 0.5f * (gr[1][2] + gr[2][1]);
             const float tr = gr[0][0] + gr[1][1] + gr[2][2];
             const float mu2 = 2.f * mu;
             float sig[3][3];
             sig[0][0] = mu2 * gr[0][0] + lam * tr;
             sig[1][1] = mu2 * gr[1][1] + lam * tr;
             sig[2][2] = mu2 * gr[2][2] + lam * tr;
-            sig[0][1] = sig[1][0] = mu2 * exy;
-            sig[0][2] = sig[2][0] = mu2 * exz;
-            sig[1][2] = sig[2][1] = mu2 * eyz;
-            // fluxes F[c][a] = sum_m sig[c][m] * dxi_a/dx_m * J w
+            sig[0][1] = sig[1][0] = mu * (gr[0][1] + gr[1][0]);
+            sig[0][2] = sig[2][0] = mu * (gr[0][2] + gr[2][0]);
+            sig[1][2] = sig[2][1] = mu * (gr[1][2] + gr[2][1]);
+            // fluxes F[c][a] = sum_m sig[c][m] dxi_a/dx_m J w
 #pragma unroll
             for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -214,15 +242,18 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
                 }
                 rv[c] = -acc;
             }
-            if (pslot >= 0) {
+            if (PML_ELEM && pslot >= 0) {
+                // element-local C-PML (pml.cpp:55-216); PML nodes are special, so a_n is stored
                 const int64_t q = int64_t(pslot) * NP + p;
-                const int64_t gn = fo + int64_t(node) * 3;
-                float un[3], vn[3], an[3], m[9], pn[3];
+                const int64_t gn = fo + node * 3;
+                const float* an_p = A.Aacc + int64_t(f) * A.astride + node * 3;
+                float upn[3], vpn[3], apn[3], m[9], pn[3];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    un[c] = A.u[gn + c];
-                    vn[c] = A.v[gn + c];
-                    an[c] = A.a[gn + c];
+                    apn[c] = an_p[c];
+                    const float w = A.W[gn + c];
+                    upn[c] = A.U[gn + c] - A.dt * w;
+                    vpn[c] = w - 0.5f * A.dt * apn[c];
                 }
                 float* pmem = A.pmem + int64_t(f) * 9 * A.Ppml;
                 float* pnew = A.pnew + int64_t(f) * 3 * A.Ppml;
@@ -237,9 +268,9 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
                 for (int c = 0; c < 5; ++c) S[c] = A.pabar[c * A.Ppml + q];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    const float oldv = pn[c] + A.c2 * an[c];   // pml.cpp:91-109
-                    const float newv = un[c] + A.c1 * vn[c];   // pml.cpp:72-89
-                    rv[c] += S[0] * vn[c] + S[1] * un[c] + S[2] * m[c * 3 + 0] +
+                    const float oldv = pn[c] + A.c2 * apn[c];  // pml.cpp:91-109
+                    const float newv = upn[c] + A.c1 * vpn[c];  // pml.cpp:72-89
+                    rv[c] += S[0] * vpn[c] + S[1] * upn[c] + S[2] * m[c * 3 + 0] +
                              S[3] * m[c * 3 + 1] + S[4] * m[c * 3 + 2];  // pml.cpp:209-212
 #pragma unroll
                     for (int d = 0; d < 3; ++d)  // pml.cpp:133-146
@@ -248,72 +279,140 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
                     pnew[c * A.Ppml + q] = newv;
                 }
             }
-            for (int s = sb; s < se; ++s) {
+            for (int s = sb; s < se; ++s) {  // solver.cpp:576-605
                 const int4 en = A.src_ent[s];
                 if (en.x == f && A.step < A.nstf) {
                     const float sv = A.stf[int64_t(f) * A.nstf + A.step];
                     rv[en.y] += sv * A.src_w[en.z + p];
                 }
             }
-            float* r = A.r + fo + int64_t(node) * 3;
+            float* r = A.R + fo + node * 3;
             if (ATOMIC) {
                 unsafeAtomicAdd(r + 0, rv[0]);
                 unsafeAtomicAdd(r + 1, rv[1]);
                 unsafeAtomicAdd(r + 2, rv[2]);
-            } else {
-                r[0] += rv[0];
-                r[1] += rv[1];
-                r[2] += rv[2];
+            } else {  // the first element (in colour order) touching a node stores
+                r[0] = rold[0] + rv[0];
+                r[1] = rold[1] + rv[1];
+                r[2] = rold[2] + rv[2];
             }
         }
-        if (f + 1 < A.nf) __syncthreads();
     }
 }
 
-// Corrector of step n fused with the predictor of step n+1
-// (forward/share/src/solver.cpp:40-69 with beta = 0, gamma = 1/2).
-__global__ void __launch_bounds__(256) k_node(NodeArgs A) {
-    const int64_t gidx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (gidx >= A.G) return;
-    const float mi = A.minv[gidx];
+// Special nodes [0, Ga): C-PML terms, legacy damping, massless rule, a_n kept.
+__global__ void __launch_bounds__(256) k_node_special(NodeArgs A) {
+    const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= A.Ga) return;
+    const float mi = A.minv[g];
     const float dt = A.dt, hdt = 0.5f * dt, hdt2 = 0.5f * dt * dt;
-    const float dmp = A.damp ? A.damp[gidx] : 0.f;
+    const float dmp = A.damp ? A.damp[g] : 0.f;
+    const bool pml = A.pml_nodal && g < A.Gpml;
+    float C[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, al[9];
+    if (pml) {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) C[q] = A.pc[q * A.Gpml + g];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) al[q] = A.palpha[q * A.Gpml + g];
+    }
     for (int f = 0; f < A.nf; ++f) {
-        const int64_t o = int64_t(f) * A.fstride + gidx * 3;
+        const int64_t o = int64_t(f) * A.fstride + g * 3;
+        const int64_t oa = int64_t(f) * A.astride + g * 3;
+        float U[3], W[3], R[3], an[3], un[3], vn[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const float rr = A.r[o + c];
-            const float v0 = A.v[o + c];
-            const float ao = A.a[o + c];
-            // legacy ramp v -= d v before the corrector (solver.cpp:565-573)
-            float vv = (dmp > 0.f) ? v0 - dmp * v0 : v0;
-            if (mi > 0.f) {
-                const float an = rr * mi;
-                // u_{n+1} = u_n + dt v + dt^2/2 a_n = predicted (beta = 0), corrected
-                // for the damped velocity when the legacy ramp is active
-                float un = A.unext[o + c];
-                if (dmp > 0.f) {
-                    un += dt * (vv - v0);
-                    A.unext[o + c] = un;
-                }
-                vv += hdt * (ao + an);
-                A.v[o + c] = vv;
-                A.a[o + c] = an;
-                A.ucur[o + c] = un + dt * vv + hdt2 * an;
-            } else {  // m <= 0: a = 0, u and v kept (solver.cpp:57-61)
-                const float uo = A.ucur[o + c];
-                A.v[o + c] = vv;
-                A.a[o + c] = 0.f;
-                A.unext[o + c] = uo;
-                A.ucur[o + c] = uo + dt * vv;
+            U[c] = A.U[o + c];
+            W[c] = A.W[o + c];
+            R[c] = A.R[o + c];
+            an[c] = A.Aacc[oa + c];
+            un[c] = U[c] - dt * W[c];
+            vn[c] = W[c] - hdt * an[c];
+        }
+        if (pml) {
+            float* pmem = A.pmem + int64_t(f) * 9 * A.Gpml;
+            float* pnew = A.pnew + int64_t(f) * 3 * A.Gpml;
+            float m[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) m[q] = pmem[q * A.Gpml + g];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float oldv = pnew[c * A.Gpml + g] + A.c2 * an[c];  // pml.cpp:91-109
+                const float newv = un[c] + A.c1 * vn[c];                 // pml.cpp:72-89
+                R[c] += C[0] * vn[c] + C[1] * un[c] + C[2] * m[c * 3 + 0] + C[3] * m[c * 3 + 1] +
+                        C[4] * m[c * 3 + 2];  // pml.cpp:209-212, assembled over PML elements
+#pragma unroll
+                for (int d = 0; d < 3; ++d)  // pml.cpp:133-146
+                    pmem[(c * 3 + d) * A.Gpml + g] =
+                        al[d * 3 + 0] * m[c * 3 + d] + al[d * 3 + 1] * newv + al[d * 3 + 2] * oldv;
+                pnew[c * A.Gpml + g] = newv;
             }
-            A.r[o + c] = 0.f;
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            // legacy ramp v -= d v before the corrector (solver.cpp:565-573)
+            const float vp = (dmp > 0.f) ? vn[c] - dmp * vn[c] : vn[c];
+            float u1, a1, v1;
+            if (mi > 0.f) {
+                u1 = (dmp > 0.f) ? un[c] + dt * vp + hdt2 * an[c] : U[c];
+                a1 = R[c] * mi;
+                v1 = vp + hdt * (an[c] + a1);
+            } else {  // m <= 0: a = 0, u and v kept (solver.cpp:57-61)
+                u1 = un[c];
+                a1 = 0.f;
+                v1 = vp;
+            }
+            const float w1 = v1 + hdt * a1;
+            A.U[o + c] = u1 + dt * w1;
+            A.W[o + c] = w1;
+            A.Aacc[oa + c] = a1;
         }
     }
 }
 
-// Strain of the recording cells (forward/share/src/solver.cpp:282-335) and
-// copies of u, v, a at their points (solver.cpp:656-674).
+// Regular nodes [Ga, G4): a = r/m, w += dt a, u~ += dt w.  4 nodes per lane,
+// 16-byte loads and stores (Ga and the field stride are multiples of 4 nodes).
+__global__ void __launch_bounds__(256) k_node_regular(NodeArgs A) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t g = A.Ga + 4 * t;
+    if (g >= A.G4) return;
+    const float dt = A.dt;
+    const float4 mi4 = *reinterpret_cast<const float4*>(A.minv + g);
+    const float mi[4] = {mi4.x, mi4.y, mi4.z, mi4.w};
+    for (int f = 0; f < A.nf; ++f) {
+        const int64_t o = int64_t(f) * A.fstride + g * 3;
+        float4* U4 = reinterpret_cast<float4*>(A.U + o);
+        float4* W4 = reinterpret_cast<float4*>(A.W + o);
+        const float4* R4 = reinterpret_cast<const float4*>(A.R + o);
+        float u[12], w[12], r[12];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const float4 a = U4[q], b = W4[q], c = R4[q];
+            u[4 * q + 0] = a.x; u[4 * q + 1] = a.y; u[4 * q + 2] = a.z; u[4 * q + 3] = a.w;
+            w[4 * q + 0] = b.x; w[4 * q + 1] = b.y; w[4 * q + 2] = b.z; w[4 * q + 3] = b.w;
+            r[4 * q + 0] = c.x; r[4 * q + 1] = c.y; r[4 * q + 2] = c.z; r[4 * q + 3] = c.w;
+        }
+#pragma unroll
+        for (int q = 0; q < 12; ++q) {
+            const float a1 = r[q] * mi[q / 3];
+            w[q] = w[q] + dt * a1;
+            u[q] = u[q] + dt * w[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            U4[q] = make_float4(u[4 * q + 0], u[4 * q + 1], u[4 * q + 2], u[4 * q + 3]);
+            W4[q] = make_float4(w[4 * q + 0], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+        }
+    }
+}
+
+// Fill kernel (used before atomic assembly; no memset nodes in a step).
+__global__ void __launch_bounds__(256) k_zero(float4* __restrict__ p, int64_t n4) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t < n4) p[t] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Strain of the recording cells (forward/share/src/solver.cpp:282-335) from
+// u_{n+1} = U - dt W, and copies of u, v, a at their points (solver.cpp:656-674).
 template <int EPB>
 __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64) k_record(RecArgs A) {
     __shared__ float sD[25];
@@ -328,17 +427,21 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64) k_record(RecArgs 
     const int i = p / 25, j = (p / 5) % 5, k = p % 5;
     const int64_t pt = int64_t(valid ? cell : 0) * NP + p;
     const int64_t nrp = int64_t(A.n_rec) * NP;
-    const int node = valid ? A.ibool[pt] : 0;
+    const int64_t node = valid ? A.ibool[pt] : 0;
     float dd[9];
 #pragma unroll
     for (int c = 0; c < 9; ++c) dd[c] = valid ? A.dd[c * nrp + pt] : 0.f;
     __syncthreads();
     for (int f = 0; f < A.nf; ++f) {
-        const int64_t go = int64_t(f) * A.fstride + int64_t(node) * 3;
+        const int64_t go = int64_t(f) * A.fstride + node * 3;
+        const int64_t ga = int64_t(f) * A.astride + node * 3;
+        float w[3] = {0.f, 0.f, 0.f};
         if (valid) {
-            su[le][0][p] = A.u[go + 0];
-            su[le][1][p] = A.u[go + 1];
-            su[le][2][p] = A.u[go + 2];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                w[c] = A.W[go + c];
+                su[le][c][p] = A.U[go + c] - A.dt * w[c];
+            }
         }
         __syncthreads();
         if (valid) {
@@ -374,9 +477,10 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64) k_record(RecArgs 
                 float* o9 = A.uva + (int64_t(f) * nrp + pt) * 9;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
+                    const float a = A.Aacc[ga + c];
                     o9[c] = su[lel][c][p];
-                    o9[3 + c] = A.v[go + c];
-                    o9[6 + c] = A.a[go + c];
+                    o9[3 + c] = w[c] - 0.5f * A.dt * a;
+                    o9[6 + c] = a;
                 }
             }
         }

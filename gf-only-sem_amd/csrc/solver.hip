@@ -4,9 +4,11 @@
 // (forward/share/include/gf/cuda_step.hpp:20-129, forward/share/src/cuda_step.cu)
 // with one handle that owns a HIP stream, FP32 SoA device arrays and the
 // launch schedule of one step:
-//   element kernel per colour -> node kernel -> (snapshot) record kernel.
-// The host-side setup (colouring, node renumbering, FP64 -> FP32 packing)
-// happens once in gfx_solver_create.
+//   [atomic mode: zero R] -> element kernel per colour -> special-node kernel
+//   -> regular-node kernel -> (snapshot steps) record kernel.
+// Host-side setup (colouring, node renumbering, C-PML assembly, FP64 -> FP32
+// packing) happens once in gfx_solver_create.
+#include <atomic>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -69,20 +71,24 @@ struct FieldSource {
 }  // namespace gfx
 
 struct gfx_solver {
-    int E = 0, G = 0, nf = 1, Ppml = 0, n_rec = 0, n_pml_elem = 0, G_pml = 0;
-    int64_t fstride = 0;
+    int E = 0, G = 0, nf = 1, n_rec = 0, n_pml_elem = 0;
+    int64_t Ga = 0, Gpml = 0, G4 = 0, Ppml = 0;
+    int64_t fstride = 0, astride = 0;
     int assembly = GFX_ASSEMBLY_COLORED, epb = 2;
     double dt = 0;
     int stride = 1;
-    bool has_cpml = false;
+    bool has_cpml = false, pml_nodal = true;
     hipStream_t stream = nullptr;
     std::vector<int32_t> int_of_elem;        // caller element -> solver element
     std::vector<int32_t> node_new_of_old;    // caller node -> solver node
+    std::vector<double> mass_node;           // assembled mass, solver numbering
     std::vector<std::pair<int, int>> ranges;  // solver element ranges launched together
-    gfx::DevBuf<float> G9, lam, mu, minv, damp, palpha, pabar, pmem, pnew;
-    gfx::DevBuf<int32_t> ibool, pml_slot, src_off;
-    gfx::DevBuf<float> U0, U1, V, A, R;
-    bool ucur_is_u0 = true;
+    gfx::DevBuf<float> G9, lam, mu, minv, damp;
+    gfx::DevBuf<uint32_t> ibool;
+    // C-PML: nodal (pc, palpha, pmem, pnew over [0,Gpml)) or element-local (Ppml points)
+    gfx::DevBuf<float> pc, palpha, pabar, pmem, pnew;
+    gfx::DevBuf<int32_t> pml_slot, src_off;
+    gfx::DevBuf<float> U, W, R, Aacc;
     std::vector<gfx::FieldSource> sources;
     gfx::DevBuf<int4> src_ent;
     gfx::DevBuf<float> src_w, stf;
@@ -103,9 +109,6 @@ struct gfx_solver {
     ~gfx_solver() {
         if (stream) (void)hipStreamDestroy(stream);
     }
-    float* ucur() { return ucur_is_u0 ? U0.p : U1.p; }
-    float* unext() { return ucur_is_u0 ? U1.p : U0.p; }
-    const float* ucur() const { return ucur_is_u0 ? U0.p : U1.p; }
 };
 
 namespace gfx {
@@ -123,7 +126,7 @@ void build_sources(gfx_solver& S) {
         nstf = std::max(nstf, int(fs.stf.size()));
         for (size_t c = 0; c < fs.cells.size(); ++c) {
             const int e = fs.cells[c];
-            if (e < 0) continue;  // solver.cpp:585-587
+            if (e < 0) continue;  // source cell not on this rank (solver.cpp:585-587)
             const int ie = S.int_of_elem[e];
             const int off = int(w.size());
             w.insert(w.end(), fs.weights.begin() + c * NP, fs.weights.begin() + (c + 1) * NP);
@@ -137,10 +140,10 @@ void build_sources(gfx_solver& S) {
         for (const int4& q : per_elem[e]) ent.push_back(q);
     }
     off[S.E] = int32_t(ent.size());
-    std::vector<float> stf(size_t(std::max(nstf, 1)) * S.nf, 0.f);
+    const int ns = std::max(nstf, 1);
+    std::vector<float> stf(size_t(ns) * S.nf, 0.f);
     for (int f = 0; f < S.nf; ++f)
-        for (size_t t = 0; t < S.sources[f].stf.size(); ++t)
-            stf[size_t(f) * std::max(nstf, 1) + t] = S.sources[f].stf[t];
+        for (size_t t = 0; t < S.sources[f].stf.size(); ++t) stf[size_t(f) * ns + t] = S.sources[f].stf[t];
     if (ent.empty()) ent.push_back(make_int4(-1, 0, 0, 0));
     if (w.empty()) w.push_back(0.f);
     S.src_off.upload(off);
@@ -151,40 +154,50 @@ void build_sources(gfx_solver& S) {
 }
 
 void init_state(gfx_solver& S) {
-    S.U0.zero(S.stream);
-    S.U1.zero(S.stream);
-    S.V.zero(S.stream);
-    S.A.zero(S.stream);
+    S.U.zero(S.stream);
+    S.W.zero(S.stream);
     S.R.zero(S.stream);
+    S.Aacc.zero(S.stream);
     S.pmem.zero(S.stream);
     S.pnew.zero(S.stream);
     GFX_HIP(hipStreamSynchronize(S.stream));
-    S.ucur_is_u0 = true;
     S.step = 0;
     S.n_snap = 0;
 }
 
-template <int EPB>
+template <int EPB, bool PE>
 void launch_element_t(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
     constexpr int NT = ((EPB * NP + 63) / 64) * 64;
     const int grid = (n + EPB - 1) / EPB;
     if (grid == 0) return;
     if (S.assembly == GFX_ASSEMBLY_ATOMIC)
-        dev::k_element<EPB, true><<<grid, NT, 0, S.stream>>>(a, b, n);
+        dev::k_element<EPB, true, PE><<<grid, NT, 0, S.stream>>>(a, b, n);
     else
-        dev::k_element<EPB, false><<<grid, NT, 0, S.stream>>>(a, b, n);
+        dev::k_element<EPB, false, PE><<<grid, NT, 0, S.stream>>>(a, b, n);
 }
 
-void launch_element(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
+template <bool PE>
+void launch_element_p(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
     switch (S.epb) {
-        case 1: launch_element_t<1>(S, a, b, n); break;
-        case 2: launch_element_t<2>(S, a, b, n); break;
-        case 4: launch_element_t<4>(S, a, b, n); break;
-        default: launch_element_t<8>(S, a, b, n); break;
+        case 1: launch_element_t<1, PE>(S, a, b, n); break;
+        case 2: launch_element_t<2, PE>(S, a, b, n); break;
+        case 4: launch_element_t<4, PE>(S, a, b, n); break;
+        default: launch_element_t<8, PE>(S, a, b, n); break;
     }
 }
 
+void launch_element(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
+    if (S.has_cpml && !S.pml_nodal)
+        launch_element_p<true>(S, a, b, n);
+    else
+        launch_element_p<false>(S, a, b, n);
+}
+
 void run_steps(gfx_solver& S, int nsteps) {
+    const double theta = 1.0 / 8.0;  // pml.cpp:35, coefficients pml.cpp:63-64
+    const float c1 = float((1.0 - 2.0 * theta) * 0.5 * S.dt);
+    const float c2 = float((1.0 - theta) * 0.5 * S.dt * S.dt);
+
     dev::ElemArgs ea{};
     ea.G = S.G9.p;
     ea.lam = S.lam.p;
@@ -193,18 +206,20 @@ void run_steps(gfx_solver& S, int nsteps) {
     ea.P = int64_t(S.E) * NP;
     ea.nf = S.nf;
     ea.fstride = S.fstride;
-    ea.v = S.V.p;
-    ea.a = S.A.p;
-    ea.r = S.R.p;
+    ea.U = S.U.p;
+    ea.R = S.R.p;
+    ea.W = S.W.p;
+    ea.Aacc = S.Aacc.p;
+    ea.astride = S.astride;
     ea.pml_slot = S.pml_slot.p;
     ea.Ppml = S.Ppml;
     ea.palpha = S.palpha.p;
     ea.pabar = S.pabar.p;
     ea.pmem = S.pmem.p;
     ea.pnew = S.pnew.p;
-    const double theta = 1.0 / 8.0;  // pml.cpp:35, coefficients pml.cpp:63-64
-    ea.c1 = float((1.0 - 2.0 * theta) * 0.5 * S.dt);
-    ea.c2 = float((1.0 - theta) * 0.5 * S.dt * S.dt);
+    ea.c1 = c1;
+    ea.c2 = c2;
+    ea.dt = float(S.dt);
     ea.src_off = S.src_off.p;
     ea.src_ent = S.src_ent.p;
     ea.src_w = S.src_w.p;
@@ -214,15 +229,30 @@ void run_steps(gfx_solver& S, int nsteps) {
 
     dev::NodeArgs na{};
     na.G = S.G;
+    na.Ga = S.Ga;
+    na.Gpml = S.Gpml;
+    na.G4 = S.G4;
     na.nf = S.nf;
     na.fstride = S.fstride;
+    na.astride = S.astride;
     na.minv = S.minv.p;
-    na.v = S.V.p;
-    na.a = S.A.p;
-    na.r = S.R.p;
     na.damp = S.has_cpml ? nullptr : S.damp.p;
+    na.U = S.U.p;
+    na.W = S.W.p;
+    na.R = S.R.p;
+    na.Aacc = S.Aacc.p;
+    na.pml_nodal = (S.has_cpml && S.pml_nodal) ? 1 : 0;
+    na.pc = S.pc.p;
+    na.palpha = S.palpha.p;
+    na.pmem = S.pmem.p;
+    na.pnew = S.pnew.p;
+    na.c1 = c1;
+    na.c2 = c2;
     na.dt = float(S.dt);
-    const int node_grid = int((int64_t(S.G) + 255) / 256);
+    const int spec_grid = int((S.Ga + 255) / 256);
+    const int reg_grid = int(((S.G4 - S.Ga) / 4 + 255) / 256);
+    const int64_t r4 = int64_t(S.nf) * S.fstride / 4;
+    const int zero_grid = int((r4 + 255) / 256);
 
     dev::RecArgs ra{};
     ra.n_rec = S.n_rec;
@@ -230,36 +260,38 @@ void run_steps(gfx_solver& S, int nsteps) {
     ra.dd = S.rec_dd.p;
     ra.nf = S.nf;
     ra.fstride = S.fstride;
-    ra.v = S.V.p;
-    ra.a = S.A.p;
+    ra.astride = S.astride;
+    ra.U = S.U.p;
+    ra.W = S.W.p;
+    ra.Aacc = S.Aacc.p;
+    ra.dt = float(S.dt);
     std::memcpy(ra.D, S.D, sizeof(S.D));
     constexpr int REPB = 2;
     const int rec_grid = (S.n_rec + REPB - 1) / REPB;
 
     std::vector<hipEvent_t> ev;
     if (S.timing) {
-        ev.resize(size_t(nsteps) * 4 + 1);
+        // ev[0] = start; step t: ev[3t+1] after elements, ev[3t+2] after nodes,
+        // ev[3t+3] after recording (= start of step t+1)
+        ev.resize(size_t(nsteps) * 3 + 1);
         for (auto& e : ev) GFX_HIP(hipEventCreate(&e));
         GFX_HIP(hipEventRecord(ev[0], S.stream));
     }
     std::vector<char> did_rec(nsteps, 0);
     for (int t = 0; t < nsteps; ++t) {
-        ea.ut = S.unext();
-        ea.u = S.ucur();
         ea.step = S.step;
+        if (S.assembly == GFX_ASSEMBLY_ATOMIC)
+            dev::k_zero<<<zero_grid, 256, 0, S.stream>>>(reinterpret_cast<float4*>(S.R.p), r4);
         for (const auto& rg : S.ranges) launch_element(S, ea, rg.first, rg.second - rg.first);
         GFX_HIP(hipGetLastError());
-        if (S.timing) GFX_HIP(hipEventRecord(ev[size_t(t) * 4 + 1], S.stream));
-        na.ucur = S.ucur();
-        na.unext = S.unext();
-        dev::k_node<<<node_grid, 256, 0, S.stream>>>(na);
+        if (S.timing) GFX_HIP(hipEventRecord(ev[size_t(t) * 3 + 1], S.stream));
+        if (spec_grid > 0) dev::k_node_special<<<spec_grid, 256, 0, S.stream>>>(na);
+        if (reg_grid > 0) dev::k_node_regular<<<reg_grid, 256, 0, S.stream>>>(na);
         GFX_HIP(hipGetLastError());
-        if (S.timing) GFX_HIP(hipEventRecord(ev[size_t(t) * 4 + 2], S.stream));
-        S.ucur_is_u0 = !S.ucur_is_u0;  // u_{n+1} now current
+        if (S.timing) GFX_HIP(hipEventRecord(ev[size_t(t) * 3 + 2], S.stream));
         if (S.stride > 0 && S.step % S.stride == 0 && S.rec_cap > 0 && S.n_rec > 0) {
             const int slot = S.n_snap % S.rec_cap;
             const int64_t nrp = int64_t(S.n_rec) * NP;
-            ra.u = S.ucur();
             ra.strain = S.rec_strain.p + int64_t(slot) * S.nf * nrp * 6;
             ra.uva = S.rec_uva_on ? S.rec_uva.p + int64_t(slot) * S.nf * nrp * 9 : nullptr;
             dev::k_record<REPB><<<rec_grid, ((REPB * NP + 63) / 64) * 64, 0, S.stream>>>(ra);
@@ -267,7 +299,7 @@ void run_steps(gfx_solver& S, int nsteps) {
             did_rec[t] = 1;
         }
         if (S.stride > 0 && S.step % S.stride == 0) ++S.n_snap;
-        if (S.timing) GFX_HIP(hipEventRecord(ev[size_t(t) * 4 + 3], S.stream));
+        if (S.timing) GFX_HIP(hipEventRecord(ev[size_t(t) * 3 + 3], S.stream));
         ++S.step;
     }
     GFX_HIP(hipStreamSynchronize(S.stream));
@@ -276,20 +308,18 @@ void run_steps(gfx_solver& S, int nsteps) {
         S.n_elem_launch = S.n_node_launch = S.n_rec_launch = 0;
         for (int t = 0; t < nsteps; ++t) {
             float a = 0, b = 0, c = 0;
-            hipEvent_t e0 = ev[size_t(t) * 4], e1 = ev[size_t(t) * 4 + 1],
-                       e2 = ev[size_t(t) * 4 + 2], e3 = ev[size_t(t) * 4 + 3];
-            GFX_HIP(hipEventElapsedTime(&a, e0, e1));
-            GFX_HIP(hipEventElapsedTime(&b, e1, e2));
-            GFX_HIP(hipEventElapsedTime(&c, e2, e3));
+            GFX_HIP(hipEventElapsedTime(&a, ev[size_t(t) * 3], ev[size_t(t) * 3 + 1]));
+            GFX_HIP(hipEventElapsedTime(&b, ev[size_t(t) * 3 + 1], ev[size_t(t) * 3 + 2]));
+            GFX_HIP(hipEventElapsedTime(&c, ev[size_t(t) * 3 + 2], ev[size_t(t) * 3 + 3]));
             S.t_elem += a;
             S.t_node += b;
             S.t_rec += c;
             S.n_elem_launch += int(S.ranges.size());
-            S.n_node_launch += 1;
+            S.n_node_launch += (spec_grid > 0) + (reg_grid > 0);
             S.n_rec_launch += did_rec[t];
         }
         float tot = 0;
-        GFX_HIP(hipEventElapsedTime(&tot, ev[0], ev[size_t(nsteps) * 4]));
+        GFX_HIP(hipEventElapsedTime(&tot, ev[0], ev[size_t(nsteps) * 3]));
         S.t_total = tot;
         S.n_total_launch = nsteps;
         for (auto& e : ev) (void)hipEventDestroy(e);
@@ -305,7 +335,7 @@ std::vector<int> colour_elements(int E, const int32_t* ibool, int G) {
         uint64_t m = 0;
         for (int p = 0; p < NP; ++p) m |= used[ibool[int64_t(e) * NP + p]];
         int c = 0;
-        while (c < 64 && (m >> c) & 1ULL) ++c;
+        while (c < 64 && ((m >> c) & 1ULL)) ++c;
         require(c < 64, "element colouring needs more than 64 colours");
         col[e] = c;
         for (int p = 0; p < NP; ++p) used[ibool[int64_t(e) * NP + p]] |= (1ULL << c);
@@ -324,6 +354,7 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
             "unknown assembly mode");
     const int epb = d.elems_per_block ? d.elems_per_block : 2;
     require(epb == 1 || epb == 2 || epb == 4 || epb == 8, "elems_per_block must be 1, 2, 4 or 8");
+    require(d.n_rank_node < int(0x7fffffff) - 8, "too many nodes");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
         (void)hipGetLastError();
@@ -341,12 +372,15 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
     S.dt = d.dt;
     S.stride = d.snapshot_stride;
     S.has_cpml = d.has_cpml != 0;
+    if (S.has_cpml)
+        require(d.pml_region && d.coef_alpha && d.coef_abar && d.density,
+                "has_cpml needs pml_region, coef_alpha, coef_abar and density");
     GFX_HIP(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
 
     const GLL gll(4);
     for (int q = 0; q < 25; ++q) S.D[q] = float(gll.D[q]);
 
-    // solver element order: by colour (coloured assembly) or caller order
+    // ---- element order: by colour (coloured assembly) or caller order ----
     std::vector<int32_t> elem_of_int(E);
     std::iota(elem_of_int.begin(), elem_of_int.end(), 0);
     if (S.assembly == GFX_ASSEMBLY_COLORED) {
@@ -365,23 +399,57 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
     S.int_of_elem.assign(E, 0);
     for (int i = 0; i < E; ++i) S.int_of_elem[elem_of_int[i]] = i;
 
-    // node numbering by first touch in solver element order
-    S.node_new_of_old.assign(G, -1);
-    int next = 0;
-    for (int i = 0; i < E; ++i) {
-        const int64_t e = elem_of_int[i];
+    // ---- nodal mass and legacy damping (solver.cpp:193-202), caller numbering ----
+    std::vector<double> mg(G, 0.0), dg(G, 0.0);
+    for (int64_t p = 0; p < P; ++p) {
+        mg[d.ibool[p]] += d.mass[p];
+        if (d.damping) dg[d.ibool[p]] = d.damping[p];  // last writer wins
+    }
+
+    // ---- node classes: 2 = C-PML node, 1 = other special, 0 = regular ----
+    std::vector<char> cls(G, 0);
+    if (S.has_cpml) {
+        for (int e = 0; e < E; ++e)
+            if (d.pml_region[e] != 0)
+                for (int p = 0; p < NP; ++p) cls[d.ibool[int64_t(e) * NP + p]] = 2;
+    }
+    for (int g = 0; g < G; ++g) {
+        if (cls[g]) continue;
+        if (mg[g] <= 0.0 || (!S.has_cpml && dg[g] > 0.0)) cls[g] = 1;
+    }
+    S.rec_cells.assign(d.rec_cell, d.rec_cell + std::max(d.n_rec_cell, 0));
+    for (int32_t e : S.rec_cells) {
+        require(e >= 0 && e < E, "recording cell out of range");
         for (int p = 0; p < NP; ++p) {
-            const int g = d.ibool[e * NP + p];
-            if (S.node_new_of_old[g] < 0) S.node_new_of_old[g] = next++;
+            char& c = cls[d.ibool[int64_t(e) * NP + p]];
+            if (!c) c = 1;
         }
     }
-    for (int g = 0; g < G; ++g)
-        if (S.node_new_of_old[g] < 0) S.node_new_of_old[g] = next++;  // nodes in no element
+    // numbering: C-PML nodes, other special nodes, regular nodes; first touch
+    // in solver element order inside each class
+    S.node_new_of_old.assign(G, -1);
+    int next = 0;
+    for (int pass = 2; pass >= 0; --pass) {
+        for (int i = 0; i < E; ++i) {
+            const int64_t e = elem_of_int[i];
+            for (int p = 0; p < NP; ++p) {
+                const int g = d.ibool[e * NP + p];
+                if (cls[g] == pass && S.node_new_of_old[g] < 0) S.node_new_of_old[g] = next++;
+            }
+        }
+        for (int g = 0; g < G; ++g)  // nodes in no element
+            if (cls[g] == pass && S.node_new_of_old[g] < 0) S.node_new_of_old[g] = next++;
+        if (pass == 2) S.Gpml = next;
+    }
+    int64_t n_special = 0;
+    for (int g = 0; g < G; ++g) n_special += cls[g] ? 1 : 0;
+    S.Ga = std::min<int64_t>(((n_special + 3) / 4) * 4, ((int64_t(G) + 3) / 4) * 4);
+    S.G4 = ((int64_t(G) + 3) / 4) * 4;
 
-    // geometry / material, FP64 -> FP32 SoA
+    // ---- geometry / material, FP64 -> FP32 SoA, first-touch flags ----
     std::vector<float> hG(size_t(9) * P), hlam(P), hmu(P);
-    std::vector<int32_t> hib(P);
-    bool neg_jac = false;
+    std::vector<uint32_t> hib(P);
+    std::atomic<bool> neg_jac{false};
     parallel_for(E, [&](int64_t ie) {
         const int64_t e = elem_of_int[ie];
         for (int p = 0; p < NP; ++p) {
@@ -394,80 +462,131 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
             const bool solid = d.mu_[src] > 0.0;  // element_cpu.cpp:42-43
             hlam[dst] = solid ? float(d.lambda_[src]) : 0.f;
             hmu[dst] = solid ? float(d.mu_[src]) : 0.f;
-            hib[dst] = S.node_new_of_old[d.ibool[src]];
+            hib[dst] = uint32_t(S.node_new_of_old[d.ibool[src]]);
         }
     });
-    if (neg_jac)
-        throw Error(GFX_ERR_UNSUPPORTED, "negative jacobian*weight (inverted element)");
+    if (neg_jac) throw Error(GFX_ERR_UNSUPPORTED, "negative jacobian*weight (inverted element)");
+    if (S.assembly == GFX_ASSEMBLY_COLORED) {
+        std::vector<char> seen(G, 0);
+        for (int64_t q = 0; q < P; ++q) {
+            const uint32_t n = hib[q];
+            if (!seen[n]) {
+                seen[n] = 1;
+                hib[q] |= dev::FIRST_TOUCH;
+            }
+        }
+    }
     S.G9.upload(hG);
     S.lam.upload(hlam);
     S.mu.upload(hmu);
     S.ibool.upload(hib);
     std::vector<float>().swap(hG);
 
-    // nodal mass (solver.cpp:193-202, accumulated in caller order, FP64)
-    std::vector<double> mg(G, 0.0), dg(G, 0.0);
-    for (int64_t p = 0; p < P; ++p) {
-        mg[d.ibool[p]] += d.mass[p];
-        if (d.damping) dg[d.ibool[p]] = d.damping[p];  // last writer wins
-    }
-    std::vector<float> hminv(G), hdamp(G);
+    S.mass_node.assign(G, 0.0);
+    std::vector<float> hminv(S.G4, 0.f), hdamp(std::max<int64_t>(S.Ga, 1), 0.f);
     for (int g = 0; g < G; ++g) {
         const int n = S.node_new_of_old[g];
+        S.mass_node[n] = mg[g];
         hminv[n] = mg[g] > 0.0 ? float(1.0 / mg[g]) : 0.f;
-        hdamp[n] = float(dg[g]);
+        if (n < S.Ga) hdamp[n] = float(dg[g]);
     }
     S.minv.upload(hminv);
     S.damp.upload(hdamp);
 
-    // C-PML, compact over PML elements in solver order
+    // ---- C-PML ----
     std::vector<int32_t> hslot(E, -1);
-    int npml = 0;
+    S.pml_nodal = true;
     if (S.has_cpml) {
-        require(d.pml_region && d.coef_alpha && d.coef_abar && d.density,
-                "has_cpml needs pml_region, coef_alpha, coef_abar and density");
-        for (int ie = 0; ie < E; ++ie)
-            if (d.pml_region[elem_of_int[ie]] != 0) hslot[ie] = npml++;
-    }
-    S.n_pml_elem = npml;
-    S.Ppml = npml * NP;
-    {
-        const int64_t Q = std::max<int64_t>(S.Ppml, 1);
-        std::vector<float> hal(size_t(9) * Q, 0.f), hab(size_t(5) * Q, 0.f);
-        std::vector<char> pml_node(G, 0);
-        for (int ie = 0; ie < E; ++ie) {
-            if (hslot[ie] < 0) continue;
-            const int64_t e = elem_of_int[ie];
+        // Nodal form: the acceleration term sum_e w J/rho (A1 v + A2 u + A3 mx + A4 my + A5 mz)
+        // assembles to per-node coefficients, and every element's memory variable at a node
+        // evolves identically when the coef_alpha of all elements that use it (A_{3+d} != 0)
+        // agree.  Checked here; otherwise the element-local form of pml.cpp is used.
+        const int64_t Gp = S.Gpml;
+        std::vector<double> C(size_t(5) * Gp, 0.0), al(size_t(9) * Gp, 0.0);
+        std::vector<char> have(size_t(3) * Gp, 0);
+        bool consistent = true;
+        for (int64_t e = 0; e < E && consistent; ++e) {
+            if (d.pml_region[e] == 0) continue;
             for (int p = 0; p < NP; ++p) {
-                const int64_t src = e * NP + p, q = int64_t(hslot[ie]) * NP + p;
+                const int64_t src = e * NP + p;
+                const int64_t n = S.node_new_of_old[d.ibool[src]];
                 const int pi = p / 25, pj = (p / 5) % 5, pk = p % 5;
                 const double wg = gll.w[pi] * gll.w[pj] * gll.w[pk];
                 const double rho = d.density[src];
                 const double scale = wg * (rho > 0.0 ? 1.0 / rho : 0.0) * d.jacobian[src];
-                for (int c = 0; c < 9; ++c) hal[c * Q + q] = float(d.coef_alpha[src * 9 + c]);
-                for (int c = 0; c < 5; ++c) hab[c * Q + q] = float(scale * d.coef_abar[src * 5 + c]);
-                pml_node[d.ibool[src]] = 1;
+                for (int q = 0; q < 5; ++q) C[q * Gp + n] += scale * d.coef_abar[src * 5 + q];
+                for (int dir = 0; dir < 3; ++dir) {
+                    if (d.coef_abar[src * 5 + 2 + dir] == 0.0) continue;  // m_dir unused here
+                    const double* a3 = &d.coef_alpha[src * 9 + dir * 3];
+                    if (!have[dir * Gp + n]) {
+                        have[dir * Gp + n] = 1;
+                        for (int q = 0; q < 3; ++q) al[(dir * 3 + q) * Gp + n] = a3[q];
+                    } else {
+                        for (int q = 0; q < 3; ++q)
+                            if (al[(dir * 3 + q) * Gp + n] != a3[q]) consistent = false;
+                    }
+                }
             }
         }
-        S.G_pml = int(std::count(pml_node.begin(), pml_node.end(), 1));
-        S.palpha.upload(hal);
-        S.pabar.upload(hab);
-        S.pmem.alloc(size_t(S.nf) * 9 * Q);
-        S.pnew.alloc(size_t(S.nf) * 3 * Q);
+        if (consistent) {
+            std::vector<float> hc(size_t(5) * std::max<int64_t>(Gp, 1), 0.f),
+                hal(size_t(9) * std::max<int64_t>(Gp, 1), 0.f);
+            for (int64_t n = 0; n < Gp; ++n) {
+                for (int q = 0; q < 5; ++q) hc[q * Gp + n] = float(C[q * Gp + n]);
+                for (int dir = 0; dir < 3; ++dir)
+                    for (int q = 0; q < 3; ++q)
+                        hal[(dir * 3 + q) * Gp + n] =
+                            have[dir * Gp + n] ? float(al[(dir * 3 + q) * Gp + n]) : (q == 0 ? 1.f : 0.f);
+            }
+            S.pc.upload(hc);
+            S.palpha.upload(hal);
+            S.pmem.alloc(size_t(S.nf) * 9 * std::max<int64_t>(Gp, 1));
+            S.pnew.alloc(size_t(S.nf) * 3 * std::max<int64_t>(Gp, 1));
+        } else {
+            S.pml_nodal = false;
+            int npml = 0;
+            for (int ie = 0; ie < E; ++ie)
+                if (d.pml_region[elem_of_int[ie]] != 0) hslot[ie] = npml++;
+            S.n_pml_elem = npml;
+            S.Ppml = int64_t(npml) * NP;
+            const int64_t Q = std::max<int64_t>(S.Ppml, 1);
+            std::vector<float> hal(size_t(9) * Q, 0.f), hab(size_t(5) * Q, 0.f);
+            for (int ie = 0; ie < E; ++ie) {
+                if (hslot[ie] < 0) continue;
+                const int64_t e = elem_of_int[ie];
+                for (int p = 0; p < NP; ++p) {
+                    const int64_t src = e * NP + p, q = int64_t(hslot[ie]) * NP + p;
+                    const int pi = p / 25, pj = (p / 5) % 5, pk = p % 5;
+                    const double wg = gll.w[pi] * gll.w[pj] * gll.w[pk];
+                    const double rho = d.density[src];
+                    const double scale = wg * (rho > 0.0 ? 1.0 / rho : 0.0) * d.jacobian[src];
+                    for (int c = 0; c < 9; ++c) hal[c * Q + q] = float(d.coef_alpha[src * 9 + c]);
+                    for (int c = 0; c < 5; ++c) hab[c * Q + q] = float(scale * d.coef_abar[src * 5 + c]);
+                }
+            }
+            S.palpha.upload(hal);
+            S.pabar.upload(hab);
+            S.pmem.alloc(size_t(S.nf) * 9 * Q);
+            S.pnew.alloc(size_t(S.nf) * 3 * Q);
+        }
+    }
+    if (S.has_cpml) {
+        int npe = 0;
+        for (int e = 0; e < E; ++e) npe += d.pml_region[e] != 0;
+        S.n_pml_elem = npe;
     }
     S.pml_slot.upload(hslot);
 
-    // nodal state: [nf][fstride], fstride padded to 64 floats
-    S.fstride = ((int64_t(G) * 3 + 63) / 64) * 64;
+    // ---- nodal state: [nf][fstride] (U, W, R) and [nf][astride] (a at special nodes) ----
+    S.fstride = ((S.G4 * 3 + 63) / 64) * 64;
+    S.astride = ((std::max<int64_t>(S.Ga, 1) * 3 + 63) / 64) * 64;
     const size_t fs = size_t(S.fstride) * S.nf;
-    S.U0.alloc(fs);
-    S.U1.alloc(fs);
-    S.V.alloc(fs);
-    S.A.alloc(fs);
+    S.U.alloc(fs);
+    S.W.alloc(fs);
     S.R.alloc(fs);
+    S.Aacc.alloc(size_t(S.astride) * S.nf);
 
-    // recording cells
-    S.rec_cells.assign(d.rec_cell, d.rec_cell + std::max(d.n_rec_cell, 0));
+    // ---- recording cells ----
     S.n_rec = int(S.rec_cells.size());
     S.rec_cap = std::max(d.record_capacity, 0);
     S.rec_uva_on = d.record_uva != 0;
@@ -477,7 +596,6 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
         std::vector<float> rdd(size_t(9) * nrp);
         for (int c = 0; c < S.n_rec; ++c) {
             const int64_t e = S.rec_cells[c];
-            require(e >= 0 && e < E, "recording cell out of range");
             for (int p = 0; p < NP; ++p) {
                 rib[int64_t(c) * NP + p] = S.node_new_of_old[d.ibool[e * NP + p]];
                 for (int q = 0; q < 9; ++q)
@@ -500,6 +618,12 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
 void check_field(const gfx_solver* s, int field) {
     require(s != nullptr, "null solver");
     require(field >= 0 && field < s->nf, "field index out of range");
+}
+
+std::vector<float> fetch(const gfx_solver* s, const float* p, size_t n) {
+    std::vector<float> h(n);
+    if (n) GFX_HIP(hipMemcpy(h.data(), p, n * sizeof(float), hipMemcpyDeviceToHost));
+    return h;
 }
 
 }  // namespace
@@ -553,6 +677,7 @@ extern "C" gfx_status gfx_solver_set_source(gfx_solver* s, int field, int direct
             for (int p = 0; p < NP; ++p) fs.weights.push_back(float(src_weights[int64_t(c) * NP + p]));
         }
         for (int t = 0; t < n_stf; ++t) fs.stf.push_back(float(stf[t]));
+        GFX_HIP(hipStreamSynchronize(s->stream));
         s->sources[field] = std::move(fs);
         build_sources(*s);
     });
@@ -580,17 +705,28 @@ extern "C" gfx_status gfx_solver_get_state(const gfx_solver* s, int field, doubl
                                            double* a) {
     return guarded([&] {
         check_field(s, field);
-        std::vector<float> h(size_t(s->G) * 3);
-        const float* srcs[3] = {s->ucur(), s->V.p, s->A.p};
-        double* dsts[3] = {u, v, a};
         GFX_HIP(hipStreamSynchronize(s->stream));
-        for (int q = 0; q < 3; ++q) {
-            if (!dsts[q]) continue;
-            GFX_HIP(hipMemcpy(h.data(), srcs[q] + int64_t(field) * s->fstride,
-                              h.size() * sizeof(float), hipMemcpyDeviceToHost));
-            for (int g = 0; g < s->G; ++g) {
-                const int64_t n = s->node_new_of_old[g];
-                for (int c = 0; c < 3; ++c) dsts[q][int64_t(g) * 3 + c] = h[n * 3 + c];
+        const size_t n3 = size_t(s->G) * 3;
+        const std::vector<float> hU = fetch(s, s->U.p + int64_t(field) * s->fstride, n3);
+        const std::vector<float> hW = fetch(s, s->W.p + int64_t(field) * s->fstride, n3);
+        const std::vector<float> hR = fetch(s, s->R.p + int64_t(field) * s->fstride, n3);
+        const std::vector<float> hA = fetch(s, s->Aacc.p + int64_t(field) * s->astride, size_t(s->Ga) * 3);
+        const float dt = float(s->dt);
+        for (int g = 0; g < s->G; ++g) {
+            const int64_t n = s->node_new_of_old[g];
+            for (int c = 0; c < 3; ++c) {
+                const float w = hW[n * 3 + c];
+                // regular nodes keep r = m a_n in R (it is not cleared between steps)
+                float an;
+                if (n < s->Ga) {
+                    an = hA[n * 3 + c];
+                } else {
+                    const double m = s->mass_node[n];
+                    an = (m > 0.0) ? hR[n * 3 + c] * float(1.0 / m) : 0.f;
+                }
+                if (u) u[int64_t(g) * 3 + c] = hU[n * 3 + c] - dt * w;
+                if (v) v[int64_t(g) * 3 + c] = w - 0.5f * dt * an;
+                if (a) a[int64_t(g) * 3 + c] = an;
             }
         }
     });
@@ -602,24 +738,27 @@ extern "C" gfx_status gfx_solver_set_state(gfx_solver* s, int field, const doubl
         check_field(s, field);
         require(u && v && a, "u, v and a are required");
         const size_t n3 = size_t(s->G) * 3;
-        std::vector<float> hu(n3), hv(n3), ha(n3), ht(n3);
+        std::vector<float> hU(n3), hW(n3), hR(n3), hA(size_t(s->Ga) * 3, 0.f);
         const double dt = s->dt;
         for (int g = 0; g < s->G; ++g) {
             const int64_t n = s->node_new_of_old[g];
+            const double m = s->mass_node[n];
             for (int c = 0; c < 3; ++c) {
                 const int64_t o = int64_t(g) * 3 + c;
-                hu[n * 3 + c] = float(u[o]);
-                hv[n * 3 + c] = float(v[o]);
-                ha[n * 3 + c] = float(a[o]);
-                ht[n * 3 + c] = float(u[o] + dt * v[o] + 0.5 * dt * dt * a[o]);  // predictor
+                const double ag = (m > 0.0) ? a[o] : 0.0;  // massless nodes carry a = 0
+                hU[n * 3 + c] = float(u[o] + dt * v[o] + 0.5 * dt * dt * ag);
+                hW[n * 3 + c] = float(v[o] + 0.5 * dt * ag);
+                hR[n * 3 + c] = float(ag * m);
+                if (n < s->Ga) hA[n * 3 + c] = float(ag);
             }
         }
         GFX_HIP(hipStreamSynchronize(s->stream));
-        const int64_t fo = int64_t(field) * s->fstride;
-        GFX_HIP(hipMemcpy(s->ucur() + fo, hu.data(), n3 * 4, hipMemcpyHostToDevice));
-        GFX_HIP(hipMemcpy(s->unext() + fo, ht.data(), n3 * 4, hipMemcpyHostToDevice));
-        GFX_HIP(hipMemcpy(s->V.p + fo, hv.data(), n3 * 4, hipMemcpyHostToDevice));
-        GFX_HIP(hipMemcpy(s->A.p + fo, ha.data(), n3 * 4, hipMemcpyHostToDevice));
+        GFX_HIP(hipMemcpy(s->U.p + int64_t(field) * s->fstride, hU.data(), n3 * 4, hipMemcpyHostToDevice));
+        GFX_HIP(hipMemcpy(s->W.p + int64_t(field) * s->fstride, hW.data(), n3 * 4, hipMemcpyHostToDevice));
+        GFX_HIP(hipMemcpy(s->R.p + int64_t(field) * s->fstride, hR.data(), n3 * 4, hipMemcpyHostToDevice));
+        if (!hA.empty())
+            GFX_HIP(hipMemcpy(s->Aacc.p + int64_t(field) * s->astride, hA.data(), hA.size() * 4,
+                              hipMemcpyHostToDevice));
     });
 }
 
@@ -636,15 +775,13 @@ extern "C" gfx_status gfx_solver_get_record(const gfx_solver* s, int field, int 
         const int64_t nrp = int64_t(s->n_rec) * NP;
         GFX_HIP(hipStreamSynchronize(s->stream));
         if (strain) {
-            std::vector<float> h(nrp * 6);
-            GFX_HIP(hipMemcpy(h.data(), s->rec_strain.p + (int64_t(slot) * s->nf + field) * nrp * 6,
-                              h.size() * 4, hipMemcpyDeviceToHost));
+            const std::vector<float> h =
+                fetch(s, s->rec_strain.p + (int64_t(slot) * s->nf + field) * nrp * 6, size_t(nrp) * 6);
             for (size_t q = 0; q < h.size(); ++q) strain[q] = h[q];
         }
         if (disp || vel || acc) {
-            std::vector<float> h(nrp * 9);
-            GFX_HIP(hipMemcpy(h.data(), s->rec_uva.p + (int64_t(slot) * s->nf + field) * nrp * 9,
-                              h.size() * 4, hipMemcpyDeviceToHost));
+            const std::vector<float> h =
+                fetch(s, s->rec_uva.p + (int64_t(slot) * s->nf + field) * nrp * 9, size_t(nrp) * 9);
             for (int64_t q = 0; q < nrp; ++q)
                 for (int c = 0; c < 3; ++c) {
                     if (disp) disp[q * 3 + c] = h[q * 9 + c];
@@ -685,17 +822,35 @@ extern "C" gfx_status gfx_solver_kernel_time(const gfx_solver* s, const char* na
     });
 }
 
+extern "C" gfx_status gfx_solver_query(const gfx_solver* s, const char* key, double* value) {
+    return guarded([&] {
+        require(s && key && value, "null argument");
+        const std::string k(key);
+        if (k == "colors") *value = double(s->ranges.size());
+        else if (k == "pml_nodal") *value = (s->has_cpml && s->pml_nodal) ? 1.0 : 0.0;
+        else if (k == "pml_elements") *value = s->n_pml_elem;
+        else if (k == "special_nodes") *value = double(s->Ga);
+        else if (k == "pml_nodes") *value = double(s->Gpml);
+        else if (k == "fields") *value = s->nf;
+        else if (k == "nodes") *value = s->G;
+        else if (k == "elements") *value = s->E;
+        else throw Error(GFX_ERR_INVALID, "unknown query key: " + k);
+    });
+}
+
 extern "C" double gfx_solver_algorithmic_bytes(const gfx_solver* s, int which) {
     if (!s) return 0.0;
-    const double P = double(s->E) * NP, G = s->G, F = s->nf, Q = s->Ppml, Gp = s->G_pml;
-    // element stage: geometry 9+1+1 floats + ibool per point; per field the
-    // predicted displacement read and residual write (12 B each per node);
-    // C-PML: alpha 9 + scaled abar 5 floats per PML point, per field the
-    // memory (9) and "new" (3) read and written, and u, v, a read at PML nodes.
-    const double elem = 48.0 * P + F * 24.0 * G + 56.0 * Q + F * (96.0 * Q + 36.0 * Gp);
-    // node stage: 1/m per node; per field r, u_{n+1}, v, a read and
-    // v, a, predicted u, zeroed r written (12 B each).
-    const double node = 4.0 * G + F * 96.0 * G;
+    const double P = double(s->E) * NP, G = s->G, F = s->nf, Ga = double(s->Ga),
+                 Gp = double(s->Gpml), Q = double(s->Ppml);
+    // element stage: geometry (9 scaled dxi_dx + lambda + mu + node id) per point;
+    // per field the predicted displacement read and the residual written once per node.
+    double elem = 48.0 * P + F * 24.0 * G;
+    if (s->has_cpml && !s->pml_nodal) elem += 56.0 * Q + F * (96.0 * Q + 36.0 * Gp);
+    // node stage: 1/m per node; per field r, u~, w read and u~, w written;
+    // special nodes also read and write a; C-PML nodes read 5 + 9 coefficients
+    // and per field read and write the memory (9) and "new" (3) values.
+    double node = 4.0 * G + F * 60.0 * G + F * 24.0 * Ga;
+    if (s->has_cpml && s->pml_nodal) node += 56.0 * Gp + F * 96.0 * Gp;
     if (which == 1) return elem;
     if (which == 2) return node;
     return elem + node;

@@ -111,6 +111,7 @@ def _load():
         "gfx_solver_kernel_time": (st, [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
         "gfx_solver_set_timing": (st, [vp, C.c_int]),
         "gfx_solver_algorithmic_bytes": (C.c_double, [vp, C.c_int]),
+        "gfx_solver_query": (st, [vp, C.c_char_p, C.POINTER(C.c_double)]),
         "gfx_box_spec_default": (st, [C.POINTER(BoxSpec)]),
         "gfx_mesh_build_box": (st, [C.POINTER(BoxSpec), C.POINTER(C.POINTER(_Mesh))]),
         "gfx_mesh_free": (st, [C.POINTER(_Mesh)]),
@@ -398,6 +399,11 @@ class Solver:
         n = C.c_int(0)
         _check(lib.gfx_solver_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def query(self, key):
+        v = C.c_double(0)
+        _check(lib.gfx_solver_query(self.h, key.encode(), C.byref(v)))
+        return v.value
 
     def algorithmic_bytes(self, which=0):
         return float(lib.gfx_solver_algorithmic_bytes(self.h, which))

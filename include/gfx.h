@@ -153,6 +153,11 @@ gfx_status gfx_solver_kernel_time(const gfx_solver* s, const char* name, double*
 /* Enable/disable per-kernel event timing (default off). */
 gfx_status gfx_solver_set_timing(gfx_solver* s, int enable);
 
+/* Introspection of the solver's setup.  key: "colors", "pml_nodal" (1 when the
+ * C-PML memory is kept per node, 0 when element-local), "pml_elements",
+ * "special_nodes", "pml_nodes", "fields", "nodes", "elements". */
+gfx_status gfx_solver_query(const gfx_solver* s, const char* key, double* value);
+
 /* Algorithmic HBM bytes per step for the current problem (each array counted
  * once per step).  which: 0 = whole step, 1 = element kernel, 2 = node kernel. */
 double gfx_solver_algorithmic_bytes(const gfx_solver* s, int which);

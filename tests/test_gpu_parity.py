@@ -139,6 +139,30 @@ def test_solver_matches_oracle_layered_jittered(direction):
     assert err <= TOL
 
 
+def test_element_local_cpml_fallback_matches_oracle():
+    """Inputs whose C-PML memory coefficients differ between elements at a shared
+    node cannot use the nodal form; the solver falls back to element-local memory."""
+    mesh = mc.small_halfspace(duration=1.5)
+    arrays = mesh.arrays
+    al = arrays["coef_alpha"].reshape(-1, 125, 9)
+    ab = arrays["coef_abar"].reshape(-1, 125, 5)
+    e = int(np.flatnonzero(mesh.pml_region == 1)[0])
+    p = int(np.flatnonzero(ab[e, :, 2] != 0)[0])
+    al[e, p, 0] *= 0.999  # x-direction alpha of one element point
+    cells, w, stf = mc.surface_centre_source(mesh)
+    ref = _oracle(mesh, cells, w, stf, 2)
+    s = pygfx.Solver(mesh, n_fields=1, record_uva=True)
+    assert s.query("pml_nodal") == 0.0
+    s.set_source(0, 2, cells, w, stf)
+    s.run(mesh.nsteps)
+    st = _records(s, 0, ref["strain"].shape[0])
+    err = mc.rel_l2(st, ref["strain"])
+    print(f"element-local C-PML strain rel L2 = {err:.3e}")
+    assert err <= TOL
+    s2 = pygfx.Solver(mc.small_halfspace(duration=1.5), n_fields=1)
+    assert s2.query("pml_nodal") == 1.0 and s2.query("colors") == 8
+
+
 def test_solver_legacy_damping_path():
     mesh = mc.small_halfspace(duration=1.5)
     cells, w, stf = mc.surface_centre_source(mesh)
