@@ -1,9 +1,10 @@
 // kernels.hpp -- CDNA4 (gfx950) device kernels of the elastic CG-SEM step.
 //
-// Nodal state per field (FP32): U = predicted displacement u~_n
-// (= u_{n+1}, beta = 0), W = v_n + dt/2 a_n, R = assembled residual, and a_n
-// only at "special" nodes (C-PML, legacy-damped, recording, massless nodes,
-// numbered first: ids [0, Ga)).  The reference's Newmark pair
+// Nodal state (FP32), field-interleaved [node][field][xyz] so one node's
+// values for all right-hand sides share cache lines: U = predicted
+// displacement u~_n (= u_{n+1}, beta = 0), W = v_n + dt/2 a_n, R = assembled
+// residual, and a_n only at "special" nodes (C-PML, legacy-damped, recording,
+// massless nodes, numbered first: ids [0, Ga)).  The reference's Newmark pair
 // (forward/share/src/solver.cpp:40-69, beta = 0, gamma = 1/2) is carried
 // algebraically unchanged:
 //   a_{n+1} = r / m,  w_{n+1} = w_n + dt a_{n+1},  u~_{n+1} = u~_n + dt w_{n+1},
@@ -37,20 +38,18 @@ struct ElemArgs {
     const float* __restrict__ mu;       // [P]   (0 where mu <= 0)
     const uint32_t* __restrict__ ibool; // [P]  node id | FIRST_TOUCH
     int64_t P;
-    int nf;
-    int64_t fstride;
-    const float* __restrict__ U;        // [nf][fstride] predicted displacement
-    float* __restrict__ R;              // [nf][fstride] assembled residual
+    int nf;                             // fields; a node holds ns = 3 nf floats
+    const float* __restrict__ U;        // [G4][nf][3] predicted displacement
+    float* __restrict__ R;              // [G4][nf][3] assembled residual
     // element-local C-PML (only when the nodal form is not applicable)
     const float* __restrict__ W;
-    const float* __restrict__ Aacc;     // [nf][astride] a_n at special nodes
-    int64_t astride;
+    const float* __restrict__ Aacc;     // [Ga][nf][3] a_n at special nodes
     const int32_t* __restrict__ pml_slot;  // [E] slot or -1
     int64_t Ppml;
     const float* __restrict__ palpha;  // [9][Ppml]
     const float* __restrict__ pabar;   // [5][Ppml] w J / rho * A_k
-    float* __restrict__ pmem;          // [nf][9][Ppml]
-    float* __restrict__ pnew;          // [nf][3][Ppml]
+    float* __restrict__ pmem;          // [Ppml][nf][9]
+    float* __restrict__ pnew;          // [Ppml][nf][3]
     float c1, c2, dt;
     // point sources
     const int32_t* __restrict__ src_off;  // [E+1]
@@ -64,19 +63,18 @@ struct ElemArgs {
 struct NodeArgs {
     int64_t G, Ga, Gpml, G4;
     int nf;
-    int64_t fstride, astride;
     const float* __restrict__ minv;   // [G4] 1/m (0 where m <= 0 or padding)
     const float* __restrict__ damp;   // [Ga] legacy ramp or nullptr
     float* __restrict__ U;
     float* __restrict__ W;
     float* __restrict__ R;
-    float* __restrict__ Aacc;         // [nf][astride]
+    float* __restrict__ Aacc;         // [Ga][nf][3]
     // nodal C-PML
     int pml_nodal;
     const float* __restrict__ pc;      // [5][Gpml] sum over PML elements of w J / rho * A_k
     const float* __restrict__ palpha;  // [9][Gpml] coef_alpha (dir*3 + {0,1,2})
-    float* __restrict__ pmem;          // [nf][9][Gpml] rmemory (comp*3 + dir)
-    float* __restrict__ pnew;          // [nf][3][Gpml] pml_displ_new
+    float* __restrict__ pmem;          // [Gpml][nf][9] rmemory (comp*3 + dir)
+    float* __restrict__ pnew;          // [Gpml][nf][3] pml_displ_new
     float c1, c2, dt;
 };
 
@@ -85,7 +83,6 @@ struct RecArgs {
     const int32_t* __restrict__ ibool;  // [n_rec*125] special node ids of recording cells
     const float* __restrict__ dd;       // [9][n_rec*125] unscaled dxi_dx
     int nf;
-    int64_t fstride, astride;
     const float* __restrict__ U;
     const float* __restrict__ W;
     const float* __restrict__ Aacc;
@@ -144,9 +141,10 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
         se = A.src_off[e + 1];
     }
     // displacement of field 0 (later fields are prefetched one iteration ahead)
+    const int64_t nbase = node * (3 * A.nf);
     float un[3] = {0.f, 0.f, 0.f};
     if (valid) {
-        const float* ut = A.U + node * 3;
+        const float* ut = A.U + nbase;
         un[0] = ut[0];
         un[1] = ut[1];
         un[2] = ut[2];
@@ -168,21 +166,21 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
     // (2), when every lane has finished reading su; writing the next fluxes
     // happens after the next (1), when every lane has finished reading sf.
     for (int f = 0; f < A.nf; ++f) {
-        const int64_t fo = int64_t(f) * A.fstride;
+        const int64_t fo = nbase + 3 * f;  // this node, field f
         if (valid) {
             su[le][0][p] = un[0];
             su[le][1][p] = un[1];
             su[le][2][p] = un[2];
         }
         if (valid && f + 1 < A.nf) {
-            const float* ut = A.U + fo + A.fstride + node * 3;
+            const float* ut = A.U + fo + 3;
             un[0] = ut[0];
             un[1] = ut[1];
             un[2] = ut[2];
         }
         float rold[3] = {0.f, 0.f, 0.f};
         if (!ATOMIC && valid && !first) {
-            const float* r = A.R + fo + node * 3;
+            const float* r = A.R + fo;
             rold[0] = r[0];
             rold[1] = r[1];
             rold[2] = r[2];
@@ -245,8 +243,8 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
             if (PML_ELEM && pslot >= 0) {
                 // element-local C-PML (pml.cpp:55-216); PML nodes are special, so a_n is stored
                 const int64_t q = int64_t(pslot) * NP + p;
-                const int64_t gn = fo + node * 3;
-                const float* an_p = A.Aacc + int64_t(f) * A.astride + node * 3;
+                const int64_t gn = fo;
+                const float* an_p = A.Aacc + fo;
                 float upn[3], vpn[3], apn[3], m[9], pn[3];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
@@ -255,12 +253,12 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
                     upn[c] = A.U[gn + c] - A.dt * w;
                     vpn[c] = w - 0.5f * A.dt * apn[c];
                 }
-                float* pmem = A.pmem + int64_t(f) * 9 * A.Ppml;
-                float* pnew = A.pnew + int64_t(f) * 3 * A.Ppml;
+                float* pmem = A.pmem + (q * A.nf + f) * 9;
+                float* pnew = A.pnew + (q * A.nf + f) * 3;
 #pragma unroll
-                for (int c = 0; c < 9; ++c) m[c] = pmem[c * A.Ppml + q];
+                for (int c = 0; c < 9; ++c) m[c] = pmem[c];
 #pragma unroll
-                for (int c = 0; c < 3; ++c) pn[c] = pnew[c * A.Ppml + q];
+                for (int c = 0; c < 3; ++c) pn[c] = pnew[c];
                 float al[9], S[5];
 #pragma unroll
                 for (int c = 0; c < 9; ++c) al[c] = A.palpha[c * A.Ppml + q];
@@ -274,9 +272,9 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
                              S[3] * m[c * 3 + 1] + S[4] * m[c * 3 + 2];  // pml.cpp:209-212
 #pragma unroll
                     for (int d = 0; d < 3; ++d)  // pml.cpp:133-146
-                        pmem[(c * 3 + d) * A.Ppml + q] =
+                        pmem[c * 3 + d] =
                             al[d * 3 + 0] * m[c * 3 + d] + al[d * 3 + 1] * newv + al[d * 3 + 2] * oldv;
-                    pnew[c * A.Ppml + q] = newv;
+                    pnew[c] = newv;
                 }
             }
             for (int s = sb; s < se; ++s) {  // solver.cpp:576-605
@@ -286,7 +284,7 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
                     rv[en.y] += sv * A.src_w[en.z + p];
                 }
             }
-            float* r = A.R + fo + node * 3;
+            float* r = A.R + fo;
             if (ATOMIC) {
                 unsafeAtomicAdd(r + 0, rv[0]);
                 unsafeAtomicAdd(r + 1, rv[1]);
@@ -295,6 +293,170 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
                 r[0] = rold[0] + rv[0];
                 r[1] = rold[1] + rv[1];
                 r[2] = rold[2] + rv[2];
+            }
+        }
+    }
+}
+
+// Wave-level LDS visibility: the lanes of one wavefront exchange data through
+// LDS with no workgroup barrier (same sequence as HIP's __syncwarp).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One wavefront per element, lane l owns points l and l + 64 (l < 61).  All
+// exchange between the element's points stays inside one wavefront, so the
+// field loop has no workgroup barrier; WPB elements (wavefronts) per block.
+template <int WPB, bool ATOMIC>
+__global__ void __launch_bounds__(64 * WPB) k_element_wave(ElemArgs A, int e0, int ne) {
+    __shared__ __align__(16) float sDr[5][8];  // rows of D, padded
+    __shared__ __align__(16) float sDc[5][8];  // columns of D, padded
+    __shared__ float su[WPB][3][128];
+    __shared__ float sf[WPB][9][128];
+    const int tid = threadIdx.x;
+    if (tid < 25) {
+        sDr[tid / 5][tid % 5] = A.D[tid];
+        sDc[tid % 5][tid / 5] = A.D[tid];
+    }
+    __syncthreads();  // the only workgroup barrier
+    const int lane = tid & 63, wv = tid >> 6;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int eloc = blk * WPB + wv;
+    if (eloc >= ne) return;  // wave-uniform
+    const int e = e0 + eloc;
+    const int pq[2] = {lane, lane + 64};
+    const bool vq[2] = {true, lane + 64 < NP};
+    int iq[2], jq[2], kq[2];
+    float g[2][9], lam[2], mu[2];
+    int64_t node[2];
+    bool first[2];
+    int sb = A.src_off[e], se = A.src_off[e + 1];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int p = vq[q] ? pq[q] : 0;
+        iq[q] = p / 25;
+        jq[q] = (p / 5) % 5;
+        kq[q] = p % 5;
+        const int64_t pt = int64_t(e) * NP + p;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) g[q][c] = vq[q] ? __builtin_nontemporal_load(&A.G[c * A.P + pt]) : 0.f;
+        lam[q] = vq[q] ? __builtin_nontemporal_load(&A.lam[pt]) : 0.f;
+        mu[q] = vq[q] ? __builtin_nontemporal_load(&A.mu[pt]) : 0.f;
+        const uint32_t nf_ = vq[q] ? __builtin_nontemporal_load(&A.ibool[pt]) : 0u;
+        node[q] = int64_t(nf_ & ~FIRST_TOUCH);
+        first[q] = (nf_ & FIRST_TOUCH) != 0;
+    }
+    const int ns = 3 * A.nf;
+    float un[2][3];
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) un[q][c] = vq[q] ? A.U[node[q] * ns + c] : 0.f;
+
+    for (int f = 0; f < A.nf; ++f) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+            if (vq[q])
+#pragma unroll
+                for (int c = 0; c < 3; ++c) su[wv][c][pq[q]] = un[q][c];
+        float rold[2][3];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const bool nx = vq[q] && f + 1 < A.nf;
+            const int64_t o = node[q] * ns + 3 * f;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                un[q][c] = nx ? A.U[o + 3 + c] : 0.f;
+                rold[q][c] = (!ATOMIC && vq[q] && !first[q]) ? A.R[o + c] : 0.f;
+            }
+        }
+        wave_sync();
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            if (!vq[q]) continue;
+            const int i = iq[q], j = jq[q], k = kq[q], p = pq[q];
+            const float4 di = *reinterpret_cast<const float4*>(&sDr[i][0]);
+            const float4 dj = *reinterpret_cast<const float4*>(&sDr[j][0]);
+            const float4 dk = *reinterpret_cast<const float4*>(&sDr[k][0]);
+            const float Di[5] = {di.x, di.y, di.z, di.w, sDr[i][4]};
+            const float Dj[5] = {dj.x, dj.y, dj.z, dj.w, sDr[j][4]};
+            const float Dk[5] = {dk.x, dk.y, dk.z, dk.w, sDr[k][4]};
+            float du[3][3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+#pragma unroll
+                for (int s = 0; s < 5; ++s) {
+                    a0 += Di[s] * su[wv][c][s * 25 + j * 5 + k];
+                    a1 += Dj[s] * su[wv][c][i * 25 + s * 5 + k];
+                    a2 += Dk[s] * su[wv][c][i * 25 + j * 5 + s];
+                }
+                du[c][0] = a0;
+                du[c][1] = a1;
+                du[c][2] = a2;
+            }
+            const float* gg = g[q];
+            float gr[3][3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int m = 0; m < 3; ++m)
+                    gr[c][m] = du[c][0] * gg[m * 3 + 0] + du[c][1] * gg[m * 3 + 1] + du[c][2] * gg[m * 3 + 2];
+            const float tr = gr[0][0] + gr[1][1] + gr[2][2];
+            const float mu2 = 2.f * mu[q], la = lam[q], m1 = mu[q];
+            float sig[3][3];
+            sig[0][0] = mu2 * gr[0][0] + la * tr;
+            sig[1][1] = mu2 * gr[1][1] + la * tr;
+            sig[2][2] = mu2 * gr[2][2] + la * tr;
+            sig[0][1] = sig[1][0] = m1 * (gr[0][1] + gr[1][0]);
+            sig[0][2] = sig[2][0] = m1 * (gr[0][2] + gr[2][0]);
+            sig[1][2] = sig[2][1] = m1 * (gr[1][2] + gr[2][1]);
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int aa = 0; aa < 3; ++aa)
+                    sf[wv][c * 3 + aa][p] = sig[c][0] * gg[0 * 3 + aa] + sig[c][1] * gg[1 * 3 + aa] +
+                                            sig[c][2] * gg[2 * 3 + aa];
+        }
+        wave_sync();
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            if (!vq[q]) continue;
+            const int i = iq[q], j = jq[q], k = kq[q], p = pq[q];
+            const float4 ci = *reinterpret_cast<const float4*>(&sDc[i][0]);
+            const float4 cj = *reinterpret_cast<const float4*>(&sDc[j][0]);
+            const float4 ck = *reinterpret_cast<const float4*>(&sDc[k][0]);
+            const float Dti[5] = {ci.x, ci.y, ci.z, ci.w, sDc[i][4]};
+            const float Dtj[5] = {cj.x, cj.y, cj.z, cj.w, sDc[j][4]};
+            const float Dtk[5] = {ck.x, ck.y, ck.z, ck.w, sDc[k][4]};
+            float rv[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float acc = 0.f;
+#pragma unroll
+                for (int s = 0; s < 5; ++s) {
+                    acc += Dti[s] * sf[wv][c * 3 + 0][s * 25 + j * 5 + k];
+                    acc += Dtj[s] * sf[wv][c * 3 + 1][i * 25 + s * 5 + k];
+                    acc += Dtk[s] * sf[wv][c * 3 + 2][i * 25 + j * 5 + s];
+                }
+                rv[c] = -acc;
+            }
+            for (int s = sb; s < se; ++s) {  // solver.cpp:576-605
+                const int4 en = A.src_ent[s];
+                if (en.x == f && A.step < A.nstf)
+                    rv[en.y] += A.stf[int64_t(f) * A.nstf + A.step] * A.src_w[en.z + p];
+            }
+            float* r = A.R + node[q] * ns + 3 * f;
+            if (ATOMIC) {
+                unsafeAtomicAdd(r + 0, rv[0]);
+                unsafeAtomicAdd(r + 1, rv[1]);
+                unsafeAtomicAdd(r + 2, rv[2]);
+            } else {
+                r[0] = rold[q][0] + rv[0];
+                r[1] = rold[q][1] + rv[1];
+                r[2] = rold[q][2] + rv[2];
             }
         }
     }
@@ -315,36 +477,36 @@ __global__ void __launch_bounds__(256) k_node_special(NodeArgs A) {
 #pragma unroll
         for (int q = 0; q < 9; ++q) al[q] = A.palpha[q * A.Gpml + g];
     }
+    const int64_t base = g * (3 * A.nf);
     for (int f = 0; f < A.nf; ++f) {
-        const int64_t o = int64_t(f) * A.fstride + g * 3;
-        const int64_t oa = int64_t(f) * A.astride + g * 3;
+        const int64_t o = base + 3 * f;
         float U[3], W[3], R[3], an[3], un[3], vn[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             U[c] = A.U[o + c];
             W[c] = A.W[o + c];
             R[c] = A.R[o + c];
-            an[c] = A.Aacc[oa + c];
+            an[c] = A.Aacc[o + c];
             un[c] = U[c] - dt * W[c];
             vn[c] = W[c] - hdt * an[c];
         }
         if (pml) {
-            float* pmem = A.pmem + int64_t(f) * 9 * A.Gpml;
-            float* pnew = A.pnew + int64_t(f) * 3 * A.Gpml;
+            float* pmem = A.pmem + (g * A.nf + f) * 9;
+            float* pnew = A.pnew + (g * A.nf + f) * 3;
             float m[9];
 #pragma unroll
-            for (int q = 0; q < 9; ++q) m[q] = pmem[q * A.Gpml + g];
+            for (int q = 0; q < 9; ++q) m[q] = pmem[q];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const float oldv = pnew[c * A.Gpml + g] + A.c2 * an[c];  // pml.cpp:91-109
-                const float newv = un[c] + A.c1 * vn[c];                 // pml.cpp:72-89
+                const float oldv = pnew[c] + A.c2 * an[c];  // pml.cpp:91-109
+                const float newv = un[c] + A.c1 * vn[c];    // pml.cpp:72-89
                 R[c] += C[0] * vn[c] + C[1] * un[c] + C[2] * m[c * 3 + 0] + C[3] * m[c * 3 + 1] +
                         C[4] * m[c * 3 + 2];  // pml.cpp:209-212, assembled over PML elements
 #pragma unroll
                 for (int d = 0; d < 3; ++d)  // pml.cpp:133-146
-                    pmem[(c * 3 + d) * A.Gpml + g] =
+                    pmem[c * 3 + d] =
                         al[d * 3 + 0] * m[c * 3 + d] + al[d * 3 + 1] * newv + al[d * 3 + 2] * oldv;
-                pnew[c * A.Gpml + g] = newv;
+                pnew[c] = newv;
             }
         }
 #pragma unroll
@@ -364,45 +526,38 @@ __global__ void __launch_bounds__(256) k_node_special(NodeArgs A) {
             const float w1 = v1 + hdt * a1;
             A.U[o + c] = u1 + dt * w1;
             A.W[o + c] = w1;
-            A.Aacc[oa + c] = a1;
+            A.Aacc[o + c] = a1;
         }
     }
 }
 
-// Regular nodes [Ga, G4): a = r/m, w += dt a, u~ += dt w.  4 nodes per lane,
-// 16-byte loads and stores (Ga and the field stride are multiples of 4 nodes).
+// Regular nodes [Ga, G4): a = r/m, w += dt a, u~ += dt w over the contiguous
+// float range [Ga*3nf, G4*3nf), 4 floats per lane with 16-byte accesses
+// (Ga and G4 are multiples of 4 nodes).
 __global__ void __launch_bounds__(256) k_node_regular(NodeArgs A) {
-    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int64_t g = A.Ga + 4 * t;
-    if (g >= A.G4) return;
+    const int64_t ns = 3 * int64_t(A.nf);
+    const int64_t q0 = A.Ga * ns + 4 * (int64_t(blockIdx.x) * blockDim.x + threadIdx.x);
+    if (q0 >= A.G4 * ns) return;
     const float dt = A.dt;
-    const float4 mi4 = *reinterpret_cast<const float4*>(A.minv + g);
-    const float mi[4] = {mi4.x, mi4.y, mi4.z, mi4.w};
-    for (int f = 0; f < A.nf; ++f) {
-        const int64_t o = int64_t(f) * A.fstride + g * 3;
-        float4* U4 = reinterpret_cast<float4*>(A.U + o);
-        float4* W4 = reinterpret_cast<float4*>(A.W + o);
-        const float4* R4 = reinterpret_cast<const float4*>(A.R + o);
-        float u[12], w[12], r[12];
+    const int64_t g0 = q0 / ns;
+    const int64_t r0 = q0 - g0 * ns;
+    const float4 r4 = *reinterpret_cast<const float4*>(A.R + q0);
+    float4 u4 = *reinterpret_cast<const float4*>(A.U + q0);
+    float4 w4 = *reinterpret_cast<const float4*>(A.W + q0);
+    const float r[4] = {r4.x, r4.y, r4.z, r4.w};
+    float u[4] = {u4.x, u4.y, u4.z, u4.w};
+    float w[4] = {w4.x, w4.y, w4.z, w4.w};
+    const float m0 = A.minv[g0];
+    const float m1 = (r0 + 3 >= ns) ? A.minv[g0 + 1] : m0;  // 4 floats span at most 2 nodes
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const float4 a = U4[q], b = W4[q], c = R4[q];
-            u[4 * q + 0] = a.x; u[4 * q + 1] = a.y; u[4 * q + 2] = a.z; u[4 * q + 3] = a.w;
-            w[4 * q + 0] = b.x; w[4 * q + 1] = b.y; w[4 * q + 2] = b.z; w[4 * q + 3] = b.w;
-            r[4 * q + 0] = c.x; r[4 * q + 1] = c.y; r[4 * q + 2] = c.z; r[4 * q + 3] = c.w;
-        }
-#pragma unroll
-        for (int q = 0; q < 12; ++q) {
-            const float a1 = r[q] * mi[q / 3];
-            w[q] = w[q] + dt * a1;
-            u[q] = u[q] + dt * w[q];
-        }
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            U4[q] = make_float4(u[4 * q + 0], u[4 * q + 1], u[4 * q + 2], u[4 * q + 3]);
-            W4[q] = make_float4(w[4 * q + 0], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
-        }
+    for (int q = 0; q < 4; ++q) {
+        const float mi = (r0 + q >= ns) ? m1 : m0;
+        const float a1 = r[q] * mi;
+        w[q] = w[q] + dt * a1;
+        u[q] = u[q] + dt * w[q];
     }
+    *reinterpret_cast<float4*>(A.U + q0) = make_float4(u[0], u[1], u[2], u[3]);
+    *reinterpret_cast<float4*>(A.W + q0) = make_float4(w[0], w[1], w[2], w[3]);
 }
 
 // Fill kernel (used before atomic assembly; no memset nodes in a step).
@@ -433,8 +588,8 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64) k_record(RecArgs 
     for (int c = 0; c < 9; ++c) dd[c] = valid ? A.dd[c * nrp + pt] : 0.f;
     __syncthreads();
     for (int f = 0; f < A.nf; ++f) {
-        const int64_t go = int64_t(f) * A.fstride + node * 3;
-        const int64_t ga = int64_t(f) * A.astride + node * 3;
+        const int64_t go = (node * A.nf + f) * 3;
+        const int64_t ga = go;
         float w[3] = {0.f, 0.f, 0.f};
         if (valid) {
 #pragma unroll

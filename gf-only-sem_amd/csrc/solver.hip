@@ -73,8 +73,7 @@ struct FieldSource {
 struct gfx_solver {
     int E = 0, G = 0, nf = 1, n_rec = 0, n_pml_elem = 0;
     int64_t Ga = 0, Gpml = 0, G4 = 0, Ppml = 0;
-    int64_t fstride = 0, astride = 0;
-    int assembly = GFX_ASSEMBLY_COLORED, epb = 2;
+    int assembly = GFX_ASSEMBLY_COLORED, epb = 2, kernel = GFX_ELEMENT_WAVE;
     double dt = 0;
     int stride = 1;
     bool has_cpml = false, pml_nodal = true;
@@ -186,11 +185,29 @@ void launch_element_p(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
     }
 }
 
-void launch_element(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
-    if (S.has_cpml && !S.pml_nodal)
-        launch_element_p<true>(S, a, b, n);
+template <int WPB>
+void launch_wave_t(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
+    const int grid = (n + WPB - 1) / WPB;
+    if (grid == 0) return;
+    if (S.assembly == GFX_ASSEMBLY_ATOMIC)
+        dev::k_element_wave<WPB, true><<<grid, 64 * WPB, 0, S.stream>>>(a, b, n);
     else
+        dev::k_element_wave<WPB, false><<<grid, 64 * WPB, 0, S.stream>>>(a, b, n);
+}
+
+void launch_element(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
+    if (S.kernel == GFX_ELEMENT_WAVE) {
+        switch (S.epb) {
+            case 1: launch_wave_t<1>(S, a, b, n); break;
+            case 2: launch_wave_t<2>(S, a, b, n); break;
+            case 4: launch_wave_t<4>(S, a, b, n); break;
+            default: launch_wave_t<8>(S, a, b, n); break;
+        }
+    } else if (S.has_cpml && !S.pml_nodal) {
+        launch_element_p<true>(S, a, b, n);
+    } else {
         launch_element_p<false>(S, a, b, n);
+    }
 }
 
 void run_steps(gfx_solver& S, int nsteps) {
@@ -205,12 +222,10 @@ void run_steps(gfx_solver& S, int nsteps) {
     ea.ibool = S.ibool.p;
     ea.P = int64_t(S.E) * NP;
     ea.nf = S.nf;
-    ea.fstride = S.fstride;
     ea.U = S.U.p;
     ea.R = S.R.p;
     ea.W = S.W.p;
     ea.Aacc = S.Aacc.p;
-    ea.astride = S.astride;
     ea.pml_slot = S.pml_slot.p;
     ea.Ppml = S.Ppml;
     ea.palpha = S.palpha.p;
@@ -233,8 +248,6 @@ void run_steps(gfx_solver& S, int nsteps) {
     na.Gpml = S.Gpml;
     na.G4 = S.G4;
     na.nf = S.nf;
-    na.fstride = S.fstride;
-    na.astride = S.astride;
     na.minv = S.minv.p;
     na.damp = S.has_cpml ? nullptr : S.damp.p;
     na.U = S.U.p;
@@ -250,8 +263,9 @@ void run_steps(gfx_solver& S, int nsteps) {
     na.c2 = c2;
     na.dt = float(S.dt);
     const int spec_grid = int((S.Ga + 255) / 256);
-    const int reg_grid = int(((S.G4 - S.Ga) / 4 + 255) / 256);
-    const int64_t r4 = int64_t(S.nf) * S.fstride / 4;
+    const int64_t ns = 3 * int64_t(S.nf);
+    const int reg_grid = int(((S.G4 - S.Ga) * ns / 4 + 255) / 256);
+    const int64_t r4 = S.G4 * ns / 4;
     const int zero_grid = int((r4 + 255) / 256);
 
     dev::RecArgs ra{};
@@ -259,8 +273,6 @@ void run_steps(gfx_solver& S, int nsteps) {
     ra.ibool = S.rec_ibool.p;
     ra.dd = S.rec_dd.p;
     ra.nf = S.nf;
-    ra.fstride = S.fstride;
-    ra.astride = S.astride;
     ra.U = S.U.p;
     ra.W = S.W.p;
     ra.Aacc = S.Aacc.p;
@@ -352,8 +364,11 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
     require(d.dt > 0, "dt must be > 0");
     require(d.assembly == GFX_ASSEMBLY_COLORED || d.assembly == GFX_ASSEMBLY_ATOMIC,
             "unknown assembly mode");
-    const int epb = d.elems_per_block ? d.elems_per_block : 2;
-    require(epb == 1 || epb == 2 || epb == 4 || epb == 8, "elems_per_block must be 1, 2, 4 or 8");
+    require(d.element_kernel >= GFX_ELEMENT_AUTO && d.element_kernel <= GFX_ELEMENT_WAVE,
+            "unknown element kernel");
+    int epb = d.elems_per_block;
+    require(epb == 0 || epb == 1 || epb == 2 || epb == 4 || epb == 8,
+            "elems_per_block must be 1, 2, 4 or 8");
     require(d.n_rank_node < int(0x7fffffff) - 8, "too many nodes");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
@@ -368,7 +383,6 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
     S.G = G;
     S.nf = d.n_fields;
     S.assembly = d.assembly;
-    S.epb = epb;
     S.dt = d.dt;
     S.stride = d.snapshot_stride;
     S.has_cpml = d.has_cpml != 0;
@@ -577,14 +591,19 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
     }
     S.pml_slot.upload(hslot);
 
-    // ---- nodal state: [nf][fstride] (U, W, R) and [nf][astride] (a at special nodes) ----
-    S.fstride = ((S.G4 * 3 + 63) / 64) * 64;
-    S.astride = ((std::max<int64_t>(S.Ga, 1) * 3 + 63) / 64) * 64;
-    const size_t fs = size_t(S.fstride) * S.nf;
+    // element kernel: the wavefront-per-element form needs no element-local C-PML state
+    const bool wave_ok = !(S.has_cpml && !S.pml_nodal);
+    if (d.element_kernel == GFX_ELEMENT_WAVE)
+        require(wave_ok, "the wave element kernel needs the nodal C-PML form");
+    S.kernel = (d.element_kernel == GFX_ELEMENT_BLOCK || !wave_ok) ? GFX_ELEMENT_BLOCK : GFX_ELEMENT_WAVE;
+    S.epb = epb ? epb : (S.kernel == GFX_ELEMENT_WAVE ? 4 : 2);
+
+    // ---- nodal state, field-interleaved: U, W, R [G4][nf][3]; a at special nodes [Ga][nf][3] ----
+    const size_t fs = size_t(S.G4) * 3 * S.nf;
     S.U.alloc(fs);
     S.W.alloc(fs);
     S.R.alloc(fs);
-    S.Aacc.alloc(size_t(S.astride) * S.nf);
+    S.Aacc.alloc(size_t(std::max<int64_t>(S.Ga, 1)) * 3 * S.nf);
 
     // ---- recording cells ----
     S.n_rec = int(S.rec_cells.size());
@@ -706,25 +725,26 @@ extern "C" gfx_status gfx_solver_get_state(const gfx_solver* s, int field, doubl
     return guarded([&] {
         check_field(s, field);
         GFX_HIP(hipStreamSynchronize(s->stream));
-        const size_t n3 = size_t(s->G) * 3;
-        const std::vector<float> hU = fetch(s, s->U.p + int64_t(field) * s->fstride, n3);
-        const std::vector<float> hW = fetch(s, s->W.p + int64_t(field) * s->fstride, n3);
-        const std::vector<float> hR = fetch(s, s->R.p + int64_t(field) * s->fstride, n3);
-        const std::vector<float> hA = fetch(s, s->Aacc.p + int64_t(field) * s->astride, size_t(s->Ga) * 3);
+        const int64_t ns = 3 * int64_t(s->nf);
+        const std::vector<float> hU = fetch(s, s->U.p, size_t(s->G4) * ns);
+        const std::vector<float> hW = fetch(s, s->W.p, size_t(s->G4) * ns);
+        const std::vector<float> hR = fetch(s, s->R.p, size_t(s->G4) * ns);
+        const std::vector<float> hA = fetch(s, s->Aacc.p, size_t(s->Ga) * ns);
         const float dt = float(s->dt);
         for (int g = 0; g < s->G; ++g) {
             const int64_t n = s->node_new_of_old[g];
+            const int64_t o = n * ns + 3 * field;
             for (int c = 0; c < 3; ++c) {
-                const float w = hW[n * 3 + c];
+                const float w = hW[o + c];
                 // regular nodes keep r = m a_n in R (it is not cleared between steps)
                 float an;
                 if (n < s->Ga) {
-                    an = hA[n * 3 + c];
+                    an = hA[o + c];
                 } else {
                     const double m = s->mass_node[n];
-                    an = (m > 0.0) ? hR[n * 3 + c] * float(1.0 / m) : 0.f;
+                    an = (m > 0.0) ? hR[o + c] * float(1.0 / m) : 0.f;
                 }
-                if (u) u[int64_t(g) * 3 + c] = hU[n * 3 + c] - dt * w;
+                if (u) u[int64_t(g) * 3 + c] = hU[o + c] - dt * w;
                 if (v) v[int64_t(g) * 3 + c] = w - 0.5f * dt * an;
                 if (a) a[int64_t(g) * 3 + c] = an;
             }
@@ -737,28 +757,30 @@ extern "C" gfx_status gfx_solver_set_state(gfx_solver* s, int field, const doubl
     return guarded([&] {
         check_field(s, field);
         require(u && v && a, "u, v and a are required");
-        const size_t n3 = size_t(s->G) * 3;
-        std::vector<float> hU(n3), hW(n3), hR(n3), hA(size_t(s->Ga) * 3, 0.f);
+        GFX_HIP(hipStreamSynchronize(s->stream));
+        const int64_t ns = 3 * int64_t(s->nf);
+        std::vector<float> hU = fetch(s, s->U.p, size_t(s->G4) * ns);
+        std::vector<float> hW = fetch(s, s->W.p, size_t(s->G4) * ns);
+        std::vector<float> hR = fetch(s, s->R.p, size_t(s->G4) * ns);
+        std::vector<float> hA = fetch(s, s->Aacc.p, size_t(s->Ga) * ns);
         const double dt = s->dt;
         for (int g = 0; g < s->G; ++g) {
             const int64_t n = s->node_new_of_old[g];
             const double m = s->mass_node[n];
+            const int64_t o = n * ns + 3 * field;
             for (int c = 0; c < 3; ++c) {
-                const int64_t o = int64_t(g) * 3 + c;
-                const double ag = (m > 0.0) ? a[o] : 0.0;  // massless nodes carry a = 0
-                hU[n * 3 + c] = float(u[o] + dt * v[o] + 0.5 * dt * dt * ag);
-                hW[n * 3 + c] = float(v[o] + 0.5 * dt * ag);
-                hR[n * 3 + c] = float(ag * m);
-                if (n < s->Ga) hA[n * 3 + c] = float(ag);
+                const int64_t i = int64_t(g) * 3 + c;
+                const double ag = (m > 0.0) ? a[i] : 0.0;  // massless nodes carry a = 0
+                hU[o + c] = float(u[i] + dt * v[i] + 0.5 * dt * dt * ag);
+                hW[o + c] = float(v[i] + 0.5 * dt * ag);
+                hR[o + c] = float(ag * m);
+                if (n < s->Ga) hA[o + c] = float(ag);
             }
         }
-        GFX_HIP(hipStreamSynchronize(s->stream));
-        GFX_HIP(hipMemcpy(s->U.p + int64_t(field) * s->fstride, hU.data(), n3 * 4, hipMemcpyHostToDevice));
-        GFX_HIP(hipMemcpy(s->W.p + int64_t(field) * s->fstride, hW.data(), n3 * 4, hipMemcpyHostToDevice));
-        GFX_HIP(hipMemcpy(s->R.p + int64_t(field) * s->fstride, hR.data(), n3 * 4, hipMemcpyHostToDevice));
-        if (!hA.empty())
-            GFX_HIP(hipMemcpy(s->Aacc.p + int64_t(field) * s->astride, hA.data(), hA.size() * 4,
-                              hipMemcpyHostToDevice));
+        GFX_HIP(hipMemcpy(s->U.p, hU.data(), hU.size() * 4, hipMemcpyHostToDevice));
+        GFX_HIP(hipMemcpy(s->W.p, hW.data(), hW.size() * 4, hipMemcpyHostToDevice));
+        GFX_HIP(hipMemcpy(s->R.p, hR.data(), hR.size() * 4, hipMemcpyHostToDevice));
+        if (!hA.empty()) GFX_HIP(hipMemcpy(s->Aacc.p, hA.data(), hA.size() * 4, hipMemcpyHostToDevice));
     });
 }
 
@@ -827,6 +849,8 @@ extern "C" gfx_status gfx_solver_query(const gfx_solver* s, const char* key, dou
         require(s && key && value, "null argument");
         const std::string k(key);
         if (k == "colors") *value = double(s->ranges.size());
+        else if (k == "element_kernel") *value = s->kernel;
+        else if (k == "elems_per_block") *value = s->epb;
         else if (k == "pml_nodal") *value = (s->has_cpml && s->pml_nodal) ? 1.0 : 0.0;
         else if (k == "pml_elements") *value = s->n_pml_elem;
         else if (k == "special_nodes") *value = double(s->Ga);

@@ -16,6 +16,9 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "libgfx.so")
 
 GFX_ASSEMBLY_COLORED = 0
 GFX_ASSEMBLY_ATOMIC = 1
+GFX_ELEMENT_AUTO = 0
+GFX_ELEMENT_BLOCK = 1
+GFX_ELEMENT_WAVE = 2
 MAX_LAYERS = 8
 
 _dp = C.POINTER(C.c_double)
@@ -81,6 +84,7 @@ class SolverDesc(C.Structure):
         ("dt", C.c_double), ("snapshot_stride", C.c_int),
         ("n_rec_cell", C.c_int), ("rec_cell", _ip),
         ("n_fields", C.c_int), ("assembly", C.c_int), ("elems_per_block", C.c_int),
+        ("element_kernel", C.c_int),
         ("record_capacity", C.c_int), ("record_uva", C.c_int),
     ]
 
@@ -290,7 +294,7 @@ class Solver:
 
     def __init__(self, mesh: Mesh, n_fields=1, assembly=GFX_ASSEMBLY_COLORED, elems_per_block=0,
                  record_capacity=None, record_uva=False, has_cpml=True, dt=None,
-                 snapshot_stride=None, rec_cell=None):
+                 snapshot_stride=None, rec_cell=None, element_kernel=GFX_ELEMENT_AUTO):
         a = mesh.arrays
         self._keep = []
 
@@ -326,6 +330,7 @@ class Solver:
         d.n_fields = n_fields
         d.assembly = assembly
         d.elems_per_block = elems_per_block
+        d.element_kernel = element_kernel
         if record_capacity is None:
             record_capacity = 0 if rk.size == 0 else (mesh.nsteps // max(d.snapshot_stride, 1) + 1)
         d.record_capacity = record_capacity

@@ -75,6 +75,12 @@ typedef enum {
     GFX_ASSEMBLY_ATOMIC = 1   /* float atomics; order not deterministic */
 } gfx_assembly;
 
+typedef enum {
+    GFX_ELEMENT_AUTO = 0,  /* wave kernel when applicable, else block kernel */
+    GFX_ELEMENT_BLOCK = 1, /* one lane per GLL point, workgroup barriers between stages */
+    GFX_ELEMENT_WAVE = 2   /* one wavefront per element, two points per lane, no barriers */
+} gfx_element_kernel;
+
 typedef struct {
     int n_elem;              /* local cells */
     int ngll;                /* must be 5 (polynomial order 4) */
@@ -102,7 +108,8 @@ typedef struct {
     /* batching */
     int n_fields;            /* right-hand sides advanced together (>= 1) */
     int assembly;            /* gfx_assembly */
-    int elems_per_block;     /* 1, 2, 4 or 8; 0 = library default */
+    int elems_per_block;     /* elements per workgroup: 1, 2, 4 or 8; 0 = library default */
+    int element_kernel;      /* gfx_element_kernel */
     /* device record ring: snapshots kept (0 = no recording); u,v,a copies on/off */
     int record_capacity;
     int record_uva;

@@ -89,10 +89,11 @@ def _oracle(mesh, cells, w, stf, direction, nsteps=None, has_cpml=True):
     return run.run(uva=True)
 
 
-def _gpu(mesh, sources, nsteps=None, assembly=pygfx.GFX_ASSEMBLY_COLORED, epb=0, has_cpml=True):
+def _gpu(mesh, sources, nsteps=None, assembly=pygfx.GFX_ASSEMBLY_COLORED, epb=0, has_cpml=True,
+         kernel=pygfx.GFX_ELEMENT_AUTO):
     nsteps = mesh.nsteps if nsteps is None else nsteps
     s = pygfx.Solver(mesh, n_fields=len(sources), assembly=assembly, elems_per_block=epb,
-                     record_uva=True, has_cpml=has_cpml)
+                     record_uva=True, has_cpml=has_cpml, element_kernel=kernel)
     for f, (cells, w, stf, d) in enumerate(sources):
         s.set_source(f, d, cells, w, stf)
     s.run(nsteps)
@@ -105,17 +106,22 @@ def _records(s, field, n_snap):
     return st
 
 
-@pytest.mark.parametrize("assembly", [pygfx.GFX_ASSEMBLY_COLORED, pygfx.GFX_ASSEMBLY_ATOMIC])
-def test_solver_matches_oracle_halfspace(assembly):
+@pytest.mark.parametrize("assembly,kernel", [
+    (pygfx.GFX_ASSEMBLY_COLORED, pygfx.GFX_ELEMENT_WAVE),
+    (pygfx.GFX_ASSEMBLY_COLORED, pygfx.GFX_ELEMENT_BLOCK),
+    (pygfx.GFX_ASSEMBLY_ATOMIC, pygfx.GFX_ELEMENT_WAVE),
+    (pygfx.GFX_ASSEMBLY_ATOMIC, pygfx.GFX_ELEMENT_BLOCK)])
+def test_solver_matches_oracle_halfspace(assembly, kernel):
     mesh = mc.small_halfspace()
     cells, w, stf = mc.surface_centre_source(mesh)
     ref = _oracle(mesh, cells, w, stf, 2)
-    s = _gpu(mesh, [(cells, w, stf, 2)], assembly=assembly)
+    s = _gpu(mesh, [(cells, w, stf, 2)], assembly=assembly, kernel=kernel)
+    assert s.query("element_kernel") == kernel
     n_snap = ref["strain"].shape[0]
     assert s.n_snapshots == n_snap
     st = _records(s, 0, n_snap)
     err = mc.rel_l2(st, ref["strain"])
-    print(f"halfspace assembly={assembly} strain rel L2 = {err:.3e}")
+    print(f"halfspace assembly={assembly} kernel={kernel} strain rel L2 = {err:.3e}")
     assert err <= TOL
     u, v, a = s.get_state(0)
     assert mc.rel_l2(u, ref["u"]) <= TOL
@@ -127,12 +133,12 @@ def test_solver_matches_oracle_halfspace(assembly):
     assert mc.rel_l2(acc, ref["acc"][-1]) <= TOL_ACC
 
 
-@pytest.mark.parametrize("direction", [0, 1])
-def test_solver_matches_oracle_layered_jittered(direction):
+@pytest.mark.parametrize("direction,kernel", [(0, pygfx.GFX_ELEMENT_WAVE), (1, pygfx.GFX_ELEMENT_BLOCK)])
+def test_solver_matches_oracle_layered_jittered(direction, kernel):
     mesh = mc.small_layered()
     cells, w, stf = mc.buried_source(mesh)
     ref = _oracle(mesh, cells, w, stf, direction)
-    s = _gpu(mesh, [(cells, w, stf, direction)])
+    s = _gpu(mesh, [(cells, w, stf, direction)], kernel=kernel)
     st = _records(s, 0, ref["strain"].shape[0])
     err = mc.rel_l2(st, ref["strain"])
     print(f"layered dir={direction} strain rel L2 = {err:.3e}")
@@ -146,9 +152,13 @@ def test_element_local_cpml_fallback_matches_oracle():
     arrays = mesh.arrays
     al = arrays["coef_alpha"].reshape(-1, 125, 9)
     ab = arrays["coef_abar"].reshape(-1, 125, 5)
-    e = int(np.flatnonzero(mesh.pml_region == 1)[0])
-    p = int(np.flatnonzero(ab[e, :, 2] != 0)[0])
-    al[e, p, 0] *= 0.999  # x-direction alpha of one element point
+    # a node shared by two elements that both use its x-direction memory (A3 != 0)
+    use = (mesh.pml_region[:, None] != 0) & (ab[:, :, 2] != 0)
+    nodes = mesh.ibool.reshape(-1, 125)
+    ids, counts = np.unique(nodes[use], return_counts=True)
+    shared = ids[counts >= 2][0]
+    e, p = (int(x[0]) for x in np.nonzero(use & (nodes == shared)))
+    al[e, p, 0] *= 0.999  # its x-direction alpha in one of the elements
     cells, w, stf = mc.surface_centre_source(mesh)
     ref = _oracle(mesh, cells, w, stf, 2)
     s = pygfx.Solver(mesh, n_fields=1, record_uva=True)
@@ -174,12 +184,13 @@ def test_solver_legacy_damping_path():
     assert mc.rel_l2(u, ref["u"]) <= TOL
 
 
-def test_coloured_assembly_is_deterministic_and_epb_invariant():
+@pytest.mark.parametrize("kernel", [pygfx.GFX_ELEMENT_WAVE, pygfx.GFX_ELEMENT_BLOCK])
+def test_coloured_assembly_is_deterministic_and_epb_invariant(kernel):
     mesh = mc.small_layered(duration=0.6)
     cells, w, stf = mc.buried_source(mesh)
     runs = []
     for epb in (1, 2, 4, 8, 2):
-        s = _gpu(mesh, [(cells, w, stf, 2)], epb=epb)
+        s = _gpu(mesh, [(cells, w, stf, 2)], epb=epb, kernel=kernel)
         runs.append(s.get_state(0))
     for r in runs[1:]:
         for x, y in zip(r, runs[0]):
