@@ -1,22 +1,25 @@
 // kernels.hpp -- CDNA4 (gfx950) device kernels of the elastic CG-SEM step.
 //
-// Nodal state (FP32), field-interleaved [node][field][xyz] so one node's
-// values for all right-hand sides share cache lines: U = predicted
-// displacement u~_n (= u_{n+1}, beta = 0), W = v_n + dt/2 a_n, R = assembled
-// residual, and a_n only at "special" nodes (C-PML, legacy-damped, recording,
-// massless nodes, numbered first: ids [0, Ga)).  The reference's Newmark pair
-// (forward/share/src/solver.cpp:40-69, beta = 0, gamma = 1/2) is carried
-// algebraically unchanged:
+// Nodal state (FP32), one array per field ([field][node][xyz], field stride
+// fstride floats): U = predicted displacement u~_n (= u_{n+1}, beta = 0),
+// W = v_n + dt/2 a_n, R = assembled residual, and a_n only at "special" nodes
+// (C-PML, legacy-damped, recording and massless nodes, numbered first:
+// ids [0, Ga)).  The reference's Newmark pair (forward/share/src/solver.cpp:40-69,
+// beta = 0, gamma = 1/2) is carried algebraically unchanged:
 //   a_{n+1} = r / m,  w_{n+1} = w_n + dt a_{n+1},  u~_{n+1} = u~_n + dt w_{n+1},
 //   u_n = u~_n - dt w_n,  v_n = w_n - dt/2 a_n.
 //
 // Per step:
-//   element kernel per colour: gather u~ into LDS, GLL contractions, scaled
-//     gradient -> strain -> isotropic stress -> fluxes (LDS) -> divergence,
-//     point sources, then store/add into R (first touch of a node stores);
-//   special-node kernel: C-PML (forward/share/src/pml.cpp:55-216, assembled
-//     per node), legacy damping, massless rule, corrector/predictor;
-//   regular-node kernel: streaming corrector/predictor, 4 nodes per lane;
+//   element kernel: gather u~ into LDS, GLL contractions, scaled gradient ->
+//     strain -> isotropic stress -> fluxes (LDS) -> divergence, point sources;
+//     then, by assembly mode,
+//       COLORED: one launch per element colour, store (first touch) or add into R;
+//       ATOMIC:  float atomics into R (zeroed first);
+//       GATHER:  one launch, element-local residual Rl[field][point][xyz];
+//   special-node kernel: [GATHER: R = fixed-order sum of Rl] C-PML
+//     (forward/share/src/pml.cpp:55-216, assembled per node), legacy damping,
+//     massless rule, corrector/predictor;
+//   regular-node kernel: [GATHER sum] streaming corrector/predictor;
 //   record kernel (snapshot steps): strain of the recording cells
 //     (forward/share/src/solver.cpp:282-335).
 #pragma once
@@ -31,6 +34,8 @@ constexpr int NG = 5;    // GLL points per axis (polynomial order 4)
 constexpr int NP = 125;  // points per element
 constexpr uint32_t FIRST_TOUCH = 0x80000000u;
 
+enum { ASM_COLORED = 0, ASM_ATOMIC = 1, ASM_GATHER = 2 };
+
 struct ElemArgs {
     // geometry, SoA over element points in the solver's element order
     const float* __restrict__ G;        // [9][P]: dxi_dx[m*3+a] * sqrt(J w_i w_j w_k)
@@ -38,18 +43,22 @@ struct ElemArgs {
     const float* __restrict__ mu;       // [P]   (0 where mu <= 0)
     const uint32_t* __restrict__ ibool; // [P]  node id | FIRST_TOUCH
     int64_t P;
-    int nf;                             // fields; a node holds ns = 3 nf floats
-    const float* __restrict__ U;        // [G4][nf][3] predicted displacement
-    float* __restrict__ R;              // [G4][nf][3] assembled residual
+    int nf;
+    int64_t fstride;                    // floats per field of the nodal arrays
+    const float* __restrict__ U;        // [nf][fstride] predicted displacement
+    float* __restrict__ R;              // [nf][fstride] assembled residual (COLORED, ATOMIC)
+    float* __restrict__ Rl;             // [nf][P][3] contributions in node-sorted slots (GATHER)
+    const int32_t* __restrict__ slot;   // [P] slot of each element point (GATHER)
     // element-local C-PML (only when the nodal form is not applicable)
     const float* __restrict__ W;
-    const float* __restrict__ Aacc;     // [Ga][nf][3] a_n at special nodes
+    const float* __restrict__ Aacc;     // [nf][astride] a_n at special nodes
+    int64_t astride;
     const int32_t* __restrict__ pml_slot;  // [E] slot or -1
     int64_t Ppml;
     const float* __restrict__ palpha;  // [9][Ppml]
     const float* __restrict__ pabar;   // [5][Ppml] w J / rho * A_k
-    float* __restrict__ pmem;          // [Ppml][nf][9]
-    float* __restrict__ pnew;          // [Ppml][nf][3]
+    float* __restrict__ pmem;          // [nf][9][Ppml]
+    float* __restrict__ pnew;          // [nf][3][Ppml]
     float c1, c2, dt;
     // point sources
     const int32_t* __restrict__ src_off;  // [E+1]
@@ -61,20 +70,26 @@ struct ElemArgs {
 };
 
 struct NodeArgs {
-    int64_t G, Ga, Gpml, G4;
+    int64_t G, Ga, Gpml, G4, P;
     int nf;
+    int64_t fstride, astride;
     const float* __restrict__ minv;   // [G4] 1/m (0 where m <= 0 or padding)
     const float* __restrict__ damp;   // [Ga] legacy ramp or nullptr
     float* __restrict__ U;
     float* __restrict__ W;
     float* __restrict__ R;
-    float* __restrict__ Aacc;         // [Ga][nf][3]
+    float* __restrict__ Aacc;         // [nf][astride]
+    // GATHER assembly: contributions of node g sit in slots [off[g], off[g+1])
+    // of Rl[f], in increasing element order
+    int gather;
+    const float* __restrict__ Rl;
+    const int32_t* __restrict__ off;  // [G4+1]
     // nodal C-PML
     int pml_nodal;
     const float* __restrict__ pc;      // [5][Gpml] sum over PML elements of w J / rho * A_k
     const float* __restrict__ palpha;  // [9][Gpml] coef_alpha (dir*3 + {0,1,2})
-    float* __restrict__ pmem;          // [Gpml][nf][9] rmemory (comp*3 + dir)
-    float* __restrict__ pnew;          // [Gpml][nf][3] pml_displ_new
+    float* __restrict__ pmem;          // [nf][9][Gpml] rmemory (comp*3 + dir)
+    float* __restrict__ pnew;          // [nf][3][Gpml] pml_displ_new
     float c1, c2, dt;
 };
 
@@ -83,6 +98,7 @@ struct RecArgs {
     const int32_t* __restrict__ ibool;  // [n_rec*125] special node ids of recording cells
     const float* __restrict__ dd;       // [9][n_rec*125] unscaled dxi_dx
     int nf;
+    int64_t fstride, astride;
     const float* __restrict__ U;
     const float* __restrict__ W;
     const float* __restrict__ Aacc;
@@ -101,7 +117,108 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return (x < r) ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
 }
 
-template <int EPB, bool ATOMIC, bool PML_ELEM>
+// Wave-level LDS visibility: the lanes of one wavefront exchange data through
+// LDS with no workgroup barrier (same sequence as HIP's __syncwarp).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Element force of one GLL point from the staged displacement su and the
+// scaled geometry g (dxi_a/dx_m sqrt(J w)): writes the 9 fluxes
+// F[c][a] = sum_m sigma[c][m] dxi_a/dx_m J w  (element_cpu.cpp:51-108).
+__device__ __forceinline__ void point_flux(const float* __restrict__ su0, const float* __restrict__ su1,
+                                           const float* __restrict__ su2, const float* Di,
+                                           const float* Dj, const float* Dk, int i, int j, int k,
+                                           const float* g, float lam, float mu, float* F) {
+    const float* su[3] = {su0, su1, su2};
+    float du[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+#pragma unroll
+        for (int s = 0; s < 5; ++s) {
+            a0 += Di[s] * su[c][s * 25 + j * 5 + k];
+            a1 += Dj[s] * su[c][i * 25 + s * 5 + k];
+            a2 += Dk[s] * su[c][i * 25 + j * 5 + s];
+        }
+        du[c][0] = a0;
+        du[c][1] = a1;
+        du[c][2] = a2;
+    }
+    float gr[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+            gr[c][m] = du[c][0] * g[m * 3 + 0] + du[c][1] * g[m * 3 + 1] + du[c][2] * g[m * 3 + 2];
+    const float tr = gr[0][0] + gr[1][1] + gr[2][2];
+    const float mu2 = 2.f * mu;
+    float sig[3][3];
+    sig[0][0] = mu2 * gr[0][0] + lam * tr;
+    sig[1][1] = mu2 * gr[1][1] + lam * tr;
+    sig[2][2] = mu2 * gr[2][2] + lam * tr;
+    sig[0][1] = sig[1][0] = mu * (gr[0][1] + gr[1][0]);
+    sig[0][2] = sig[2][0] = mu * (gr[0][2] + gr[2][0]);
+    sig[1][2] = sig[2][1] = mu * (gr[1][2] + gr[2][1]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            F[c * 3 + a] = sig[c][0] * g[0 * 3 + a] + sig[c][1] * g[1 * 3 + a] + sig[c][2] * g[2 * 3 + a];
+}
+
+// Weighted divergence of the staged fluxes at point (i,j,k):
+// r_c = -sum_s (D[s][i] F_c0(s,j,k) + D[s][j] F_c1(i,s,k) + D[s][k] F_c2(i,j,s))
+// (element_cpu.cpp:110-157 rewritten as a gather over the point's neighbours).
+__device__ __forceinline__ void point_div(const float* __restrict__ sf, int stride, const float* Dti,
+                                          const float* Dtj, const float* Dtk, int i, int j, int k,
+                                          float* rv) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float acc = 0.f;
+#pragma unroll
+        for (int s = 0; s < 5; ++s) {
+            acc += Dti[s] * sf[(c * 3 + 0) * stride + s * 25 + j * 5 + k];
+            acc += Dtj[s] * sf[(c * 3 + 1) * stride + i * 25 + s * 5 + k];
+            acc += Dtk[s] * sf[(c * 3 + 2) * stride + i * 25 + j * 5 + s];
+        }
+        rv[c] = -acc;
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ void emit_residual(const ElemArgs& A, int f, int64_t node, int64_t slot,
+                                              const float* rold, const float* rv) {
+    if (MODE == ASM_GATHER) {
+        float* o = A.Rl + (int64_t(f) * A.P + slot) * 3;
+        o[0] = rv[0];
+        o[1] = rv[1];
+        o[2] = rv[2];
+    } else if (MODE == ASM_ATOMIC) {
+        float* r = A.R + int64_t(f) * A.fstride + node * 3;
+        unsafeAtomicAdd(r + 0, rv[0]);
+        unsafeAtomicAdd(r + 1, rv[1]);
+        unsafeAtomicAdd(r + 2, rv[2]);
+    } else {  // the first element (in colour order) touching a node stores (rold = 0)
+        float* r = A.R + int64_t(f) * A.fstride + node * 3;
+        r[0] = rold[0] + rv[0];
+        r[1] = rold[1] + rv[1];
+        r[2] = rold[2] + rv[2];
+    }
+}
+
+__device__ __forceinline__ void add_sources(const ElemArgs& A, int f, int sb, int se, int p, float* rv) {
+    for (int s = sb; s < se; ++s) {  // solver.cpp:576-605
+        const int4 en = A.src_ent[s];
+        if (en.x == f && A.step < A.nstf) rv[en.y] += A.stf[int64_t(f) * A.nstf + A.step] * A.src_w[en.z + p];
+    }
+}
+
+// One lane per GLL point, EPB elements per workgroup, workgroup barriers
+// between the stages.  Supports the element-local C-PML fallback.
+template <int EPB, int MODE, bool PML_ELEM>
 __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
     k_element(ElemArgs A, int e0, int ne) {
     __shared__ float sD[25];
@@ -134,17 +251,16 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
     }
     const int64_t node = int64_t(nodef & ~FIRST_TOUCH);
     const bool first = (nodef & FIRST_TOUCH) != 0;
+    const int64_t rslot = (MODE == ASM_GATHER && valid) ? A.slot[pt] : 0;
     const int pslot = (PML_ELEM && valid) ? A.pml_slot[e] : -1;
     int sb = 0, se = 0;
     if (valid) {
         sb = A.src_off[e];
         se = A.src_off[e + 1];
     }
-    // displacement of field 0 (later fields are prefetched one iteration ahead)
-    const int64_t nbase = node * (3 * A.nf);
-    float un[3] = {0.f, 0.f, 0.f};
+    float un[3] = {0.f, 0.f, 0.f};  // field 0; later fields are prefetched one iteration ahead
     if (valid) {
-        const float* ut = A.U + nbase;
+        const float* ut = A.U + node * 3;
         un[0] = ut[0];
         un[1] = ut[1];
         un[2] = ut[2];
@@ -166,85 +282,41 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
     // (2), when every lane has finished reading su; writing the next fluxes
     // happens after the next (1), when every lane has finished reading sf.
     for (int f = 0; f < A.nf; ++f) {
-        const int64_t fo = nbase + 3 * f;  // this node, field f
+        const int64_t fo = int64_t(f) * A.fstride;
         if (valid) {
             su[le][0][p] = un[0];
             su[le][1][p] = un[1];
             su[le][2][p] = un[2];
         }
         if (valid && f + 1 < A.nf) {
-            const float* ut = A.U + fo + 3;
+            const float* ut = A.U + fo + A.fstride + node * 3;
             un[0] = ut[0];
             un[1] = ut[1];
             un[2] = ut[2];
         }
         float rold[3] = {0.f, 0.f, 0.f};
-        if (!ATOMIC && valid && !first) {
-            const float* r = A.R + fo;
+        if (MODE == ASM_COLORED && valid && !first) {
+            const float* r = A.R + fo + node * 3;
             rold[0] = r[0];
             rold[1] = r[1];
             rold[2] = r[2];
         }
         __syncthreads();
         if (valid) {
-            float du[3][3];
+            float F[9];
+            point_flux(su[lel][0], su[lel][1], su[lel][2], Di, Dj, Dk, i, j, k, g, lam, mu, F);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                float a0 = 0.f, a1 = 0.f, a2 = 0.f;
-#pragma unroll
-                for (int s = 0; s < 5; ++s) {
-                    a0 += Di[s] * su[lel][c][s * 25 + j * 5 + k];
-                    a1 += Dj[s] * su[lel][c][i * 25 + s * 5 + k];
-                    a2 += Dk[s] * su[lel][c][i * 25 + j * 5 + s];
-                }
-                du[c][0] = a0;
-                du[c][1] = a1;
-                du[c][2] = a2;
-            }
-            // scaled physical gradient gr[c][m] = sum_a du[c][a] dxi_a/dx_m sqrt(Jw)
-            float gr[3][3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-                for (int m = 0; m < 3; ++m)
-                    gr[c][m] = du[c][0] * g[m * 3 + 0] + du[c][1] * g[m * 3 + 1] +
-                               du[c][2] * g[m * 3 + 2];
-            const float tr = gr[0][0] + gr[1][1] + gr[2][2];
-            const float mu2 = 2.f * mu;
-            float sig[3][3];
-            sig[0][0] = mu2 * gr[0][0] + lam * tr;
-            sig[1][1] = mu2 * gr[1][1] + lam * tr;
-            sig[2][2] = mu2 * gr[2][2] + lam * tr;
-            sig[0][1] = sig[1][0] = mu * (gr[0][1] + gr[1][0]);
-            sig[0][2] = sig[2][0] = mu * (gr[0][2] + gr[2][0]);
-            sig[1][2] = sig[2][1] = mu * (gr[1][2] + gr[2][1]);
-            // fluxes F[c][a] = sum_m sig[c][m] dxi_a/dx_m J w
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-                for (int aa = 0; aa < 3; ++aa)
-                    sf[le][c * 3 + aa][p] = sig[c][0] * g[0 * 3 + aa] + sig[c][1] * g[1 * 3 + aa] +
-                                            sig[c][2] * g[2 * 3 + aa];
+            for (int q = 0; q < 9; ++q) sf[le][q][p] = F[q];
         }
         __syncthreads();
         if (valid) {
             float rv[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                float acc = 0.f;
-#pragma unroll
-                for (int s = 0; s < 5; ++s) {
-                    acc += Dti[s] * sf[lel][c * 3 + 0][s * 25 + j * 5 + k];
-                    acc += Dtj[s] * sf[lel][c * 3 + 1][i * 25 + s * 5 + k];
-                    acc += Dtk[s] * sf[lel][c * 3 + 2][i * 25 + j * 5 + s];
-                }
-                rv[c] = -acc;
-            }
+            point_div(&sf[lel][0][0], NP, Dti, Dtj, Dtk, i, j, k, rv);
             if (PML_ELEM && pslot >= 0) {
                 // element-local C-PML (pml.cpp:55-216); PML nodes are special, so a_n is stored
                 const int64_t q = int64_t(pslot) * NP + p;
-                const int64_t gn = fo;
-                const float* an_p = A.Aacc + fo;
+                const int64_t gn = fo + node * 3;
+                const float* an_p = A.Aacc + int64_t(f) * A.astride + node * 3;
                 float upn[3], vpn[3], apn[3], m[9], pn[3];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
@@ -253,12 +325,12 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
                     upn[c] = A.U[gn + c] - A.dt * w;
                     vpn[c] = w - 0.5f * A.dt * apn[c];
                 }
-                float* pmem = A.pmem + (q * A.nf + f) * 9;
-                float* pnew = A.pnew + (q * A.nf + f) * 3;
+                float* pmem = A.pmem + int64_t(f) * 9 * A.Ppml;
+                float* pnew = A.pnew + int64_t(f) * 3 * A.Ppml;
 #pragma unroll
-                for (int c = 0; c < 9; ++c) m[c] = pmem[c];
+                for (int c = 0; c < 9; ++c) m[c] = pmem[c * A.Ppml + q];
 #pragma unroll
-                for (int c = 0; c < 3; ++c) pn[c] = pnew[c];
+                for (int c = 0; c < 3; ++c) pn[c] = pnew[c * A.Ppml + q];
                 float al[9], S[5];
 #pragma unroll
                 for (int c = 0; c < 9; ++c) al[c] = A.palpha[c * A.Ppml + q];
@@ -266,50 +338,27 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
                 for (int c = 0; c < 5; ++c) S[c] = A.pabar[c * A.Ppml + q];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    const float oldv = pn[c] + A.c2 * apn[c];  // pml.cpp:91-109
+                    const float oldv = pn[c] + A.c2 * apn[c];   // pml.cpp:91-109
                     const float newv = upn[c] + A.c1 * vpn[c];  // pml.cpp:72-89
                     rv[c] += S[0] * vpn[c] + S[1] * upn[c] + S[2] * m[c * 3 + 0] +
                              S[3] * m[c * 3 + 1] + S[4] * m[c * 3 + 2];  // pml.cpp:209-212
 #pragma unroll
                     for (int d = 0; d < 3; ++d)  // pml.cpp:133-146
-                        pmem[c * 3 + d] =
+                        pmem[(c * 3 + d) * A.Ppml + q] =
                             al[d * 3 + 0] * m[c * 3 + d] + al[d * 3 + 1] * newv + al[d * 3 + 2] * oldv;
-                    pnew[c] = newv;
+                    pnew[c * A.Ppml + q] = newv;
                 }
             }
-            for (int s = sb; s < se; ++s) {  // solver.cpp:576-605
-                const int4 en = A.src_ent[s];
-                if (en.x == f && A.step < A.nstf) {
-                    const float sv = A.stf[int64_t(f) * A.nstf + A.step];
-                    rv[en.y] += sv * A.src_w[en.z + p];
-                }
-            }
-            float* r = A.R + fo;
-            if (ATOMIC) {
-                unsafeAtomicAdd(r + 0, rv[0]);
-                unsafeAtomicAdd(r + 1, rv[1]);
-                unsafeAtomicAdd(r + 2, rv[2]);
-            } else {  // the first element (in colour order) touching a node stores
-                r[0] = rold[0] + rv[0];
-                r[1] = rold[1] + rv[1];
-                r[2] = rold[2] + rv[2];
-            }
+            add_sources(A, f, sb, se, p, rv);
+            emit_residual<MODE>(A, f, node, rslot, rold, rv);
         }
     }
-}
-
-// Wave-level LDS visibility: the lanes of one wavefront exchange data through
-// LDS with no workgroup barrier (same sequence as HIP's __syncwarp).
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // One wavefront per element, lane l owns points l and l + 64 (l < 61).  All
 // exchange between the element's points stays inside one wavefront, so the
 // field loop has no workgroup barrier; WPB elements (wavefronts) per block.
-template <int WPB, bool ATOMIC>
+template <int WPB, int MODE>
 __global__ void __launch_bounds__(64 * WPB) k_element_wave(ElemArgs A, int e0, int ne) {
     __shared__ __align__(16) float sDr[5][8];  // rows of D, padded
     __shared__ __align__(16) float sDc[5][8];  // columns of D, padded
@@ -330,9 +379,9 @@ __global__ void __launch_bounds__(64 * WPB) k_element_wave(ElemArgs A, int e0, i
     const bool vq[2] = {true, lane + 64 < NP};
     int iq[2], jq[2], kq[2];
     float g[2][9], lam[2], mu[2];
-    int64_t node[2];
+    int64_t node[2], ptq[2];
     bool first[2];
-    int sb = A.src_off[e], se = A.src_off[e + 1];
+    const int sb = A.src_off[e], se = A.src_off[e + 1];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         const int p = vq[q] ? pq[q] : 0;
@@ -347,15 +396,16 @@ __global__ void __launch_bounds__(64 * WPB) k_element_wave(ElemArgs A, int e0, i
         const uint32_t nf_ = vq[q] ? __builtin_nontemporal_load(&A.ibool[pt]) : 0u;
         node[q] = int64_t(nf_ & ~FIRST_TOUCH);
         first[q] = (nf_ & FIRST_TOUCH) != 0;
+        ptq[q] = (MODE == ASM_GATHER && vq[q]) ? A.slot[pt] : 0;
     }
-    const int ns = 3 * A.nf;
     float un[2][3];
 #pragma unroll
     for (int q = 0; q < 2; ++q)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) un[q][c] = vq[q] ? A.U[node[q] * ns + c] : 0.f;
+        for (int c = 0; c < 3; ++c) un[q][c] = vq[q] ? A.U[node[q] * 3 + c] : 0.f;
 
     for (int f = 0; f < A.nf; ++f) {
+        const int64_t fo = int64_t(f) * A.fstride;
 #pragma unroll
         for (int q = 0; q < 2; ++q)
             if (vq[q])
@@ -365,66 +415,33 @@ __global__ void __launch_bounds__(64 * WPB) k_element_wave(ElemArgs A, int e0, i
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             const bool nx = vq[q] && f + 1 < A.nf;
-            const int64_t o = node[q] * ns + 3 * f;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                un[q][c] = nx ? A.U[o + 3 + c] : 0.f;
-                rold[q][c] = (!ATOMIC && vq[q] && !first[q]) ? A.R[o + c] : 0.f;
+                un[q][c] = nx ? A.U[fo + A.fstride + node[q] * 3 + c] : 0.f;
+                rold[q][c] = (MODE == ASM_COLORED && vq[q] && !first[q]) ? A.R[fo + node[q] * 3 + c] : 0.f;
             }
         }
         wave_sync();
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             if (!vq[q]) continue;
-            const int i = iq[q], j = jq[q], k = kq[q], p = pq[q];
+            const int i = iq[q], j = jq[q], k = kq[q];
             const float4 di = *reinterpret_cast<const float4*>(&sDr[i][0]);
             const float4 dj = *reinterpret_cast<const float4*>(&sDr[j][0]);
             const float4 dk = *reinterpret_cast<const float4*>(&sDr[k][0]);
             const float Di[5] = {di.x, di.y, di.z, di.w, sDr[i][4]};
             const float Dj[5] = {dj.x, dj.y, dj.z, dj.w, sDr[j][4]};
             const float Dk[5] = {dk.x, dk.y, dk.z, dk.w, sDr[k][4]};
-            float du[3][3];
+            float F[9];
+            point_flux(su[wv][0], su[wv][1], su[wv][2], Di, Dj, Dk, i, j, k, g[q], lam[q], mu[q], F);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                float a0 = 0.f, a1 = 0.f, a2 = 0.f;
-#pragma unroll
-                for (int s = 0; s < 5; ++s) {
-                    a0 += Di[s] * su[wv][c][s * 25 + j * 5 + k];
-                    a1 += Dj[s] * su[wv][c][i * 25 + s * 5 + k];
-                    a2 += Dk[s] * su[wv][c][i * 25 + j * 5 + s];
-                }
-                du[c][0] = a0;
-                du[c][1] = a1;
-                du[c][2] = a2;
-            }
-            const float* gg = g[q];
-            float gr[3][3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-                for (int m = 0; m < 3; ++m)
-                    gr[c][m] = du[c][0] * gg[m * 3 + 0] + du[c][1] * gg[m * 3 + 1] + du[c][2] * gg[m * 3 + 2];
-            const float tr = gr[0][0] + gr[1][1] + gr[2][2];
-            const float mu2 = 2.f * mu[q], la = lam[q], m1 = mu[q];
-            float sig[3][3];
-            sig[0][0] = mu2 * gr[0][0] + la * tr;
-            sig[1][1] = mu2 * gr[1][1] + la * tr;
-            sig[2][2] = mu2 * gr[2][2] + la * tr;
-            sig[0][1] = sig[1][0] = m1 * (gr[0][1] + gr[1][0]);
-            sig[0][2] = sig[2][0] = m1 * (gr[0][2] + gr[2][0]);
-            sig[1][2] = sig[2][1] = m1 * (gr[1][2] + gr[2][1]);
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-                for (int aa = 0; aa < 3; ++aa)
-                    sf[wv][c * 3 + aa][p] = sig[c][0] * gg[0 * 3 + aa] + sig[c][1] * gg[1 * 3 + aa] +
-                                            sig[c][2] * gg[2 * 3 + aa];
+            for (int c = 0; c < 9; ++c) sf[wv][c][pq[q]] = F[c];
         }
         wave_sync();
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             if (!vq[q]) continue;
-            const int i = iq[q], j = jq[q], k = kq[q], p = pq[q];
+            const int i = iq[q], j = jq[q], k = kq[q];
             const float4 ci = *reinterpret_cast<const float4*>(&sDc[i][0]);
             const float4 cj = *reinterpret_cast<const float4*>(&sDc[j][0]);
             const float4 ck = *reinterpret_cast<const float4*>(&sDc[k][0]);
@@ -432,33 +449,22 @@ __global__ void __launch_bounds__(64 * WPB) k_element_wave(ElemArgs A, int e0, i
             const float Dtj[5] = {cj.x, cj.y, cj.z, cj.w, sDc[j][4]};
             const float Dtk[5] = {ck.x, ck.y, ck.z, ck.w, sDc[k][4]};
             float rv[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                float acc = 0.f;
-#pragma unroll
-                for (int s = 0; s < 5; ++s) {
-                    acc += Dti[s] * sf[wv][c * 3 + 0][s * 25 + j * 5 + k];
-                    acc += Dtj[s] * sf[wv][c * 3 + 1][i * 25 + s * 5 + k];
-                    acc += Dtk[s] * sf[wv][c * 3 + 2][i * 25 + j * 5 + s];
-                }
-                rv[c] = -acc;
-            }
-            for (int s = sb; s < se; ++s) {  // solver.cpp:576-605
-                const int4 en = A.src_ent[s];
-                if (en.x == f && A.step < A.nstf)
-                    rv[en.y] += A.stf[int64_t(f) * A.nstf + A.step] * A.src_w[en.z + p];
-            }
-            float* r = A.R + node[q] * ns + 3 * f;
-            if (ATOMIC) {
-                unsafeAtomicAdd(r + 0, rv[0]);
-                unsafeAtomicAdd(r + 1, rv[1]);
-                unsafeAtomicAdd(r + 2, rv[2]);
-            } else {
-                r[0] = rold[q][0] + rv[0];
-                r[1] = rold[q][1] + rv[1];
-                r[2] = rold[q][2] + rv[2];
-            }
+            point_div(&sf[wv][0][0], 128, Dti, Dtj, Dtk, i, j, k, rv);
+            add_sources(A, f, sb, se, pq[q], rv);
+            emit_residual<MODE>(A, f, node[q], ptq[q], rold[q], rv);
         }
+    }
+}
+
+// Fixed-order sum of a node's element contributions (GATHER assembly).
+__device__ __forceinline__ void gather_r(const NodeArgs& A, int f, int kb, int ke, float* r) {
+    r[0] = r[1] = r[2] = 0.f;
+    const float* Rl = A.Rl + int64_t(f) * A.P * 3;
+    for (int q = kb; q < ke; ++q) {
+        const float* c = Rl + int64_t(q) * 3;
+        r[0] += c[0];
+        r[1] += c[1];
+        r[2] += c[2];
     }
 }
 
@@ -477,36 +483,42 @@ __global__ void __launch_bounds__(256) k_node_special(NodeArgs A) {
 #pragma unroll
         for (int q = 0; q < 9; ++q) al[q] = A.palpha[q * A.Gpml + g];
     }
-    const int64_t base = g * (3 * A.nf);
+    int kb = 0, ke = 0;
+    if (A.gather) {
+        kb = A.off[g];
+        ke = A.off[g + 1];
+    }
     for (int f = 0; f < A.nf; ++f) {
-        const int64_t o = base + 3 * f;
+        const int64_t o = int64_t(f) * A.fstride + g * 3;
+        const int64_t oa = int64_t(f) * A.astride + g * 3;
         float U[3], W[3], R[3], an[3], un[3], vn[3];
+        if (A.gather) gather_r(A, f, kb, ke, R);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             U[c] = A.U[o + c];
             W[c] = A.W[o + c];
-            R[c] = A.R[o + c];
-            an[c] = A.Aacc[o + c];
+            if (!A.gather) R[c] = A.R[o + c];
+            an[c] = A.Aacc[oa + c];
             un[c] = U[c] - dt * W[c];
             vn[c] = W[c] - hdt * an[c];
         }
         if (pml) {
-            float* pmem = A.pmem + (g * A.nf + f) * 9;
-            float* pnew = A.pnew + (g * A.nf + f) * 3;
+            float* pmem = A.pmem + int64_t(f) * 9 * A.Gpml;
+            float* pnew = A.pnew + int64_t(f) * 3 * A.Gpml;
             float m[9];
 #pragma unroll
-            for (int q = 0; q < 9; ++q) m[q] = pmem[q];
+            for (int q = 0; q < 9; ++q) m[q] = pmem[q * A.Gpml + g];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const float oldv = pnew[c] + A.c2 * an[c];  // pml.cpp:91-109
-                const float newv = un[c] + A.c1 * vn[c];    // pml.cpp:72-89
+                const float oldv = pnew[c * A.Gpml + g] + A.c2 * an[c];  // pml.cpp:91-109
+                const float newv = un[c] + A.c1 * vn[c];                 // pml.cpp:72-89
                 R[c] += C[0] * vn[c] + C[1] * un[c] + C[2] * m[c * 3 + 0] + C[3] * m[c * 3 + 1] +
                         C[4] * m[c * 3 + 2];  // pml.cpp:209-212, assembled over PML elements
 #pragma unroll
                 for (int d = 0; d < 3; ++d)  // pml.cpp:133-146
-                    pmem[c * 3 + d] =
+                    pmem[(c * 3 + d) * A.Gpml + g] =
                         al[d * 3 + 0] * m[c * 3 + d] + al[d * 3 + 1] * newv + al[d * 3 + 2] * oldv;
-                pnew[c] = newv;
+                pnew[c * A.Gpml + g] = newv;
             }
         }
 #pragma unroll
@@ -526,38 +538,60 @@ __global__ void __launch_bounds__(256) k_node_special(NodeArgs A) {
             const float w1 = v1 + hdt * a1;
             A.U[o + c] = u1 + dt * w1;
             A.W[o + c] = w1;
-            A.Aacc[o + c] = a1;
+            A.Aacc[oa + c] = a1;
         }
     }
 }
 
-// Regular nodes [Ga, G4): a = r/m, w += dt a, u~ += dt w over the contiguous
-// float range [Ga*3nf, G4*3nf), 4 floats per lane with 16-byte accesses
-// (Ga and G4 are multiples of 4 nodes).
+// Regular nodes [Ga, G4): a = r/m, w += dt a, u~ += dt w.  4 nodes per lane,
+// 16-byte accesses (Ga and the field stride are multiples of 4 nodes).
+template <bool GATHER>
 __global__ void __launch_bounds__(256) k_node_regular(NodeArgs A) {
-    const int64_t ns = 3 * int64_t(A.nf);
-    const int64_t q0 = A.Ga * ns + 4 * (int64_t(blockIdx.x) * blockDim.x + threadIdx.x);
-    if (q0 >= A.G4 * ns) return;
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t g = A.Ga + 4 * t;
+    if (g >= A.G4) return;
     const float dt = A.dt;
-    const int64_t g0 = q0 / ns;
-    const int64_t r0 = q0 - g0 * ns;
-    const float4 r4 = *reinterpret_cast<const float4*>(A.R + q0);
-    float4 u4 = *reinterpret_cast<const float4*>(A.U + q0);
-    float4 w4 = *reinterpret_cast<const float4*>(A.W + q0);
-    const float r[4] = {r4.x, r4.y, r4.z, r4.w};
-    float u[4] = {u4.x, u4.y, u4.z, u4.w};
-    float w[4] = {w4.x, w4.y, w4.z, w4.w};
-    const float m0 = A.minv[g0];
-    const float m1 = (r0 + 3 >= ns) ? A.minv[g0 + 1] : m0;  // 4 floats span at most 2 nodes
+    const float4 mi4 = *reinterpret_cast<const float4*>(A.minv + g);
+    const float mi[4] = {mi4.x, mi4.y, mi4.z, mi4.w};
+    int kb[5] = {0, 0, 0, 0, 0};
+    if (GATHER) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const float mi = (r0 + q >= ns) ? m1 : m0;
-        const float a1 = r[q] * mi;
-        w[q] = w[q] + dt * a1;
-        u[q] = u[q] + dt * w[q];
+        for (int q = 0; q < 5; ++q) kb[q] = A.off[g + q];
     }
-    *reinterpret_cast<float4*>(A.U + q0) = make_float4(u[0], u[1], u[2], u[3]);
-    *reinterpret_cast<float4*>(A.W + q0) = make_float4(w[0], w[1], w[2], w[3]);
+    for (int f = 0; f < A.nf; ++f) {
+        const int64_t o = int64_t(f) * A.fstride + g * 3;
+        float4* U4 = reinterpret_cast<float4*>(A.U + o);
+        float4* W4 = reinterpret_cast<float4*>(A.W + o);
+        float u[12], w[12], r[12];
+        if (GATHER) {
+#pragma unroll
+            for (int n = 0; n < 4; ++n) gather_r(A, f, kb[n], kb[n + 1], &r[3 * n]);
+        } else {
+            const float4* R4 = reinterpret_cast<const float4*>(A.R + o);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const float4 c = R4[q];
+                r[4 * q + 0] = c.x; r[4 * q + 1] = c.y; r[4 * q + 2] = c.z; r[4 * q + 3] = c.w;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const float4 a = U4[q], b = W4[q];
+            u[4 * q + 0] = a.x; u[4 * q + 1] = a.y; u[4 * q + 2] = a.z; u[4 * q + 3] = a.w;
+            w[4 * q + 0] = b.x; w[4 * q + 1] = b.y; w[4 * q + 2] = b.z; w[4 * q + 3] = b.w;
+        }
+#pragma unroll
+        for (int q = 0; q < 12; ++q) {
+            const float a1 = r[q] * mi[q / 3];
+            w[q] = w[q] + dt * a1;
+            u[q] = u[q] + dt * w[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            U4[q] = make_float4(u[4 * q + 0], u[4 * q + 1], u[4 * q + 2], u[4 * q + 3]);
+            W4[q] = make_float4(w[4 * q + 0], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+        }
+    }
 }
 
 // Fill kernel (used before atomic assembly; no memset nodes in a step).
@@ -588,8 +622,8 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64) k_record(RecArgs 
     for (int c = 0; c < 9; ++c) dd[c] = valid ? A.dd[c * nrp + pt] : 0.f;
     __syncthreads();
     for (int f = 0; f < A.nf; ++f) {
-        const int64_t go = (node * A.nf + f) * 3;
-        const int64_t ga = go;
+        const int64_t go = int64_t(f) * A.fstride + node * 3;
+        const int64_t ga = int64_t(f) * A.astride + node * 3;
         float w[3] = {0.f, 0.f, 0.f};
         if (valid) {
 #pragma unroll

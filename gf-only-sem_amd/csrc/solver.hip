@@ -73,6 +73,7 @@ struct FieldSource {
 struct gfx_solver {
     int E = 0, G = 0, nf = 1, n_rec = 0, n_pml_elem = 0;
     int64_t Ga = 0, Gpml = 0, G4 = 0, Ppml = 0;
+    int64_t fstride = 0, astride = 0;
     int assembly = GFX_ASSEMBLY_COLORED, epb = 2, kernel = GFX_ELEMENT_WAVE;
     double dt = 0;
     int stride = 1;
@@ -87,7 +88,9 @@ struct gfx_solver {
     // C-PML: nodal (pc, palpha, pmem, pnew over [0,Gpml)) or element-local (Ppml points)
     gfx::DevBuf<float> pc, palpha, pabar, pmem, pnew;
     gfx::DevBuf<int32_t> pml_slot, src_off;
-    gfx::DevBuf<float> U, W, R, Aacc;
+    gfx::DevBuf<float> U, W, R, Aacc, Rl;
+    gfx::DevBuf<int32_t> goff, gslot;  // GATHER assembly: node slot ranges, point -> slot
+    std::vector<int32_t> goff_host;
     std::vector<gfx::FieldSource> sources;
     gfx::DevBuf<int4> src_ent;
     gfx::DevBuf<float> src_w, stf;
@@ -157,6 +160,7 @@ void init_state(gfx_solver& S) {
     S.W.zero(S.stream);
     S.R.zero(S.stream);
     S.Aacc.zero(S.stream);
+    S.Rl.zero(S.stream);
     S.pmem.zero(S.stream);
     S.pnew.zero(S.stream);
     GFX_HIP(hipStreamSynchronize(S.stream));
@@ -169,10 +173,16 @@ void launch_element_t(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
     constexpr int NT = ((EPB * NP + 63) / 64) * 64;
     const int grid = (n + EPB - 1) / EPB;
     if (grid == 0) return;
-    if (S.assembly == GFX_ASSEMBLY_ATOMIC)
-        dev::k_element<EPB, true, PE><<<grid, NT, 0, S.stream>>>(a, b, n);
-    else
-        dev::k_element<EPB, false, PE><<<grid, NT, 0, S.stream>>>(a, b, n);
+    switch (S.assembly) {
+        case GFX_ASSEMBLY_ATOMIC:
+            dev::k_element<EPB, dev::ASM_ATOMIC, PE><<<grid, NT, 0, S.stream>>>(a, b, n);
+            break;
+        case GFX_ASSEMBLY_GATHER:
+            dev::k_element<EPB, dev::ASM_GATHER, PE><<<grid, NT, 0, S.stream>>>(a, b, n);
+            break;
+        default:
+            dev::k_element<EPB, dev::ASM_COLORED, PE><<<grid, NT, 0, S.stream>>>(a, b, n);
+    }
 }
 
 template <bool PE>
@@ -189,10 +199,16 @@ template <int WPB>
 void launch_wave_t(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
     const int grid = (n + WPB - 1) / WPB;
     if (grid == 0) return;
-    if (S.assembly == GFX_ASSEMBLY_ATOMIC)
-        dev::k_element_wave<WPB, true><<<grid, 64 * WPB, 0, S.stream>>>(a, b, n);
-    else
-        dev::k_element_wave<WPB, false><<<grid, 64 * WPB, 0, S.stream>>>(a, b, n);
+    switch (S.assembly) {
+        case GFX_ASSEMBLY_ATOMIC:
+            dev::k_element_wave<WPB, dev::ASM_ATOMIC><<<grid, 64 * WPB, 0, S.stream>>>(a, b, n);
+            break;
+        case GFX_ASSEMBLY_GATHER:
+            dev::k_element_wave<WPB, dev::ASM_GATHER><<<grid, 64 * WPB, 0, S.stream>>>(a, b, n);
+            break;
+        default:
+            dev::k_element_wave<WPB, dev::ASM_COLORED><<<grid, 64 * WPB, 0, S.stream>>>(a, b, n);
+    }
 }
 
 void launch_element(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
@@ -222,10 +238,14 @@ void run_steps(gfx_solver& S, int nsteps) {
     ea.ibool = S.ibool.p;
     ea.P = int64_t(S.E) * NP;
     ea.nf = S.nf;
+    ea.fstride = S.fstride;
     ea.U = S.U.p;
     ea.R = S.R.p;
+    ea.Rl = S.Rl.p;
+    ea.slot = S.gslot.p;
     ea.W = S.W.p;
     ea.Aacc = S.Aacc.p;
+    ea.astride = S.astride;
     ea.pml_slot = S.pml_slot.p;
     ea.Ppml = S.Ppml;
     ea.palpha = S.palpha.p;
@@ -247,7 +267,13 @@ void run_steps(gfx_solver& S, int nsteps) {
     na.Ga = S.Ga;
     na.Gpml = S.Gpml;
     na.G4 = S.G4;
+    na.P = int64_t(S.E) * NP;
     na.nf = S.nf;
+    na.fstride = S.fstride;
+    na.astride = S.astride;
+    na.gather = (S.assembly == GFX_ASSEMBLY_GATHER) ? 1 : 0;
+    na.Rl = S.Rl.p;
+    na.off = S.goff.p;
     na.minv = S.minv.p;
     na.damp = S.has_cpml ? nullptr : S.damp.p;
     na.U = S.U.p;
@@ -263,9 +289,8 @@ void run_steps(gfx_solver& S, int nsteps) {
     na.c2 = c2;
     na.dt = float(S.dt);
     const int spec_grid = int((S.Ga + 255) / 256);
-    const int64_t ns = 3 * int64_t(S.nf);
-    const int reg_grid = int(((S.G4 - S.Ga) * ns / 4 + 255) / 256);
-    const int64_t r4 = S.G4 * ns / 4;
+    const int reg_grid = int(((S.G4 - S.Ga) / 4 + 255) / 256);
+    const int64_t r4 = int64_t(S.nf) * S.fstride / 4;
     const int zero_grid = int((r4 + 255) / 256);
 
     dev::RecArgs ra{};
@@ -273,6 +298,8 @@ void run_steps(gfx_solver& S, int nsteps) {
     ra.ibool = S.rec_ibool.p;
     ra.dd = S.rec_dd.p;
     ra.nf = S.nf;
+    ra.fstride = S.fstride;
+    ra.astride = S.astride;
     ra.U = S.U.p;
     ra.W = S.W.p;
     ra.Aacc = S.Aacc.p;
@@ -298,7 +325,12 @@ void run_steps(gfx_solver& S, int nsteps) {
         GFX_HIP(hipGetLastError());
         if (S.timing) GFX_HIP(hipEventRecord(ev[size_t(t) * 3 + 1], S.stream));
         if (spec_grid > 0) dev::k_node_special<<<spec_grid, 256, 0, S.stream>>>(na);
-        if (reg_grid > 0) dev::k_node_regular<<<reg_grid, 256, 0, S.stream>>>(na);
+        if (reg_grid > 0) {
+            if (na.gather)
+                dev::k_node_regular<true><<<reg_grid, 256, 0, S.stream>>>(na);
+            else
+                dev::k_node_regular<false><<<reg_grid, 256, 0, S.stream>>>(na);
+        }
         GFX_HIP(hipGetLastError());
         if (S.timing) GFX_HIP(hipEventRecord(ev[size_t(t) * 3 + 2], S.stream));
         if (S.stride > 0 && S.step % S.stride == 0 && S.rec_cap > 0 && S.n_rec > 0) {
@@ -362,7 +394,8 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
             "missing geometry/material arrays");
     require(d.n_fields >= 1 && d.n_fields <= 4096, "n_fields must be in [1, 4096]");
     require(d.dt > 0, "dt must be > 0");
-    require(d.assembly == GFX_ASSEMBLY_COLORED || d.assembly == GFX_ASSEMBLY_ATOMIC,
+    require(d.assembly == GFX_ASSEMBLY_COLORED || d.assembly == GFX_ASSEMBLY_ATOMIC ||
+                d.assembly == GFX_ASSEMBLY_GATHER,
             "unknown assembly mode");
     require(d.element_kernel >= GFX_ELEMENT_AUTO && d.element_kernel <= GFX_ELEMENT_WAVE,
             "unknown element kernel");
@@ -394,13 +427,37 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
     const GLL gll(4);
     for (int q = 0; q < 25; ++q) S.D[q] = float(gll.D[q]);
 
-    // ---- element order: by colour (coloured assembly) or caller order ----
+    // ---- element order: Morton curve of the centroids (when given), colour-major
+    //      for coloured assembly; caller order otherwise ----
+    std::vector<uint64_t> key(E, 0);
+    if (d.elem_centroid) {
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int e = 0; e < E; ++e)
+            for (int c = 0; c < 3; ++c) {
+                lo[c] = std::min(lo[c], d.elem_centroid[e * 3 + c]);
+                hi[c] = std::max(hi[c], d.elem_centroid[e * 3 + c]);
+            }
+        for (int e = 0; e < E; ++e) {
+            uint64_t q[3];
+            for (int c = 0; c < 3; ++c) {
+                const double span = hi[c] > lo[c] ? hi[c] - lo[c] : 1.0;
+                const double t = (d.elem_centroid[e * 3 + c] - lo[c]) / span;
+                q[c] = uint64_t(std::min(std::max(t, 0.0), 1.0) * 2097151.0);
+            }
+            uint64_t m = 0;
+            for (int b = 20; b >= 0; --b)
+                for (int c = 2; c >= 0; --c) m = (m << 1) | ((q[c] >> b) & 1ULL);
+            key[e] = m;
+        }
+    }
     std::vector<int32_t> elem_of_int(E);
     std::iota(elem_of_int.begin(), elem_of_int.end(), 0);
+    std::vector<int> col(E, 0);
+    if (S.assembly == GFX_ASSEMBLY_COLORED) col = colour_elements(E, d.ibool, G);
+    std::stable_sort(elem_of_int.begin(), elem_of_int.end(), [&](int a, int b) {
+        return col[a] != col[b] ? col[a] < col[b] : key[a] < key[b];
+    });
     if (S.assembly == GFX_ASSEMBLY_COLORED) {
-        const std::vector<int> col = colour_elements(E, d.ibool, G);
-        std::stable_sort(elem_of_int.begin(), elem_of_int.end(),
-                         [&](int a, int b) { return col[a] < col[b]; });
         int b = 0;
         for (int i = 1; i <= E; ++i)
             if (i == E || col[elem_of_int[i]] != col[elem_of_int[b]]) {
@@ -598,12 +655,27 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
     S.kernel = (d.element_kernel == GFX_ELEMENT_BLOCK || !wave_ok) ? GFX_ELEMENT_BLOCK : GFX_ELEMENT_WAVE;
     S.epb = epb ? epb : (S.kernel == GFX_ELEMENT_WAVE ? 4 : 2);
 
-    // ---- nodal state, field-interleaved: U, W, R [G4][nf][3]; a at special nodes [Ga][nf][3] ----
-    const size_t fs = size_t(S.G4) * 3 * S.nf;
+    // ---- nodal state: U, W, R [nf][fstride]; a at special nodes [nf][astride] ----
+    S.fstride = ((S.G4 * 3 + 63) / 64) * 64;
+    S.astride = ((std::max<int64_t>(S.Ga, 1) * 3 + 63) / 64) * 64;
+    const size_t fs = size_t(S.fstride) * S.nf;
     S.U.alloc(fs);
     S.W.alloc(fs);
     S.R.alloc(fs);
-    S.Aacc.alloc(size_t(std::max<int64_t>(S.Ga, 1)) * 3 * S.nf);
+    S.Aacc.alloc(size_t(S.astride) * S.nf);
+    if (S.assembly == GFX_ASSEMBLY_GATHER) {
+        // each node owns the contiguous slots [off[n], off[n+1]); the element points
+        // of a node take its slots in increasing solver order (fixed summation order)
+        std::vector<int32_t> off(S.G4 + 1, 0), slot(P);
+        for (int64_t q = 0; q < P; ++q) ++off[hib[q] + 1];
+        for (int64_t n = 0; n < S.G4; ++n) off[n + 1] += off[n];
+        std::vector<int32_t> fill(off.begin(), off.end() - 1);
+        for (int64_t q = 0; q < P; ++q) slot[q] = fill[hib[q]]++;
+        S.goff.upload(off);
+        S.gslot.upload(slot);
+        S.goff_host = off;
+        S.Rl.alloc(size_t(S.nf) * P * 3);
+    }
 
     // ---- recording cells ----
     S.n_rec = int(S.rec_cells.size());
@@ -725,26 +797,38 @@ extern "C" gfx_status gfx_solver_get_state(const gfx_solver* s, int field, doubl
     return guarded([&] {
         check_field(s, field);
         GFX_HIP(hipStreamSynchronize(s->stream));
-        const int64_t ns = 3 * int64_t(s->nf);
-        const std::vector<float> hU = fetch(s, s->U.p, size_t(s->G4) * ns);
-        const std::vector<float> hW = fetch(s, s->W.p, size_t(s->G4) * ns);
-        const std::vector<float> hR = fetch(s, s->R.p, size_t(s->G4) * ns);
-        const std::vector<float> hA = fetch(s, s->Aacc.p, size_t(s->Ga) * ns);
+        const size_t n3 = size_t(s->G) * 3;
+        const std::vector<float> hU = fetch(s, s->U.p + int64_t(field) * s->fstride, n3);
+        const std::vector<float> hW = fetch(s, s->W.p + int64_t(field) * s->fstride, n3);
+        std::vector<float> hR;
+        if (s->assembly == GFX_ASSEMBLY_GATHER) {
+            // assembled r of the last step: fixed-order sum of the node's slots, as the
+            // node kernels form it
+            const int64_t P = int64_t(s->E) * NP;
+            const std::vector<float> hl = fetch(s, s->Rl.p + int64_t(field) * P * 3, size_t(P) * 3);
+            const std::vector<int32_t> off = s->goff_host;
+            hR.assign(size_t(s->G4) * 3, 0.f);
+            for (int64_t n = 0; n < s->G; ++n)
+                for (int q = off[n]; q < off[n + 1]; ++q)
+                    for (int c = 0; c < 3; ++c) hR[n * 3 + c] += hl[int64_t(q) * 3 + c];
+        } else {
+            hR = fetch(s, s->R.p + int64_t(field) * s->fstride, n3);
+        }
+        const std::vector<float> hA = fetch(s, s->Aacc.p + int64_t(field) * s->astride, size_t(s->Ga) * 3);
         const float dt = float(s->dt);
         for (int g = 0; g < s->G; ++g) {
             const int64_t n = s->node_new_of_old[g];
-            const int64_t o = n * ns + 3 * field;
             for (int c = 0; c < 3; ++c) {
-                const float w = hW[o + c];
-                // regular nodes keep r = m a_n in R (it is not cleared between steps)
+                const float w = hW[n * 3 + c];
+                // regular nodes keep the assembled r = m a_n in R
                 float an;
                 if (n < s->Ga) {
-                    an = hA[o + c];
+                    an = hA[n * 3 + c];
                 } else {
                     const double m = s->mass_node[n];
-                    an = (m > 0.0) ? hR[o + c] * float(1.0 / m) : 0.f;
+                    an = (m > 0.0) ? hR[n * 3 + c] * float(1.0 / m) : 0.f;
                 }
-                if (u) u[int64_t(g) * 3 + c] = hU[o + c] - dt * w;
+                if (u) u[int64_t(g) * 3 + c] = hU[n * 3 + c] - dt * w;
                 if (v) v[int64_t(g) * 3 + c] = w - 0.5f * dt * an;
                 if (a) a[int64_t(g) * 3 + c] = an;
             }
@@ -757,30 +841,38 @@ extern "C" gfx_status gfx_solver_set_state(gfx_solver* s, int field, const doubl
     return guarded([&] {
         check_field(s, field);
         require(u && v && a, "u, v and a are required");
-        GFX_HIP(hipStreamSynchronize(s->stream));
-        const int64_t ns = 3 * int64_t(s->nf);
-        std::vector<float> hU = fetch(s, s->U.p, size_t(s->G4) * ns);
-        std::vector<float> hW = fetch(s, s->W.p, size_t(s->G4) * ns);
-        std::vector<float> hR = fetch(s, s->R.p, size_t(s->G4) * ns);
-        std::vector<float> hA = fetch(s, s->Aacc.p, size_t(s->Ga) * ns);
+        const size_t n3 = size_t(s->G) * 3;
+        std::vector<float> hU(n3), hW(n3), hR(n3), hA(size_t(s->Ga) * 3, 0.f);
         const double dt = s->dt;
         for (int g = 0; g < s->G; ++g) {
             const int64_t n = s->node_new_of_old[g];
             const double m = s->mass_node[n];
-            const int64_t o = n * ns + 3 * field;
             for (int c = 0; c < 3; ++c) {
-                const int64_t i = int64_t(g) * 3 + c;
-                const double ag = (m > 0.0) ? a[i] : 0.0;  // massless nodes carry a = 0
-                hU[o + c] = float(u[i] + dt * v[i] + 0.5 * dt * dt * ag);
-                hW[o + c] = float(v[i] + 0.5 * dt * ag);
-                hR[o + c] = float(ag * m);
-                if (n < s->Ga) hA[o + c] = float(ag);
+                const int64_t o = int64_t(g) * 3 + c;
+                const double ag = (m > 0.0) ? a[o] : 0.0;  // massless nodes carry a = 0
+                hU[n * 3 + c] = float(u[o] + dt * v[o] + 0.5 * dt * dt * ag);
+                hW[n * 3 + c] = float(v[o] + 0.5 * dt * ag);
+                hR[n * 3 + c] = float(ag * m);
+                if (n < s->Ga) hA[n * 3 + c] = float(ag);
             }
         }
-        GFX_HIP(hipMemcpy(s->U.p, hU.data(), hU.size() * 4, hipMemcpyHostToDevice));
-        GFX_HIP(hipMemcpy(s->W.p, hW.data(), hW.size() * 4, hipMemcpyHostToDevice));
-        GFX_HIP(hipMemcpy(s->R.p, hR.data(), hR.size() * 4, hipMemcpyHostToDevice));
-        if (!hA.empty()) GFX_HIP(hipMemcpy(s->Aacc.p, hA.data(), hA.size() * 4, hipMemcpyHostToDevice));
+        GFX_HIP(hipStreamSynchronize(s->stream));
+        const int64_t fo = int64_t(field) * s->fstride;
+        GFX_HIP(hipMemcpy(s->U.p + fo, hU.data(), n3 * 4, hipMemcpyHostToDevice));
+        GFX_HIP(hipMemcpy(s->W.p + fo, hW.data(), n3 * 4, hipMemcpyHostToDevice));
+        GFX_HIP(hipMemcpy(s->R.p + fo, hR.data(), n3 * 4, hipMemcpyHostToDevice));
+        if (!hA.empty())
+            GFX_HIP(hipMemcpy(s->Aacc.p + int64_t(field) * s->astride, hA.data(), hA.size() * 4,
+                              hipMemcpyHostToDevice));
+        if (s->assembly == GFX_ASSEMBLY_GATHER) {  // r = m a in the first slot of each node
+            const int64_t P = int64_t(s->E) * NP;
+            std::vector<float> hl(size_t(P) * 3, 0.f);
+            for (int64_t n = 0; n < s->G; ++n)
+                if (s->goff_host[n + 1] > s->goff_host[n])
+                    for (int c = 0; c < 3; ++c) hl[int64_t(s->goff_host[n]) * 3 + c] = hR[n * 3 + c];
+            GFX_HIP(hipMemcpy(s->Rl.p + int64_t(field) * P * 3, hl.data(), hl.size() * 4,
+                              hipMemcpyHostToDevice));
+        }
     });
 }
 
@@ -858,6 +950,7 @@ extern "C" gfx_status gfx_solver_query(const gfx_solver* s, const char* key, dou
         else if (k == "fields") *value = s->nf;
         else if (k == "nodes") *value = s->G;
         else if (k == "elements") *value = s->E;
+        else if (k == "assembly") *value = s->assembly;
         else throw Error(GFX_ERR_INVALID, "unknown query key: " + k);
     });
 }

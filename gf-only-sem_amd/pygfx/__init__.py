@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "libgfx.so")
 
 GFX_ASSEMBLY_COLORED = 0
 GFX_ASSEMBLY_ATOMIC = 1
+GFX_ASSEMBLY_GATHER = 2
 GFX_ELEMENT_AUTO = 0
 GFX_ELEMENT_BLOCK = 1
 GFX_ELEMENT_WAVE = 2
@@ -80,6 +81,7 @@ class SolverDesc(C.Structure):
         ("n_elem", C.c_int), ("ngll", C.c_int), ("n_rank_node", C.c_int),
         ("dxi_dx", _dp), ("jacobian", _dp), ("lambda_", _dp), ("mu_", _dp),
         ("mass", _dp), ("density", _dp), ("damping", _dp), ("ibool", _ip),
+        ("elem_centroid", _dp),
         ("has_cpml", C.c_int), ("pml_region", _ip), ("coef_alpha", _dp), ("coef_abar", _dp),
         ("dt", C.c_double), ("snapshot_stride", C.c_int),
         ("n_rec_cell", C.c_int), ("rec_cell", _ip),
@@ -191,6 +193,13 @@ class Mesh:
             return arrays[k]
         raise AttributeError(k)
 
+    def centroids(self):
+        """Element centroids [E,3] from the 8 corner nodes."""
+        corners = np.array([0, 4, 20, 24, 100, 104, 120, 124])
+        ib = self.arrays["ibool"].reshape(self.n_elem, 125)[:, corners]
+        xyz = self.arrays["node_xyz"].reshape(-1, 3)
+        return np.ascontiguousarray(xyz[ib].mean(axis=1))
+
     def locate_source(self, x, y, z, max_cells=8):
         nn = self.ngll ** 3
         cells = np.zeros(max_cells, dtype=np.int32)
@@ -294,7 +303,8 @@ class Solver:
 
     def __init__(self, mesh: Mesh, n_fields=1, assembly=GFX_ASSEMBLY_COLORED, elems_per_block=0,
                  record_capacity=None, record_uva=False, has_cpml=True, dt=None,
-                 snapshot_stride=None, rec_cell=None, element_kernel=GFX_ELEMENT_AUTO):
+                 snapshot_stride=None, rec_cell=None, element_kernel=GFX_ELEMENT_AUTO,
+                 morton=True):
         a = mesh.arrays
         self._keep = []
 
@@ -318,6 +328,8 @@ class Solver:
         d.density = _ptr(keep(a["density"]))
         d.damping = _ptr(keep(a["damping"]))
         d.ibool = _ptr(keep(a["ibool"], np.int32), C.c_int32)
+        if morton:
+            d.elem_centroid = _ptr(keep(mesh.centroids()))
         d.has_cpml = 1 if has_cpml else 0
         d.pml_region = _ptr(keep(a["pml_region"], np.int32), C.c_int32)
         d.coef_alpha = _ptr(keep(a["coef_alpha"]))

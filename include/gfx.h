@@ -72,7 +72,9 @@ typedef struct gfx_solver gfx_solver;
 
 typedef enum {
     GFX_ASSEMBLY_COLORED = 0, /* deterministic: element colouring, plain read-modify-write */
-    GFX_ASSEMBLY_ATOMIC = 1   /* float atomics; order not deterministic */
+    GFX_ASSEMBLY_ATOMIC = 1,  /* float atomics; order not deterministic */
+    GFX_ASSEMBLY_GATHER = 2   /* deterministic: element-local residual, per-node
+                                 fixed-order sum fused into the node update */
 } gfx_assembly;
 
 typedef enum {
@@ -94,6 +96,8 @@ typedef struct {
     const double* density;   /* used by C-PML only */
     const double* damping;   /* legacy ramp; used only when has_cpml == 0 */
     const int32_t* ibool;    /* local_cell2global_node */
+    const double* elem_centroid; /* [n_elem*3] optional; elements are then processed
+                                    along a Morton curve of their centroids */
     /* C-PML (pml_coef_abar non-empty in the partition file => has_cpml) */
     int has_cpml;
     const int32_t* pml_region; /* [n_elem] 0 = interior */

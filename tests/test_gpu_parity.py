@@ -107,6 +107,8 @@ def _records(s, field, n_snap):
 
 
 @pytest.mark.parametrize("assembly,kernel", [
+    (pygfx.GFX_ASSEMBLY_GATHER, pygfx.GFX_ELEMENT_WAVE),
+    (pygfx.GFX_ASSEMBLY_GATHER, pygfx.GFX_ELEMENT_BLOCK),
     (pygfx.GFX_ASSEMBLY_COLORED, pygfx.GFX_ELEMENT_WAVE),
     (pygfx.GFX_ASSEMBLY_COLORED, pygfx.GFX_ELEMENT_BLOCK),
     (pygfx.GFX_ASSEMBLY_ATOMIC, pygfx.GFX_ELEMENT_WAVE),
@@ -184,27 +186,32 @@ def test_solver_legacy_damping_path():
     assert mc.rel_l2(u, ref["u"]) <= TOL
 
 
-@pytest.mark.parametrize("kernel", [pygfx.GFX_ELEMENT_WAVE, pygfx.GFX_ELEMENT_BLOCK])
-def test_coloured_assembly_is_deterministic_and_epb_invariant(kernel):
+@pytest.mark.parametrize("assembly,kernel", [
+    (pygfx.GFX_ASSEMBLY_COLORED, pygfx.GFX_ELEMENT_WAVE),
+    (pygfx.GFX_ASSEMBLY_COLORED, pygfx.GFX_ELEMENT_BLOCK),
+    (pygfx.GFX_ASSEMBLY_GATHER, pygfx.GFX_ELEMENT_WAVE),
+    (pygfx.GFX_ASSEMBLY_GATHER, pygfx.GFX_ELEMENT_BLOCK)])
+def test_deterministic_assembly_is_bitwise_reproducible_and_epb_invariant(assembly, kernel):
     mesh = mc.small_layered(duration=0.6)
     cells, w, stf = mc.buried_source(mesh)
     runs = []
     for epb in (1, 2, 4, 8, 2):
-        s = _gpu(mesh, [(cells, w, stf, 2)], epb=epb, kernel=kernel)
+        s = _gpu(mesh, [(cells, w, stf, 2)], epb=epb, kernel=kernel, assembly=assembly)
         runs.append(s.get_state(0))
     for r in runs[1:]:
         for x, y in zip(r, runs[0]):
             assert np.array_equal(x, y)
 
 
-def test_batched_fields_equal_single_runs():
+@pytest.mark.parametrize("assembly", [pygfx.GFX_ASSEMBLY_COLORED, pygfx.GFX_ASSEMBLY_GATHER])
+def test_batched_fields_equal_single_runs(assembly):
     mesh = mc.small_halfspace(duration=1.0)
     cells, w, stf = mc.surface_centre_source(mesh)
     c2, w2, stf2 = mc.buried_source(mesh, frac=(0.45, 0.4, 0.0), depth=1500.0, f0=1.2)
     sources = [(cells, w, stf, 0), (cells, w, stf, 1), (cells, w, stf, 2), (c2, w2, stf2, 2)]
-    sb = _gpu(mesh, sources)
+    sb = _gpu(mesh, sources, assembly=assembly)
     for f, src in enumerate(sources):
-        s1 = _gpu(mesh, [src])
+        s1 = _gpu(mesh, [src], assembly=assembly)
         for x, y in zip(sb.get_state(f), s1.get_state(0)):
             assert np.array_equal(x, y)
         n_snap = s1.n_snapshots
