@@ -106,8 +106,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--fields", type=int, default=24)
-    ap.add_argument("--assembly", choices=["colored", "atomic"], default="colored")
+    ap.add_argument("--assembly", choices=["auto", "colored", "gather", "atomic"], default="auto",
+                    help="auto = the library autotuner picks assembly, element kernel and block size")
     ap.add_argument("--epb", type=int, default=0)
+    ap.add_argument("--kernel", choices=["auto", "block", "wave"], default="auto")
     ap.add_argument("--small", action="store_true", help="40x40x20 mesh instead of 100x100x50 (smoke)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -122,7 +124,24 @@ def main():
     if pygfx.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
-    assembly = pygfx.GFX_ASSEMBLY_ATOMIC if args.assembly == "atomic" else pygfx.GFX_ASSEMBLY_COLORED
+    asm_of = {"colored": pygfx.GFX_ASSEMBLY_COLORED, "gather": pygfx.GFX_ASSEMBLY_GATHER,
+              "atomic": pygfx.GFX_ASSEMBLY_ATOMIC}
+    kern_of = {"auto": pygfx.GFX_ELEMENT_AUTO, "block": pygfx.GFX_ELEMENT_BLOCK,
+               "wave": pygfx.GFX_ELEMENT_WAVE}
+
+    def launch_config(n_fields):
+        if args.assembly == "auto":
+            t0 = time.perf_counter()
+            pick = pygfx.autotune(mesh, n_fields=n_fields, trial_steps=3, record_capacity=0)
+            log(f"autotune F={n_fields}: {pick} ({time.perf_counter() - t0:.1f} s)")
+            return dict(assembly=pick["assembly"], element_kernel=pick["element_kernel"],
+                        elems_per_block=pick["elems_per_block"])
+        return dict(assembly=asm_of[args.assembly], element_kernel=kern_of[args.kernel],
+                    elems_per_block=args.epb)
+
+    names = {pygfx.GFX_ASSEMBLY_COLORED: "colored", pygfx.GFX_ASSEMBLY_GATHER: "gather",
+             pygfx.GFX_ASSEMBLY_ATOMIC: "atomic"}
+    knames = {pygfx.GFX_ELEMENT_BLOCK: "block", pygfx.GFX_ELEMENT_WAVE: "wave", 0: "auto"}
 
     t0 = time.perf_counter()
     dims = (40, 40, 20) if args.small else (100, 100, 50)
@@ -139,9 +158,9 @@ def main():
     result = {}
     plain = None
     if not args.no_plain:
+        lc1 = launch_config(1)
         t0 = time.perf_counter()
-        s1 = pygfx.Solver(mesh, n_fields=1, assembly=assembly, elems_per_block=args.epb,
-                          record_capacity=2)
+        s1 = pygfx.Solver(mesh, n_fields=1, record_capacity=2, **lc1)
         cells, w = mesh.locate_source(cx, cy, 0.0)
         s1.set_source(0, 2, cells, w, stf)
         log(f"plain solver ready ({time.perf_counter() - t0:.1f} s)")
@@ -151,15 +170,19 @@ def main():
                      ms_element_per_step=tp["ms_elem"] / args.steps,
                      ms_node_per_step=tp["ms_node"] / args.steps,
                      node_updates_per_s=G * args.steps / tp["wall_s"],
-                     bytes_per_step=s1.algorithmic_bytes(0))
+                     bytes_per_step=s1.algorithmic_bytes(0),
+                     launch={"assembly": names[int(s1.query("assembly"))],
+                             "element_kernel": knames[int(s1.query("element_kernel"))],
+                             "elems_per_block": int(s1.query("elems_per_block"))})
         plain["hbm_gbs_whole_step"] = plain["bytes_per_step"] / (plain["ms_per_step"] * 1e-3) / 1e9
         log("plain:", json.dumps(plain))
         s1.close()
         del s1
 
     F = args.fields
+    lcb = launch_config(F)
     t0 = time.perf_counter()
-    sb = pygfx.Solver(mesh, n_fields=F, assembly=assembly, elems_per_block=args.epb, record_capacity=2)
+    sb = pygfx.Solver(mesh, n_fields=F, record_capacity=2, **lcb)
     st = stations(mesh, n=max(1, (F + 2) // 3))
     for f in range(F):
         x, y = st[f // 3]
@@ -192,7 +215,10 @@ def main():
                                f"C-PML 5, x/y/z forces at {len(st)} surface stations = {F} fields",
                    "elements": E, "nodes": G, "fields": F, "pml_elements": mesh.n_pml_elem,
                    "recording_cells": int(len(mesh.rec_cell)), "snapshot_stride": mesh.snapshot_stride,
-                   "dt_s": mesh.dt, "assembly": args.assembly, "parallelism": "none (1 GPU)"},
+                   "dt_s": mesh.dt, "parallelism": "none (1 GPU)",
+                   "launch": {"assembly": names[int(sb.query("assembly"))],
+                              "element_kernel": knames[int(sb.query("element_kernel"))],
+                              "elems_per_block": int(sb.query("elems_per_block"))}},
         "node_updates_per_s": G * F * args.steps / tb["wall_s"],
         "plain": plain,
         "batched_vs_plain": (value / plain["value"]) if plain else None,

@@ -936,6 +936,75 @@ extern "C" gfx_status gfx_solver_kernel_time(const gfx_solver* s, const char* na
     });
 }
 
+extern "C" gfx_status gfx_solver_set_launch(gfx_solver* s, int element_kernel, int elems_per_block) {
+    return guarded([&] {
+        require(s != nullptr, "null solver");
+        require(element_kernel >= GFX_ELEMENT_AUTO && element_kernel <= GFX_ELEMENT_WAVE,
+                "unknown element kernel");
+        require(elems_per_block == 0 || elems_per_block == 1 || elems_per_block == 2 ||
+                    elems_per_block == 4 || elems_per_block == 8,
+                "elems_per_block must be 1, 2, 4 or 8");
+        const bool wave_ok = !(s->has_cpml && !s->pml_nodal);
+        if (element_kernel == GFX_ELEMENT_WAVE)
+            require(wave_ok, "the wave element kernel needs the nodal C-PML form");
+        s->kernel = (element_kernel == GFX_ELEMENT_BLOCK || !wave_ok) ? GFX_ELEMENT_BLOCK : GFX_ELEMENT_WAVE;
+        s->epb = elems_per_block ? elems_per_block : (s->kernel == GFX_ELEMENT_WAVE ? 4 : 2);
+    });
+}
+
+extern "C" gfx_status gfx_autotune(const gfx_solver_desc* base, int trial_steps,
+                                   gfx_solver_desc* best, double* best_ms) {
+    return guarded([&] {
+        require(base && best && best_ms, "null argument");
+        require(trial_steps >= 1, "trial_steps must be >= 1");
+        double best_t = 1e300;
+        gfx_solver_desc pick = *base;
+        const int asms[2] = {GFX_ASSEMBLY_COLORED, GFX_ASSEMBLY_GATHER};
+        const int launch[6][2] = {{GFX_ELEMENT_BLOCK, 1}, {GFX_ELEMENT_BLOCK, 2}, {GFX_ELEMENT_BLOCK, 4},
+                                  {GFX_ELEMENT_WAVE, 2},  {GFX_ELEMENT_WAVE, 4},  {GFX_ELEMENT_WAVE, 8}};
+        for (int asm_mode : asms) {
+            gfx_solver_desc d = *base;
+            d.assembly = asm_mode;
+            d.record_capacity = 0;
+            d.element_kernel = GFX_ELEMENT_AUTO;
+            d.elems_per_block = 0;
+            auto S = std::make_unique<gfx_solver>();
+            create(d, *S);
+            for (const auto& lc : launch) {
+                if (lc[0] == GFX_ELEMENT_WAVE && S->has_cpml && !S->pml_nodal) continue;
+                S->kernel = lc[0];
+                S->epb = lc[1];
+                run_steps(*S, 1);  // warm-up
+                hipEvent_t e0, e1;
+                GFX_HIP(hipEventCreate(&e0));
+                GFX_HIP(hipEventCreate(&e1));
+                GFX_HIP(hipEventRecord(e0, S->stream));
+                const bool keep_timing = S->timing;
+                S->timing = false;
+                run_steps(*S, trial_steps);
+                S->timing = keep_timing;
+                GFX_HIP(hipEventRecord(e1, S->stream));
+                GFX_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                GFX_HIP(hipEventElapsedTime(&ms, e0, e1));
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                const double per = double(ms) / trial_steps;
+                if (per < best_t) {
+                    best_t = per;
+                    pick = *base;
+                    pick.assembly = asm_mode;
+                    pick.element_kernel = lc[0];
+                    pick.elems_per_block = lc[1];
+                }
+            }
+            GFX_HIP(hipStreamSynchronize(S->stream));
+        }
+        *best = pick;
+        *best_ms = best_t;
+    });
+}
+
 extern "C" gfx_status gfx_solver_query(const gfx_solver* s, const char* key, double* value) {
     return guarded([&] {
         require(s && key && value, "null argument");

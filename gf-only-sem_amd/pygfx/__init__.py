@@ -118,6 +118,8 @@ def _load():
         "gfx_solver_set_timing": (st, [vp, C.c_int]),
         "gfx_solver_algorithmic_bytes": (C.c_double, [vp, C.c_int]),
         "gfx_solver_query": (st, [vp, C.c_char_p, C.POINTER(C.c_double)]),
+        "gfx_solver_set_launch": (st, [vp, C.c_int, C.c_int]),
+        "gfx_autotune": (st, [C.POINTER(SolverDesc), C.c_int, C.POINTER(SolverDesc), C.POINTER(C.c_double)]),
         "gfx_box_spec_default": (st, [C.POINTER(BoxSpec)]),
         "gfx_mesh_build_box": (st, [C.POINTER(BoxSpec), C.POINTER(C.POINTER(_Mesh))]),
         "gfx_mesh_free": (st, [C.POINTER(_Mesh)]),
@@ -298,6 +300,59 @@ def build_box(spec: BoxSpec | None = None, keep_handle=True, copy=True, **kw) ->
     return mesh
 
 
+def make_desc(mesh: Mesh, keep: list, n_fields=1, assembly=GFX_ASSEMBLY_COLORED, elems_per_block=0,
+              record_capacity=None, record_uva=False, has_cpml=True, dt=None,
+              snapshot_stride=None, rec_cell=None, element_kernel=GFX_ELEMENT_AUTO, morton=True):
+    """Solver descriptor over mesh arrays; arrays it points to are appended to keep."""
+    a = mesh.arrays
+
+    def kp(x, dt_=np.float64):
+        if x is None:
+            return None
+        x = np.ascontiguousarray(x, dtype=dt_)
+        keep.append(x)
+        return x
+
+    rec = mesh.rec_cell if rec_cell is None else np.asarray(rec_cell, np.int32)
+    d = SolverDesc()
+    d.n_elem = mesh.n_elem
+    d.ngll = mesh.ngll
+    d.n_rank_node = mesh.n_node
+    for name in ("dxi_dx", "jacobian", "lambda_", "mu_", "mass", "density", "damping",
+                 "coef_alpha", "coef_abar"):
+        setattr(d, name, _ptr(kp(a[name])))
+    d.ibool = _ptr(kp(a["ibool"], np.int32), C.c_int32)
+    if morton:
+        d.elem_centroid = _ptr(kp(mesh.centroids()))
+    d.has_cpml = 1 if has_cpml else 0
+    d.pml_region = _ptr(kp(a["pml_region"], np.int32), C.c_int32)
+    d.dt = mesh.dt if dt is None else dt
+    d.snapshot_stride = mesh.snapshot_stride if snapshot_stride is None else snapshot_stride
+    rk = kp(rec, np.int32)
+    d.n_rec_cell = rk.size
+    d.rec_cell = _ptr(rk, C.c_int32)
+    d.n_fields = n_fields
+    d.assembly = assembly
+    d.elems_per_block = elems_per_block
+    d.element_kernel = element_kernel
+    if record_capacity is None:
+        record_capacity = 0 if rk.size == 0 else (mesh.nsteps // max(d.snapshot_stride, 1) + 1)
+    d.record_capacity = record_capacity
+    d.record_uva = 1 if record_uva else 0
+    return d
+
+
+def autotune(mesh: Mesh, n_fields=1, trial_steps=3, **kw):
+    """Fastest (assembly, element_kernel, elems_per_block) for this mesh and field count."""
+    keep = []
+    base = make_desc(mesh, keep, n_fields=n_fields, **kw)
+    best = SolverDesc()
+    ms = C.c_double(0)
+    _check(lib.gfx_autotune(C.byref(base), trial_steps, C.byref(best), C.byref(ms)))
+    return dict(assembly=best.assembly, element_kernel=best.element_kernel,
+                elems_per_block=best.elems_per_block, ms_per_step=ms.value)
+
+
 class Solver:
     """Device-resident solver over one mesh and n_fields right-hand sides."""
 
@@ -305,55 +360,20 @@ class Solver:
                  record_capacity=None, record_uva=False, has_cpml=True, dt=None,
                  snapshot_stride=None, rec_cell=None, element_kernel=GFX_ELEMENT_AUTO,
                  morton=True):
-        a = mesh.arrays
         self._keep = []
-
-        def keep(x, dt=np.float64):
-            if x is None:
-                return None
-            x = np.ascontiguousarray(x, dtype=dt)
-            self._keep.append(x)
-            return x
-
-        rec = mesh.rec_cell if rec_cell is None else np.asarray(rec_cell, np.int32)
-        d = SolverDesc()
-        d.n_elem = mesh.n_elem
-        d.ngll = mesh.ngll
-        d.n_rank_node = mesh.n_node
-        d.dxi_dx = _ptr(keep(a["dxi_dx"]))
-        d.jacobian = _ptr(keep(a["jacobian"]))
-        d.lambda_ = _ptr(keep(a["lambda_"]))
-        d.mu_ = _ptr(keep(a["mu_"]))
-        d.mass = _ptr(keep(a["mass"]))
-        d.density = _ptr(keep(a["density"]))
-        d.damping = _ptr(keep(a["damping"]))
-        d.ibool = _ptr(keep(a["ibool"], np.int32), C.c_int32)
-        if morton:
-            d.elem_centroid = _ptr(keep(mesh.centroids()))
-        d.has_cpml = 1 if has_cpml else 0
-        d.pml_region = _ptr(keep(a["pml_region"], np.int32), C.c_int32)
-        d.coef_alpha = _ptr(keep(a["coef_alpha"]))
-        d.coef_abar = _ptr(keep(a["coef_abar"]))
-        d.dt = mesh.dt if dt is None else dt
-        d.snapshot_stride = mesh.snapshot_stride if snapshot_stride is None else snapshot_stride
-        rk = keep(rec, np.int32)
-        d.n_rec_cell = rk.size
-        d.rec_cell = _ptr(rk, C.c_int32)
-        d.n_fields = n_fields
-        d.assembly = assembly
-        d.elems_per_block = elems_per_block
-        d.element_kernel = element_kernel
-        if record_capacity is None:
-            record_capacity = 0 if rk.size == 0 else (mesh.nsteps // max(d.snapshot_stride, 1) + 1)
-        d.record_capacity = record_capacity
-        d.record_uva = 1 if record_uva else 0
+        d = make_desc(mesh, self._keep, n_fields=n_fields, assembly=assembly,
+                      elems_per_block=elems_per_block, record_capacity=record_capacity,
+                      record_uva=record_uva, has_cpml=has_cpml, dt=dt,
+                      snapshot_stride=snapshot_stride, rec_cell=rec_cell,
+                      element_kernel=element_kernel, morton=morton)
+        rk_size = d.n_rec_cell
         h = C.c_void_p()
         _check(lib.gfx_solver_create(C.byref(d), C.byref(h)))
         self._keep = []
         self.h = h
         self.mesh = mesh
         self.n_fields = n_fields
-        self.n_rec = int(rk.size)
+        self.n_rec = int(rk_size)
         self.ngll = mesh.ngll
 
     def close(self):
@@ -416,6 +436,9 @@ class Solver:
         n = C.c_int(0)
         _check(lib.gfx_solver_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def set_launch(self, element_kernel, elems_per_block=0):
+        _check(lib.gfx_solver_set_launch(self.h, element_kernel, elems_per_block))
 
     def query(self, key):
         v = C.c_double(0)

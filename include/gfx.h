@@ -164,6 +164,19 @@ gfx_status gfx_solver_kernel_time(const gfx_solver* s, const char* name, double*
 /* Enable/disable per-kernel event timing (default off). */
 gfx_status gfx_solver_set_timing(gfx_solver* s, int enable);
 
+/* Switch the element kernel form and elements per workgroup of an existing
+ * solver (same contract as the descriptor fields; results do not depend on it). */
+gfx_status gfx_solver_set_launch(gfx_solver* s, int element_kernel, int elems_per_block);
+
+/* Autotuner: times trial_steps steps of every launch configuration
+ * (assembly COLORED / GATHER x element kernel BLOCK / WAVE x elements per
+ * workgroup) for the problem in `base` and writes the fastest into *best
+ * (a copy of base with assembly, element_kernel and elems_per_block set);
+ * *best_ms receives its device time per step.  Sources and recording are
+ * not part of the trial. */
+gfx_status gfx_autotune(const gfx_solver_desc* base, int trial_steps, gfx_solver_desc* best,
+                        double* best_ms);
+
 /* Introspection of the solver's setup.  key: "colors", "pml_nodal" (1 when the
  * C-PML memory is kept per node, 0 when element-local), "pml_elements",
  * "special_nodes", "pml_nodes", "fields", "nodes", "elements". */

@@ -218,10 +218,31 @@ def test_batched_fields_equal_single_runs(assembly):
         assert np.array_equal(_records(sb, f, n_snap), _records(s1, 0, n_snap))
 
 
-def test_reset_and_restart_state_roundtrip():
+@pytest.mark.parametrize("assembly", [pygfx.GFX_ASSEMBLY_COLORED, pygfx.GFX_ASSEMBLY_GATHER])
+def test_set_state_get_state_roundtrip_and_continuation(assembly):
+    """Loading a state and continuing equals running through (restart semantics)."""
     mesh = mc.small_halfspace(duration=1.0)
     cells, w, stf = mc.surface_centre_source(mesh)
-    s = _gpu(mesh, [(cells, w, stf, 2)], nsteps=30)
+    ref = _oracle(mesh, cells, w, stf, 2)
+    s = _gpu(mesh, [(cells, w, stf, 2)], nsteps=40, assembly=assembly)
+    u, v, a = s.get_state(0)
+    s2 = pygfx.Solver(mesh, n_fields=1, assembly=assembly)
+    s2.set_state(0, u, v, a)
+    u2, v2, a2 = s2.get_state(0)
+    for x, y in ((u, u2), (v, v2), (a, a2)):
+        assert mc.rel_l2(y, x) < 1e-6
+    # continue from step 40 with the source shifted in time
+    s2.set_source(0, 2, cells, w, stf[40:])
+    s2.run(mesh.nsteps - 40)
+    uf = s2.get_state(0)[0]
+    assert mc.rel_l2(uf, ref["u"]) <= TOL
+
+
+@pytest.mark.parametrize("assembly", [pygfx.GFX_ASSEMBLY_COLORED, pygfx.GFX_ASSEMBLY_GATHER])
+def test_reset_and_restart_state_roundtrip(assembly):
+    mesh = mc.small_halfspace(duration=1.0)
+    cells, w, stf = mc.surface_centre_source(mesh)
+    s = _gpu(mesh, [(cells, w, stf, 2)], nsteps=30, assembly=assembly)
     u1, v1, a1 = s.get_state(0)
     s.reset()
     assert s.step == 0 and s.n_snapshots == 0
@@ -229,3 +250,20 @@ def test_reset_and_restart_state_roundtrip():
     s.run(30)
     for x, y in zip(s.get_state(0), (u1, v1, a1)):
         assert np.array_equal(x, y)
+
+
+def test_autotuned_configuration_matches_oracle():
+    mesh = mc.small_layered(duration=1.0)
+    cells, w, stf = mc.buried_source(mesh)
+    pick = pygfx.autotune(mesh, n_fields=2, trial_steps=2)
+    assert pick["assembly"] in (pygfx.GFX_ASSEMBLY_COLORED, pygfx.GFX_ASSEMBLY_GATHER)
+    assert pick["element_kernel"] in (pygfx.GFX_ELEMENT_BLOCK, pygfx.GFX_ELEMENT_WAVE)
+    assert pick["elems_per_block"] in (1, 2, 4, 8) and pick["ms_per_step"] > 0
+    ref = _oracle(mesh, cells, w, stf, 1)
+    s = pygfx.Solver(mesh, n_fields=2, assembly=pick["assembly"], element_kernel=pick["element_kernel"],
+                     elems_per_block=pick["elems_per_block"])
+    s.set_source(0, 1, cells, w, stf)
+    s.set_source(1, 1, cells, w, stf)
+    s.run(mesh.nsteps)
+    st = _records(s, 1, ref["strain"].shape[0])
+    assert mc.rel_l2(st, ref["strain"]) <= TOL
