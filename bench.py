@@ -63,12 +63,92 @@ def time_run(pygfx, solver, steps, warmup):
     solver.run(steps)  # returns after hipStreamSynchronize on the solver stream
     t1 = time.perf_counter()
     solver.set_timing(False)
-    ms_el, n_el = solver.kernel_time("element")
-    ms_nd, n_nd = solver.kernel_time("node")
-    ms_rc, n_rc = solver.kernel_time("record")
-    ms_tot, _ = solver.kernel_time("total")
-    return dict(wall_s=t1 - t0, ms_elem=ms_el, n_elem_launch=n_el, ms_node=ms_nd, n_node=n_nd,
-                ms_rec=ms_rc, n_rec=n_rc, ms_events=ms_tot)
+    out = dict(wall_s=t1 - t0)
+    for name in ("element", "node", "node_special", "node_regular", "record", "total"):
+        ms, n = solver.kernel_time(name)
+        out[name] = (ms, n)
+    return out
+
+
+def copy_calibration(n_copies=3, nbytes=1 << 30):
+    """Device-to-device copies of a known size through the HIP runtime, run
+    first under a rocprofv3 --pmc pass: what FETCH_SIZE / WRITE_SIZE read for
+    them calibrates the counters against bytes actually moved."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(a, 1, C.c_size_t(nbytes)):
+            raise RuntimeError("hipMemset failed")
+        for _ in range(n_copies):
+            if hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3):  # hipMemcpyDeviceToDevice
+                raise RuntimeError("hipMemcpy failed")
+        hip.hipDeviceSynchronize()
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    log(f"copy calibration: {n_copies} x {nbytes} B device-to-device")
+
+
+def pmc_summary(fetch_csv, write_csv, n_copies=3, nbytes=1 << 30):
+    """Per-kernel HBM traffic from two rocprofv3 --pmc passes (FETCH_SIZE, then
+    WRITE_SIZE; counter values in KiB per dispatch), each scaled by the factor
+    that makes the calibration copies read as the bytes they moved."""
+    import csv
+
+    def load(path, counter):
+        per = {}
+        for r in csv.DictReader(open(path)):
+            if r.get("Counter_Name", counter) != counter:
+                continue
+            name = r["Kernel_Name"].split("(")[0].replace("void ", "").strip()
+            per.setdefault(name, []).append(float(r["Counter_Value"]) * 1024.0)
+        return per
+
+    fetch = load(fetch_csv, "FETCH_SIZE")
+    write = load(write_csv, "WRITE_SIZE")
+
+    def factor(per, expect):
+        copies = [v for k, v in per.items() if "copybuffer" in k.lower() or "copy_buffer" in k.lower()]
+        tot = sum(sum(v) for v in copies)
+        return (expect * n_copies / tot) if tot > 0 else None
+
+    ff = factor(fetch, nbytes)
+    fw = factor(write, nbytes)
+    lines = [f"calibration: {n_copies} device-to-device copies of {nbytes} B; "
+             f"FETCH_SIZE factor {ff if ff is None else round(ff, 4)}, "
+             f"WRITE_SIZE factor {fw if fw is None else round(fw, 4)}",
+             f"{'kernel':72s} {'dispatches':>10s} {'fetch GB':>10s} {'write GB':>10s} {'total GB':>10s}"]
+    for k in sorted(set(fetch) | set(write)):
+        f = fetch.get(k, [])
+        w = write.get(k, [])
+        fm = (sum(f) / len(f) * (ff or 1.0)) if f else 0.0
+        wm = (sum(w) / len(w) * (fw or 1.0)) if w else 0.0
+        lines.append(f"{k[:72]:72s} {max(len(f), len(w)):10d} {fm / 1e9:10.4f} {wm / 1e9:10.4f} "
+                     f"{(fm + wm) / 1e9:10.4f}")
+    return "\n".join(lines)
+
+
+def traffic_from_profiles(kernel_prefix, launch, fields, dims):
+    """HBM bytes per dispatch of a kernel as committed from the PMC passes
+    (profiles/*.txt lines 'traffic <kernel> <launch> <fields> <mesh> <bytes>');
+    None when no pass covers this kernel, launch configuration and workload."""
+    tag = f"{launch['assembly']}/{launch['element_kernel']}/{launch['elems_per_block']}"
+    mesh = "x".join(str(d) for d in dims)
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
+        return None
+    for fn in sorted(os.listdir(pdir)):
+        if not fn.endswith(".txt"):
+            continue
+        for line in open(os.path.join(pdir, fn)):
+            p = line.split()
+            if len(p) == 6 and p[0] == "traffic" and p[1] == kernel_prefix and p[2] == tag \
+                    and p[3] == str(fields) and p[4] == mesh:
+                return float(p[5])
+    return None
 
 
 def cpu_baseline(seconds=15.0):
@@ -113,7 +193,14 @@ def main():
     ap.add_argument("--small", action="store_true", help="40x40x20 mesh instead of 100x100x50 (smoke)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run 3 device-to-device copies of 1 GiB (PMC counter calibration)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="print per-kernel HBM traffic from two rocprofv3 --pmc passes and exit")
     args = ap.parse_args()
+    if args.pmc_summary:
+        print(pmc_summary(*args.pmc_summary))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would run "
               "its own fields or meshes with no communication); run with --gpus 1.", file=sys.stderr)
@@ -124,6 +211,8 @@ def main():
     if pygfx.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.calibrate_copy:
+        copy_calibration()
     asm_of = {"colored": pygfx.GFX_ASSEMBLY_COLORED, "gather": pygfx.GFX_ASSEMBLY_GATHER,
               "atomic": pygfx.GFX_ASSEMBLY_ATOMIC}
     kern_of = {"auto": pygfx.GFX_ELEMENT_AUTO, "block": pygfx.GFX_ELEMENT_BLOCK,
@@ -167,8 +256,8 @@ def main():
         tp = time_run(pygfx, s1, args.steps, args.warmup)
         plain = dict(value=E * 125 * 1 * args.steps / tp["wall_s"],
                      ms_per_step=1e3 * tp["wall_s"] / args.steps,
-                     ms_element_per_step=tp["ms_elem"] / args.steps,
-                     ms_node_per_step=tp["ms_node"] / args.steps,
+                     ms_element_per_step=tp["element"][0] / args.steps,
+                     ms_node_per_step=tp["node"][0] / args.steps,
                      node_updates_per_s=G * args.steps / tp["wall_s"],
                      bytes_per_step=s1.algorithmic_bytes(0),
                      launch={"assembly": names[int(s1.query("assembly"))],
@@ -193,11 +282,21 @@ def main():
     value = E * 125 * F * args.steps / tb["wall_s"]
     ms_step = 1e3 * tb["wall_s"] / args.steps
 
-    # roofline of the dominant kernel (element stage), HIP events on the solver stream
-    bytes_elem_step = sb.algorithmic_bytes(1)
-    n_launch_per_step = tb["n_elem_launch"] / args.steps
-    ms_elem_launch = tb["ms_elem"] / tb["n_elem_launch"]
-    achieved = (bytes_elem_step / n_launch_per_step) / (ms_elem_launch * 1e-3) / 1e9
+    # roofline of the dominant kernel: the one with the most device time per
+    # step (HIP events on the solver stream around each kernel)
+    launch = {"assembly": names[int(sb.query("assembly"))],
+              "element_kernel": knames[int(sb.query("element_kernel"))],
+              "elems_per_block": int(sb.query("elems_per_block"))}
+    kern = {"element": ("k_element_wave" if launch["element_kernel"] == "wave" else "k_element", 1),
+            "node_special": ("k_node_special", 3), "node_regular": ("k_node_regular", 4)}
+    dom = max(kern, key=lambda k: tb[k][0])
+    kname, which = kern[dom]
+    ms_dom, n_dom = tb[dom]
+    launches_per_step = n_dom / args.steps
+    alg_launch = sb.algorithmic_bytes(which) / launches_per_step
+    ms_launch = ms_dom / n_dom
+    achieved = alg_launch / (ms_launch * 1e-3) / 1e9
+    traffic = traffic_from_profiles(kname, launch, F, dims)
     result = {
         "metric": METRIC,
         "value": value,
@@ -216,20 +315,20 @@ def main():
                    "elements": E, "nodes": G, "fields": F, "pml_elements": mesh.n_pml_elem,
                    "recording_cells": int(len(mesh.rec_cell)), "snapshot_stride": mesh.snapshot_stride,
                    "dt_s": mesh.dt, "parallelism": "none (1 GPU)",
-                   "launch": {"assembly": names[int(sb.query("assembly"))],
-                              "element_kernel": knames[int(sb.query("element_kernel"))],
-                              "elems_per_block": int(sb.query("elems_per_block"))}},
+                   "launch": launch},
         "node_updates_per_s": G * F * args.steps / tb["wall_s"],
         "plain": plain,
         "batched_vs_plain": (value / plain["value"]) if plain else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_element", "algorithmic_bytes_per_launch": bytes_elem_step / n_launch_per_step,
-                     "ms_per_launch": ms_elem_launch, "launches_per_step": n_launch_per_step},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": kname, "algorithmic_bytes_per_launch": alg_launch,
+                     "ms_per_launch": ms_launch, "launches_per_step": launches_per_step},
         "whole_step_hbm_gbs": sb.algorithmic_bytes(0) / (ms_step * 1e-3) / 1e9,
-        "ms_element_per_step": tb["ms_elem"] / args.steps,
-        "ms_node_per_step": tb["ms_node"] / args.steps,
-        "ms_record_per_snapshot": (tb["ms_rec"] / tb["n_rec"]) if tb["n_rec"] else None,
+        "ms_per_step_by_kernel": {k: tb[k][0] / args.steps for k in ("element", "node_special",
+                                                                      "node_regular", "record")},
+        "gbs_by_kernel": {k: sb.algorithmic_bytes(w) / (tb[k][0] / args.steps * 1e-3) / 1e9
+                          for k, (_, w) in kern.items() if tb[k][0] > 0},
+        "ms_record_per_snapshot": (tb["record"][0] / tb["record"][1]) if tb["record"][1] else None,
     }
     sb.close()
     del sb

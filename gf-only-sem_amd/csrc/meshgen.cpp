@@ -21,17 +21,10 @@
 #include <set>
 
 #include "common.hpp"
+#include "mesh.hpp"
 
 namespace gfx {
 namespace {
-
-struct MeshHolder {
-    gfx_mesh m{};
-    std::vector<double> coords, jacobian, dxi_dx, mass, vp, vs, density, lambda_, mu_, damping,
-        node_xyz, coef_alpha, coef_beta, coef_abar;
-    std::vector<int32_t> ibool, pml_region, rec_cell;
-    gfx_box_spec spec{};
-};
 
 // --- deterministic RNG (splitmix64) -------------------------------------
 struct Rng {
@@ -596,24 +589,7 @@ void build(MeshHolder& H) {
         }
     }
 
-    m.coords = H.coords.empty() ? nullptr : H.coords.data();
-    m.jacobian = H.jacobian.data();
-    m.dxi_dx = H.dxi_dx.data();
-    m.mass = H.mass.data();
-    m.vp = H.vp.empty() ? nullptr : H.vp.data();
-    m.vs = H.vs.empty() ? nullptr : H.vs.data();
-    m.density = H.density.data();
-    m.lambda_ = H.lambda_.data();
-    m.mu_ = H.mu_.data();
-    m.damping = H.damping.data();
-    m.ibool = H.ibool.data();
-    m.node_xyz = H.node_xyz.data();
-    m.pml_region = H.pml_region.data();
-    m.coef_alpha = H.coef_alpha.data();
-    m.coef_beta = H.coef_beta.empty() ? nullptr : H.coef_beta.data();
-    m.coef_abar = H.coef_abar.data();
-    m.n_rec_cell = int(H.rec_cell.size());
-    m.rec_cell = H.rec_cell.empty() ? nullptr : H.rec_cell.data();
+    wire_mesh(H);
 }
 
 // Newton solve for natural coordinates of x in a trilinear hex
@@ -719,20 +695,25 @@ extern "C" gfx_status gfx_mesh_locate_source(const gfx_mesh* m, double x, double
                                              int* n_src) {
     return guarded([&] {
         require(m && cells && weights && n_src, "null argument");
-        const MeshHolder& H = *reinterpret_cast<const MeshHolder*>(m);
-        const gfx_box_spec& s = H.spec;
-        const GLL gll(s.order);
-        const int ng = m->ngll, nn = ng * ng * ng, nx = m->nx, ny = m->ny;
+        const GLL gll(m->ngll - 1);
+        const int ng = m->ngll, nn = ng * ng * ng;
         const double xs[3] = {x, y, z};
         // surface mode when z is the free surface (source_locator.py:273-275)
         const bool buried = std::abs(z - m->zmin) > 1e-8 * 1.0;
+        const int idx = ng - 1;
+        const int corner[8] = {0, idx, idx * ng, idx * ng + idx, idx * ng * ng, idx * ng * ng + idx,
+                               (idx * ng + idx) * ng, (idx * ng + idx) * ng + idx};
+        const double ztol = 1e-6 * std::max(1.0, std::abs(m->zmax - m->zmin));
         std::vector<int64_t> cand;
         for (int64_t e = 0; e < m->n_elem; ++e) {
-            const int ez = int(e / (int64_t(nx) * ny));
             if (!buried) {
-                if (ez == 0) cand.push_back(e);  // cells with a face on z = zmin
+                // cells with a face on z = zmin: four corners on the surface
+                int on = 0;
+                for (int a = 0; a < 8; ++a)
+                    on += std::abs(m->node_xyz[int64_t(m->ibool[e * nn + corner[a]]) * 3 + 2] - m->zmin) <= ztol;
+                if (on >= 4) cand.push_back(e);
             } else {
-                if (H.pml_region[e] != 0) continue;
+                if (m->pml_region && m->pml_region[e] != 0) continue;
                 double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
                 for (int n = 0; n < nn; ++n) {
                     const double* X = &m->node_xyz[int64_t(m->ibool[e * nn + n]) * 3];
@@ -752,7 +733,6 @@ extern "C" gfx_status gfx_mesh_locate_source(const gfx_mesh* m, double x, double
         double total = 0;
         for (int64_t e : cand) {
             double cv[8][3];
-            const int idx = ng - 1;
             const int cidx[8][3] = {{0, 0, 0}, {idx, 0, 0}, {idx, idx, 0}, {0, idx, 0},
                                     {0, 0, idx}, {idx, 0, idx}, {idx, idx, idx}, {0, idx, idx}};
             for (int a = 0; a < 8; ++a) {
@@ -801,3 +781,29 @@ extern "C" gfx_status gfx_ricker(double f0, double t0, double amplitude, double 
         }
     });
 }
+
+namespace gfx {
+
+void wire_mesh(MeshHolder& H) {
+    gfx_mesh& m = H.m;
+    m.coords = H.coords.empty() ? nullptr : H.coords.data();
+    m.jacobian = H.jacobian.data();
+    m.dxi_dx = H.dxi_dx.data();
+    m.mass = H.mass.data();
+    m.vp = H.vp.empty() ? nullptr : H.vp.data();
+    m.vs = H.vs.empty() ? nullptr : H.vs.data();
+    m.density = H.density.data();
+    m.lambda_ = H.lambda_.data();
+    m.mu_ = H.mu_.data();
+    m.damping = H.damping.data();
+    m.ibool = H.ibool.data();
+    m.node_xyz = H.node_xyz.data();
+    m.pml_region = H.pml_region.empty() ? nullptr : H.pml_region.data();
+    m.coef_alpha = H.coef_alpha.empty() ? nullptr : H.coef_alpha.data();
+    m.coef_beta = H.coef_beta.empty() ? nullptr : H.coef_beta.data();
+    m.coef_abar = H.coef_abar.empty() ? nullptr : H.coef_abar.data();
+    m.n_rec_cell = int(H.rec_cell.size());
+    m.rec_cell = H.rec_cell.empty() ? nullptr : H.rec_cell.data();
+}
+
+}  // namespace gfx

@@ -105,8 +105,8 @@ struct gfx_solver {
     int step = 0;
     // timing
     bool timing = false;
-    double t_elem = 0, t_node = 0, t_rec = 0, t_total = 0;
-    int n_elem_launch = 0, n_node_launch = 0, n_rec_launch = 0, n_total_launch = 0;
+    double t_elem = 0, t_spec = 0, t_reg = 0, t_rec = 0, t_total = 0;
+    int n_elem_launch = 0, n_spec_launch = 0, n_reg_launch = 0, n_rec_launch = 0, n_total_launch = 0;
 
     ~gfx_solver() {
         if (stream) (void)hipStreamDestroy(stream);
@@ -309,10 +309,12 @@ void run_steps(gfx_solver& S, int nsteps) {
     const int rec_grid = (S.n_rec + REPB - 1) / REPB;
 
     std::vector<hipEvent_t> ev;
+    constexpr size_t EV = 4;
     if (S.timing) {
-        // ev[0] = start; step t: ev[3t+1] after elements, ev[3t+2] after nodes,
-        // ev[3t+3] after recording (= start of step t+1)
-        ev.resize(size_t(nsteps) * 3 + 1);
+        // ev[0] = start; step t: ev[4t+1] after elements, ev[4t+2] after the
+        // special-node kernel, ev[4t+3] after the regular-node kernel, ev[4t+4]
+        // after recording (= start of step t+1)
+        ev.resize(size_t(nsteps) * EV + 1);
         for (auto& e : ev) GFX_HIP(hipEventCreate(&e));
         GFX_HIP(hipEventRecord(ev[0], S.stream));
     }
@@ -323,8 +325,9 @@ void run_steps(gfx_solver& S, int nsteps) {
             dev::k_zero<<<zero_grid, 256, 0, S.stream>>>(reinterpret_cast<float4*>(S.R.p), r4);
         for (const auto& rg : S.ranges) launch_element(S, ea, rg.first, rg.second - rg.first);
         GFX_HIP(hipGetLastError());
-        if (S.timing) GFX_HIP(hipEventRecord(ev[size_t(t) * 3 + 1], S.stream));
+        if (S.timing) GFX_HIP(hipEventRecord(ev[size_t(t) * EV + 1], S.stream));
         if (spec_grid > 0) dev::k_node_special<<<spec_grid, 256, 0, S.stream>>>(na);
+        if (S.timing) GFX_HIP(hipEventRecord(ev[size_t(t) * EV + 2], S.stream));
         if (reg_grid > 0) {
             if (na.gather)
                 dev::k_node_regular<true><<<reg_grid, 256, 0, S.stream>>>(na);
@@ -332,7 +335,7 @@ void run_steps(gfx_solver& S, int nsteps) {
                 dev::k_node_regular<false><<<reg_grid, 256, 0, S.stream>>>(na);
         }
         GFX_HIP(hipGetLastError());
-        if (S.timing) GFX_HIP(hipEventRecord(ev[size_t(t) * 3 + 2], S.stream));
+        if (S.timing) GFX_HIP(hipEventRecord(ev[size_t(t) * EV + 3], S.stream));
         if (S.stride > 0 && S.step % S.stride == 0 && S.rec_cap > 0 && S.n_rec > 0) {
             const int slot = S.n_snap % S.rec_cap;
             const int64_t nrp = int64_t(S.n_rec) * NP;
@@ -343,27 +346,31 @@ void run_steps(gfx_solver& S, int nsteps) {
             did_rec[t] = 1;
         }
         if (S.stride > 0 && S.step % S.stride == 0) ++S.n_snap;
-        if (S.timing) GFX_HIP(hipEventRecord(ev[size_t(t) * 3 + 3], S.stream));
+        if (S.timing) GFX_HIP(hipEventRecord(ev[size_t(t) * EV + 4], S.stream));
         ++S.step;
     }
     GFX_HIP(hipStreamSynchronize(S.stream));
     if (S.timing) {
-        S.t_elem = S.t_node = S.t_rec = S.t_total = 0;
-        S.n_elem_launch = S.n_node_launch = S.n_rec_launch = 0;
+        S.t_elem = S.t_spec = S.t_reg = S.t_rec = S.t_total = 0;
+        S.n_elem_launch = S.n_spec_launch = S.n_reg_launch = S.n_rec_launch = 0;
         for (int t = 0; t < nsteps; ++t) {
-            float a = 0, b = 0, c = 0;
-            GFX_HIP(hipEventElapsedTime(&a, ev[size_t(t) * 3], ev[size_t(t) * 3 + 1]));
-            GFX_HIP(hipEventElapsedTime(&b, ev[size_t(t) * 3 + 1], ev[size_t(t) * 3 + 2]));
-            GFX_HIP(hipEventElapsedTime(&c, ev[size_t(t) * 3 + 2], ev[size_t(t) * 3 + 3]));
+            float a = 0, b = 0, c = 0, d = 0;
+            const size_t o = size_t(t) * EV;
+            GFX_HIP(hipEventElapsedTime(&a, ev[o], ev[o + 1]));
+            GFX_HIP(hipEventElapsedTime(&b, ev[o + 1], ev[o + 2]));
+            GFX_HIP(hipEventElapsedTime(&c, ev[o + 2], ev[o + 3]));
+            GFX_HIP(hipEventElapsedTime(&d, ev[o + 3], ev[o + 4]));
             S.t_elem += a;
-            S.t_node += b;
-            S.t_rec += c;
+            S.t_spec += b;
+            S.t_reg += c;
+            S.t_rec += d;
             S.n_elem_launch += int(S.ranges.size());
-            S.n_node_launch += (spec_grid > 0) + (reg_grid > 0);
+            S.n_spec_launch += spec_grid > 0;
+            S.n_reg_launch += reg_grid > 0;
             S.n_rec_launch += did_rec[t];
         }
         float tot = 0;
-        GFX_HIP(hipEventElapsedTime(&tot, ev[0], ev[size_t(nsteps) * 3]));
+        GFX_HIP(hipEventElapsedTime(&tot, ev[0], ev[size_t(nsteps) * EV]));
         S.t_total = tot;
         S.n_total_launch = nsteps;
         for (auto& e : ev) (void)hipEventDestroy(e);
@@ -790,6 +797,16 @@ extern "C" gfx_status gfx_solver_run(gfx_solver* s, int n_steps) {
 }
 
 extern "C" int gfx_solver_step_index(const gfx_solver* s) { return s ? s->step : -1; }
+
+extern "C" gfx_status gfx_solver_set_step(gfx_solver* s, int step) {
+    return guarded([&] {
+        require(s != nullptr, "null solver");
+        require(step >= 0, "step must be >= 0");
+        s->step = step;
+        // snapshots are taken after the updates with index % stride == 0
+        s->n_snap = s->stride > 0 ? (step + s->stride - 1) / s->stride : 0;
+    });
+}
 extern "C" int gfx_solver_n_snapshots(const gfx_solver* s) { return s ? s->n_snap : -1; }
 
 extern "C" gfx_status gfx_solver_get_state(const gfx_solver* s, int field, double* u, double* v,
@@ -906,6 +923,132 @@ extern "C" gfx_status gfx_solver_get_record(const gfx_solver* s, int field, int 
     });
 }
 
+namespace gfx {
+namespace {
+
+// Checkpoint image: a header of int64 words, then the device arrays that
+// carry state from one step to the next, as raw FP32.
+constexpr int64_t CKPT_MAGIC = 0x31544b5043584647LL;  // "GFXCPKT1"
+constexpr int CKPT_WORDS = 16;
+
+uint64_t layout_signature(const gfx_solver& S) {
+    uint64_t h = 1469598103934665603ULL;  // FNV-1a over the solver's numbering
+    auto mix = [&](const void* p, size_t n) {
+        const unsigned char* c = static_cast<const unsigned char*>(p);
+        for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ULL;
+    };
+    mix(S.node_new_of_old.data(), S.node_new_of_old.size() * sizeof(int32_t));
+    mix(S.int_of_elem.data(), S.int_of_elem.size() * sizeof(int32_t));
+    const int64_t k[6] = {S.E, S.G, S.nf, S.Ga, S.Gpml, int64_t(S.assembly)};
+    mix(k, sizeof(k));
+    return h;
+}
+
+size_t checkpoint_bytes(const gfx_solver& S) {
+    const size_t fl = size_t(S.nf) * S.fstride;
+    return CKPT_WORDS * 8 + 4 * (3 * fl + size_t(S.nf) * S.astride + S.pmem.n + S.pnew.n);
+}
+
+}  // namespace
+}  // namespace gfx
+
+extern "C" gfx_status gfx_solver_checkpoint_size(const gfx_solver* s, size_t* bytes) {
+    return guarded([&] {
+        require(s && bytes, "null argument");
+        *bytes = checkpoint_bytes(*s);
+    });
+}
+
+extern "C" gfx_status gfx_solver_checkpoint_save(const gfx_solver* s, void* host, size_t bytes) {
+    return guarded([&] {
+        require(s && host, "null argument");
+        require(bytes >= checkpoint_bytes(*s), "checkpoint buffer too small");
+        GFX_HIP(hipStreamSynchronize(s->stream));
+        int64_t* hdr = static_cast<int64_t*>(host);
+        std::memset(hdr, 0, CKPT_WORDS * 8);
+        hdr[0] = CKPT_MAGIC;
+        hdr[1] = s->E;
+        hdr[2] = s->G;
+        hdr[3] = s->nf;
+        hdr[4] = s->assembly;
+        hdr[5] = s->step;
+        hdr[6] = s->n_snap;
+        hdr[7] = int64_t(layout_signature(*s));
+        hdr[8] = int64_t(s->pmem.n);
+        hdr[9] = int64_t(s->pnew.n);
+        float* p = reinterpret_cast<float*>(static_cast<char*>(host) + CKPT_WORDS * 8);
+        const size_t fl = size_t(s->nf) * s->fstride, al = size_t(s->nf) * s->astride;
+        auto get = [&](const float* d, size_t n) {
+            if (n) GFX_HIP(hipMemcpy(p, d, n * 4, hipMemcpyDeviceToHost));
+            p += n;
+        };
+        get(s->U.p, fl);
+        get(s->W.p, fl);
+        get(s->Aacc.p, al);
+        get(s->pmem.p, s->pmem.n);
+        get(s->pnew.p, s->pnew.n);
+        // assembled residual of the last step (what get_state reads for a at
+        // regular nodes): R itself, or the fixed-order slot sums
+        if (s->assembly != GFX_ASSEMBLY_GATHER) {
+            get(s->R.p, fl);
+        } else {
+            const int64_t P = int64_t(s->E) * NP;
+            std::fill(p, p + fl, 0.f);
+            for (int f = 0; f < s->nf; ++f) {
+                const std::vector<float> hl = fetch(s, s->Rl.p + int64_t(f) * P * 3, size_t(P) * 3);
+                float* r = p + int64_t(f) * s->fstride;
+                for (int64_t n = 0; n < s->G; ++n)
+                    for (int q = s->goff_host[n]; q < s->goff_host[n + 1]; ++q)
+                        for (int c = 0; c < 3; ++c) r[n * 3 + c] += hl[int64_t(q) * 3 + c];
+            }
+        }
+    });
+}
+
+extern "C" gfx_status gfx_solver_checkpoint_load(gfx_solver* s, const void* host, size_t bytes) {
+    return guarded([&] {
+        require(s && host, "null argument");
+        require(bytes >= size_t(CKPT_WORDS) * 8, "checkpoint image too small");
+        const int64_t* hdr = static_cast<const int64_t*>(host);
+        require(hdr[0] == CKPT_MAGIC, "not a solver checkpoint image");
+        require(hdr[1] == s->E && hdr[2] == s->G && hdr[3] == s->nf && hdr[4] == s->assembly &&
+                    hdr[7] == int64_t(layout_signature(*s)) && hdr[8] == int64_t(s->pmem.n) &&
+                    hdr[9] == int64_t(s->pnew.n),
+                "checkpoint was taken from a solver with another mesh, field count or assembly");
+        require(bytes >= checkpoint_bytes(*s), "checkpoint image truncated");
+        const float* p = reinterpret_cast<const float*>(static_cast<const char*>(host) + CKPT_WORDS * 8);
+        const size_t fl = size_t(s->nf) * s->fstride, al = size_t(s->nf) * s->astride;
+        GFX_HIP(hipStreamSynchronize(s->stream));
+        auto put = [&](float* d, size_t n) {
+            if (n) GFX_HIP(hipMemcpy(d, p, n * 4, hipMemcpyHostToDevice));
+            p += n;
+        };
+        put(s->U.p, fl);
+        put(s->W.p, fl);
+        put(s->Aacc.p, al);
+        put(s->pmem.p, s->pmem.n);
+        put(s->pnew.p, s->pnew.n);
+        if (s->assembly != GFX_ASSEMBLY_GATHER) {
+            put(s->R.p, fl);
+        } else {
+            // the sum in the first slot of each node, zeros elsewhere: get_state
+            // then forms the same r
+            const int64_t P = int64_t(s->E) * NP;
+            std::vector<float> hl(size_t(P) * 3);
+            for (int f = 0; f < s->nf; ++f) {
+                std::fill(hl.begin(), hl.end(), 0.f);
+                const float* r = p + int64_t(f) * s->fstride;
+                for (int64_t n = 0; n < s->G; ++n)
+                    if (s->goff_host[n + 1] > s->goff_host[n])
+                        for (int c = 0; c < 3; ++c) hl[int64_t(s->goff_host[n]) * 3 + c] = r[n * 3 + c];
+                GFX_HIP(hipMemcpy(s->Rl.p + int64_t(f) * P * 3, hl.data(), hl.size() * 4, hipMemcpyHostToDevice));
+            }
+        }
+        s->step = int(hdr[5]);
+        s->n_snap = int(hdr[6]);
+    });
+}
+
 extern "C" gfx_status gfx_solver_set_timing(gfx_solver* s, int enable) {
     return guarded([&] {
         require(s != nullptr, "null solver");
@@ -922,8 +1065,14 @@ extern "C" gfx_status gfx_solver_kernel_time(const gfx_solver* s, const char* na
             *ms = s->t_elem;
             *launches = s->n_elem_launch;
         } else if (n == "node") {
-            *ms = s->t_node;
-            *launches = s->n_node_launch;
+            *ms = s->t_spec + s->t_reg;
+            *launches = s->n_spec_launch + s->n_reg_launch;
+        } else if (n == "node_special") {
+            *ms = s->t_spec;
+            *launches = s->n_spec_launch;
+        } else if (n == "node_regular") {
+            *ms = s->t_reg;
+            *launches = s->n_reg_launch;
         } else if (n == "record") {
             *ms = s->t_rec;
             *launches = s->n_rec_launch;
@@ -1037,7 +1186,12 @@ extern "C" double gfx_solver_algorithmic_bytes(const gfx_solver* s, int which) {
     // and per field read and write the memory (9) and "new" (3) values.
     double node = 4.0 * G + F * 60.0 * G + F * 24.0 * Ga;
     if (s->has_cpml && s->pml_nodal) node += 56.0 * Gp + F * 96.0 * Gp;
+    // the node stage split by kernel: special nodes [0, Ga) carry a and the
+    // C-PML terms, regular nodes [Ga, G) only r, u~ and w
+    const double node_reg = (4.0 + F * 60.0) * (G - Ga);
     if (which == 1) return elem;
     if (which == 2) return node;
+    if (which == 3) return node - node_reg;
+    if (which == 4) return node_reg;
     return elem + node;
 }

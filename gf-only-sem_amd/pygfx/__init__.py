@@ -91,6 +91,15 @@ class SolverDesc(C.Structure):
     ]
 
 
+class ForwardOptions(C.Structure):
+    _fields_ = [
+        ("workdir", C.c_char_p), ("directions", C.c_char_p), ("stations", C.c_char_p),
+        ("resume", C.c_int), ("record_uva", C.c_int), ("assembly", C.c_int),
+        ("element_kernel", C.c_int), ("elems_per_block", C.c_int), ("ring", C.c_int),
+        ("stop_after_step", C.c_int), ("restart_stride", C.c_int), ("verbose", C.c_int),
+    ]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
@@ -126,6 +135,27 @@ def _load():
         "gfx_mesh_locate_source": (st, [C.POINTER(_Mesh), C.c_double, C.c_double, C.c_double,
                                         C.c_int, _ip, _dp, C.POINTER(C.c_int)]),
         "gfx_ricker": (st, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, _dp]),
+        "gfx_solver_set_step": (st, [vp, C.c_int]),
+        "gfx_solver_checkpoint_size": (st, [vp, C.POINTER(C.c_size_t)]),
+        "gfx_solver_checkpoint_save": (st, [vp, vp, C.c_size_t]),
+        "gfx_solver_checkpoint_load": (st, [vp, vp, C.c_size_t]),
+        "gfx_run_forward": (C.c_int, [C.c_char_p, C.c_int, C.c_int]),
+        "gfx_forward_options_default": (st, [C.POINTER(ForwardOptions)]),
+        "gfx_run_forward_ex": (st, [C.POINTER(ForwardOptions)]),
+        "gfx_mesh_write_inputs": (st, [C.POINTER(_Mesh), C.c_char_p, C.c_int, C.c_double, C.c_double,
+                                       C.c_double, C.c_int, _dp, C.c_char_p, C.c_int]),
+        "gfx_mesh_read_inputs": (st, [C.c_char_p, C.POINTER(C.POINTER(_Mesh))]),
+        "gfx_record_write_step": (st, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, _dp, _dp, _dp, _dp]),
+        "gfx_write_stations": (st, [C.c_char_p, C.POINTER(_Mesh), C.c_int, _dp]),
+        "gfx_h5_available": (C.c_int, []),
+        "gfx_h5_describe": (st, [C.c_char_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+        "gfx_h5_read_f64": (st, [C.c_char_p, C.c_char_p, _dp, C.c_size_t, C.POINTER(C.c_size_t)]),
+        "gfx_h5_read_attr_f64": (st, [C.c_char_p, C.c_char_p, C.c_char_p, _dp]),
+        "gfx_h5_read_attr_str": (st, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_size_t]),
+        "gfx_h5_write": (st, [C.c_char_p, C.c_int, C.c_char_p, C.c_char_p, C.c_int, C.POINTER(C.c_uint64), vp]),
+        "gfx_h5_write_attr": (st, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, vp]),
+        "gfx_config_summary": (st, [C.c_char_p, _dp, C.c_int, C.c_char_p, C.c_char_p, C.c_size_t]),
+        "gfx_partition_summary": (st, [C.c_char_p, C.c_int, _dp, C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -254,6 +284,18 @@ def build_box(spec: BoxSpec | None = None, keep_handle=True, copy=True, **kw) ->
         spec = make_spec(**kw)
     h = C.POINTER(_Mesh)()
     _check(lib.gfx_mesh_build_box(C.byref(spec), C.byref(h)))
+    return _wrap_mesh(h, keep_handle, copy)
+
+
+def read_inputs(directory, keep_handle=True, copy=True) -> Mesh:
+    """Load config.h5 + partitions/partition_{r}.h5 (the reference's input
+    files, merged as read_partition_all does) as a Mesh."""
+    h = C.POINTER(_Mesh)()
+    _check(lib.gfx_mesh_read_inputs(os.fsencode(directory), C.byref(h)))
+    return _wrap_mesh(h, keep_handle, copy)
+
+
+def _wrap_mesh(h, keep_handle, copy) -> Mesh:
     m = h.contents
     E, ng = m.n_elem, m.ngll
     nn = ng ** 3
@@ -298,6 +340,128 @@ def build_box(spec: BoxSpec | None = None, keep_handle=True, copy=True, **kw) ->
             mesh.arrays = {k: (None if v is None else v.copy()) for k, v in arrays.items()}
         mesh.close()
     return mesh
+
+
+def write_inputs(mesh: Mesh, directory, src_xyz, stf, n_parts=1, precision="float32", restart_stride=0):
+    """Write config.h5 + partitions/partition_{r}.h5 for this mesh and a point
+    force at src_xyz (the files the reference solver reads)."""
+    stf = np.ascontiguousarray(stf, np.float64)
+    _check(lib.gfx_mesh_write_inputs(mesh._handle, os.fsencode(directory), n_parts, *map(float, src_xyz),
+                                     stf.size, _ptr(stf), precision.encode(), restart_stride))
+
+
+def write_stations(path, mesh: Mesh, xyz):
+    xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+    _check(lib.gfx_write_stations(os.fsencode(path), mesh._handle, xyz.shape[0], _ptr(xyz)))
+
+
+def write_record(inputs_dir, out_dir, direction, step, strain, disp=None, vel=None, acc=None):
+    """One record file in the reference schema for the recording map of inputs_dir."""
+    arrs = [None if x is None else np.ascontiguousarray(x, np.float64) for x in (strain, disp, vel, acc)]
+    _check(lib.gfx_record_write_step(os.fsencode(inputs_dir), os.fsencode(out_dir), direction.encode(), step,
+                                     *[_ptr(a) for a in arrs]))
+
+
+def run_forward(workdir=None, directions="z", stations=None, resume=False, record_uva=True, assembly=-1,
+                element_kernel=GFX_ELEMENT_AUTO, elems_per_block=0, ring=8, stop_after_step=-1,
+                restart_stride=-1, verbose=False):
+    """The forward driver over the reference's files (gfx_run_forward_ex)."""
+    o = ForwardOptions()
+    _check(lib.gfx_forward_options_default(C.byref(o)))
+    keep = [None if workdir is None else os.fsencode(workdir), directions.encode(),
+            None if stations is None else os.fsencode(stations)]
+    o.workdir, o.directions, o.stations = keep
+    o.resume = 1 if resume else 0
+    o.record_uva = 1 if record_uva else 0
+    o.assembly = assembly
+    o.element_kernel = element_kernel
+    o.elems_per_block = elems_per_block
+    o.ring = ring
+    o.stop_after_step = stop_after_step
+    o.restart_stride = restart_stride
+    o.verbose = 1 if verbose else 0
+    _check(lib.gfx_run_forward_ex(C.byref(o)))
+
+
+def h5_available() -> bool:
+    return bool(lib.gfx_h5_available())
+
+
+def h5_describe(path) -> str:
+    need = C.c_size_t(0)
+    _check(lib.gfx_h5_describe(os.fsencode(path), None, 0, C.byref(need)))
+    buf = C.create_string_buffer(need.value + 1)
+    _check(lib.gfx_h5_describe(os.fsencode(path), buf, need.value + 1, C.byref(need)))
+    return buf.value.decode()
+
+
+def h5_read(path, dataset) -> np.ndarray:
+    n = C.c_size_t(0)
+    _check(lib.gfx_h5_read_f64(os.fsencode(path), dataset.encode(), _ptr(None), 0, C.byref(n)))
+    out = np.zeros(n.value)
+    _check(lib.gfx_h5_read_f64(os.fsencode(path), dataset.encode(), _ptr(out), out.size, C.byref(n)))
+    return out
+
+
+_H5T = {np.dtype(np.float64): "f64", np.dtype(np.float32): "f32", np.dtype(np.int32): "i32",
+        np.dtype(np.int64): "i64"}
+
+
+def h5_write(path, dataset, data, append=True):
+    """Write a numpy array as a dataset (parent groups created)."""
+    a = np.ascontiguousarray(data)
+    dims = (C.c_uint64 * max(a.ndim, 1))(*(a.shape if a.ndim else (1,)))
+    _check(lib.gfx_h5_write(os.fsencode(path), 1 if append else 0, dataset.encode(), _H5T[a.dtype].encode(),
+                            max(a.ndim, 1), dims, a.ctypes.data_as(C.c_void_p)))
+
+
+def h5_write_attr(path, obj, name, value):
+    """Scalar attribute: str, int (int32) or float (float64)."""
+    if isinstance(value, str):
+        b = C.create_string_buffer(value.encode())
+        _check(lib.gfx_h5_write_attr(os.fsencode(path), obj.encode(), name.encode(), b"str", C.cast(b, C.c_void_p)))
+        return
+    if isinstance(value, (int, np.integer)) and not isinstance(value, bool):
+        v, t = C.c_int32(int(value)), b"i32"
+    else:
+        v, t = C.c_double(float(value)), b"f64"
+    _check(lib.gfx_h5_write_attr(os.fsencode(path), obj.encode(), name.encode(), t, C.cast(C.byref(v), C.c_void_p)))
+
+
+_CONFIG_KEYS = ("polynomial_order", "solver_dt", "output_dt_s", "snapshot_stride", "nsteps", "cfl_safety",
+                "xmin", "xmax", "ymin", "ymax", "zmin", "zmax", "n_stf_t", "n_stf_values", "source_x",
+                "source_y", "source_z", "n_src_cell", "record_depth_max_m", "record_depth_actual_m",
+                "restart_stride", "restart_dt_s", "log_stride", "n_tilex")
+_PART_KEYS = ("n_local_cell", "ngll", "n_rank_node", "has_cpml", "has_recording", "n_rec_cell", "n_unique_gll",
+              "n_local_cell_ids", "n_exchange_neighbors", "n_jacobian", "n_local_cell2rank_node",
+              "n_local_cell2global_node")
+
+
+def read_config(path) -> dict:
+    """What the config reader makes of config.h5 (defaults and legacy layout included)."""
+    out = np.zeros(24)
+    t, p = C.create_string_buffer(1024), C.create_string_buffer(64)
+    _check(lib.gfx_config_summary(os.fsencode(path), _ptr(out), 24, t, p, 64))
+    d = dict(zip(_CONFIG_KEYS, out.tolist()))
+    d["title"], d["snapshot_precision"] = t.value.decode(), p.value.decode()
+    return d
+
+
+def read_partition(path, merge=False) -> dict:
+    """What the partition reader makes of one file, or of a directory merged."""
+    out = np.zeros(12)
+    _check(lib.gfx_partition_summary(os.fsencode(path), 1 if merge else 0, _ptr(out), 12))
+    return dict(zip(_PART_KEYS, out.tolist()))
+
+
+def h5_attr(path, obj, name, string=False):
+    if string:
+        buf = C.create_string_buffer(4096)
+        _check(lib.gfx_h5_read_attr_str(os.fsencode(path), obj.encode(), name.encode(), buf, 4096))
+        return buf.value.decode()
+    v = C.c_double(0)
+    _check(lib.gfx_h5_read_attr_f64(os.fsencode(path), obj.encode(), name.encode(), C.byref(v)))
+    return v.value
 
 
 def make_desc(mesh: Mesh, keep: list, n_fields=1, assembly=GFX_ASSEMBLY_COLORED, elems_per_block=0,
@@ -447,3 +611,18 @@ class Solver:
 
     def algorithmic_bytes(self, which=0):
         return float(lib.gfx_solver_algorithmic_bytes(self.h, which))
+
+    def set_step(self, step):
+        _check(lib.gfx_solver_set_step(self.h, int(step)))
+
+    def checkpoint(self) -> bytes:
+        """Complete time-stepping state as an opaque image (bit-exact resume)."""
+        n = C.c_size_t(0)
+        _check(lib.gfx_solver_checkpoint_size(self.h, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        _check(lib.gfx_solver_checkpoint_save(self.h, buf, n.value))
+        return buf.raw
+
+    def restore(self, image: bytes):
+        buf = C.create_string_buffer(image, len(image))
+        _check(lib.gfx_solver_checkpoint_load(self.h, buf, len(image)))

@@ -140,6 +140,9 @@ gfx_status gfx_solver_run(gfx_solver* s, int n_steps);
 
 /* Current step counter (number of steps taken). */
 int gfx_solver_step_index(const gfx_solver* s);
+/* Set the step counter (steps already taken, e.g. after gfx_solver_set_state
+ * on restart) and the snapshot counter that follows from it. */
+gfx_status gfx_solver_set_step(gfx_solver* s, int step);
 
 /* Copy a field's nodal state to host FP64 [n_rank_node*3]; NULL skips
  * (cuda_copy_state_to_host, forward/share/src/cuda_step.cu:260-279). */
@@ -156,9 +159,21 @@ int gfx_solver_n_snapshots(const gfx_solver* s);
 gfx_status gfx_solver_get_record(const gfx_solver* s, int field, int snap, double* strain,
                                  double* disp, double* vel, double* acc);
 
+/* Complete time-stepping state as an opaque host image (predicted
+ * displacement, velocity, acceleration where kept, C-PML memory, the last
+ * assembled residual, step and snapshot counters).  Loading it into a solver
+ * built from the same inputs, field count and assembly continues the run bit
+ * for bit; any other solver rejects it (GFX_ERR_INVALID).  This goes beyond
+ * the reference's restart file (forward/share/src/restart.cpp:120-152), which
+ * keeps neither the C-PML memory nor the exact step state. */
+gfx_status gfx_solver_checkpoint_size(const gfx_solver* s, size_t* bytes);
+gfx_status gfx_solver_checkpoint_save(const gfx_solver* s, void* host, size_t bytes);
+gfx_status gfx_solver_checkpoint_load(gfx_solver* s, const void* host, size_t bytes);
+
 /* Per-kernel device time of the last gfx_solver_run (HIP events on the
- * solver's stream).  name is "element", "node", "record" or "total";
- * returns milliseconds summed over the run and the launch count. */
+ * solver's stream).  name is "element", "node" (both node kernels),
+ * "node_special", "node_regular", "record" or "total"; returns milliseconds
+ * summed over the run and the launch count. */
 gfx_status gfx_solver_kernel_time(const gfx_solver* s, const char* name, double* ms,
                                   int* launches);
 /* Enable/disable per-kernel event timing (default off). */
@@ -183,7 +198,8 @@ gfx_status gfx_autotune(const gfx_solver_desc* base, int trial_steps, gfx_solver
 gfx_status gfx_solver_query(const gfx_solver* s, const char* key, double* value);
 
 /* Algorithmic HBM bytes per step for the current problem (each array counted
- * once per step).  which: 0 = whole step, 1 = element kernel, 2 = node kernel. */
+ * once per step).  which: 0 = whole step, 1 = element stage, 2 = node stage
+ * (both node kernels), 3 = special-node kernel, 4 = regular-node kernel. */
 double gfx_solver_algorithmic_bytes(const gfx_solver* s, int which);
 
 /* -------------------------------------------------------------------------
