@@ -13,11 +13,11 @@
  *
  * Pinning (what checks this restatement, by function):
  *   gfo_gll_*             reference tests/test_gll.cpp known answers + closed
- *                         forms (N=4 nodes/weights), tests/test_oracle_gll.py
+ *                         forms (N=4 nodes/weights)
  *   gfo_element_residual  tests/test_element.cpp invariants (rigid translation,
  *                         rotation, force balance), brute-force weak form by an
- *                         independent numpy quadrature, symmetry of K,
- *                         tests/test_oracle_element.py
+ *                         independent numpy Voigt-matrix quadrature, the six
+ *                         rigid-body null modes of K, the mu <= 0 skip
  *   gfo_scatter/gather    tests/test_assembly.cpp exact sums
  *   gfo_newmark_*         tests/test_newmark.cpp closed forms, discrete
  *                         eigenmode cos(n*theta) of the whole step
@@ -28,6 +28,7 @@
  *                         sequences; no reference test exists for C-PML
  *                         (parity partially pinned: formula-level only)
  *   gfo_full_strain       exact strain of affine displacement fields
+ * The checks live in tests/test_oracle.py (one test per line above).
  */
 #ifndef GFX_ORACLE_H
 #define GFX_ORACLE_H

@@ -16,6 +16,7 @@ def _declared(header):
 
 
 @pytest.mark.parametrize("header,lib", [("include/gfx.h", "gf-only-sem_amd/libgfx.so"),
+                                        ("include/gfx_io.h", "gf-only-sem_amd/libgfx.so"),
                                         ("oracle/gfx_oracle.h", "oracle/liboracle.so")])
 def test_library_exports_every_declared_symbol(header, lib):
     names = _declared(header)
@@ -49,3 +50,21 @@ def test_bad_arguments_rejected_before_device_use():
         pygfx.element_residual(np.zeros(9), np.zeros(1), np.zeros(1), np.zeros(1), np.zeros(1),
                                np.zeros(1), 1, np.zeros(3), r)
     assert ei.value.status == 1
+
+
+def test_cpp_glue_header_compiles_and_links(tmp_path):
+    """include/gfx_backend_hip.hpp is what a C++ caller includes; it must build
+    with the host compiler alone and link against libgfx.so."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("no g++")
+    src = tmp_path / "use.cpp"
+    src.write_text('#include "gfx_backend_hip.hpp"\n'
+                   'int main() { return gfx_hip::run_forward("q") == 1 ? 0 : 1; }\n')
+    exe = tmp_path / "use"
+    pkg = os.path.join(ROOT, "gf-only-sem_amd")
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src),
+                    "-o", str(exe), "-L", pkg, "-lgfx", "-Wl,-rpath," + pkg], check=True)
+    # a bad direction is rejected before any device or file access
+    assert subprocess.run([str(exe)]).returncode == 0
