@@ -151,6 +151,78 @@ def traffic_from_profiles(kernel_prefix, launch, fields, dims):
     return None
 
 
+def uneven_specs(pygfx, n=16, seed=4, h=600.0):
+    """16 independent boxes (seeded): n_target log-uniform in [8k, 200k] elements,
+    nx = ny = round((2n)^(1/3)), nz = round(nx/2), 0.6 km hexes; 2-4 layers with
+    Vs ~ U(1500, 4500) increasing with depth, Vp = 1.73 Vs, rho ~ U(2300, 3300);
+    C-PML U{3..6}; run length U(5, 30) s; Ricker f0 = 1.5 Hz."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(n):
+        nt = float(np.exp(rng.uniform(np.log(8e3), np.log(2e5))))
+        nx = int(round((2 * nt) ** (1 / 3)))
+        nz = max(4, int(round(nx / 2)))
+        nl = int(rng.integers(2, 5))
+        vs = np.sort(rng.uniform(1500.0, 4500.0, nl))
+        rho = rng.uniform(2300.0, 3300.0, nl)
+        lz = h * nz
+        tops = np.sort(rng.uniform(0.1, 0.7, nl - 1)) * lz
+        layers = [(float(tops[i]), 1.73 * vs[i], vs[i], rho[i]) for i in range(nl - 1)]
+        layers.append((0.0, 1.73 * vs[-1], vs[-1], rho[-1]))
+        pml = int(rng.integers(3, 7))
+        dur = float(rng.uniform(5.0, 30.0))
+        specs.append(pygfx.make_spec(nx=nx, ny=nx, nz=nz, lx=h * nx, ly=h * nx, lz=lz, layers=layers,
+                                     pml_xmin=pml, pml_xmax=pml, pml_ymin=pml, pml_ymax=pml, pml_zmax=pml,
+                                     f0_pml=1.5, cfl_safety=0.5, output_dt=0.05, duration=dur,
+                                     record_depth_max=2000.0, want_coords=0, want_beta=0))
+    return specs
+
+
+def uneven_workload(pygfx, steps, warmup):
+    """x, y and z forces at the surface centre of each of 16 uneven meshes:
+    the meshes advanced together (one captured graph with a branch per mesh)
+    against the same solvers advanced one after another, kernel by kernel."""
+    t0 = time.perf_counter()
+    meshes = [pygfx.build_box(sp, copy=False) for sp in uneven_specs(pygfx)]
+    solvers = []
+    for m in meshes:
+        s = pygfx.Solver(m, n_fields=3, assembly=pygfx.GFX_ASSEMBLY_GATHER, record_capacity=2)
+        cx = 0.5 * (m.bounds[0] + m.bounds[1])
+        cy = 0.5 * (m.bounds[2] + m.bounds[3])
+        cells, w = m.locate_source(cx, cy, 0.0)
+        stf = pygfx.ricker(1.5, 1.0, 1e20, m.dt, m.nsteps)
+        for f in range(3):
+            s.set_source(f, f, cells, w, stf)
+        solvers.append(s)
+    E = sum(m.n_elem for m in meshes)
+    log(f"uneven: {len(meshes)} meshes, {E} elements ({min(m.n_elem for m in meshes)}..."
+        f"{max(m.n_elem for m in meshes)}), dt {min(m.dt for m in meshes):.4g}..{max(m.dt for m in meshes):.4g} s "
+        f"({time.perf_counter() - t0:.1f} s setup)")
+    for s in solvers:
+        s.run(warmup)
+    t0 = time.perf_counter()
+    for s in solvers:
+        s.run(steps)
+    t_seq = time.perf_counter() - t0
+    batch = pygfx.Batch(solvers)
+    batch.run(warmup)
+    t0 = time.perf_counter()
+    batch.run(steps)
+    t_bat = time.perf_counter() - t0
+    batch.close()
+    pts = E * 125 * 3 * steps
+    out = {"meshes": len(meshes), "elements": E, "fields_per_mesh": 3, "steps": steps,
+           "elements_min": min(m.n_elem for m in meshes), "elements_max": max(m.n_elem for m in meshes),
+           "value": pts / t_bat, "unit": "GLL-point updates/s",
+           "ms_per_step_batched": 1e3 * t_bat / steps, "ms_per_step_sequential": 1e3 * t_seq / steps,
+           "speedup_vs_sequential": t_seq / t_bat}
+    for s in solvers:
+        s.close()
+    for m in meshes:
+        m.close()
+    return out
+
+
 def cpu_baseline(seconds=15.0):
     """FP64 CPU restatement of the reference solver (oracle/), one core, on a
     20 x 20 x 10 sub-box of the same 0.6 km layered discretisation."""
@@ -190,9 +262,11 @@ def main():
                     help="auto = the library autotuner picks assembly, element kernel and block size")
     ap.add_argument("--epb", type=int, default=0)
     ap.add_argument("--kernel", choices=["auto", "block", "wave"], default="auto")
+    ap.add_argument("--node-groups", type=int, default=0, help="node-kernel field groups (1, 2, 4, 8)")
     ap.add_argument("--small", action="store_true", help="40x40x20 mesh instead of 100x100x50 (smoke)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the 16-mesh uneven batch line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run 3 device-to-device copies of 1 GiB (PMC counter calibration)")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -224,9 +298,9 @@ def main():
             pick = pygfx.autotune(mesh, n_fields=n_fields, trial_steps=3, record_capacity=0)
             log(f"autotune F={n_fields}: {pick} ({time.perf_counter() - t0:.1f} s)")
             return dict(assembly=pick["assembly"], element_kernel=pick["element_kernel"],
-                        elems_per_block=pick["elems_per_block"])
+                        elems_per_block=pick["elems_per_block"], node_field_groups=pick["node_field_groups"])
         return dict(assembly=asm_of[args.assembly], element_kernel=kern_of[args.kernel],
-                    elems_per_block=args.epb)
+                    elems_per_block=args.epb, node_field_groups=args.node_groups)
 
     names = {pygfx.GFX_ASSEMBLY_COLORED: "colored", pygfx.GFX_ASSEMBLY_GATHER: "gather",
              pygfx.GFX_ASSEMBLY_ATOMIC: "atomic"}
@@ -262,7 +336,8 @@ def main():
                      bytes_per_step=s1.algorithmic_bytes(0),
                      launch={"assembly": names[int(s1.query("assembly"))],
                              "element_kernel": knames[int(s1.query("element_kernel"))],
-                             "elems_per_block": int(s1.query("elems_per_block"))})
+                             "elems_per_block": int(s1.query("elems_per_block")),
+                             "node_field_groups": int(s1.query("node_field_groups"))})
         plain["hbm_gbs_whole_step"] = plain["bytes_per_step"] / (plain["ms_per_step"] * 1e-3) / 1e9
         log("plain:", json.dumps(plain))
         s1.close()
@@ -286,7 +361,8 @@ def main():
     # step (HIP events on the solver stream around each kernel)
     launch = {"assembly": names[int(sb.query("assembly"))],
               "element_kernel": knames[int(sb.query("element_kernel"))],
-              "elems_per_block": int(sb.query("elems_per_block"))}
+              "elems_per_block": int(sb.query("elems_per_block")),
+              "node_field_groups": int(sb.query("node_field_groups"))}
     kern = {"element": ("k_element_wave" if launch["element_kernel"] == "wave" else "k_element", 1),
             "node_special": ("k_node_special", 3), "node_regular": ("k_node_regular", 4)}
     dom = max(kern, key=lambda k: tb[k][0])
@@ -332,6 +408,9 @@ def main():
     }
     sb.close()
     del sb
+    if not args.no_uneven and not args.small:
+        result["uneven_batch"] = uneven_workload(pygfx, args.steps, args.warmup)
+        log("uneven:", json.dumps(result["uneven_batch"]))
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline()

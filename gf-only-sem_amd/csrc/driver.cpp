@@ -287,6 +287,7 @@ int run_forward_impl(const gfx_forward_options& o) {
             d.assembly = best.assembly;
             d.element_kernel = best.element_kernel;
             d.elems_per_block = best.elems_per_block;
+            d.node_field_groups = best.node_field_groups;
             std::ostringstream os;
             os << "  autotune: assembly=" << d.assembly << " element_kernel=" << d.element_kernel
                << " elems_per_block=" << d.elems_per_block << " (" << ms << " ms/step)";

@@ -65,7 +65,8 @@ struct ElemArgs {
     const int4* __restrict__ src_ent;     // (field, dir, weight offset, 0)
     const float* __restrict__ src_w;
     const float* __restrict__ stf;        // [nf][nstf]
-    int nstf, step;
+    int nstf;
+    const int* __restrict__ ctr;          // device counters: [0] step index, [1] snapshots taken
     float D[25];
 };
 
@@ -94,6 +95,8 @@ struct NodeArgs {
 };
 
 struct RecArgs {
+    const int* __restrict__ ctr;  // device counters: [0] step index, [1] snapshots taken
+    int stride, cap;              // snapshot stride; ring capacity (snapshots)
     int n_rec;
     const int32_t* __restrict__ ibool;  // [n_rec*125] special node ids of recording cells
     const float* __restrict__ dd;       // [9][n_rec*125] unscaled dxi_dx
@@ -103,8 +106,8 @@ struct RecArgs {
     const float* __restrict__ W;
     const float* __restrict__ Aacc;
     float dt;
-    float* __restrict__ strain;  // [nf][n_rec*125][6]
-    float* __restrict__ uva;     // [nf][n_rec*125][9] or nullptr
+    float* __restrict__ strain;  // ring [cap][nf][n_rec*125][6]
+    float* __restrict__ uva;     // ring [cap][nf][n_rec*125][9] or nullptr
     float D[25];
 };
 
@@ -209,10 +212,21 @@ __device__ __forceinline__ void emit_residual(const ElemArgs& A, int f, int64_t 
     }
 }
 
-__device__ __forceinline__ void add_sources(const ElemArgs& A, int f, int sb, int se, int p, float* rv) {
+__device__ __forceinline__ void add_sources(const ElemArgs& A, int step, int f, int sb, int se, int p,
+                                            float* rv) {
     for (int s = sb; s < se; ++s) {  // solver.cpp:576-605
         const int4 en = A.src_ent[s];
-        if (en.x == f && A.step < A.nstf) rv[en.y] += A.stf[int64_t(f) * A.nstf + A.step] * A.src_w[en.z + p];
+        if (en.x == f && step < A.nstf) rv[en.y] += A.stf[int64_t(f) * A.nstf + step] * A.src_w[en.z + p];
+    }
+}
+
+// Closes a step: the step counter advances, and the snapshot counter when this
+// step was a snapshot step (step % stride == 0), as the host mirrors it.
+__global__ void k_advance(int* ctr, int stride) {
+    if (threadIdx.x == 0) {
+        const int st = ctr[0];
+        if (stride > 0 && st % stride == 0) ctr[1] = ctr[1] + 1;
+        ctr[0] = st + 1;
     }
 }
 
@@ -258,6 +272,7 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
         sb = A.src_off[e];
         se = A.src_off[e + 1];
     }
+    const int step = A.ctr[0];
     float un[3] = {0.f, 0.f, 0.f};  // field 0; later fields are prefetched one iteration ahead
     if (valid) {
         const float* ut = A.U + node * 3;
@@ -349,7 +364,7 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
                     pnew[c * A.Ppml + q] = newv;
                 }
             }
-            add_sources(A, f, sb, se, p, rv);
+            add_sources(A, step, f, sb, se, p, rv);
             emit_residual<MODE>(A, f, node, rslot, rold, rv);
         }
     }
@@ -382,6 +397,7 @@ __global__ void __launch_bounds__(64 * WPB) k_element_wave(ElemArgs A, int e0, i
     int64_t node[2], ptq[2];
     bool first[2];
     const int sb = A.src_off[e], se = A.src_off[e + 1];
+    const int step = A.ctr[0];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         const int p = vq[q] ? pq[q] : 0;
@@ -450,7 +466,7 @@ __global__ void __launch_bounds__(64 * WPB) k_element_wave(ElemArgs A, int e0, i
             const float Dtk[5] = {ck.x, ck.y, ck.z, ck.w, sDc[k][4]};
             float rv[3];
             point_div(&sf[wv][0][0], 128, Dti, Dtj, Dtk, i, j, k, rv);
-            add_sources(A, f, sb, se, pq[q], rv);
+            add_sources(A, step, f, sb, se, pq[q], rv);
             emit_residual<MODE>(A, f, node[q], ptq[q], rold[q], rv);
         }
     }
@@ -469,8 +485,14 @@ __device__ __forceinline__ void gather_r(const NodeArgs& A, int f, int kb, int k
 }
 
 // Special nodes [0, Ga): C-PML terms, legacy damping, massless rule, a_n kept.
+// FG field groups per workgroup: the 256 lanes cover 256/FG nodes, and lane
+// group fy handles fields fy, fy + FG, ... (each wavefront stays on one field
+// group, so its loads remain contiguous over nodes).
+template <int FG>
 __global__ void __launch_bounds__(256) k_node_special(NodeArgs A) {
-    const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    constexpr int NPB = 256 / FG;
+    const int fy = threadIdx.x / NPB;
+    const int64_t g = int64_t(blockIdx.x) * NPB + (threadIdx.x - fy * NPB);
     if (g >= A.Ga) return;
     const float mi = A.minv[g];
     const float dt = A.dt, hdt = 0.5f * dt, hdt2 = 0.5f * dt * dt;
@@ -488,7 +510,7 @@ __global__ void __launch_bounds__(256) k_node_special(NodeArgs A) {
         kb = A.off[g];
         ke = A.off[g + 1];
     }
-    for (int f = 0; f < A.nf; ++f) {
+    for (int f = fy; f < A.nf; f += FG) {
         const int64_t o = int64_t(f) * A.fstride + g * 3;
         const int64_t oa = int64_t(f) * A.astride + g * 3;
         float U[3], W[3], R[3], an[3], un[3], vn[3];
@@ -545,9 +567,11 @@ __global__ void __launch_bounds__(256) k_node_special(NodeArgs A) {
 
 // Regular nodes [Ga, G4): a = r/m, w += dt a, u~ += dt w.  4 nodes per lane,
 // 16-byte accesses (Ga and the field stride are multiples of 4 nodes).
-template <bool GATHER>
+template <bool GATHER, int FG>
 __global__ void __launch_bounds__(256) k_node_regular(NodeArgs A) {
-    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    constexpr int NPB = 256 / FG;
+    const int fy = threadIdx.x / NPB;
+    const int64_t t = int64_t(blockIdx.x) * NPB + (threadIdx.x - fy * NPB);
     const int64_t g = A.Ga + 4 * t;
     if (g >= A.G4) return;
     const float dt = A.dt;
@@ -558,7 +582,7 @@ __global__ void __launch_bounds__(256) k_node_regular(NodeArgs A) {
 #pragma unroll
         for (int q = 0; q < 5; ++q) kb[q] = A.off[g + q];
     }
-    for (int f = 0; f < A.nf; ++f) {
+    for (int f = fy; f < A.nf; f += FG) {
         const int64_t o = int64_t(f) * A.fstride + g * 3;
         float4* U4 = reinterpret_cast<float4*>(A.U + o);
         float4* W4 = reinterpret_cast<float4*>(A.W + o);
@@ -606,6 +630,14 @@ template <int EPB>
 __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64) k_record(RecArgs A) {
     __shared__ float sD[25];
     __shared__ float su[EPB][3][NP];
+    // snapshot steps only (step % stride == 0), into ring slot n_snap % cap;
+    // the condition is uniform over the grid
+    const int st = A.ctr[0];
+    if (A.stride <= 0 || A.cap <= 0 || st % A.stride != 0) return;
+    const int64_t nrp_all = int64_t(A.n_rec) * NP;
+    const int64_t slot = A.ctr[1] % A.cap;
+    float* const strain_slot = A.strain + slot * A.nf * nrp_all * 6;
+    float* const uva_slot = A.uva ? A.uva + slot * A.nf * nrp_all * 9 : nullptr;
     const int tid = threadIdx.x;
     if (tid < 25) sD[tid] = A.D[tid];
     const int le = tid / NP;
@@ -655,15 +687,15 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64) k_record(RecArgs 
                 for (int m = 0; m < 3; ++m)
                     gr[c][m] = du[c][0] * dd[m * 3 + 0] + du[c][1] * dd[m * 3 + 1] +
                                du[c][2] * dd[m * 3 + 2];
-            float* s6 = A.strain + (int64_t(f) * nrp + pt) * 6;
+            float* s6 = strain_slot + (int64_t(f) * nrp + pt) * 6;
             s6[0] = gr[0][0];
             s6[1] = gr[1][1];
             s6[2] = gr[2][2];
             s6[3] = 0.5f * (gr[0][1] + gr[1][0]);
             s6[4] = 0.5f * (gr[0][2] + gr[2][0]);
             s6[5] = 0.5f * (gr[1][2] + gr[2][1]);
-            if (A.uva) {
-                float* o9 = A.uva + (int64_t(f) * nrp + pt) * 9;
+            if (uva_slot) {
+                float* o9 = uva_slot + (int64_t(f) * nrp + pt) * 9;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const float a = A.Aacc[ga + c];

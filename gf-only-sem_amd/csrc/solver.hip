@@ -75,6 +75,12 @@ struct gfx_solver {
     int64_t Ga = 0, Gpml = 0, G4 = 0, Ppml = 0;
     int64_t fstride = 0, astride = 0;
     int assembly = GFX_ASSEMBLY_COLORED, epb = 2, kernel = GFX_ELEMENT_WAVE;
+    int node_fg = 1;  // field groups per node-kernel workgroup (1, 2, 4, 8)
+    gfx::DevBuf<int> ctr;  // device step / snapshot counters
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int gen = 0;  // bumped whenever captured launches become stale
     double dt = 0;
     int stride = 1;
     bool has_cpml = false, pml_nodal = true;
@@ -109,6 +115,8 @@ struct gfx_solver {
     int n_elem_launch = 0, n_spec_launch = 0, n_reg_launch = 0, n_rec_launch = 0, n_total_launch = 0;
 
     ~gfx_solver() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -163,6 +171,7 @@ void init_state(gfx_solver& S) {
     S.Rl.zero(S.stream);
     S.pmem.zero(S.stream);
     S.pnew.zero(S.stream);
+    S.ctr.zero(S.stream);
     GFX_HIP(hipStreamSynchronize(S.stream));
     S.step = 0;
     S.n_snap = 0;
@@ -226,12 +235,49 @@ void launch_element(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
     }
 }
 
-void run_steps(gfx_solver& S, int nsteps) {
+void launch_node_special(gfx_solver& S, const dev::NodeArgs& na, int grid) {
+    switch (S.node_fg) {
+        case 2: dev::k_node_special<2><<<grid, 256, 0, S.stream>>>(na); break;
+        case 4: dev::k_node_special<4><<<grid, 256, 0, S.stream>>>(na); break;
+        case 8: dev::k_node_special<8><<<grid, 256, 0, S.stream>>>(na); break;
+        default: dev::k_node_special<1><<<grid, 256, 0, S.stream>>>(na); break;
+    }
+}
+
+template <bool GATHER>
+void launch_node_regular_t(gfx_solver& S, const dev::NodeArgs& na, int grid) {
+    switch (S.node_fg) {
+        case 2: dev::k_node_regular<GATHER, 2><<<grid, 256, 0, S.stream>>>(na); break;
+        case 4: dev::k_node_regular<GATHER, 4><<<grid, 256, 0, S.stream>>>(na); break;
+        case 8: dev::k_node_regular<GATHER, 8><<<grid, 256, 0, S.stream>>>(na); break;
+        default: dev::k_node_regular<GATHER, 1><<<grid, 256, 0, S.stream>>>(na); break;
+    }
+}
+
+void launch_node_regular(gfx_solver& S, const dev::NodeArgs& na, int grid) {
+    if (na.gather)
+        launch_node_regular_t<true>(S, na, grid);
+    else
+        launch_node_regular_t<false>(S, na, grid);
+}
+
+struct StepArgs {
+    dev::ElemArgs ea{};
+    dev::NodeArgs na{};
+    dev::RecArgs ra{};
+    int spec_grid = 0, reg_grid = 0, zero_grid = 0, rec_grid = 0, rec_threads = 0;
+    int64_t r4 = 0;
+    bool recording = false;
+};
+constexpr int REPB = 2;  // recording cells per workgroup
+
+StepArgs make_step_args(gfx_solver& S) {
+    StepArgs a;
     const double theta = 1.0 / 8.0;  // pml.cpp:35, coefficients pml.cpp:63-64
     const float c1 = float((1.0 - 2.0 * theta) * 0.5 * S.dt);
     const float c2 = float((1.0 - theta) * 0.5 * S.dt * S.dt);
 
-    dev::ElemArgs ea{};
+    dev::ElemArgs& ea = a.ea;
     ea.G = S.G9.p;
     ea.lam = S.lam.p;
     ea.mu = S.mu.p;
@@ -260,9 +306,10 @@ void run_steps(gfx_solver& S, int nsteps) {
     ea.src_w = S.src_w.p;
     ea.stf = S.stf.p;
     ea.nstf = S.nstf;
+    ea.ctr = S.ctr.p;
     std::memcpy(ea.D, S.D, sizeof(S.D));
 
-    dev::NodeArgs na{};
+    dev::NodeArgs& na = a.na;
     na.G = S.G;
     na.Ga = S.Ga;
     na.Gpml = S.Gpml;
@@ -288,12 +335,16 @@ void run_steps(gfx_solver& S, int nsteps) {
     na.c1 = c1;
     na.c2 = c2;
     na.dt = float(S.dt);
-    const int spec_grid = int((S.Ga + 255) / 256);
-    const int reg_grid = int(((S.G4 - S.Ga) / 4 + 255) / 256);
-    const int64_t r4 = int64_t(S.nf) * S.fstride / 4;
-    const int zero_grid = int((r4 + 255) / 256);
+    const int fg = S.node_fg;
+    a.spec_grid = int((S.Ga + 256 / fg - 1) / (256 / fg));
+    a.reg_grid = int(((S.G4 - S.Ga) / 4 + 256 / fg - 1) / (256 / fg));
+    a.r4 = int64_t(S.nf) * S.fstride / 4;
+    a.zero_grid = int((a.r4 + 255) / 256);
 
-    dev::RecArgs ra{};
+    dev::RecArgs& ra = a.ra;
+    ra.ctr = S.ctr.p;
+    ra.stride = S.stride;
+    ra.cap = S.rec_cap;
     ra.n_rec = S.n_rec;
     ra.ibool = S.rec_ibool.p;
     ra.dd = S.rec_dd.p;
@@ -304,10 +355,74 @@ void run_steps(gfx_solver& S, int nsteps) {
     ra.W = S.W.p;
     ra.Aacc = S.Aacc.p;
     ra.dt = float(S.dt);
+    ra.strain = S.rec_strain.p;
+    ra.uva = S.rec_uva_on ? S.rec_uva.p : nullptr;
     std::memcpy(ra.D, S.D, sizeof(S.D));
-    constexpr int REPB = 2;
-    const int rec_grid = (S.n_rec + REPB - 1) / REPB;
+    a.recording = S.stride > 0 && S.rec_cap > 0 && S.n_rec > 0;
+    a.rec_grid = (S.n_rec + REPB - 1) / REPB;
+    a.rec_threads = ((REPB * NP + 63) / 64) * 64;
+    return a;
+}
 
+// Enqueues one whole step on S.stream.  The step index and snapshot counter
+// live on the device (S.ctr), so the same enqueue can be captured once into a
+// graph and replayed.  `record` launches the record kernel (which itself does
+// nothing off snapshot steps); ev, when given, receives 4 events: after the
+// elements, the special nodes, the regular nodes and the end of the step.
+void enqueue_step(gfx_solver& S, const StepArgs& a, bool record, hipEvent_t* ev) {
+    if (S.assembly == GFX_ASSEMBLY_ATOMIC)
+        dev::k_zero<<<a.zero_grid, 256, 0, S.stream>>>(reinterpret_cast<float4*>(S.R.p), a.r4);
+    for (const auto& rg : S.ranges) launch_element(S, a.ea, rg.first, rg.second - rg.first);
+    GFX_HIP(hipGetLastError());
+    if (ev) GFX_HIP(hipEventRecord(ev[0], S.stream));
+    if (a.spec_grid > 0) launch_node_special(S, a.na, a.spec_grid);
+    if (ev) GFX_HIP(hipEventRecord(ev[1], S.stream));
+    if (a.reg_grid > 0) launch_node_regular(S, a.na, a.reg_grid);
+    GFX_HIP(hipGetLastError());
+    if (ev) GFX_HIP(hipEventRecord(ev[2], S.stream));
+    if (record && a.recording && a.rec_grid > 0)
+        dev::k_record<REPB><<<a.rec_grid, a.rec_threads, 0, S.stream>>>(a.ra);
+    dev::k_advance<<<1, 64, 0, S.stream>>>(S.ctr.p, S.stride);
+    GFX_HIP(hipGetLastError());
+    if (ev) GFX_HIP(hipEventRecord(ev[3], S.stream));
+}
+
+// Host mirror of k_advance.
+void advance_host(gfx_solver& S) {
+    if (S.stride > 0 && S.step % S.stride == 0) ++S.n_snap;
+    ++S.step;
+}
+
+void drop_graph(gfx_solver& S) {
+    ++S.gen;
+    if (S.graph_exec) (void)hipGraphExecDestroy(S.graph_exec);
+    if (S.graph) (void)hipGraphDestroy(S.graph);
+    S.graph_exec = nullptr;
+    S.graph = nullptr;
+}
+
+// One step captured into a graph (record kernel included; it checks the
+// snapshot condition on the device).  Recaptured after any launch change.
+void ensure_graph(gfx_solver& S) {
+    if (S.graph_exec) return;
+    const StepArgs a = make_step_args(S);
+    GFX_HIP(hipStreamBeginCapture(S.stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(S, a, true, nullptr);
+    GFX_HIP(hipStreamEndCapture(S.stream, &S.graph));
+    GFX_HIP(hipGraphInstantiate(&S.graph_exec, S.graph, nullptr, nullptr, 0));
+}
+
+void run_steps(gfx_solver& S, int nsteps) {
+    if (S.use_graph && !S.timing) {
+        ensure_graph(S);
+        for (int t = 0; t < nsteps; ++t) {
+            GFX_HIP(hipGraphLaunch(S.graph_exec, S.stream));
+            advance_host(S);
+        }
+        GFX_HIP(hipStreamSynchronize(S.stream));
+        return;
+    }
+    const StepArgs a = make_step_args(S);
     std::vector<hipEvent_t> ev;
     constexpr size_t EV = 4;
     if (S.timing) {
@@ -320,53 +435,29 @@ void run_steps(gfx_solver& S, int nsteps) {
     }
     std::vector<char> did_rec(nsteps, 0);
     for (int t = 0; t < nsteps; ++t) {
-        ea.step = S.step;
-        if (S.assembly == GFX_ASSEMBLY_ATOMIC)
-            dev::k_zero<<<zero_grid, 256, 0, S.stream>>>(reinterpret_cast<float4*>(S.R.p), r4);
-        for (const auto& rg : S.ranges) launch_element(S, ea, rg.first, rg.second - rg.first);
-        GFX_HIP(hipGetLastError());
-        if (S.timing) GFX_HIP(hipEventRecord(ev[size_t(t) * EV + 1], S.stream));
-        if (spec_grid > 0) dev::k_node_special<<<spec_grid, 256, 0, S.stream>>>(na);
-        if (S.timing) GFX_HIP(hipEventRecord(ev[size_t(t) * EV + 2], S.stream));
-        if (reg_grid > 0) {
-            if (na.gather)
-                dev::k_node_regular<true><<<reg_grid, 256, 0, S.stream>>>(na);
-            else
-                dev::k_node_regular<false><<<reg_grid, 256, 0, S.stream>>>(na);
-        }
-        GFX_HIP(hipGetLastError());
-        if (S.timing) GFX_HIP(hipEventRecord(ev[size_t(t) * EV + 3], S.stream));
-        if (S.stride > 0 && S.step % S.stride == 0 && S.rec_cap > 0 && S.n_rec > 0) {
-            const int slot = S.n_snap % S.rec_cap;
-            const int64_t nrp = int64_t(S.n_rec) * NP;
-            ra.strain = S.rec_strain.p + int64_t(slot) * S.nf * nrp * 6;
-            ra.uva = S.rec_uva_on ? S.rec_uva.p + int64_t(slot) * S.nf * nrp * 9 : nullptr;
-            dev::k_record<REPB><<<rec_grid, ((REPB * NP + 63) / 64) * 64, 0, S.stream>>>(ra);
-            GFX_HIP(hipGetLastError());
-            did_rec[t] = 1;
-        }
-        if (S.stride > 0 && S.step % S.stride == 0) ++S.n_snap;
-        if (S.timing) GFX_HIP(hipEventRecord(ev[size_t(t) * EV + 4], S.stream));
-        ++S.step;
+        const bool snap = a.recording && S.step % S.stride == 0;
+        enqueue_step(S, a, snap, S.timing ? &ev[size_t(t) * EV + 1] : nullptr);
+        did_rec[t] = snap ? 1 : 0;
+        advance_host(S);
     }
     GFX_HIP(hipStreamSynchronize(S.stream));
     if (S.timing) {
         S.t_elem = S.t_spec = S.t_reg = S.t_rec = S.t_total = 0;
         S.n_elem_launch = S.n_spec_launch = S.n_reg_launch = S.n_rec_launch = 0;
         for (int t = 0; t < nsteps; ++t) {
-            float a = 0, b = 0, c = 0, d = 0;
+            float e = 0, b = 0, c = 0, d = 0;
             const size_t o = size_t(t) * EV;
-            GFX_HIP(hipEventElapsedTime(&a, ev[o], ev[o + 1]));
+            GFX_HIP(hipEventElapsedTime(&e, ev[o], ev[o + 1]));
             GFX_HIP(hipEventElapsedTime(&b, ev[o + 1], ev[o + 2]));
             GFX_HIP(hipEventElapsedTime(&c, ev[o + 2], ev[o + 3]));
             GFX_HIP(hipEventElapsedTime(&d, ev[o + 3], ev[o + 4]));
-            S.t_elem += a;
+            S.t_elem += e;
             S.t_spec += b;
             S.t_reg += c;
             S.t_rec += d;
             S.n_elem_launch += int(S.ranges.size());
-            S.n_spec_launch += spec_grid > 0;
-            S.n_reg_launch += reg_grid > 0;
+            S.n_spec_launch += a.spec_grid > 0;
+            S.n_reg_launch += a.reg_grid > 0;
             S.n_rec_launch += did_rec[t];
         }
         float tot = 0;
@@ -375,6 +466,13 @@ void run_steps(gfx_solver& S, int nsteps) {
         S.n_total_launch = nsteps;
         for (auto& e : ev) (void)hipEventDestroy(e);
     }
+}
+
+// Writes the host step / snapshot counters to the device.
+void upload_counters(gfx_solver& S) {
+    const int h[2] = {S.step, S.n_snap};
+    GFX_HIP(hipStreamSynchronize(S.stream));
+    GFX_HIP(hipMemcpy(S.ctr.p, h, sizeof(h), hipMemcpyHostToDevice));
 }
 
 // Greedy element colouring: no two elements of one colour share a node.
@@ -409,6 +507,9 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
     int epb = d.elems_per_block;
     require(epb == 0 || epb == 1 || epb == 2 || epb == 4 || epb == 8,
             "elems_per_block must be 1, 2, 4 or 8");
+    const int nfg = d.node_field_groups;
+    require(nfg == 0 || nfg == 1 || nfg == 2 || nfg == 4 || nfg == 8, "node_field_groups must be 1, 2, 4 or 8");
+    S.node_fg = nfg ? nfg : 1;
     require(d.n_rank_node < int(0x7fffffff) - 8, "too many nodes");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
@@ -666,6 +767,7 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
     S.fstride = ((S.G4 * 3 + 63) / 64) * 64;
     S.astride = ((std::max<int64_t>(S.Ga, 1) * 3 + 63) / 64) * 64;
     const size_t fs = size_t(S.fstride) * S.nf;
+    S.ctr.alloc(2);
     S.U.alloc(fs);
     S.W.alloc(fs);
     S.R.alloc(fs);
@@ -777,6 +879,7 @@ extern "C" gfx_status gfx_solver_set_source(gfx_solver* s, int field, int direct
         for (int t = 0; t < n_stf; ++t) fs.stf.push_back(float(stf[t]));
         GFX_HIP(hipStreamSynchronize(s->stream));
         s->sources[field] = std::move(fs);
+        drop_graph(*s);
         build_sources(*s);
     });
 }
@@ -805,6 +908,7 @@ extern "C" gfx_status gfx_solver_set_step(gfx_solver* s, int step) {
         s->step = step;
         // snapshots are taken after the updates with index % stride == 0
         s->n_snap = s->stride > 0 ? (step + s->stride - 1) / s->stride : 0;
+        upload_counters(*s);
     });
 }
 extern "C" int gfx_solver_n_snapshots(const gfx_solver* s) { return s ? s->n_snap : -1; }
@@ -1046,6 +1150,7 @@ extern "C" gfx_status gfx_solver_checkpoint_load(gfx_solver* s, const void* host
         }
         s->step = int(hdr[5]);
         s->n_snap = int(hdr[6]);
+        upload_counters(*s);
     });
 }
 
@@ -1098,6 +1203,26 @@ extern "C" gfx_status gfx_solver_set_launch(gfx_solver* s, int element_kernel, i
             require(wave_ok, "the wave element kernel needs the nodal C-PML form");
         s->kernel = (element_kernel == GFX_ELEMENT_BLOCK || !wave_ok) ? GFX_ELEMENT_BLOCK : GFX_ELEMENT_WAVE;
         s->epb = elems_per_block ? elems_per_block : (s->kernel == GFX_ELEMENT_WAVE ? 4 : 2);
+        drop_graph(*s);
+    });
+}
+
+extern "C" gfx_status gfx_solver_set_graph(gfx_solver* s, int enable) {
+    return guarded([&] {
+        require(s != nullptr, "null solver");
+        GFX_HIP(hipStreamSynchronize(s->stream));
+        if (!enable) drop_graph(*s);
+        s->use_graph = enable != 0;
+    });
+}
+
+extern "C" gfx_status gfx_solver_set_node_groups(gfx_solver* s, int node_field_groups) {
+    return guarded([&] {
+        require(s != nullptr, "null solver");
+        const int g = node_field_groups;
+        require(g == 1 || g == 2 || g == 4 || g == 8, "node_field_groups must be 1, 2, 4 or 8");
+        s->node_fg = g;
+        drop_graph(*s);
     });
 }
 
@@ -1111,41 +1236,67 @@ extern "C" gfx_status gfx_autotune(const gfx_solver_desc* base, int trial_steps,
         const int asms[2] = {GFX_ASSEMBLY_COLORED, GFX_ASSEMBLY_GATHER};
         const int launch[6][2] = {{GFX_ELEMENT_BLOCK, 1}, {GFX_ELEMENT_BLOCK, 2}, {GFX_ELEMENT_BLOCK, 4},
                                   {GFX_ELEMENT_WAVE, 2},  {GFX_ELEMENT_WAVE, 4},  {GFX_ELEMENT_WAVE, 8}};
+        auto time_steps = [&](gfx_solver& S) {
+            run_steps(S, 1);  // warm-up
+            hipEvent_t e0, e1;
+            GFX_HIP(hipEventCreate(&e0));
+            GFX_HIP(hipEventCreate(&e1));
+            GFX_HIP(hipEventRecord(e0, S.stream));
+            const bool keep_timing = S.timing;
+            S.timing = false;
+            run_steps(S, trial_steps);
+            S.timing = keep_timing;
+            GFX_HIP(hipEventRecord(e1, S.stream));
+            GFX_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            GFX_HIP(hipEventElapsedTime(&ms, e0, e1));
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return double(ms) / trial_steps;
+        };
+        const int fg0 = base->node_field_groups ? base->node_field_groups : 1;
         for (int asm_mode : asms) {
             gfx_solver_desc d = *base;
             d.assembly = asm_mode;
             d.record_capacity = 0;
             d.element_kernel = GFX_ELEMENT_AUTO;
             d.elems_per_block = 0;
+            d.node_field_groups = fg0;
             auto S = std::make_unique<gfx_solver>();
             create(d, *S);
-            for (const auto& lc : launch) {
+            double best_here = 1e300;
+            int best_lc = -1;
+            for (int li = 0; li < 6; ++li) {
+                const auto& lc = launch[li];
                 if (lc[0] == GFX_ELEMENT_WAVE && S->has_cpml && !S->pml_nodal) continue;
                 S->kernel = lc[0];
                 S->epb = lc[1];
-                run_steps(*S, 1);  // warm-up
-                hipEvent_t e0, e1;
-                GFX_HIP(hipEventCreate(&e0));
-                GFX_HIP(hipEventCreate(&e1));
-                GFX_HIP(hipEventRecord(e0, S->stream));
-                const bool keep_timing = S->timing;
-                S->timing = false;
-                run_steps(*S, trial_steps);
-                S->timing = keep_timing;
-                GFX_HIP(hipEventRecord(e1, S->stream));
-                GFX_HIP(hipEventSynchronize(e1));
-                float ms = 0;
-                GFX_HIP(hipEventElapsedTime(&ms, e0, e1));
-                (void)hipEventDestroy(e0);
-                (void)hipEventDestroy(e1);
-                const double per = double(ms) / trial_steps;
-                if (per < best_t) {
-                    best_t = per;
-                    pick = *base;
-                    pick.assembly = asm_mode;
-                    pick.element_kernel = lc[0];
-                    pick.elems_per_block = lc[1];
+                const double per = time_steps(*S);
+                if (per < best_here) {
+                    best_here = per;
+                    best_lc = li;
                 }
+            }
+            // node-kernel field groups for this assembly's best element launch
+            S->kernel = launch[best_lc][0];
+            S->epb = launch[best_lc][1];
+            int best_fg = fg0;
+            for (int fg : {1, 2, 4, 8}) {
+                if (fg == fg0 || fg > 2 * S->nf) continue;
+                S->node_fg = fg;
+                const double per = time_steps(*S);
+                if (per < best_here) {
+                    best_here = per;
+                    best_fg = fg;
+                }
+            }
+            if (best_here < best_t) {
+                best_t = best_here;
+                pick = *base;
+                pick.assembly = asm_mode;
+                pick.element_kernel = launch[best_lc][0];
+                pick.elems_per_block = launch[best_lc][1];
+                pick.node_field_groups = best_fg;
             }
             GFX_HIP(hipStreamSynchronize(S->stream));
         }
@@ -1161,6 +1312,7 @@ extern "C" gfx_status gfx_solver_query(const gfx_solver* s, const char* key, dou
         if (k == "colors") *value = double(s->ranges.size());
         else if (k == "element_kernel") *value = s->kernel;
         else if (k == "elems_per_block") *value = s->epb;
+        else if (k == "node_field_groups") *value = s->node_fg;
         else if (k == "pml_nodal") *value = (s->has_cpml && s->pml_nodal) ? 1.0 : 0.0;
         else if (k == "pml_elements") *value = s->n_pml_elem;
         else if (k == "special_nodes") *value = double(s->Ga);
@@ -1195,3 +1347,109 @@ extern "C" double gfx_solver_algorithmic_bytes(const gfx_solver* s, int which) {
     if (which == 4) return node_reg;
     return elem + node;
 }
+
+// ---------------------------------------------------------------------------
+// Batches of independent solvers (uneven meshes): one step of every active
+// instance is captured as parallel branches of one graph on the batch stream.
+// ---------------------------------------------------------------------------
+struct gfx_batch {
+    std::vector<gfx_solver*> s;
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    struct Cached {
+        std::vector<char> active;
+        std::vector<int> gens;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<Cached> cache;
+    ~gfx_batch() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& c : cache) {
+            if (c.exec) (void)hipGraphExecDestroy(c.exec);
+            if (c.graph) (void)hipGraphDestroy(c.graph);
+        }
+        for (auto e : join)
+            if (e) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace gfx {
+namespace {
+
+hipGraphExec_t batch_graph(gfx_batch& B, const std::vector<char>& active) {
+    std::vector<int> gens(B.s.size());
+    for (size_t i = 0; i < B.s.size(); ++i) gens[i] = B.s[i]->gen;
+    for (auto& c : B.cache)
+        if (c.active == active && c.gens == gens) return c.exec;
+    gfx_batch::Cached c;
+    c.active = active;
+    c.gens = gens;
+    std::vector<StepArgs> args(B.s.size());
+    for (size_t i = 0; i < B.s.size(); ++i)
+        if (active[i]) args[i] = make_step_args(*B.s[i]);
+    GFX_HIP(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
+    GFX_HIP(hipEventRecord(B.fork, B.stream));
+    for (size_t i = 0; i < B.s.size(); ++i) {
+        if (!active[i]) continue;
+        gfx_solver& S = *B.s[i];
+        GFX_HIP(hipStreamWaitEvent(S.stream, B.fork, 0));
+        enqueue_step(S, args[i], true, nullptr);
+        GFX_HIP(hipEventRecord(B.join[i], S.stream));
+        GFX_HIP(hipStreamWaitEvent(B.stream, B.join[i], 0));
+    }
+    GFX_HIP(hipStreamEndCapture(B.stream, &c.graph));
+    GFX_HIP(hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0));
+    B.cache.push_back(c);
+    return c.exec;
+}
+
+}  // namespace
+}  // namespace gfx
+
+extern "C" gfx_status gfx_batch_create(int n, gfx_solver* const* solvers, gfx_batch** out) {
+    return guarded([&] {
+        require(n >= 1 && solvers && out, "bad argument");
+        *out = nullptr;
+        auto B = std::make_unique<gfx_batch>();
+        for (int i = 0; i < n; ++i) {
+            require(solvers[i] != nullptr, "null solver in batch");
+            for (int k = 0; k < i; ++k) require(solvers[k] != solvers[i], "a solver appears twice in the batch");
+            B->s.push_back(solvers[i]);
+        }
+        GFX_HIP(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+        GFX_HIP(hipEventCreateWithFlags(&B->fork, hipEventDisableTiming));
+        B->join.resize(n, nullptr);
+        for (auto& e : B->join) GFX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        *out = B.release();
+    });
+}
+
+extern "C" gfx_status gfx_batch_run(gfx_batch* b, int n_steps, const int* until) {
+    return guarded([&] {
+        require(b != nullptr, "null batch");
+        require(n_steps >= 0, "n_steps must be >= 0");
+        for (auto* S : b->s) GFX_HIP(hipStreamSynchronize(S->stream));
+        std::vector<char> active(b->s.size());
+        for (int t = 0; t < n_steps; ++t) {
+            bool any = false;
+            for (size_t i = 0; i < b->s.size(); ++i) {
+                active[i] = (!until || b->s[i]->step < until[i]) ? 1 : 0;
+                any = any || active[i];
+            }
+            if (!any) break;
+            GFX_HIP(hipGraphLaunch(batch_graph(*b, active), b->stream));
+            for (size_t i = 0; i < b->s.size(); ++i)
+                if (active[i]) advance_host(*b->s[i]);
+        }
+        GFX_HIP(hipStreamSynchronize(b->stream));
+    });
+}
+
+extern "C" gfx_status gfx_batch_destroy(gfx_batch* b) {
+    return guarded([&] { delete b; });
+}
+

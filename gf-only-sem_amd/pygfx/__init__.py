@@ -88,6 +88,7 @@ class SolverDesc(C.Structure):
         ("n_fields", C.c_int), ("assembly", C.c_int), ("elems_per_block", C.c_int),
         ("element_kernel", C.c_int),
         ("record_capacity", C.c_int), ("record_uva", C.c_int),
+        ("node_field_groups", C.c_int),
     ]
 
 
@@ -128,6 +129,11 @@ def _load():
         "gfx_solver_algorithmic_bytes": (C.c_double, [vp, C.c_int]),
         "gfx_solver_query": (st, [vp, C.c_char_p, C.POINTER(C.c_double)]),
         "gfx_solver_set_launch": (st, [vp, C.c_int, C.c_int]),
+        "gfx_solver_set_node_groups": (st, [vp, C.c_int]),
+        "gfx_solver_set_graph": (st, [vp, C.c_int]),
+        "gfx_batch_create": (st, [C.c_int, C.POINTER(vp), C.POINTER(vp)]),
+        "gfx_batch_run": (st, [vp, C.c_int, _ip]),
+        "gfx_batch_destroy": (st, [vp]),
         "gfx_autotune": (st, [C.POINTER(SolverDesc), C.c_int, C.POINTER(SolverDesc), C.POINTER(C.c_double)]),
         "gfx_box_spec_default": (st, [C.POINTER(BoxSpec)]),
         "gfx_mesh_build_box": (st, [C.POINTER(BoxSpec), C.POINTER(C.POINTER(_Mesh))]),
@@ -466,7 +472,8 @@ def h5_attr(path, obj, name, string=False):
 
 def make_desc(mesh: Mesh, keep: list, n_fields=1, assembly=GFX_ASSEMBLY_COLORED, elems_per_block=0,
               record_capacity=None, record_uva=False, has_cpml=True, dt=None,
-              snapshot_stride=None, rec_cell=None, element_kernel=GFX_ELEMENT_AUTO, morton=True):
+              snapshot_stride=None, rec_cell=None, element_kernel=GFX_ELEMENT_AUTO, morton=True,
+              node_field_groups=0):
     """Solver descriptor over mesh arrays; arrays it points to are appended to keep."""
     a = mesh.arrays
 
@@ -503,6 +510,7 @@ def make_desc(mesh: Mesh, keep: list, n_fields=1, assembly=GFX_ASSEMBLY_COLORED,
         record_capacity = 0 if rk.size == 0 else (mesh.nsteps // max(d.snapshot_stride, 1) + 1)
     d.record_capacity = record_capacity
     d.record_uva = 1 if record_uva else 0
+    d.node_field_groups = node_field_groups
     return d
 
 
@@ -514,7 +522,8 @@ def autotune(mesh: Mesh, n_fields=1, trial_steps=3, **kw):
     ms = C.c_double(0)
     _check(lib.gfx_autotune(C.byref(base), trial_steps, C.byref(best), C.byref(ms)))
     return dict(assembly=best.assembly, element_kernel=best.element_kernel,
-                elems_per_block=best.elems_per_block, ms_per_step=ms.value)
+                elems_per_block=best.elems_per_block, node_field_groups=best.node_field_groups,
+                ms_per_step=ms.value)
 
 
 class Solver:
@@ -523,13 +532,14 @@ class Solver:
     def __init__(self, mesh: Mesh, n_fields=1, assembly=GFX_ASSEMBLY_COLORED, elems_per_block=0,
                  record_capacity=None, record_uva=False, has_cpml=True, dt=None,
                  snapshot_stride=None, rec_cell=None, element_kernel=GFX_ELEMENT_AUTO,
-                 morton=True):
+                 morton=True, node_field_groups=0):
         self._keep = []
         d = make_desc(mesh, self._keep, n_fields=n_fields, assembly=assembly,
                       elems_per_block=elems_per_block, record_capacity=record_capacity,
                       record_uva=record_uva, has_cpml=has_cpml, dt=dt,
                       snapshot_stride=snapshot_stride, rec_cell=rec_cell,
-                      element_kernel=element_kernel, morton=morton)
+                      element_kernel=element_kernel, morton=morton,
+                      node_field_groups=node_field_groups)
         rk_size = d.n_rec_cell
         h = C.c_void_p()
         _check(lib.gfx_solver_create(C.byref(d), C.byref(h)))
@@ -601,8 +611,10 @@ class Solver:
         _check(lib.gfx_solver_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
-    def set_launch(self, element_kernel, elems_per_block=0):
+    def set_launch(self, element_kernel, elems_per_block=0, node_field_groups=None):
         _check(lib.gfx_solver_set_launch(self.h, element_kernel, elems_per_block))
+        if node_field_groups is not None:
+            _check(lib.gfx_solver_set_node_groups(self.h, node_field_groups))
 
     def query(self, key):
         v = C.c_double(0)
@@ -611,6 +623,9 @@ class Solver:
 
     def algorithmic_bytes(self, which=0):
         return float(lib.gfx_solver_algorithmic_bytes(self.h, which))
+
+    def set_graph(self, on=True):
+        _check(lib.gfx_solver_set_graph(self.h, 1 if on else 0))
 
     def set_step(self, step):
         _check(lib.gfx_solver_set_step(self.h, int(step)))
@@ -626,3 +641,31 @@ class Solver:
     def restore(self, image: bytes):
         buf = C.create_string_buffer(image, len(image))
         _check(lib.gfx_solver_checkpoint_load(self.h, buf, len(image)))
+
+
+class Batch:
+    """Independent solvers (uneven meshes) advanced together, one captured
+    graph replay per step with one branch per instance."""
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        arr = (C.c_void_p * len(self.solvers))(*[s.h.value for s in self.solvers])
+        h = C.c_void_p()
+        _check(lib.gfx_batch_create(len(self.solvers), arr, C.byref(h)))
+        self.h = h
+
+    def run(self, n_steps, until=None):
+        u = None if until is None else np.ascontiguousarray(until, np.int32)
+        _check(lib.gfx_batch_run(self.h, int(n_steps), _ptr(u, C.c_int32)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.gfx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

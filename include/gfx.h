@@ -117,6 +117,9 @@ typedef struct {
     /* device record ring: snapshots kept (0 = no recording); u,v,a copies on/off */
     int record_capacity;
     int record_uva;
+    /* node kernels: field groups per workgroup (1, 2, 4 or 8; 0 = 1): the
+       fields of one node block are spread over that many lane groups */
+    int node_field_groups;
 } gfx_solver_desc;
 
 gfx_status gfx_solver_create(const gfx_solver_desc* desc, gfx_solver** out);
@@ -182,11 +185,20 @@ gfx_status gfx_solver_set_timing(gfx_solver* s, int enable);
 /* Switch the element kernel form and elements per workgroup of an existing
  * solver (same contract as the descriptor fields; results do not depend on it). */
 gfx_status gfx_solver_set_launch(gfx_solver* s, int element_kernel, int elems_per_block);
+/* Capture one whole step into a HIP graph and replay it for every step of
+ * gfx_solver_run (1), or launch kernel by kernel (0, default).  The step and
+ * snapshot counters live on the device, so one captured step serves every
+ * step; the graph is recaptured after any launch or source change.  Per-kernel
+ * timing (gfx_solver_set_timing) uses the kernel-by-kernel path. */
+gfx_status gfx_solver_set_graph(gfx_solver* s, int enable);
+/* Switch the node kernels' field groups per workgroup (1, 2, 4 or 8). */
+gfx_status gfx_solver_set_node_groups(gfx_solver* s, int node_field_groups);
 
 /* Autotuner: times trial_steps steps of every launch configuration
  * (assembly COLORED / GATHER x element kernel BLOCK / WAVE x elements per
- * workgroup) for the problem in `base` and writes the fastest into *best
- * (a copy of base with assembly, element_kernel and elems_per_block set);
+ * workgroup, then node-kernel field groups for the best of those) for the
+ * problem in `base` and writes the fastest into *best (a copy of base with
+ * assembly, element_kernel, elems_per_block and node_field_groups set);
  * *best_ms receives its device time per step.  Sources and recording are
  * not part of the trial. */
 gfx_status gfx_autotune(const gfx_solver_desc* base, int trial_steps, gfx_solver_desc* best,
@@ -201,6 +213,22 @@ gfx_status gfx_solver_query(const gfx_solver* s, const char* key, double* value)
  * once per step).  which: 0 = whole step, 1 = element stage, 2 = node stage
  * (both node kernels), 3 = special-node kernel, 4 = regular-node kernel. */
 double gfx_solver_algorithmic_bytes(const gfx_solver* s, int which);
+
+/* -------------------------------------------------------------------------
+ * Batches of independent solvers (uneven meshes, each with its own dt,
+ * C-PML, record ring and run length).  No counterpart in the reference, whose
+ * pipeline runs meshes and directions one process at a time.
+ * ---------------------------------------------------------------------------- */
+typedef struct gfx_batch gfx_batch;
+/* The solvers stay owned by the caller and must outlive the batch; while they
+ * belong to a batch they must not be run or changed concurrently with it. */
+gfx_status gfx_batch_create(int n, gfx_solver* const* solvers, gfx_batch** out);
+/* Advance every instance by up to n_steps steps, instance i only while its
+ * step counter is below until[i] (until may be NULL).  Each step of all active
+ * instances is one replay of a captured graph with one branch per instance;
+ * graphs are cached per active set.  Returns when all work has finished. */
+gfx_status gfx_batch_run(gfx_batch* b, int n_steps, const int* until);
+gfx_status gfx_batch_destroy(gfx_batch* b);
 
 /* -------------------------------------------------------------------------
  * Structured-box inputs (restates preprocess/ geometry, material, CFL,

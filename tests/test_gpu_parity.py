@@ -90,10 +90,11 @@ def _oracle(mesh, cells, w, stf, direction, nsteps=None, has_cpml=True):
 
 
 def _gpu(mesh, sources, nsteps=None, assembly=pygfx.GFX_ASSEMBLY_COLORED, epb=0, has_cpml=True,
-         kernel=pygfx.GFX_ELEMENT_AUTO):
+         kernel=pygfx.GFX_ELEMENT_AUTO, node_groups=0):
     nsteps = mesh.nsteps if nsteps is None else nsteps
     s = pygfx.Solver(mesh, n_fields=len(sources), assembly=assembly, elems_per_block=epb,
-                     record_uva=True, has_cpml=has_cpml, element_kernel=kernel)
+                     record_uva=True, has_cpml=has_cpml, element_kernel=kernel,
+                     node_field_groups=node_groups)
     for f, (cells, w, stf, d) in enumerate(sources):
         s.set_source(f, d, cells, w, stf)
     s.run(nsteps)
@@ -216,6 +217,25 @@ def test_batched_fields_equal_single_runs(assembly):
             assert np.array_equal(x, y)
         n_snap = s1.n_snapshots
         assert np.array_equal(_records(sb, f, n_snap), _records(s1, 0, n_snap))
+
+
+@pytest.mark.parametrize("assembly", [pygfx.GFX_ASSEMBLY_COLORED, pygfx.GFX_ASSEMBLY_GATHER])
+def test_node_field_groups_do_not_change_results(assembly):
+    """Spreading the fields of a node block over 1, 2, 4 or 8 lane groups only
+    changes which lane does the arithmetic, never the arithmetic."""
+    mesh = mc.small_halfspace(duration=0.8)
+    cells, w, stf = mc.surface_centre_source(mesh)
+    c2, w2, stf2 = mc.buried_source(mesh, frac=(0.45, 0.4, 0.0), depth=1500.0, f0=1.2)
+    sources = [(cells, w, stf, 0), (cells, w, stf, 1), (cells, w, stf, 2), (c2, w2, stf2, 2), (c2, w2, stf2, 0)]
+    base = _gpu(mesh, sources, assembly=assembly, node_groups=1)
+    n_snap = base.n_snapshots
+    for fg in (2, 4, 8):
+        s = _gpu(mesh, sources, assembly=assembly, node_groups=fg)
+        assert int(s.query("node_field_groups")) == fg
+        for f in range(len(sources)):
+            for x, y in zip(s.get_state(f), base.get_state(f)):
+                assert np.array_equal(x, y)
+            assert np.array_equal(_records(s, f, n_snap), _records(base, f, n_snap))
 
 
 @pytest.mark.parametrize("assembly", [pygfx.GFX_ASSEMBLY_COLORED, pygfx.GFX_ASSEMBLY_GATHER])
