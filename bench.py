@@ -223,6 +223,31 @@ def uneven_workload(pygfx, steps, warmup):
     return out
 
 
+def surrogate_run(pygfx, args):
+    """Learned Green's-function surrogate on the layered mesh: device tile of
+    strain Green's tensors (6 x 3) at the GLL nodes of the recording cells to
+    1.8 km, one sample per snapshot; MLP (64 Fourier frequencies, 6 x 256
+    GELU, BF16 MFMA) trained on it; 1M seeded queries scored against GLL
+    interpolation of the tile, with per-cell fallback above 1 %."""
+    from pygfx import greens
+    dims = (40, 40, 20) if args.small else (100, 100, 50)
+    t0 = time.perf_counter()
+    mesh = pygfx.build_box(cfg2_spec(pygfx, *dims), copy=False)
+    log(f"mesh: {mesh.n_elem} elements ({time.perf_counter() - t0:.1f} s)")
+    stf = pygfx.ricker(1.5, 1.0, 1e20, mesh.dt, mesh.nsteps)
+    src = (0.5 * (mesh.bounds[0] + mesh.bounds[1]), 0.5 * (mesh.bounds[2] + mesh.bounds[3]), 0.0)
+    tile, info = greens.build_tile(mesh, src, stf, depth_max=1800.0, log=log,
+                                   assembly=pygfx.GFX_ASSEMBLY_GATHER)
+    res = greens.experiment(tile, info["dt_snap"], iters=args.surrogate_iters, n_query=1_000_000, bound=0.01,
+                            log=log)
+    out = {"metric": "surrogate rel L2 vs GLL interpolation of the tile (1M seeded queries)",
+           "workload": f"layered {dims[0]}x{dims[1]}x{dims[2]}, x/y/z forces at the surface centre, "
+                       f"tile = GLL nodes of recording cells to 1.8 km",
+           "tile": {k: v for k, v in info.items() if k != "cell_ids"}}
+    out.update(res)
+    return out
+
+
 def cpu_baseline(seconds=15.0):
     """FP64 CPU restatement of the reference solver (oracle/), one core, on a
     20 x 20 x 10 sub-box of the same 0.6 km layered discretisation."""
@@ -267,6 +292,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-uneven", action="store_true", help="skip the 16-mesh uneven batch line")
+    ap.add_argument("--surrogate", action="store_true",
+                    help="instead of the step benchmark: Green tile of the layered mesh (x, y, z forces at the "
+                         "surface centre, cells to 1.8 km, every snapshot), MLP training, 1M scored queries")
+    ap.add_argument("--surrogate-iters", type=int, default=4000)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run 3 device-to-device copies of 1 GiB (PMC counter calibration)")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -287,6 +316,9 @@ def main():
         sys.exit(1)
     if args.calibrate_copy:
         copy_calibration()
+    if args.surrogate:
+        print(json.dumps(surrogate_run(pygfx, args)), flush=True)
+        return
     asm_of = {"colored": pygfx.GFX_ASSEMBLY_COLORED, "gather": pygfx.GFX_ASSEMBLY_GATHER,
               "atomic": pygfx.GFX_ASSEMBLY_ATOMIC}
     kern_of = {"auto": pygfx.GFX_ELEMENT_AUTO, "block": pygfx.GFX_ELEMENT_BLOCK,

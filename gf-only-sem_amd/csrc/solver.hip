@@ -16,48 +16,11 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
-#define GFX_HIP(x)                                                                         \
-    do {                                                                                   \
-        hipError_t e_ = (x);                                                               \
-        if (e_ != hipSuccess)                                                              \
-            throw ::gfx::Error(GFX_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
-    } while (0)
+#include "device.hpp"
+#include "green.hpp"
+#include "gfx_surrogate.h"
 
 namespace gfx {
-
-template <typename T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    void alloc(size_t count) {
-        release();
-        if (count == 0) return;
-        hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
-        if (e != hipSuccess) {
-            p = nullptr;
-            (void)hipGetLastError();
-            throw Error(GFX_ERR_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) +
-                                           " bytes failed: " + hipGetErrorString(e));
-        }
-        n = count;
-    }
-    void upload(const std::vector<T>& h) {
-        alloc(h.size());
-        if (!h.empty()) GFX_HIP(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
-    }
-    void zero(hipStream_t st) {
-        if (p) GFX_HIP(hipMemsetAsync(p, 0, n * sizeof(T), st));
-    }
-};
 
 namespace {
 struct FieldSource {
@@ -1451,5 +1414,89 @@ extern "C" gfx_status gfx_batch_run(gfx_batch* b, int n_steps, const int* until)
 
 extern "C" gfx_status gfx_batch_destroy(gfx_batch* b) {
     return guarded([&] { delete b; });
+}
+
+// ---------------------------------------------------------------------------
+// Green's tensor capture: node values of a tile from one snapshot of the
+// solver's record ring (mass-weighted merge, postprocess/cpp/main.cpp:354-426).
+// ---------------------------------------------------------------------------
+namespace gfx {
+namespace dev {
+__global__ void k_green_capture(const float* __restrict__ ring, int64_t nrp, const int64_t* __restrict__ off,
+                                const int32_t* __restrict__ pt, const float* __restrict__ w, int n_node, int f0,
+                                int f1, int f2, float* __restrict__ out) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= n_node) return;
+    const int fd[3] = {f0, f1, f2};
+    float acc[18];
+#pragma unroll
+    for (int k = 0; k < 18; ++k) acc[k] = 0.f;
+    for (int64_t e = off[n]; e < off[n + 1]; ++e) {
+        const float we = w[e];
+        const int64_t q = pt[e];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (fd[d] < 0) continue;
+            const float* s6 = ring + (int64_t(fd[d]) * nrp + q) * 6;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) acc[c * 3 + d] += we * s6[c];
+        }
+    }
+    float* o = out + int64_t(n) * 18;
+#pragma unroll
+    for (int k = 0; k < 18; ++k) o[k] = acc[k];
+}
+}  // namespace dev
+}  // namespace gfx
+
+extern "C" gfx_status gfx_green_set_capture_map(gfx_green* g, const int64_t* off, const int32_t* rec_point,
+                                                const double* weight) {
+    return guarded([&] {
+        require(g && off && rec_point && weight, "null argument");
+        require(off[0] == 0, "off[0] must be 0");
+        const int64_t ne = off[g->n_node];
+        std::vector<int64_t> ho(off, off + g->n_node + 1);
+        std::vector<int32_t> hp(rec_point, rec_point + ne);
+        std::vector<float> hw(ne);
+        for (int n = 0; n < g->n_node; ++n) {
+            require(ho[n + 1] >= ho[n], "off must be non-decreasing");
+            double sw = 0;
+            for (int64_t e = ho[n]; e < ho[n + 1]; ++e) sw += weight[e];
+            for (int64_t e = ho[n]; e < ho[n + 1]; ++e) hw[e] = sw > 0 ? float(weight[e] / sw) : 0.f;
+        }
+        int64_t mx = -1;
+        for (int32_t q : hp) {
+            require(q >= 0, "negative recording point");
+            mx = std::max<int64_t>(mx, q);
+        }
+        g->map_max = mx;
+        g->map_off.upload(ho);
+        g->map_pt.upload(hp.empty() ? std::vector<int32_t>{0} : hp);
+        g->map_w.upload(hw.empty() ? std::vector<float>{0.f} : hw);
+    });
+}
+
+extern "C" gfx_status gfx_green_capture(gfx_green* g, const gfx_solver* s, int snap, const int32_t* field_of_dir,
+                                        int time_index) {
+    return guarded([&] {
+        require(g && s && field_of_dir, "null argument");
+        require(g->map_off.p != nullptr, "capture map not set");
+        require(time_index >= 0 && time_index < g->n_time, "time index out of range");
+        require(s->rec_cap > 0 && s->n_rec > 0, "the solver records nothing");
+        require(snap >= 0 && snap < s->n_snap && snap >= s->n_snap - s->rec_cap, "snapshot not held in the record ring");
+        for (int d = 0; d < 3; ++d) require(field_of_dir[d] >= -1 && field_of_dir[d] < s->nf, "bad field index");
+        const int64_t nrp = int64_t(s->n_rec) * NP;
+        // every map entry must address the solver's recording points
+        require(g->map_max < nrp, "capture map addresses recording points the solver does not have");
+        const int slot = snap % s->rec_cap;
+        const float* ring = s->rec_strain.p + int64_t(slot) * s->nf * nrp * 6;
+        GFX_HIP(hipStreamSynchronize(s->stream));
+        GFX_HIP(hipStreamSynchronize(g->stream));
+        dev::k_green_capture<<<(g->n_node + 255) / 256, 256, 0, g->stream>>>(
+            ring, nrp, g->map_off.p, g->map_pt.p, g->map_w.p, g->n_node, field_of_dir[0], field_of_dir[1],
+            field_of_dir[2], g->val.p + int64_t(time_index) * g->n_node * 18);
+        GFX_HIP(hipGetLastError());
+        GFX_HIP(hipStreamSynchronize(g->stream));
+    });
 }
 

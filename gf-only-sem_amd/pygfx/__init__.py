@@ -101,6 +101,12 @@ class ForwardOptions(C.Structure):
     ]
 
 
+class MlpConfig(C.Structure):
+    _fields_ = [("n_freq", C.c_int), ("width", C.c_int), ("depth", C.c_int), ("n_out", C.c_int),
+                ("sigma", C.c_float), ("seed", C.c_uint64), ("lr", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float), ("batch", C.c_int)]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
@@ -134,6 +140,26 @@ def _load():
         "gfx_batch_create": (st, [C.c_int, C.POINTER(vp), C.POINTER(vp)]),
         "gfx_batch_run": (st, [vp, C.c_int, _ip]),
         "gfx_batch_destroy": (st, [vp]),
+        "gfx_green_create": (st, [C.c_int, C.c_int, _dp, C.c_int, _ip, C.POINTER(vp)]),
+        "gfx_green_destroy": (st, [vp]),
+        "gfx_green_set_values": (st, [vp, C.c_int, C.POINTER(C.c_float)]),
+        "gfx_green_get_values": (st, [vp, C.c_int, C.POINTER(C.c_float)]),
+        "gfx_green_set_capture_map": (st, [vp, C.POINTER(C.c_int64), _ip, _dp]),
+        "gfx_green_capture": (st, [vp, vp, C.c_int, _ip, C.c_int]),
+        "gfx_green_interpolate": (st, [vp, C.c_int, _dp, _ip, C.POINTER(C.c_float), _ip]),
+        "gfx_gll_interpolate_host": (st, [C.c_int, _dp, C.c_int, _ip, C.c_int, _dp, _dp, _dp]),
+        "gfx_mlp_config_default": (st, [C.POINTER(MlpConfig)]),
+        "gfx_mlp_create": (st, [C.POINTER(MlpConfig), _dp, _dp, _dp, C.POINTER(vp)]),
+        "gfx_mlp_destroy": (st, [vp]),
+        "gfx_mlp_n_params": (st, [vp, C.POINTER(C.c_size_t)]),
+        "gfx_mlp_get_params": (st, [vp, C.POINTER(C.c_float), C.c_size_t, C.POINTER(C.c_float)]),
+        "gfx_mlp_set_params": (st, [vp, C.POINTER(C.c_float), C.c_size_t]),
+        "gfx_mlp_forward": (st, [vp, C.c_int, _dp, C.POINTER(C.c_float)]),
+        "gfx_mlp_gradient": (st, [vp, C.c_int, _dp, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                  C.POINTER(C.c_double)]),
+        "gfx_mlp_train": (st, [vp, vp, C.c_double, C.c_double, C.c_int, C.c_uint64, C.POINTER(C.c_double)]),
+        "gfx_surrogate_query": (st, [vp, vp, C.POINTER(C.c_uint8), C.c_double, C.c_double, C.c_int, _dp, _ip,
+                                     C.POINTER(C.c_float), C.POINTER(C.c_int)]),
         "gfx_autotune": (st, [C.POINTER(SolverDesc), C.c_int, C.POINTER(SolverDesc), C.POINTER(C.c_double)]),
         "gfx_box_spec_default": (st, [C.POINTER(BoxSpec)]),
         "gfx_mesh_build_box": (st, [C.POINTER(BoxSpec), C.POINTER(C.POINTER(_Mesh))]),
@@ -661,6 +687,162 @@ class Batch:
     def close(self):
         if getattr(self, "h", None):
             lib.gfx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_fp = C.POINTER(C.c_float)
+
+
+def gll_interpolate_host(node_xyz, cell_node_index, values, point):
+    """FP64 host restatement of GLLInterpolator.interpolate for one point."""
+    xyz = np.ascontiguousarray(node_xyz, np.float64).reshape(-1, 3)
+    cidx = np.ascontiguousarray(cell_node_index, np.int32).reshape(-1, 125)
+    vals = np.ascontiguousarray(values, np.float64).reshape(xyz.shape[0], -1)
+    p = np.ascontiguousarray(point, np.float64)
+    out = np.zeros(vals.shape[1])
+    _check(lib.gfx_gll_interpolate_host(xyz.shape[0], _ptr(xyz), cidx.shape[0], _ptr(cidx, C.c_int32), vals.shape[1],
+                                        _ptr(vals), _ptr(p), _ptr(out)))
+    return out
+
+
+class GreenTile:
+    """Device-resident Green's tensor tile [n_time][n_node][6][3] with its cells."""
+
+    def __init__(self, n_time, node_xyz, cell_node_index):
+        self.xyz = np.ascontiguousarray(node_xyz, np.float64).reshape(-1, 3)
+        self.cells = np.ascontiguousarray(cell_node_index, np.int32).reshape(-1, 125)
+        self.n_time, self.n_node, self.n_cell = int(n_time), self.xyz.shape[0], self.cells.shape[0]
+        h = C.c_void_p()
+        _check(lib.gfx_green_create(self.n_time, self.n_node, _ptr(self.xyz), self.n_cell,
+                                    _ptr(self.cells, C.c_int32), C.byref(h)))
+        self.h = h
+
+    def set_values(self, t, v):
+        v = np.ascontiguousarray(v, np.float32).reshape(self.n_node, 18)
+        _check(lib.gfx_green_set_values(self.h, int(t), v.ctypes.data_as(_fp)))
+
+    def get_values(self, t):
+        v = np.zeros((self.n_node, 18), np.float32)
+        _check(lib.gfx_green_get_values(self.h, int(t), v.ctypes.data_as(_fp)))
+        return v
+
+    def set_capture_map(self, off, rec_point, weight):
+        off = np.ascontiguousarray(off, np.int64)
+        rp = np.ascontiguousarray(rec_point, np.int32)
+        w = np.ascontiguousarray(weight, np.float64)
+        _check(lib.gfx_green_set_capture_map(self.h, off.ctypes.data_as(C.POINTER(C.c_int64)), _ptr(rp, C.c_int32),
+                                             _ptr(w)))
+
+    def capture(self, solver, snap, field_of_dir, t):
+        fd = np.ascontiguousarray(field_of_dir, np.int32)
+        _check(lib.gfx_green_capture(self.h, solver.h, int(snap), _ptr(fd, C.c_int32), int(t)))
+
+    def interpolate(self, xyz, tix):
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        tix = np.ascontiguousarray(np.broadcast_to(tix, (xyz.shape[0],)), np.int32)
+        out = np.zeros((xyz.shape[0], 18), np.float32)
+        cell = np.zeros(xyz.shape[0], np.int32)
+        _check(lib.gfx_green_interpolate(self.h, xyz.shape[0], _ptr(xyz), _ptr(tix, C.c_int32),
+                                         out.ctypes.data_as(_fp), _ptr(cell, C.c_int32)))
+        return out, cell
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.gfx_green_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def green_capture_map(mesh, rec_cell, tile_nodes):
+    """CSR map tile node -> (recording point, lumped mass) over every point of
+    the recording cells that sits on the node (the mass-weighted merge)."""
+    rec_cell = np.asarray(rec_cell, np.int64)
+    ib = mesh.ibool.reshape(-1, 125)[rec_cell].ravel()
+    mass = mesh.mass.reshape(-1, 125)[rec_cell].ravel()
+    pos = np.full(mesh.n_node, -1, np.int64)
+    pos[np.asarray(tile_nodes, np.int64)] = np.arange(len(tile_nodes))
+    sel = pos[ib] >= 0
+    tn = pos[ib][sel]
+    rp = np.flatnonzero(sel).astype(np.int64)
+    order = np.argsort(tn, kind="stable")
+    tn, rp, w = tn[order], rp[order], mass[sel][order]
+    off = np.zeros(len(tile_nodes) + 1, np.int64)
+    np.add.at(off, tn + 1, 1)
+    return np.cumsum(off), rp.astype(np.int32), w
+
+
+class MLP:
+    """Fourier-feature coordinate MLP on the device (BF16 MFMA, FP32 master weights)."""
+
+    def __init__(self, in_lo, in_hi, out_scale, **cfg):
+        c = MlpConfig()
+        _check(lib.gfx_mlp_config_default(C.byref(c)))
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        self.cfg = c
+        lo, hi, sc = (np.ascontiguousarray(a, np.float64) for a in (in_lo, in_hi, out_scale))
+        h = C.c_void_p()
+        _check(lib.gfx_mlp_create(C.byref(c), _ptr(lo), _ptr(hi), _ptr(sc), C.byref(h)))
+        self.h = h
+        n = C.c_size_t(0)
+        _check(lib.gfx_mlp_n_params(self.h, C.byref(n)))
+        self.n_params = n.value
+
+    def params(self):
+        w = np.zeros(self.n_params, np.float32)
+        b = np.zeros((self.cfg.n_freq, 4), np.float32)
+        _check(lib.gfx_mlp_get_params(self.h, w.ctypes.data_as(_fp), self.n_params, b.ctypes.data_as(_fp)))
+        return w, b
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        _check(lib.gfx_mlp_set_params(self.h, w.ctypes.data_as(_fp), w.size))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, 4)
+        y = np.zeros((x.shape[0], 18), np.float32)
+        _check(lib.gfx_mlp_forward(self.h, x.shape[0], _ptr(x), y.ctypes.data_as(_fp)))
+        return y
+
+    def gradient(self, x, y):
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, 4)
+        y = np.ascontiguousarray(y, np.float32).reshape(-1, 18)
+        g = np.zeros(self.n_params, np.float32)
+        loss = C.c_double(0)
+        _check(lib.gfx_mlp_gradient(self.h, x.shape[0], _ptr(x), y.ctypes.data_as(_fp), g.ctypes.data_as(_fp),
+                                    C.byref(loss)))
+        return g, loss.value
+
+    def train(self, tile, t0, dt, iters, seed=1):
+        loss = C.c_double(0)
+        _check(lib.gfx_mlp_train(self.h, tile.h, t0, dt, int(iters), int(seed), C.byref(loss)))
+        return loss.value
+
+    def query(self, tile, fallback, t0, dt, xyz, tix):
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        tix = np.ascontiguousarray(np.broadcast_to(tix, (xyz.shape[0],)), np.int32)
+        fb = np.ascontiguousarray(fallback, np.uint8)
+        out = np.zeros((xyz.shape[0], 18), np.float32)
+        nf = C.c_int(0)
+        _check(lib.gfx_surrogate_query(self.h, tile.h, fb.ctypes.data_as(C.POINTER(C.c_uint8)), t0, dt,
+                                       xyz.shape[0], _ptr(xyz), _ptr(tix, C.c_int32), out.ctypes.data_as(_fp),
+                                       C.byref(nf)))
+        return out, nf.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.gfx_mlp_destroy(self.h)
             self.h = None
 
     def __del__(self):
