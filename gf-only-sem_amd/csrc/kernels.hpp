@@ -370,6 +370,129 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
     }
 }
 
+// Block kernel, two fields per pass: the same per-point work as k_element
+// (nodal C-PML only), with fields f and f+1 staged and computed together, so
+// each pass has two independent dependency chains per lane and one pair of
+// workgroup barriers serves two fields.
+template <int EPB, int MODE>
+__global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
+    k_element_pair(ElemArgs A, int e0, int ne) {
+    __shared__ float sD[25];
+    __shared__ float su[2][EPB][3][NP];
+    __shared__ float sf[2][EPB][9][NP];
+    const int tid = threadIdx.x;
+    if (tid < 25) sD[tid] = A.D[tid];
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int le = tid / NP;
+    const int p = tid - le * NP;
+    const int eloc = blk * EPB + le;
+    const bool valid = (le < EPB) && (eloc < ne);
+    const int e = e0 + (valid ? eloc : 0);
+    const int i = p / 25, j = (p / 5) % 5, k = p % 5;
+    const int lel = valid ? le : 0;
+
+    float g[9];
+    float lam = 0.f, mu = 0.f;
+    uint32_t nodef = 0;
+    const int64_t pt = int64_t(e) * NP + p;
+    if (valid) {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) g[c] = __builtin_nontemporal_load(&A.G[c * A.P + pt]);
+        lam = __builtin_nontemporal_load(&A.lam[pt]);
+        mu = __builtin_nontemporal_load(&A.mu[pt]);
+        nodef = __builtin_nontemporal_load(&A.ibool[pt]);
+    } else {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) g[c] = 0.f;
+    }
+    const int64_t node = int64_t(nodef & ~FIRST_TOUCH);
+    const bool first = (nodef & FIRST_TOUCH) != 0;
+    const int64_t rslot = (MODE == ASM_GATHER && valid) ? A.slot[pt] : 0;
+    int sb = 0, se = 0;
+    if (valid) {
+        sb = A.src_off[e];
+        se = A.src_off[e + 1];
+    }
+    const int step = A.ctr[0];
+    const int nf = A.nf;
+    float un[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};  // fields f, f+1 (prefetched a pass ahead)
+    if (valid) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+            if (h < nf) {
+                const float* ut = A.U + int64_t(h) * A.fstride + node * 3;
+                un[h][0] = ut[0];
+                un[h][1] = ut[1];
+                un[h][2] = ut[2];
+            }
+    }
+    __syncthreads();
+    float Di[5], Dj[5], Dk[5], Dti[5], Dtj[5], Dtk[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        Di[q] = sD[i * 5 + q];
+        Dj[q] = sD[j * 5 + q];
+        Dk[q] = sD[k * 5 + q];
+        Dti[q] = sD[q * 5 + i];
+        Dtj[q] = sD[q * 5 + j];
+        Dtk[q] = sD[q * 5 + k];
+    }
+
+    for (int f = 0; f < nf; f += 2) {
+        const bool two = f + 1 < nf;
+        if (valid) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                su[h][le][0][p] = un[h][0];
+                su[h][le][1][p] = un[h][1];
+                su[h][le][2][p] = un[h][2];
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                if (f + 2 + h < nf) {
+                    const float* ut = A.U + int64_t(f + 2 + h) * A.fstride + node * 3;
+                    un[h][0] = ut[0];
+                    un[h][1] = ut[1];
+                    un[h][2] = ut[2];
+                }
+        }
+        float rold[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+        if (MODE == ASM_COLORED && valid && !first) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                if (h == 0 || two) {
+                    const float* r = A.R + int64_t(f + h) * A.fstride + node * 3;
+                    rold[h][0] = r[0];
+                    rold[h][1] = r[1];
+                    rold[h][2] = r[2];
+                }
+        }
+        __syncthreads();
+        if (valid) {
+            float F0[9], F1[9];
+            point_flux(su[0][lel][0], su[0][lel][1], su[0][lel][2], Di, Dj, Dk, i, j, k, g, lam, mu, F0);
+            point_flux(su[1][lel][0], su[1][lel][1], su[1][lel][2], Di, Dj, Dk, i, j, k, g, lam, mu, F1);
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                sf[0][le][q][p] = F0[q];
+                sf[1][le][q][p] = F1[q];
+            }
+        }
+        __syncthreads();
+        if (valid) {
+            float rv0[3], rv1[3];
+            point_div(&sf[0][lel][0][0], NP, Dti, Dtj, Dtk, i, j, k, rv0);
+            point_div(&sf[1][lel][0][0], NP, Dti, Dtj, Dtk, i, j, k, rv1);
+            add_sources(A, step, f, sb, se, p, rv0);
+            emit_residual<MODE>(A, f, node, rslot, rold[0], rv0);
+            if (two) {
+                add_sources(A, step, f + 1, sb, se, p, rv1);
+                emit_residual<MODE>(A, f + 1, node, rslot, rold[1], rv1);
+            }
+        }
+    }
+}
+
 // One wavefront per element, lane l owns points l and l + 64 (l < 61).  All
 // exchange between the element's points stays inside one wavefront, so the
 // field loop has no workgroup barrier; WPB elements (wavefronts) per block.

@@ -183,8 +183,46 @@ void launch_wave_t(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
     }
 }
 
+template <int EPB>
+void launch_pair_t(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
+    constexpr int NT = ((EPB * NP + 63) / 64) * 64;
+    const int grid = (n + EPB - 1) / EPB;
+    if (grid == 0) return;
+    switch (S.assembly) {
+        case GFX_ASSEMBLY_ATOMIC:
+            dev::k_element_pair<EPB, dev::ASM_ATOMIC><<<grid, NT, 0, S.stream>>>(a, b, n);
+            break;
+        case GFX_ASSEMBLY_GATHER:
+            dev::k_element_pair<EPB, dev::ASM_GATHER><<<grid, NT, 0, S.stream>>>(a, b, n);
+            break;
+        default:
+            dev::k_element_pair<EPB, dev::ASM_COLORED><<<grid, NT, 0, S.stream>>>(a, b, n);
+    }
+}
+
+// Element kernel form for a request: AUTO = wave when the nodal C-PML form
+// applies, else block; WAVE and PAIR need the nodal form.
+int resolve_kernel(int requested, bool nodal_ok) {
+    require(requested >= GFX_ELEMENT_AUTO && requested <= GFX_ELEMENT_PAIR, "unknown element kernel");
+    if (requested == GFX_ELEMENT_WAVE || requested == GFX_ELEMENT_PAIR)
+        require(nodal_ok, "the wave and pair element kernels need the nodal C-PML form");
+    if (requested == GFX_ELEMENT_AUTO) return nodal_ok ? GFX_ELEMENT_WAVE : GFX_ELEMENT_BLOCK;
+    return requested;
+}
+
+void check_epb(int kernel, int epb) {
+    require(epb == 0 || epb == 1 || epb == 2 || epb == 4 || epb == 8, "elems_per_block must be 1, 2, 4 or 8");
+    require(!(kernel == GFX_ELEMENT_PAIR && epb == 8), "the pair element kernel takes 1, 2 or 4 elements per block");
+}
+
 void launch_element(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
-    if (S.kernel == GFX_ELEMENT_WAVE) {
+    if (S.kernel == GFX_ELEMENT_PAIR) {
+        switch (S.epb) {
+            case 1: launch_pair_t<1>(S, a, b, n); break;
+            case 4: launch_pair_t<4>(S, a, b, n); break;
+            default: launch_pair_t<2>(S, a, b, n); break;
+        }
+    } else if (S.kernel == GFX_ELEMENT_WAVE) {
         switch (S.epb) {
             case 1: launch_wave_t<1>(S, a, b, n); break;
             case 2: launch_wave_t<2>(S, a, b, n); break;
@@ -465,7 +503,7 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
     require(d.assembly == GFX_ASSEMBLY_COLORED || d.assembly == GFX_ASSEMBLY_ATOMIC ||
                 d.assembly == GFX_ASSEMBLY_GATHER,
             "unknown assembly mode");
-    require(d.element_kernel >= GFX_ELEMENT_AUTO && d.element_kernel <= GFX_ELEMENT_WAVE,
+    require(d.element_kernel >= GFX_ELEMENT_AUTO && d.element_kernel <= GFX_ELEMENT_PAIR,
             "unknown element kernel");
     int epb = d.elems_per_block;
     require(epb == 0 || epb == 1 || epb == 2 || epb == 4 || epb == 8,
@@ -721,9 +759,8 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
 
     // element kernel: the wavefront-per-element form needs no element-local C-PML state
     const bool wave_ok = !(S.has_cpml && !S.pml_nodal);
-    if (d.element_kernel == GFX_ELEMENT_WAVE)
-        require(wave_ok, "the wave element kernel needs the nodal C-PML form");
-    S.kernel = (d.element_kernel == GFX_ELEMENT_BLOCK || !wave_ok) ? GFX_ELEMENT_BLOCK : GFX_ELEMENT_WAVE;
+    S.kernel = resolve_kernel(d.element_kernel, wave_ok);
+    check_epb(S.kernel, epb);
     S.epb = epb ? epb : (S.kernel == GFX_ELEMENT_WAVE ? 4 : 2);
 
     // ---- nodal state: U, W, R [nf][fstride]; a at special nodes [nf][astride] ----
@@ -1156,15 +1193,10 @@ extern "C" gfx_status gfx_solver_kernel_time(const gfx_solver* s, const char* na
 extern "C" gfx_status gfx_solver_set_launch(gfx_solver* s, int element_kernel, int elems_per_block) {
     return guarded([&] {
         require(s != nullptr, "null solver");
-        require(element_kernel >= GFX_ELEMENT_AUTO && element_kernel <= GFX_ELEMENT_WAVE,
-                "unknown element kernel");
-        require(elems_per_block == 0 || elems_per_block == 1 || elems_per_block == 2 ||
-                    elems_per_block == 4 || elems_per_block == 8,
-                "elems_per_block must be 1, 2, 4 or 8");
         const bool wave_ok = !(s->has_cpml && !s->pml_nodal);
-        if (element_kernel == GFX_ELEMENT_WAVE)
-            require(wave_ok, "the wave element kernel needs the nodal C-PML form");
-        s->kernel = (element_kernel == GFX_ELEMENT_BLOCK || !wave_ok) ? GFX_ELEMENT_BLOCK : GFX_ELEMENT_WAVE;
+        const int kern = resolve_kernel(element_kernel, wave_ok);
+        check_epb(kern, elems_per_block);
+        s->kernel = kern;
         s->epb = elems_per_block ? elems_per_block : (s->kernel == GFX_ELEMENT_WAVE ? 4 : 2);
         drop_graph(*s);
     });
@@ -1197,8 +1229,9 @@ extern "C" gfx_status gfx_autotune(const gfx_solver_desc* base, int trial_steps,
         double best_t = 1e300;
         gfx_solver_desc pick = *base;
         const int asms[2] = {GFX_ASSEMBLY_COLORED, GFX_ASSEMBLY_GATHER};
-        const int launch[6][2] = {{GFX_ELEMENT_BLOCK, 1}, {GFX_ELEMENT_BLOCK, 2}, {GFX_ELEMENT_BLOCK, 4},
-                                  {GFX_ELEMENT_WAVE, 2},  {GFX_ELEMENT_WAVE, 4},  {GFX_ELEMENT_WAVE, 8}};
+        const int launch[8][2] = {{GFX_ELEMENT_BLOCK, 1}, {GFX_ELEMENT_BLOCK, 2}, {GFX_ELEMENT_BLOCK, 4},
+                                  {GFX_ELEMENT_WAVE, 2},  {GFX_ELEMENT_WAVE, 4},  {GFX_ELEMENT_WAVE, 8},
+                                  {GFX_ELEMENT_PAIR, 1},  {GFX_ELEMENT_PAIR, 2}};
         auto time_steps = [&](gfx_solver& S) {
             run_steps(S, 1);  // warm-up
             hipEvent_t e0, e1;
@@ -1229,9 +1262,10 @@ extern "C" gfx_status gfx_autotune(const gfx_solver_desc* base, int trial_steps,
             create(d, *S);
             double best_here = 1e300;
             int best_lc = -1;
-            for (int li = 0; li < 6; ++li) {
+            for (int li = 0; li < 8; ++li) {
                 const auto& lc = launch[li];
-                if (lc[0] == GFX_ELEMENT_WAVE && S->has_cpml && !S->pml_nodal) continue;
+                if (lc[0] != GFX_ELEMENT_BLOCK && S->has_cpml && !S->pml_nodal) continue;
+                if (lc[0] == GFX_ELEMENT_PAIR && S->nf < 2) continue;
                 S->kernel = lc[0];
                 S->epb = lc[1];
                 const double per = time_steps(*S);

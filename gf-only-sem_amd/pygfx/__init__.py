@@ -20,6 +20,7 @@ GFX_ASSEMBLY_GATHER = 2
 GFX_ELEMENT_AUTO = 0
 GFX_ELEMENT_BLOCK = 1
 GFX_ELEMENT_WAVE = 2
+GFX_ELEMENT_PAIR = 3
 MAX_LAYERS = 8
 
 _dp = C.POINTER(C.c_double)

@@ -80,7 +80,8 @@ typedef enum {
 typedef enum {
     GFX_ELEMENT_AUTO = 0,  /* wave kernel when applicable, else block kernel */
     GFX_ELEMENT_BLOCK = 1, /* one lane per GLL point, workgroup barriers between stages */
-    GFX_ELEMENT_WAVE = 2   /* one wavefront per element, two points per lane, no barriers */
+    GFX_ELEMENT_WAVE = 2,  /* one wavefront per element, two points per lane, no barriers */
+    GFX_ELEMENT_PAIR = 3   /* as BLOCK, two fields per pass (nodal C-PML only) */
 } gfx_element_kernel;
 
 typedef struct {

@@ -113,7 +113,9 @@ def _records(s, field, n_snap):
     (pygfx.GFX_ASSEMBLY_COLORED, pygfx.GFX_ELEMENT_WAVE),
     (pygfx.GFX_ASSEMBLY_COLORED, pygfx.GFX_ELEMENT_BLOCK),
     (pygfx.GFX_ASSEMBLY_ATOMIC, pygfx.GFX_ELEMENT_WAVE),
-    (pygfx.GFX_ASSEMBLY_ATOMIC, pygfx.GFX_ELEMENT_BLOCK)])
+    (pygfx.GFX_ASSEMBLY_ATOMIC, pygfx.GFX_ELEMENT_BLOCK),
+    (pygfx.GFX_ASSEMBLY_GATHER, pygfx.GFX_ELEMENT_PAIR),
+    (pygfx.GFX_ASSEMBLY_COLORED, pygfx.GFX_ELEMENT_PAIR)])
 def test_solver_matches_oracle_halfspace(assembly, kernel):
     mesh = mc.small_halfspace()
     cells, w, stf = mc.surface_centre_source(mesh)
@@ -217,6 +219,25 @@ def test_batched_fields_equal_single_runs(assembly):
             assert np.array_equal(x, y)
         n_snap = s1.n_snapshots
         assert np.array_equal(_records(sb, f, n_snap), _records(s1, 0, n_snap))
+
+
+@pytest.mark.parametrize("assembly,nf", [(pygfx.GFX_ASSEMBLY_COLORED, 3), (pygfx.GFX_ASSEMBLY_GATHER, 4)])
+def test_pair_kernel_matches_block_kernel_and_oracle(assembly, nf):
+    """Two fields per pass (odd and even field counts) against one per pass,
+    and field 0 against the oracle."""
+    mesh = mc.small_layered(duration=0.8)
+    cells, w, stf = mc.buried_source(mesh)
+    c2, w2, stf2 = mc.buried_source(mesh, frac=(0.4, 0.6, 0.0), depth=900.0)
+    srcs = [(cells, w, stf, 2), (c2, w2, stf2, 0), (cells, w, stf, 1), (c2, w2, stf2, 2)][:nf]
+    a = _gpu(mesh, srcs, assembly=assembly, kernel=pygfx.GFX_ELEMENT_PAIR, epb=2)
+    b = _gpu(mesh, srcs, assembly=assembly, kernel=pygfx.GFX_ELEMENT_BLOCK, epb=2)
+    assert int(a.query("element_kernel")) == pygfx.GFX_ELEMENT_PAIR
+    for f in range(nf):
+        for x, y in zip(a.get_state(f), b.get_state(f)):
+            assert mc.rel_l2(x, y) <= 1e-6
+    ref = _oracle(mesh, cells, w, stf, 2)
+    got = np.stack([a.get_record(0, k) for k in range(a.n_snapshots)])
+    assert mc.rel_l2(got, ref["strain"][:a.n_snapshots]) <= TOL
 
 
 @pytest.mark.parametrize("assembly", [pygfx.GFX_ASSEMBLY_COLORED, pygfx.GFX_ASSEMBLY_GATHER])
