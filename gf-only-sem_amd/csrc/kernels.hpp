@@ -607,6 +607,40 @@ __device__ __forceinline__ void gather_r(const NodeArgs& A, int f, int kb, int k
     }
 }
 
+// LDS-staged fixed-order slot sums for the nodes of one workgroup: the
+// workgroup's slots [s0, s1) of field f come into LDS in chunks with coalesced
+// loads, then each lane adds its nodes' slots in increasing slot order -- the
+// order gather_r uses, so the sums are bit-identical.  Every lane of the
+// workgroup must call it (it holds workgroup barriers); kb[0..NN] are the
+// lane's node slot bounds.
+constexpr int SLOT_CHUNK = 2048;
+template <int NN>
+__device__ __forceinline__ void staged_sums(const NodeArgs& A, int f, int64_t s0, int64_t s1, const int* kb,
+                                            bool active, float* __restrict__ sl, float* r) {
+#pragma unroll
+    for (int q = 0; q < 3 * NN; ++q) r[q] = 0.f;
+    const float* Rl = A.Rl + int64_t(f) * A.P * 3;
+    for (int64_t c0 = s0; c0 < s1; c0 += SLOT_CHUNK) {
+        const int64_t c1 = min(s1, c0 + SLOT_CHUNK);
+        const int cnt = int(c1 - c0) * 3;
+        __syncthreads();  // the previous chunk has been consumed
+        for (int q = threadIdx.x; q < cnt; q += blockDim.x) sl[q] = Rl[c0 * 3 + q];
+        __syncthreads();
+        if (active) {
+#pragma unroll
+            for (int n = 0; n < NN; ++n) {
+                const int64_t qb = max(int64_t(kb[n]), c0), qe = min(int64_t(kb[n + 1]), c1);
+                for (int64_t q = qb; q < qe; ++q) {
+                    const float* c = sl + (q - c0) * 3;
+                    r[3 * n + 0] += c[0];
+                    r[3 * n + 1] += c[1];
+                    r[3 * n + 2] += c[2];
+                }
+            }
+        }
+    }
+}
+
 // Special nodes [0, Ga): C-PML terms, legacy damping, massless rule, a_n kept.
 // FG field groups per workgroup: the 256 lanes cover 256/FG nodes, and lane
 // group fy handles fields fy, fy + FG, ... (each wavefront stays on one field
@@ -614,9 +648,15 @@ __device__ __forceinline__ void gather_r(const NodeArgs& A, int f, int kb, int k
 template <int FG>
 __global__ void __launch_bounds__(256) k_node_special(NodeArgs A) {
     constexpr int NPB = 256 / FG;
+    // one field group: the workgroup's slot sums are staged through LDS
+    constexpr bool STAGE = FG == 1;
+    __shared__ float sl[STAGE ? SLOT_CHUNK * 3 : 1];
     const int fy = threadIdx.x / NPB;
-    const int64_t g = int64_t(blockIdx.x) * NPB + (threadIdx.x - fy * NPB);
-    if (g >= A.Ga) return;
+    const int64_t g0 = int64_t(blockIdx.x) * NPB + (threadIdx.x - fy * NPB);
+    const bool active = g0 < A.Ga;
+    const bool staged = STAGE && A.gather;  // uniform over the workgroup
+    if (!active && !staged) return;
+    const int64_t g = active ? g0 : A.Ga - 1;
     const float mi = A.minv[g];
     const float dt = A.dt, hdt = 0.5f * dt, hdt2 = 0.5f * dt * dt;
     const float dmp = A.damp ? A.damp[g] : 0.f;
@@ -628,16 +668,27 @@ __global__ void __launch_bounds__(256) k_node_special(NodeArgs A) {
 #pragma unroll
         for (int q = 0; q < 9; ++q) al[q] = A.palpha[q * A.Gpml + g];
     }
-    int kb = 0, ke = 0;
+    int kb[2] = {0, 0};
+    int64_t s0 = 0, s1 = 0;
     if (A.gather) {
-        kb = A.off[g];
-        ke = A.off[g + 1];
+        kb[0] = A.off[g];
+        kb[1] = A.off[g + 1];
+        if (staged) {
+            const int64_t gb = int64_t(blockIdx.x) * NPB;
+            s0 = A.off[gb];
+            s1 = A.off[min(A.Ga, gb + NPB)];
+        }
     }
     for (int f = fy; f < A.nf; f += FG) {
         const int64_t o = int64_t(f) * A.fstride + g * 3;
         const int64_t oa = int64_t(f) * A.astride + g * 3;
         float U[3], W[3], R[3], an[3], un[3], vn[3];
-        if (A.gather) gather_r(A, f, kb, ke, R);
+        if (staged) {
+            staged_sums<1>(A, f, s0, s1, kb, active, sl, R);
+            if (!active) continue;
+        } else if (A.gather) {
+            gather_r(A, f, kb[0], kb[1], R);
+        }
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             U[c] = A.U[o + c];
@@ -693,24 +744,37 @@ __global__ void __launch_bounds__(256) k_node_special(NodeArgs A) {
 template <bool GATHER, int FG>
 __global__ void __launch_bounds__(256) k_node_regular(NodeArgs A) {
     constexpr int NPB = 256 / FG;
+    constexpr bool STAGE = GATHER && FG == 1;  // slot sums staged through LDS
+    __shared__ float sl[STAGE ? SLOT_CHUNK * 3 : 1];
     const int fy = threadIdx.x / NPB;
     const int64_t t = int64_t(blockIdx.x) * NPB + (threadIdx.x - fy * NPB);
-    const int64_t g = A.Ga + 4 * t;
-    if (g >= A.G4) return;
+    const int64_t g0 = A.Ga + 4 * t;
+    const bool active = g0 < A.G4;
+    if (!active && !STAGE) return;
+    const int64_t g = active ? g0 : A.G4 - 4;
     const float dt = A.dt;
     const float4 mi4 = *reinterpret_cast<const float4*>(A.minv + g);
     const float mi[4] = {mi4.x, mi4.y, mi4.z, mi4.w};
     int kb[5] = {0, 0, 0, 0, 0};
+    int64_t s0 = 0, s1 = 0;
     if (GATHER) {
 #pragma unroll
         for (int q = 0; q < 5; ++q) kb[q] = A.off[g + q];
+        if (STAGE) {
+            const int64_t gb = A.Ga + int64_t(blockIdx.x) * NPB * 4;
+            s0 = A.off[gb];
+            s1 = A.off[min(A.G4, gb + int64_t(NPB) * 4)];
+        }
     }
     for (int f = fy; f < A.nf; f += FG) {
         const int64_t o = int64_t(f) * A.fstride + g * 3;
         float4* U4 = reinterpret_cast<float4*>(A.U + o);
         float4* W4 = reinterpret_cast<float4*>(A.W + o);
         float u[12], w[12], r[12];
-        if (GATHER) {
+        if (STAGE) {
+            staged_sums<4>(A, f, s0, s1, kb, active, sl, r);
+            if (!active) continue;
+        } else if (GATHER) {
 #pragma unroll
             for (int n = 0; n < 4; ++n) gather_r(A, f, kb[n], kb[n + 1], &r[3 * n]);
         } else {
