@@ -249,8 +249,10 @@ def surrogate_run(pygfx, args):
 
 
 def cpu_baseline(seconds=15.0):
-    """FP64 CPU restatement of the reference solver (oracle/), one core, on a
-    20 x 20 x 10 sub-box of the same 0.6 km layered discretisation."""
+    """FP64 CPU restatement of the reference solver (oracle/) on a 20 x 20 x 10
+    sub-box of the same 0.6 km layered discretisation: one thread (the
+    reference's single-process solver) and all host threads (its element-local and nodal loops
+    parallel; results identical).  The all-threads figure is the baseline."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pygfx
     import pyoracle
@@ -258,23 +260,31 @@ def cpu_baseline(seconds=15.0):
     mesh = pygfx.build_box(cfg2_spec(pygfx, nx=20, ny=20, nz=10, duration=30.0))
     cells, w = mesh.locate_source(6000.0, 6000.0, 0.0)
     stf = pygfx.ricker(1.5, 1.0, 1e20, mesh.dt, mesh.nsteps)
-    # calibrate: 5 steps, then as many as fit the time budget
-    probe = pyoracle.Run(mesh.arrays, mesh.n_elem, 5, mesh.n_node, mesh.dt, 5, mesh.snapshot_stride,
-                         2, cells, w, stf, mesh.rec_cell)
-    t0 = time.perf_counter()
-    probe.run()
-    per = (time.perf_counter() - t0) / 5
-    n = max(5, min(mesh.nsteps, int(seconds / per)))
-    run = pyoracle.Run(mesh.arrays, mesh.n_elem, 5, mesh.n_node, mesh.dt, n, mesh.snapshot_stride,
-                       2, cells, w, stf, mesh.rec_cell)
-    t0 = time.perf_counter()
-    run.run()
-    dt = time.perf_counter() - t0
-    return {"value": mesh.n_elem * 125 * n / dt, "unit": "GLL-point updates/s", "cores": 1,
-            "kind": "port",
-            "sample": f"oracle (FP64 C restatement of the reference CPU step), 1 thread, "
-                      f"{mesh.n_elem}-element sub-box of the 0.6 km layered mesh, z force, "
-                      f"{n} steps with C-PML and recording, {dt:.1f} s"}
+
+    def timed(threads, budget):
+        got = pyoracle.set_threads(threads)
+        probe = pyoracle.Run(mesh.arrays, mesh.n_elem, 5, mesh.n_node, mesh.dt, 5, mesh.snapshot_stride,
+                             2, cells, w, stf, mesh.rec_cell)
+        t0 = time.perf_counter()
+        probe.run()
+        per = (time.perf_counter() - t0) / 5
+        n = max(5, min(mesh.nsteps, int(budget / per)))
+        run = pyoracle.Run(mesh.arrays, mesh.n_elem, 5, mesh.n_node, mesh.dt, n, mesh.snapshot_stride,
+                           2, cells, w, stf, mesh.rec_cell)
+        t0 = time.perf_counter()
+        run.run()
+        dt = time.perf_counter() - t0
+        return got, n, dt, mesh.n_elem * 125 * n / dt
+
+    one = timed(1, seconds * 0.5)
+    nthreads = min(16, os.cpu_count() or 1)
+    many = timed(nthreads, seconds * 0.5)
+    return {"value": many[3], "unit": "GLL-point updates/s", "cores": many[0], "kind": "port",
+            "sample": f"oracle (FP64 C restatement of the reference CPU step, element-local and nodal loops on {many[0]} "
+                      f"threads), {mesh.n_elem}-element sub-box of the 0.6 km layered mesh, z force, "
+                      f"{many[1]} steps with C-PML and recording, {many[2]:.1f} s",
+            "value_1_core": one[3],
+            "sample_1_core": f"same, 1 thread, {one[1]} steps, {one[2]:.1f} s"}
 
 
 def main():

@@ -18,6 +18,10 @@
  */
 #include "gfx_oracle.h"
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -122,6 +126,8 @@ void gfo_element_residual(int n_elem, const double* dxi_dx, const double* jacobi
                           const double* lambda_, const double* mu_, const double* D,
                           const double* weights, int ngll, const double* u, double* r) {
     const int n_node = ngll * ngll * ngll;
+    /* elements are independent (each writes its own block of r) */
+#pragma omp parallel for schedule(static)
     for (long e = 0; e < n_elem; ++e) {
         const double* edd = dxi_dx + e * n_node * 9;
         const double* ejac = jacobian + e * n_node;
@@ -205,6 +211,7 @@ void gfo_scatter_to_rank(const double* loc, const int32_t* ibool, int n_elem, in
 
 void gfo_gather_from_rank(const double* glob, const int32_t* ibool, int n_elem, int n_node,
                           double* loc) {
+#pragma omp parallel for schedule(static)
     for (long e = 0; e < n_elem; ++e)
         for (int n = 0; n < n_node; ++n) {
             const long g = ibool[e * n_node + n];
@@ -219,11 +226,13 @@ void gfo_gather_from_rank(const double* glob, const int32_t* ibool, int n_elem, 
 void gfo_newmark_predict(int n_dof, double dt, double beta, const double* u, const double* v,
                          const double* a, double* ut) {
     const double c = 0.5 * dt * dt * (1.0 - 2.0 * beta);
+#pragma omp parallel for schedule(static)
     for (long i = 0; i < n_dof; ++i) ut[i] = u[i] + dt * v[i] + c * a[i];
 }
 
 void gfo_newmark_correct(int n_dof, double dt, double beta, double gamma, const double* mass,
                          double* u, double* v, double* a, const double* r) {
+#pragma omp parallel for schedule(static)
     for (long i = 0; i < n_dof; ++i) {
         const double m = mass[i / 3];
         if (m <= 0.0) {
@@ -257,6 +266,7 @@ void gfo_cpml_update_displ_fields(gfo_cpml* p, const double* u, const double* v,
     double* t = p->displ_old;
     p->displ_old = p->displ_new;
     p->displ_new = t;
+#pragma omp parallel for schedule(static)
     for (long e = 0; e < p->n_elem; ++e) {
         if (region_of(p, (int)e) == 0) continue;
         for (int n = 0; n < n_node; ++n) {
@@ -265,6 +275,7 @@ void gfo_cpml_update_displ_fields(gfo_cpml* p, const double* u, const double* v,
                 p->displ_new[(e * n_node + n) * 3 + d] = u[g * 3 + d] + c1 * v[g * 3 + d];
         }
     }
+#pragma omp parallel for schedule(static)
     for (long e = 0; e < p->n_elem; ++e) {
         if (region_of(p, (int)e) == 0) continue;
         for (int n = 0; n < n_node; ++n) {
@@ -277,6 +288,7 @@ void gfo_cpml_update_displ_fields(gfo_cpml* p, const double* u, const double* v,
 /* pml.cpp:112-149: m[c*3+d] = a0_d m + a1_d new_c + a2_d old_c. */
 void gfo_cpml_update_displ_memory(gfo_cpml* p) {
     const int n_node = p->ngll * p->ngll * p->ngll;
+#pragma omp parallel for schedule(static)
     for (long e = 0; e < p->n_elem; ++e) {
         if (region_of(p, (int)e) == 0) continue;
         for (int n = 0; n < n_node; ++n) {
@@ -299,6 +311,7 @@ void gfo_cpml_accel_contribution(const gfo_cpml* p, const double* u, const doubl
                                  const double* gll_w, double* r_loc) {
     const int ngll = p->ngll;
     const int n_node = ngll * ngll * ngll;
+#pragma omp parallel for schedule(static)
     for (long e = 0; e < p->n_elem; ++e) {
         if (region_of(p, (int)e) == 0) continue;
         for (int n = 0; n < n_node; ++n) {
@@ -327,6 +340,7 @@ void gfo_cpml_accel_contribution(const gfo_cpml* p, const double* u, const doubl
 void gfo_full_strain(int n_cell, const int32_t* cells, const double* dxi_dx, const double* D,
                      int ngll, const double* u_loc_all, double* strain) {
     const int n_node = ngll * ngll * ngll;
+#pragma omp parallel for schedule(static)
     for (long ci = 0; ci < n_cell; ++ci) {
         const long e = cells ? cells[ci] : ci;
         const double* eu = u_loc_all + e * n_node * 3;
@@ -558,3 +572,15 @@ void gfo_random_fill(double* v, int n, unsigned seed) {
     srand(seed);
     for (int i = 0; i < n; ++i) v[i] = 2.0 * (double)rand() / RAND_MAX - 1.0;
 }
+
+/* Threads used by the parallel loops (OpenMP; 1 when built without it). */
+int gfo_set_threads(int n) {
+#ifdef _OPENMP
+    if (n >= 1) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+

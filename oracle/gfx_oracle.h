@@ -129,6 +129,11 @@ int gfo_run_state(const gfo_problem* pb, double* u, double* v, double* a);
  * tests/test_element_cuda.cu:27-32 (random_fill). */
 void gfo_random_fill(double* v, int n, unsigned seed);
 
+/* Threads for the element-local and nodal loops (OpenMP; the scatter stays
+ * serial); returns the count in effect.  Results do not depend on it: every
+ * parallel loop writes disjoint entries. */
+int gfo_set_threads(int n);
+
 #ifdef __cplusplus
 }
 #endif

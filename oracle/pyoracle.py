@@ -60,6 +60,7 @@ def _load():
         "gfo_run": (C.c_int, [C.POINTER(Problem), _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
         "gfo_run_state": (C.c_int, [C.POINTER(Problem), _dp, _dp, _dp]),
         "gfo_random_fill": (None, [_dp, C.c_int, C.c_uint]),
+        "gfo_set_threads": (C.c_int, [C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -176,3 +177,9 @@ class Run:
         u, v, a = (np.ascontiguousarray(x, np.float64).copy() for x in (u, v, a))
         assert lib.gfo_run_state(C.byref(self.pb), P(u), P(v), P(a)) == 0
         return u, v, a
+
+
+def set_threads(n):
+    """Threads of the oracle's element loop; returns the count in effect."""
+    return int(lib.gfo_set_threads(int(n)))
+
