@@ -89,6 +89,7 @@ struct NodeArgs {
     int pml_nodal;
     const float* __restrict__ pc;      // [5][Gpml] sum over PML elements of w J / rho * A_k
     const float* __restrict__ palpha;  // [9][Gpml] coef_alpha (dir*3 + {0,1,2})
+    const uint8_t* __restrict__ pmask; // [Gpml] memory directions in use (unused ones stay 0)
     float* __restrict__ pmem;          // [nf][9][Gpml] rmemory (comp*3 + dir)
     float* __restrict__ pnew;          // [nf][3][Gpml] pml_displ_new
     float c1, c2, dt;
@@ -662,11 +663,15 @@ __global__ void __launch_bounds__(256) k_node_special(NodeArgs A) {
     const float dmp = A.damp ? A.damp[g] : 0.f;
     const bool pml = A.pml_nodal && g < A.Gpml;
     float C[5] = {0.f, 0.f, 0.f, 0.f, 0.f}, al[9];
+    bool use[3] = {false, false, false};
     if (pml) {
 #pragma unroll
         for (int q = 0; q < 5; ++q) C[q] = A.pc[q * A.Gpml + g];
 #pragma unroll
         for (int q = 0; q < 9; ++q) al[q] = A.palpha[q * A.Gpml + g];
+        const uint32_t mk = A.pmask[g];
+#pragma unroll
+        for (int dd = 0; dd < 3; ++dd) use[dd] = (mk >> dd) & 1u;
     }
     int kb[2] = {0, 0};
     int64_t s0 = 0, s1 = 0;
@@ -701,9 +706,11 @@ __global__ void __launch_bounds__(256) k_node_special(NodeArgs A) {
         if (pml) {
             float* pmem = A.pmem + int64_t(f) * 9 * A.Gpml;
             float* pnew = A.pnew + int64_t(f) * 3 * A.Gpml;
+            // memory of unused directions is zero and stays zero (alpha = (1, 0, 0),
+            // C = 0): neither read nor written
             float m[9];
 #pragma unroll
-            for (int q = 0; q < 9; ++q) m[q] = pmem[q * A.Gpml + g];
+            for (int q = 0; q < 9; ++q) m[q] = use[q % 3] ? pmem[q * A.Gpml + g] : 0.f;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const float oldv = pnew[c * A.Gpml + g] + A.c2 * an[c];  // pml.cpp:91-109
@@ -712,8 +719,9 @@ __global__ void __launch_bounds__(256) k_node_special(NodeArgs A) {
                         C[4] * m[c * 3 + 2];  // pml.cpp:209-212, assembled over PML elements
 #pragma unroll
                 for (int d = 0; d < 3; ++d)  // pml.cpp:133-146
-                    pmem[(c * 3 + d) * A.Gpml + g] =
-                        al[d * 3 + 0] * m[c * 3 + d] + al[d * 3 + 1] * newv + al[d * 3 + 2] * oldv;
+                    if (use[d])
+                        pmem[(c * 3 + d) * A.Gpml + g] =
+                            al[d * 3 + 0] * m[c * 3 + d] + al[d * 3 + 1] * newv + al[d * 3 + 2] * oldv;
                 pnew[c * A.Gpml + g] = newv;
             }
         }

@@ -56,6 +56,7 @@ struct gfx_solver {
     gfx::DevBuf<uint32_t> ibool;
     // C-PML: nodal (pc, palpha, pmem, pnew over [0,Gpml)) or element-local (Ppml points)
     gfx::DevBuf<float> pc, palpha, pabar, pmem, pnew;
+    gfx::DevBuf<uint8_t> pmask;  // nodal C-PML: memory directions in use (bits x, y, z)
     gfx::DevBuf<int32_t> pml_slot, src_off;
     gfx::DevBuf<float> U, W, R, Aacc, Rl;
     gfx::DevBuf<int32_t> goff, gslot;  // GATHER assembly: node slot ranges, point -> slot
@@ -331,6 +332,7 @@ StepArgs make_step_args(gfx_solver& S) {
     na.pml_nodal = (S.has_cpml && S.pml_nodal) ? 1 : 0;
     na.pc = S.pc.p;
     na.palpha = S.palpha.p;
+    na.pmask = S.pmask.p;
     na.pmem = S.pmem.p;
     na.pnew = S.pnew.p;
     na.c1 = c1;
@@ -605,20 +607,46 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
             if (!c) c = 1;
         }
     }
+    // C-PML memory directions used at each node: direction d when some PML
+    // element point on the node has A_{3+d} != 0 (the others keep zero memory)
+    std::vector<uint8_t> pm_old;
+    if (S.has_cpml) {
+        pm_old.assign(G, 0);
+        for (int64_t e = 0; e < E; ++e) {
+            if (d.pml_region[e] == 0) continue;
+            for (int p = 0; p < NP; ++p) {
+                const int64_t src = e * NP + p;
+                for (int dir = 0; dir < 3; ++dir)
+                    if (d.coef_abar[src * 5 + 2 + dir] != 0.0) pm_old[d.ibool[src]] |= uint8_t(1u << dir);
+            }
+        }
+    }
     // numbering: C-PML nodes, other special nodes, regular nodes; first touch
-    // in solver element order inside each class
+    // in solver element order inside each class; C-PML nodes grouped by their
+    // direction mask (first touch inside a group), so wavefronts share a mask
     S.node_new_of_old.assign(G, -1);
     int next = 0;
     for (int pass = 2; pass >= 0; --pass) {
+        std::vector<int> order;
         for (int i = 0; i < E; ++i) {
             const int64_t e = elem_of_int[i];
             for (int p = 0; p < NP; ++p) {
                 const int g = d.ibool[e * NP + p];
-                if (cls[g] == pass && S.node_new_of_old[g] < 0) S.node_new_of_old[g] = next++;
+                if (cls[g] == pass && S.node_new_of_old[g] < 0) {
+                    S.node_new_of_old[g] = -2;  // seen
+                    order.push_back(g);
+                }
             }
         }
         for (int g = 0; g < G; ++g)  // nodes in no element
-            if (cls[g] == pass && S.node_new_of_old[g] < 0) S.node_new_of_old[g] = next++;
+            if (cls[g] == pass && S.node_new_of_old[g] == -1) {
+                S.node_new_of_old[g] = -2;
+                order.push_back(g);
+            }
+        if (pass == 2)
+            std::stable_sort(order.begin(), order.end(),
+                             [&](int a, int b) { return pm_old[a] < pm_old[b]; });
+        for (int g : order) S.node_new_of_old[g] = next++;
         if (pass == 2) S.Gpml = next;
     }
     int64_t n_special = 0;
@@ -720,6 +748,10 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
             }
             S.pc.upload(hc);
             S.palpha.upload(hal);
+            std::vector<uint8_t> hm(std::max<int64_t>(Gp, 1), 0);
+            for (int64_t n = 0; n < Gp; ++n)
+                hm[n] = uint8_t((have[n] ? 1 : 0) | (have[Gp + n] ? 2 : 0) | (have[2 * Gp + n] ? 4 : 0));
+            S.pmask.upload(hm);
             S.pmem.alloc(size_t(S.nf) * 9 * std::max<int64_t>(Gp, 1));
             S.pnew.alloc(size_t(S.nf) * 3 * std::max<int64_t>(Gp, 1));
         } else {
