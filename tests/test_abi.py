@@ -17,6 +17,7 @@ def _declared(header):
 
 @pytest.mark.parametrize("header,lib", [("include/gfx.h", "gf-only-sem_amd/libgfx.so"),
                                         ("include/gfx_io.h", "gf-only-sem_amd/libgfx.so"),
+                                        ("include/gfx_surrogate.h", "gf-only-sem_amd/libgfx.so"),
                                         ("oracle/gfx_oracle.h", "oracle/liboracle.so")])
 def test_library_exports_every_declared_symbol(header, lib):
     names = _declared(header)
