@@ -10,6 +10,7 @@
 // packing) happens once in gfx_solver_create.
 #include <atomic>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <numeric>
 
@@ -495,6 +496,25 @@ std::vector<int> colour_elements(int E, const int32_t* ibool, int G) {
     return col;
 }
 
+// Device allocations and copies of create(), queued while the host plan is
+// built and run only once it is complete: every host-side check runs before
+// the first device call (also on a machine without a device).
+struct DeviceWork {
+    std::vector<std::function<void()>> ops;
+    template <typename T>
+    void upload(DevBuf<T>& b, std::vector<T> h) {
+        ops.emplace_back([&b, h = std::move(h)] { b.upload(h); });
+    }
+    template <typename T>
+    void alloc(DevBuf<T>& b, size_t n) {
+        ops.emplace_back([&b, n] { b.alloc(n); });
+    }
+    void run() {
+        for (auto& f : ops) f();
+        ops.clear();
+    }
+};
+
 void create(const gfx_solver_desc& d, gfx_solver& S) {
     require(d.ngll == 5, "the device solver supports NGLL = 5 (polynomial order 4) only");
     require(d.n_elem > 0 && d.n_rank_node > 0, "empty mesh");
@@ -514,11 +534,7 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
     require(nfg == 0 || nfg == 1 || nfg == 2 || nfg == 4 || nfg == 8, "node_field_groups must be 1, 2, 4 or 8");
     S.node_fg = nfg ? nfg : 1;
     require(d.n_rank_node < int(0x7fffffff) - 8, "too many nodes");
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
-        (void)hipGetLastError();
-        throw Error(GFX_ERR_DEVICE, "no HIP device available");
-    }
+    DeviceWork dv;
     const int E = d.n_elem, G = d.n_rank_node;
     const int64_t P = int64_t(E) * NP;
     for (int64_t p = 0; p < P; ++p)
@@ -533,7 +549,6 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
     if (S.has_cpml)
         require(d.pml_region && d.coef_alpha && d.coef_abar && d.density,
                 "has_cpml needs pml_region, coef_alpha, coef_abar and density");
-    GFX_HIP(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
 
     const GLL gll(4);
     for (int q = 0; q < 25; ++q) S.D[q] = float(gll.D[q]);
@@ -624,6 +639,7 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
     // numbering: C-PML nodes, other special nodes, regular nodes; first touch
     // in solver element order inside each class; C-PML nodes grouped by their
     // direction mask (first touch inside a group), so wavefronts share a mask
+    // (-1 = not yet seen, -2 = queued in this class, >= 0 = numbered)
     S.node_new_of_old.assign(G, -1);
     int next = 0;
     for (int pass = 2; pass >= 0; --pass) {
@@ -632,8 +648,8 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
             const int64_t e = elem_of_int[i];
             for (int p = 0; p < NP; ++p) {
                 const int g = d.ibool[e * NP + p];
-                if (cls[g] == pass && S.node_new_of_old[g] < 0) {
-                    S.node_new_of_old[g] = -2;  // seen
+                if (cls[g] == pass && S.node_new_of_old[g] == -1) {
+                    S.node_new_of_old[g] = -2;
                     order.push_back(g);
                 }
             }
@@ -648,6 +664,16 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
                              [&](int a, int b) { return pm_old[a] < pm_old[b]; });
         for (int g : order) S.node_new_of_old[g] = next++;
         if (pass == 2) S.Gpml = next;
+    }
+    // every node numbered exactly once: the numbering is a permutation of [0, G)
+    require(next == G, "internal: node numbering is not a permutation");
+    {
+        std::vector<char> hit(G, 0);
+        for (int g = 0; g < G; ++g) {
+            const int n = S.node_new_of_old[g];
+            require(n >= 0 && n < G && !hit[n], "internal: node numbering is not a permutation");
+            hit[n] = 1;
+        }
     }
     int64_t n_special = 0;
     for (int g = 0; g < G; ++g) n_special += cls[g] ? 1 : 0;
@@ -684,11 +710,10 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
             }
         }
     }
-    S.G9.upload(hG);
-    S.lam.upload(hlam);
-    S.mu.upload(hmu);
-    S.ibool.upload(hib);
-    std::vector<float>().swap(hG);
+    dv.upload(S.G9, std::move(hG));
+    dv.upload(S.lam, std::move(hlam));
+    dv.upload(S.mu, std::move(hmu));
+    dv.upload(S.ibool, hib);  // hib is read again for the gather slots
 
     S.mass_node.assign(G, 0.0);
     std::vector<float> hminv(S.G4, 0.f), hdamp(std::max<int64_t>(S.Ga, 1), 0.f);
@@ -698,8 +723,8 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
         hminv[n] = mg[g] > 0.0 ? float(1.0 / mg[g]) : 0.f;
         if (n < S.Ga) hdamp[n] = float(dg[g]);
     }
-    S.minv.upload(hminv);
-    S.damp.upload(hdamp);
+    dv.upload(S.minv, std::move(hminv));
+    dv.upload(S.damp, std::move(hdamp));
 
     // ---- C-PML ----
     std::vector<int32_t> hslot(E, -1);
@@ -746,14 +771,14 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
                         hal[(dir * 3 + q) * Gp + n] =
                             have[dir * Gp + n] ? float(al[(dir * 3 + q) * Gp + n]) : (q == 0 ? 1.f : 0.f);
             }
-            S.pc.upload(hc);
-            S.palpha.upload(hal);
+            dv.upload(S.pc, std::move(hc));
+            dv.upload(S.palpha, std::move(hal));
             std::vector<uint8_t> hm(std::max<int64_t>(Gp, 1), 0);
             for (int64_t n = 0; n < Gp; ++n)
                 hm[n] = uint8_t((have[n] ? 1 : 0) | (have[Gp + n] ? 2 : 0) | (have[2 * Gp + n] ? 4 : 0));
-            S.pmask.upload(hm);
-            S.pmem.alloc(size_t(S.nf) * 9 * std::max<int64_t>(Gp, 1));
-            S.pnew.alloc(size_t(S.nf) * 3 * std::max<int64_t>(Gp, 1));
+            dv.upload(S.pmask, std::move(hm));
+            dv.alloc(S.pmem, size_t(S.nf) * 9 * std::max<int64_t>(Gp, 1));
+            dv.alloc(S.pnew, size_t(S.nf) * 3 * std::max<int64_t>(Gp, 1));
         } else {
             S.pml_nodal = false;
             int npml = 0;
@@ -776,10 +801,10 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
                     for (int c = 0; c < 5; ++c) hab[c * Q + q] = float(scale * d.coef_abar[src * 5 + c]);
                 }
             }
-            S.palpha.upload(hal);
-            S.pabar.upload(hab);
-            S.pmem.alloc(size_t(S.nf) * 9 * Q);
-            S.pnew.alloc(size_t(S.nf) * 3 * Q);
+            dv.upload(S.palpha, std::move(hal));
+            dv.upload(S.pabar, std::move(hab));
+            dv.alloc(S.pmem, size_t(S.nf) * 9 * Q);
+            dv.alloc(S.pnew, size_t(S.nf) * 3 * Q);
         }
     }
     if (S.has_cpml) {
@@ -787,7 +812,7 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
         for (int e = 0; e < E; ++e) npe += d.pml_region[e] != 0;
         S.n_pml_elem = npe;
     }
-    S.pml_slot.upload(hslot);
+    dv.upload(S.pml_slot, std::move(hslot));
 
     // element kernel: the wavefront-per-element form needs no element-local C-PML state
     const bool wave_ok = !(S.has_cpml && !S.pml_nodal);
@@ -799,24 +824,26 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
     S.fstride = ((S.G4 * 3 + 63) / 64) * 64;
     S.astride = ((std::max<int64_t>(S.Ga, 1) * 3 + 63) / 64) * 64;
     const size_t fs = size_t(S.fstride) * S.nf;
-    S.ctr.alloc(2);
-    S.U.alloc(fs);
-    S.W.alloc(fs);
-    S.R.alloc(fs);
-    S.Aacc.alloc(size_t(S.astride) * S.nf);
+    dv.alloc(S.ctr, 2);
+    dv.alloc(S.U, fs);
+    dv.alloc(S.W, fs);
+    dv.alloc(S.R, fs);
+    dv.alloc(S.Aacc, size_t(S.astride) * S.nf);
     if (S.assembly == GFX_ASSEMBLY_GATHER) {
         // each node owns the contiguous slots [off[n], off[n+1]); the element points
         // of a node take its slots in increasing solver order (fixed summation order)
         std::vector<int32_t> off(S.G4 + 1, 0), slot(P);
         for (int64_t q = 0; q < P; ++q) ++off[hib[q] + 1];
         for (int64_t n = 0; n < S.G4; ++n) off[n + 1] += off[n];
+        require(off[S.G4] == P, "internal: gather slots do not cover every element point");
         std::vector<int32_t> fill(off.begin(), off.end() - 1);
         for (int64_t q = 0; q < P; ++q) slot[q] = fill[hib[q]]++;
-        S.goff.upload(off);
-        S.gslot.upload(slot);
         S.goff_host = off;
-        S.Rl.alloc(size_t(S.nf) * P * 3);
+        dv.upload(S.goff, std::move(off));
+        dv.upload(S.gslot, std::move(slot));
+        dv.alloc(S.Rl, size_t(S.nf) * P * 3);
     }
+    std::vector<uint32_t>().swap(hib);
 
     // ---- recording cells ----
     S.n_rec = int(S.rec_cells.size());
@@ -834,14 +861,22 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
                     rdd[q * nrp + int64_t(c) * NP + p] = float(d.dxi_dx[(e * NP + p) * 9 + q]);
             }
         }
-        S.rec_ibool.upload(rib);
-        S.rec_dd.upload(rdd);
+        dv.upload(S.rec_ibool, std::move(rib));
+        dv.upload(S.rec_dd, std::move(rdd));
         if (S.rec_cap > 0) {
-            S.rec_strain.alloc(size_t(S.rec_cap) * S.nf * nrp * 6);
-            if (S.rec_uva_on) S.rec_uva.alloc(size_t(S.rec_cap) * S.nf * nrp * 9);
+            dv.alloc(S.rec_strain, size_t(S.rec_cap) * S.nf * nrp * 6);
+            if (S.rec_uva_on) dv.alloc(S.rec_uva, size_t(S.rec_cap) * S.nf * nrp * 9);
         }
     }
 
+    // ---- the host plan is complete: device buffers, sources, zero state ----
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        throw Error(GFX_ERR_DEVICE, "no HIP device available");
+    }
+    GFX_HIP(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking));
+    dv.run();
     S.sources.assign(S.nf, FieldSource{});
     build_sources(S);
     init_state(S);

@@ -1,0 +1,50 @@
+"""Host-side solver set-up without a device.
+
+gfx_solver_create builds its whole host plan (element order, node numbering
+with the C-PML direction-mask grouping, nodal C-PML coefficients, gather
+slots, recording tables) and checks its invariants before the first device
+call, so on a machine with no HIP device the call must end in
+GFX_ERR_DEVICE ("no HIP device available"), never in an internal check or a
+crash.  A numbering that is not a permutation of the nodes (a node queued
+once per element that touches it) fails the plan's own check instead."""
+import numpy as np
+import pytest
+
+import meshcases as mc
+import pygfx
+
+GFX_ERR_DEVICE = 2
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _no_device():
+    if pygfx.device_count() > 0:
+        pytest.skip("a HIP device is visible: the device-free path is not reached")
+
+
+@pytest.mark.parametrize("assembly", [pygfx.GFX_ASSEMBLY_COLORED, pygfx.GFX_ASSEMBLY_ATOMIC,
+                                      pygfx.GFX_ASSEMBLY_GATHER])
+@pytest.mark.parametrize("has_cpml", [True, False])
+def test_create_plans_fully_before_the_device_check(assembly, has_cpml):
+    mesh = mc.small_layered(nx=5, ny=4, nz=4, pml=1)
+    with pytest.raises(pygfx.GfxError) as ei:
+        pygfx.Solver(mesh, n_fields=3, assembly=assembly, has_cpml=has_cpml, record_capacity=2)
+    assert ei.value.status == GFX_ERR_DEVICE, str(ei.value)
+    assert "no HIP device" in str(ei.value)
+
+
+def test_create_still_rejects_bad_arguments_first():
+    mesh = mc.small_halfspace(nx=4, ny=4, nz=3, pml=1)
+    with pytest.raises(pygfx.GfxError, match="elems_per_block"):
+        pygfx.Solver(mesh, elems_per_block=3)
+    with pytest.raises(pygfx.GfxError, match="pair element kernel"):
+        pygfx.Solver(mesh, n_fields=2, element_kernel=pygfx.GFX_ELEMENT_PAIR, elems_per_block=8)
+    good = mesh.arrays["ibool"]
+    bad = np.array(good, copy=True)
+    bad.reshape(-1)[7] = mesh.n_node
+    mesh.arrays["ibool"] = bad
+    try:
+        with pytest.raises(pygfx.GfxError, match="ibool"):
+            pygfx.Solver(mesh)
+    finally:
+        mesh.arrays["ibool"] = good
