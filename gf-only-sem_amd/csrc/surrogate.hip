@@ -92,16 +92,24 @@ __device__ inline int find_cell(const JudgeArgs& A, const double p[3]) {
     return best;
 }
 
+// Region (cell, time window) of tile time index t: window t * n_window / n_time.
+__host__ __device__ inline int64_t region_of(int cell, int t, int n_time, int n_window) {
+    const int w = (t >= 0 && t < n_time) ? int(int64_t(t) * n_window / n_time) : 0;
+    return int64_t(cell) * n_window + w;
+}
+
+// only (may be null): evaluate just the points whose region is marked there
 __global__ void k_judge(JudgeArgs A, int n, const double* __restrict__ q, const int32_t* __restrict__ tix,
-                        const uint8_t* __restrict__ only, float* __restrict__ out, int32_t* __restrict__ cell_out) {
+                        const uint8_t* __restrict__ only, int n_window, float* __restrict__ out,
+                        int32_t* __restrict__ cell_out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double p[3] = {q[3 * i], q[3 * i + 1], q[3 * i + 2]};
     const int c = find_cell(A, p);
     if (cell_out) cell_out[i] = c;
-    if (only && (c < 0 || !only[c])) return;
-    float* o = out + int64_t(i) * 18;
     const int t = tix[i];
+    if (only && (c < 0 || !only[region_of(c, t, A.n_time, n_window)])) return;
+    float* o = out + int64_t(i) * 18;
     if (c < 0 || t < 0 || t >= A.n_time) {
         for (int k = 0; k < 18; ++k) o[k] = __builtin_nanf("");
         return;
@@ -307,6 +315,32 @@ __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float
     w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
+// Output envelope e(r), r = horizontal distance from (cx, cy): piecewise
+// linear through env[i] at r_i = (i + 0.5) h, h = r_max / n, constant outside
+// [r_0, r_{n-1}]; 1 when n == 0.
+// The network's time input is the retarded time t - slowness * r (moveout).
+struct Envelope {
+    const float* env;  // device or host array of n knots
+    int n;
+    double cx, cy, r_max, slowness;
+};
+
+__host__ __device__ inline double retarded_time(const Envelope& E, double x, double y, double t) {
+    if (E.slowness == 0.0) return t;
+    return t - E.slowness * sqrt((x - E.cx) * (x - E.cx) + (y - E.cy) * (y - E.cy));
+}
+
+__host__ __device__ inline float envelope_at(const Envelope& E, double x, double y) {
+    if (E.n <= 0) return 1.f;
+    const double r = sqrt((x - E.cx) * (x - E.cx) + (y - E.cy) * (y - E.cy));
+    const double u = r * E.n / E.r_max - 0.5;
+    if (u <= 0.0) return E.env[0];
+    if (u >= E.n - 1) return E.env[E.n - 1];
+    const int i = int(u);
+    const float f = float(u - i);
+    return E.env[i] * (1.f - f) + E.env[i + 1] * f;
+}
+
 __device__ inline uint64_t mix64(uint64_t z) {
     z += 0x9E3779B97F4A7C15ULL;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
@@ -315,20 +349,22 @@ __device__ inline uint64_t mix64(uint64_t z) {
 }
 
 // training batch from a tile: random (node, time) pairs; inputs scaled to
-// [-1, 1], targets divided by the per-output scale
+// [-1, 1], targets divided by the per-output scale and the envelope
 __global__ void k_sample_tile(const float* __restrict__ val, const double* __restrict__ xyz, int n_node, int n_time,
                               double t0, double dt, const double* __restrict__ lo, const double* __restrict__ hi,
-                              const float* __restrict__ inv_scale, uint64_t seed, int rows, float* __restrict__ X,
-                              float* __restrict__ Y) {
+                              const float* __restrict__ inv_scale, Envelope E, uint64_t seed, int rows,
+                              float* __restrict__ X, float* __restrict__ Y) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= rows) return;
     const uint64_t h = mix64(seed ^ (uint64_t(r) * 0xD1B54A32D192ED03ULL));
     const int node = int(h % uint64_t(n_node));
     const int t = int((h >> 32) % uint64_t(n_time));
-    const double in[4] = {xyz[int64_t(node) * 3], xyz[int64_t(node) * 3 + 1], xyz[int64_t(node) * 3 + 2], t0 + t * dt};
+    const double px = xyz[int64_t(node) * 3], py = xyz[int64_t(node) * 3 + 1];
+    const double in[4] = {px, py, xyz[int64_t(node) * 3 + 2], retarded_time(E, px, py, t0 + t * dt)};
     for (int d = 0; d < 4; ++d) X[int64_t(r) * 4 + d] = float(2.0 * (in[d] - lo[d]) / (hi[d] - lo[d]) - 1.0);
+    const float ie = 1.f / envelope_at(E, px, py);
     const float* v = val + (int64_t(t) * n_node + node) * 18;
-    for (int k = 0; k < 18; ++k) Y[int64_t(r) * 18 + k] = v[k] * inv_scale[k];
+    for (int k = 0; k < 18; ++k) Y[int64_t(r) * 18 + k] = v[k] * inv_scale[k] * ie;
 }
 
 // raw (x, y, z, t) -> scaled inputs
@@ -353,6 +389,14 @@ struct gfx_mlp {
     gfx::DevBuf<double> lo, hi;
     gfx::DevBuf<float> inv_scale;
     std::vector<double> h_scale;
+    // output envelope (knots on the host and the device)
+    std::vector<float> h_env;
+    gfx::DevBuf<float> env;
+    double env_cx = 0, env_cy = 0, env_rmax = 1, slowness = 0;
+    gfx::sur::Envelope envelope(bool device) const {
+        return gfx::sur::Envelope{device ? env.p : h_env.data(), int(h_env.size()), env_cx, env_cy, env_rmax,
+                                  slowness};
+    }
     // activations for up to cap rows
     int cap = 0;
     gfx::DevBuf<float> X, H0, Y, work, part, lossp;
@@ -476,12 +520,12 @@ double backward(gfx_mlp& M, int rows) {
     return loss / double(n);
 }
 
-void adam(gfx_mlp& M) {
+void adam(gfx_mlp& M, float lr) {
     ++M.adam_t;
     const float b1 = M.cfg.beta1, b2 = M.cfg.beta2;
     const float c1 = 1.f - std::pow(b1, float(M.adam_t)), c2 = 1.f - std::pow(b2, float(M.adam_t));
     sur::k_adam<<<int((M.n_params + 255) / 256), 256, 0, M.stream>>>(M.w.p, M.g.p, M.m.p, M.v.p, M.n_params,
-                                                                      M.cfg.lr, b1, b2, M.cfg.eps, c1, c2);
+                                                                      lr, b1, b2, M.cfg.eps, c1, c2);
 }
 
 }  // namespace
@@ -605,7 +649,7 @@ extern "C" gfx_status gfx_green_interpolate(const gfx_green* g, int n, const dou
         if (cell) c.alloc(n);
         GFX_HIP(hipMemcpy(q.p, xyz, size_t(n) * 24, hipMemcpyHostToDevice));
         GFX_HIP(hipMemcpy(t.p, tix, size_t(n) * 4, hipMemcpyHostToDevice));
-        sur::k_judge<<<(n + 127) / 128, 128, 0, g->stream>>>(sur::judge_args(*g), n, q.p, t.p, nullptr, o.p,
+        sur::k_judge<<<(n + 127) / 128, 128, 0, g->stream>>>(sur::judge_args(*g), n, q.p, t.p, nullptr, 1, o.p,
                                                               cell ? c.p : nullptr);
         GFX_HIP(hipGetLastError());
         GFX_HIP(hipStreamSynchronize(g->stream));
@@ -700,6 +744,7 @@ extern "C" gfx_status gfx_mlp_config_default(gfx_mlp_config* c) {
         c->beta2 = 0.999f;
         c->eps = 1e-8f;
         c->batch = 8192;
+        c->lr_final = 5e-5f;
     });
 }
 
@@ -711,6 +756,7 @@ extern "C" gfx_status gfx_mlp_create(const gfx_mlp_config* cfg, const double* in
                     cfg->depth >= 1 && cfg->depth <= 32 && cfg->n_out == 18,
                 "bad MLP shape (n_out must be 18: 6 strain components x 3 force directions)");
         for (int d = 0; d < 4; ++d) require(in_hi[d] > in_lo[d], "input range must be non-empty");
+        require(cfg->lr > 0 && cfg->lr_final > 0 && cfg->lr_final <= cfg->lr, "need 0 < lr_final <= lr");
         *out = nullptr;
         auto M = std::make_unique<gfx_mlp>();
         M->cfg = *cfg;
@@ -787,6 +833,44 @@ extern "C" gfx_status gfx_mlp_set_params(gfx_mlp* m, const float* w, size_t n) {
     });
 }
 
+extern "C" gfx_status gfx_mlp_set_envelope(gfx_mlp* m, double cx, double cy, double slowness, double r_max,
+                                           int n_bins, const float* env) {
+    return guarded([&] {
+        require(m != nullptr, "null argument");
+        require(n_bins >= 0 && n_bins <= 65536, "n_bins must be in [0, 65536]");
+        require(std::isfinite(slowness) && slowness >= 0, "slowness must be >= 0");
+        GFX_HIP(hipStreamSynchronize(m->stream));
+        m->env_cx = cx;
+        m->env_cy = cy;
+        m->slowness = slowness;
+        if (n_bins == 0) {
+            m->h_env.clear();
+            m->env.release();
+            return;
+        }
+        require(env != nullptr && r_max > 0, "envelope needs knots and r_max > 0");
+        for (int i = 0; i < n_bins; ++i) require(env[i] > 0.f && std::isfinite(env[i]), "envelope knots must be > 0");
+        m->h_env.assign(env, env + n_bins);
+        m->env.upload(m->h_env);
+        m->env_rmax = r_max;
+    });
+}
+
+namespace gfx {
+namespace sur {
+namespace {
+// raw (x, y, z, t) rows with t replaced by the network's time input
+std::vector<double> network_inputs(const gfx_mlp& m, const double* x, int rows) {
+    std::vector<double> v(x, x + size_t(rows) * 4);
+    const Envelope E = m.envelope(false);
+    if (E.slowness != 0.0)
+        for (int r = 0; r < rows; ++r) v[size_t(r) * 4 + 3] = retarded_time(E, v[size_t(r) * 4], v[size_t(r) * 4 + 1], v[size_t(r) * 4 + 3]);
+    return v;
+}
+}  // namespace
+}  // namespace sur
+}  // namespace gfx
+
 // inputs raw (x, y, z, t) [n][4] FP64; outputs unscaled [n][18]
 extern "C" gfx_status gfx_mlp_forward(gfx_mlp* m, int n, const double* x, float* y) {
     return guarded([&] {
@@ -795,17 +879,23 @@ extern "C" gfx_status gfx_mlp_forward(gfx_mlp* m, int n, const double* x, float*
         DevBuf<double> raw;
         raw.alloc(size_t(std::min(n, chunk)) * 4 + 4);
         std::vector<float> ho;
+        const sur::Envelope E = m->envelope(false);
         for (int b = 0; b < n; b += chunk) {
             const int rows = std::min(chunk, n - b);
             sur::ensure_rows(*m, rows);
-            GFX_HIP(hipMemcpyAsync(raw.p, x + size_t(b) * 4, size_t(rows) * 32, hipMemcpyHostToDevice, m->stream));
+            const std::vector<double> xin = sur::network_inputs(*m, x + size_t(b) * 4, rows);  // lives past the sync
+            GFX_HIP(hipMemcpyAsync(raw.p, xin.data(), size_t(rows) * 32, hipMemcpyHostToDevice, m->stream));
             sur::k_scale_in<<<(rows + 255) / 256, 256, 0, m->stream>>>(raw.p, rows, m->lo.p, m->hi.p, m->X.p);
             sur::forward(*m, rows);
             ho.resize(size_t(rows) * 18);
             GFX_HIP(hipMemcpyAsync(ho.data(), m->out.p, ho.size() * 4, hipMemcpyDeviceToHost, m->stream));
             GFX_HIP(hipStreamSynchronize(m->stream));
-            for (int r = 0; r < rows; ++r)
-                for (int k = 0; k < 18; ++k) y[(size_t(b) + r) * 18 + k] = float(ho[size_t(r) * 18 + k] * m->h_scale[k]);
+            for (int r = 0; r < rows; ++r) {
+                const double* xr = x + (size_t(b) + r) * 4;
+                const double e = sur::envelope_at(E, xr[0], xr[1]);
+                for (int k = 0; k < 18; ++k)
+                    y[(size_t(b) + r) * 18 + k] = float(ho[size_t(r) * 18 + k] * m->h_scale[k] * e);
+            }
         }
     });
 }
@@ -819,10 +909,14 @@ extern "C" gfx_status gfx_mlp_gradient(gfx_mlp* m, int n, const double* x, const
         sur::ensure_rows(*m, n);
         DevBuf<double> raw;
         raw.alloc(size_t(n) * 4);
-        GFX_HIP(hipMemcpy(raw.p, x, size_t(n) * 32, hipMemcpyHostToDevice));
+        const std::vector<double> xin = sur::network_inputs(*m, x, n);
+        GFX_HIP(hipMemcpy(raw.p, xin.data(), size_t(n) * 32, hipMemcpyHostToDevice));
         std::vector<float> ys(size_t(n) * 18);
-        for (int r = 0; r < n; ++r)
-            for (int k = 0; k < 18; ++k) ys[size_t(r) * 18 + k] = float(y[size_t(r) * 18 + k] / m->h_scale[k]);
+        const sur::Envelope E = m->envelope(false);
+        for (int r = 0; r < n; ++r) {
+            const double e = sur::envelope_at(E, x[size_t(r) * 4], x[size_t(r) * 4 + 1]);
+            for (int k = 0; k < 18; ++k) ys[size_t(r) * 18 + k] = float(y[size_t(r) * 18 + k] / (m->h_scale[k] * e));
+        }
         GFX_HIP(hipMemcpy(m->Y.p, ys.data(), ys.size() * 4, hipMemcpyHostToDevice));
         sur::k_scale_in<<<(n + 255) / 256, 256, 0, m->stream>>>(raw.p, n, m->lo.p, m->hi.p, m->X.p);
         sur::forward(*m, n);
@@ -844,13 +938,17 @@ extern "C" gfx_status gfx_mlp_train(gfx_mlp* m, const gfx_green* g, double t0, d
         GFX_HIP(hipStreamSynchronize(g->stream));
         double tail = 0.0;
         int nt = 0;
+        const sur::Envelope E = m->envelope(true);
+        const double lr0 = m->cfg.lr, lr1 = m->cfg.lr_final;
         for (int it = 0; it < iters; ++it) {
             sur::k_sample_tile<<<(rows + 255) / 256, 256, 0, m->stream>>>(
-                g->val.p, g->xyz.p, g->n_node, g->n_time, t0, dt, m->lo.p, m->hi.p, m->inv_scale.p,
+                g->val.p, g->xyz.p, g->n_node, g->n_time, t0, dt, m->lo.p, m->hi.p, m->inv_scale.p, E,
                 seed * 0x9E3779B97F4A7C15ULL + uint64_t(it), rows, m->X.p, m->Y.p);
             sur::forward(*m, rows);
             const double l = sur::backward(*m, rows);
-            sur::adam(*m);
+            // cosine decay from lr to lr_final over this call
+            const double ph = iters > 1 ? double(it) / double(iters - 1) : 1.0;
+            sur::adam(*m, float(lr1 + 0.5 * (lr0 - lr1) * (1.0 + std::cos(3.14159265358979323846 * ph))));
             if (it >= iters - std::max(1, iters / 10)) {
                 tail += l;
                 ++nt;
@@ -861,13 +959,15 @@ extern "C" gfx_status gfx_mlp_train(gfx_mlp* m, const gfx_green* g, double t0, d
     });
 }
 
-// Surrogate query: the MLP everywhere, the judge for points in cells marked
-// for fallback (or outside the tile).  *n_fallback counts judge answers.
-extern "C" gfx_status gfx_surrogate_query(gfx_mlp* m, const gfx_green* g, const uint8_t* fallback, double t0,
-                                          double dt, int n, const double* xyz, const int32_t* tix, float* out,
-                                          int* n_fallback) {
+// Surrogate query: the MLP everywhere, the judge for points in (cell, time
+// window) regions marked for fallback (or outside the tile).  *n_fallback
+// counts judge answers.
+extern "C" gfx_status gfx_surrogate_query(gfx_mlp* m, const gfx_green* g, const uint8_t* fallback, int n_window,
+                                          double t0, double dt, int n, const double* xyz, const int32_t* tix,
+                                          float* out, int* n_fallback) {
     return guarded([&] {
         require(m && g && fallback && xyz && tix && out && n_fallback && n >= 0, "null argument");
+        require(n_window >= 1 && n_window <= g->n_time, "n_window must be in [1, n_time]");
         if (n == 0) {
             *n_fallback = 0;
             return;
@@ -892,18 +992,19 @@ extern "C" gfx_status gfx_surrogate_query(gfx_mlp* m, const gfx_green* g, const 
         t.alloc(n);
         c.alloc(n);
         o.alloc(size_t(n) * 18);
-        fb.upload(std::vector<uint8_t>(fallback, fallback + g->n_cell));
+        fb.upload(std::vector<uint8_t>(fallback, fallback + size_t(g->n_cell) * n_window));
         GFX_HIP(hipMemcpy(q.p, xyz, size_t(n) * 24, hipMemcpyHostToDevice));
         GFX_HIP(hipMemcpy(t.p, tix, size_t(n) * 4, hipMemcpyHostToDevice));
         GFX_HIP(hipMemsetAsync(o.p, 0, size_t(n) * 72, g->stream));
-        sur::k_judge<<<(n + 127) / 128, 128, 0, g->stream>>>(sur::judge_args(*g), n, q.p, t.p, fb.p, o.p, c.p);
+        sur::k_judge<<<(n + 127) / 128, 128, 0, g->stream>>>(sur::judge_args(*g), n, q.p, t.p, fb.p, n_window, o.p,
+                                                              c.p);
         GFX_HIP(hipGetLastError());
         GFX_HIP(hipStreamSynchronize(g->stream));
         GFX_HIP(hipMemcpy(jo.data(), o.p, jo.size() * 4, hipMemcpyDeviceToHost));
         GFX_HIP(hipMemcpy(cell.data(), c.p, size_t(n) * 4, hipMemcpyDeviceToHost));
         int nf = 0;
         for (int i = 0; i < n; ++i) {
-            const bool use_judge = cell[i] < 0 || fallback[cell[i]];
+            const bool use_judge = cell[i] < 0 || fallback[sur::region_of(cell[i], tix[i], g->n_time, n_window)];
             const float* src = use_judge ? &jo[size_t(i) * 18] : &mo[size_t(i) * 18];
             if (use_judge) {
                 ++nf;

@@ -105,7 +105,7 @@ class ForwardOptions(C.Structure):
 class MlpConfig(C.Structure):
     _fields_ = [("n_freq", C.c_int), ("width", C.c_int), ("depth", C.c_int), ("n_out", C.c_int),
                 ("sigma", C.c_float), ("seed", C.c_uint64), ("lr", C.c_float), ("beta1", C.c_float),
-                ("beta2", C.c_float), ("eps", C.c_float), ("batch", C.c_int)]
+                ("beta2", C.c_float), ("eps", C.c_float), ("batch", C.c_int), ("lr_final", C.c_float)]
 
 
 def _load():
@@ -159,8 +159,10 @@ def _load():
         "gfx_mlp_gradient": (st, [vp, C.c_int, _dp, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                   C.POINTER(C.c_double)]),
         "gfx_mlp_train": (st, [vp, vp, C.c_double, C.c_double, C.c_int, C.c_uint64, C.POINTER(C.c_double)]),
-        "gfx_surrogate_query": (st, [vp, vp, C.POINTER(C.c_uint8), C.c_double, C.c_double, C.c_int, _dp, _ip,
-                                     C.POINTER(C.c_float), C.POINTER(C.c_int)]),
+        "gfx_mlp_set_envelope": (st, [vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                      C.POINTER(C.c_float)]),
+        "gfx_surrogate_query": (st, [vp, vp, C.POINTER(C.c_uint8), C.c_int, C.c_double, C.c_double, C.c_int, _dp,
+                                     _ip, C.POINTER(C.c_float), C.POINTER(C.c_int)]),
         "gfx_autotune": (st, [C.POINTER(SolverDesc), C.c_int, C.POINTER(SolverDesc), C.POINTER(C.c_double)]),
         "gfx_box_spec_default": (st, [C.POINTER(BoxSpec)]),
         "gfx_mesh_build_box": (st, [C.POINTER(BoxSpec), C.POINTER(C.POINTER(_Mesh))]),
@@ -810,6 +812,14 @@ class MLP:
         w = np.ascontiguousarray(w, np.float32)
         _check(lib.gfx_mlp_set_params(self.h, w.ctypes.data_as(_fp), w.size))
 
+    def set_envelope(self, cx, cy, r_max, env, slowness=0.0):
+        """Source frame: output envelope e(r) over the horizontal distance r
+        from (cx, cy), piecewise linear through env[i] at (i + 0.5) r_max /
+        len(env) (empty: none), and time input t - slowness * r."""
+        env = np.ascontiguousarray(env, np.float32).ravel()
+        _check(lib.gfx_mlp_set_envelope(self.h, float(cx), float(cy), float(slowness), float(r_max), env.size,
+                                        env.ctypes.data_as(_fp) if env.size else None))
+
     def forward(self, x):
         x = np.ascontiguousarray(x, np.float64).reshape(-1, 4)
         y = np.zeros((x.shape[0], 18), np.float32)
@@ -831,13 +841,15 @@ class MLP:
         return loss.value
 
     def query(self, tile, fallback, t0, dt, xyz, tix):
+        """fallback: [n_cell] or [n_cell][n_window] flags (regions answered by the judge)."""
         xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
         tix = np.ascontiguousarray(np.broadcast_to(tix, (xyz.shape[0],)), np.int32)
         fb = np.ascontiguousarray(fallback, np.uint8)
+        fb = fb.reshape(tile.n_cell, -1)
         out = np.zeros((xyz.shape[0], 18), np.float32)
         nf = C.c_int(0)
-        _check(lib.gfx_surrogate_query(self.h, tile.h, fb.ctypes.data_as(C.POINTER(C.c_uint8)), t0, dt,
-                                       xyz.shape[0], _ptr(xyz), _ptr(tix, C.c_int32), out.ctypes.data_as(_fp),
+        _check(lib.gfx_surrogate_query(self.h, tile.h, fb.ctypes.data_as(C.POINTER(C.c_uint8)), fb.shape[1], t0,
+                                       dt, xyz.shape[0], _ptr(xyz), _ptr(tix, C.c_int32), out.ctypes.data_as(_fp),
                                        C.byref(nf)))
         return out, nf.value
 

@@ -50,15 +50,18 @@ gfx_status gfx_gll_interpolate_host(int n_node, const double* node_xyz, int n_ce
 /* ---- MLP ----------------------------------------------------------------
  * (x, y, z, t) scaled to [-1, 1] by [in_lo, in_hi] -> [sin, cos](2 pi B x),
  * B ~ N(0, sigma^2) [n_freq][4] -> depth x (Linear(width) + GELU) ->
- * Linear(18), outputs multiplied by out_scale[18].  Parameters are one FP32
- * array (per linear layer: W [in][out] row-major, then b [out]); products use
- * BF16 operands on MFMA with FP32 accumulation. */
+ * Linear(18), outputs multiplied by out_scale[18] and by the envelope e(r)
+ * (gfx_mlp_set_envelope; 1 until set).  Parameters are one FP32 array (per
+ * linear layer: W [in][out] row-major, then b [out]); products use BF16
+ * operands on MFMA with FP32 accumulation.  Training runs Adam with the step
+ * size decaying from lr to lr_final on a cosine over each gfx_mlp_train call. */
 typedef struct {
     int n_freq, width, depth, n_out;
     float sigma;
     uint64_t seed;
     float lr, beta1, beta2, eps;
     int batch;
+    float lr_final;
 } gfx_mlp_config;
 typedef struct gfx_mlp gfx_mlp;
 gfx_status gfx_mlp_config_default(gfx_mlp_config* c);
@@ -69,19 +72,35 @@ gfx_status gfx_mlp_n_params(const gfx_mlp* m, size_t* n);
 /* fourier (may be NULL) receives B [n_freq][4]. */
 gfx_status gfx_mlp_get_params(const gfx_mlp* m, float* params, size_t n, float* fourier);
 gfx_status gfx_mlp_set_params(gfx_mlp* m, const float* params, size_t n);
+/* Source frame of the network, r = horizontal distance from (cx, cy):
+ *  - moveout: the network's time input is t - slowness * r (0 = plain t), so
+ *    arrivals travelling at about 1/slowness line up;
+ *  - amplitude envelope: e(r) piecewise linear through env[i] at
+ *    r_i = (i + 0.5) * r_max / n_bins, constant beyond the first and last
+ *    knots, env[i] > 0; the network is trained on (value / out_scale) / e(r)
+ *    and its output is multiplied back, so it fits the Green's functions with
+ *    their geometric decay taken out.  n_bins = 0: no envelope.
+ * lo/hi of the time input (gfx_mlp_create) refer to the network's time input. */
+gfx_status gfx_mlp_set_envelope(gfx_mlp* m, double cx, double cy, double slowness, double r_max, int n_bins,
+                                const float* env);
 /* x raw [n][4] (x, y, z, t); y [n][18]. */
 gfx_status gfx_mlp_forward(gfx_mlp* m, int n, const double* x, float* y);
-/* Loss = mean over n x 18 of ((out - y) / out_scale)^2 and its gradient
- * [n_params] (for checking against a CPU model). */
+/* Loss = mean over n x 18 of ((out - y) / (out_scale e(r)))^2, out the
+ * network's scaled output, and its gradient [n_params] (for checking against
+ * a CPU model). */
 gfx_status gfx_mlp_gradient(gfx_mlp* m, int n, const double* x, const float* y, float* grad, double* loss);
 /* Adam on random (node, time) samples of the tile (time index k -> t0 + k dt);
  * *loss = mean loss of the last 10 % of iterations. */
 gfx_status gfx_mlp_train(gfx_mlp* m, const gfx_green* g, double t0, double dt, int iters, uint64_t seed,
                          double* loss);
-/* Query: the MLP, except for points in cells with fallback[cell] != 0 (or
- * outside the tile), which the judge answers; *n_fallback counts those. */
-gfx_status gfx_surrogate_query(gfx_mlp* m, const gfx_green* g, const uint8_t* fallback, double t0, double dt,
-                               int n, const double* xyz, const int32_t* tix, float* out, int* n_fallback);
+/* Query: the MLP, except in regions marked for fallback (and outside the
+ * tile), where the judge answers; *n_fallback counts judge answers.  Regions
+ * are (cell, time window): the tile's n_time samples split into n_window
+ * windows, time index k in window k * n_window / n_time, and
+ * fallback[cell * n_window + window] != 0 marks one. */
+gfx_status gfx_surrogate_query(gfx_mlp* m, const gfx_green* g, const uint8_t* fallback, int n_window, double t0,
+                               double dt, int n, const double* xyz, const int32_t* tix, float* out,
+                               int* n_fallback);
 
 #ifdef __cplusplus
 }
