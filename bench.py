@@ -313,7 +313,7 @@ def main():
     ap.add_argument("--assembly", choices=["auto", "colored", "gather", "atomic"], default="auto",
                     help="auto = the library autotuner picks assembly, element kernel and block size")
     ap.add_argument("--epb", type=int, default=0)
-    ap.add_argument("--kernel", choices=["auto", "block", "wave", "pair"], default="auto")
+    ap.add_argument("--kernel", choices=["auto", "block", "wave", "pair", "pencil"], default="auto")
     ap.add_argument("--node-groups", type=int, default=0, help="node-kernel field groups (1, 2, 4, 8)")
     ap.add_argument("--small", action="store_true", help="40x40x20 mesh instead of 100x100x50 (smoke)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -352,7 +352,7 @@ def main():
     asm_of = {"colored": pygfx.GFX_ASSEMBLY_COLORED, "gather": pygfx.GFX_ASSEMBLY_GATHER,
               "atomic": pygfx.GFX_ASSEMBLY_ATOMIC}
     kern_of = {"auto": pygfx.GFX_ELEMENT_AUTO, "block": pygfx.GFX_ELEMENT_BLOCK,
-               "wave": pygfx.GFX_ELEMENT_WAVE, "pair": pygfx.GFX_ELEMENT_PAIR}
+               "wave": pygfx.GFX_ELEMENT_WAVE, "pair": pygfx.GFX_ELEMENT_PAIR, "pencil": pygfx.GFX_ELEMENT_PENCIL}
 
     def launch_config(n_fields):
         if args.assembly == "auto":
@@ -367,7 +367,7 @@ def main():
     names = {pygfx.GFX_ASSEMBLY_COLORED: "colored", pygfx.GFX_ASSEMBLY_GATHER: "gather",
              pygfx.GFX_ASSEMBLY_ATOMIC: "atomic"}
     knames = {pygfx.GFX_ELEMENT_BLOCK: "block", pygfx.GFX_ELEMENT_WAVE: "wave", pygfx.GFX_ELEMENT_PAIR: "pair",
-              0: "auto"}
+              pygfx.GFX_ELEMENT_PENCIL: "pencil", 0: "auto"}
 
     t0 = time.perf_counter()
     dims = (40, 40, 20) if args.small else (100, 100, 50)
@@ -426,8 +426,8 @@ def main():
               "element_kernel": knames[int(sb.query("element_kernel"))],
               "elems_per_block": int(sb.query("elems_per_block")),
               "node_field_groups": int(sb.query("node_field_groups"))}
-    kern = {"element": ({"wave": "k_element_wave", "pair": "k_element_pair"}.get(launch["element_kernel"],
-                                                                                "k_element"), 1),
+    kern = {"element": ({"wave": "k_element_wave", "pair": "k_element_pair",
+                         "pencil": "k_element_pencil"}.get(launch["element_kernel"], "k_element"), 1),
             "node_special": ("k_node_special", 3), "node_regular": ("k_node_regular", 4)}
     dom = max(kern, key=lambda k: tb[k][0])
     kname, which = kern[dom]

@@ -47,8 +47,7 @@ struct ElemArgs {
     int64_t fstride;                    // floats per field of the nodal arrays
     const float* __restrict__ U;        // [nf][fstride] predicted displacement
     float* __restrict__ R;              // [nf][fstride] assembled residual (COLORED, ATOMIC)
-    float* __restrict__ Rl;             // [nf][P][3] contributions in node-sorted slots (GATHER)
-    const int32_t* __restrict__ slot;   // [P] slot of each element point (GATHER)
+    float* __restrict__ Rl;             // [nf][P][3] per element point, solver point order (GATHER)
     // element-local C-PML (only when the nodal form is not applicable)
     const float* __restrict__ W;
     const float* __restrict__ Aacc;     // [nf][astride] a_n at special nodes
@@ -80,11 +79,13 @@ struct NodeArgs {
     float* __restrict__ W;
     float* __restrict__ R;
     float* __restrict__ Aacc;         // [nf][astride]
-    // GATHER assembly: contributions of node g sit in slots [off[g], off[g+1])
-    // of Rl[f], in increasing element order
+    // GATHER assembly: node g owns the slots [off[g], off[g+1]); slot q holds
+    // the element point pts[q] (increasing in solver order), whose residual
+    // is Rl[f][pts[q]]
     int gather;
     const float* __restrict__ Rl;
     const int32_t* __restrict__ off;  // [G4+1]
+    const int32_t* __restrict__ pts;  // [P]
     // nodal C-PML
     int pml_nodal;
     const float* __restrict__ pc;      // [5][Gpml] sum over PML elements of w J / rho * A_k
@@ -129,23 +130,23 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Element force of one GLL point from the staged displacement su and the
-// scaled geometry g (dxi_a/dx_m sqrt(J w)): writes the 9 fluxes
-// F[c][a] = sum_m sigma[c][m] dxi_a/dx_m J w  (element_cpu.cpp:51-108).
-__device__ __forceinline__ void point_flux(const float* __restrict__ su0, const float* __restrict__ su1,
-                                           const float* __restrict__ su2, const float* Di,
-                                           const float* Dj, const float* Dk, int i, int j, int k,
-                                           const float* g, float lam, float mu, float* F) {
-    const float* su[3] = {su0, su1, su2};
+// Element force of one GLL point (i,j,k) from its three rows of displacement
+// neighbours ui[c][s] = u_c(s,j,k), uj[c][s] = u_c(i,s,k), uk[c][s] = u_c(i,j,s)
+// and the scaled geometry g (dxi_a/dx_m sqrt(J w)): writes the 9 fluxes
+// F[c][a] = sum_m sigma[c][m] dxi_a/dx_m J w  (element_cpu.cpp:51-108).  Every
+// element kernel form goes through this one sequence of operations.
+__device__ __forceinline__ void flux_core(const float (&ui)[3][5], const float (&uj)[3][5], const float (&uk)[3][5],
+                                          const float* Di, const float* Dj, const float* Dk, const float* g,
+                                          float lam, float mu, float* F) {
     float du[3][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         float a0 = 0.f, a1 = 0.f, a2 = 0.f;
 #pragma unroll
         for (int s = 0; s < 5; ++s) {
-            a0 += Di[s] * su[c][s * 25 + j * 5 + k];
-            a1 += Dj[s] * su[c][i * 25 + s * 5 + k];
-            a2 += Dk[s] * su[c][i * 25 + j * 5 + s];
+            a0 += Di[s] * ui[c][s];
+            a1 += Dj[s] * uj[c][s];
+            a2 += Dk[s] * uk[c][s];
         }
         du[c][0] = a0;
         du[c][1] = a1;
@@ -173,6 +174,39 @@ __device__ __forceinline__ void point_flux(const float* __restrict__ su0, const 
             F[c * 3 + a] = sig[c][0] * g[0 * 3 + a] + sig[c][1] * g[1 * 3 + a] + sig[c][2] * g[2 * 3 + a];
 }
 
+// point_flux with the neighbours read from a staged element (su_c[p], p = (i*5+j)*5+k).
+__device__ __forceinline__ void point_flux(const float* __restrict__ su0, const float* __restrict__ su1,
+                                           const float* __restrict__ su2, const float* Di,
+                                           const float* Dj, const float* Dk, int i, int j, int k,
+                                           const float* g, float lam, float mu, float* F) {
+    const float* su[3] = {su0, su1, su2};
+    float ui[3][5], uj[3][5], uk[3][5];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int s = 0; s < 5; ++s) {
+            ui[c][s] = su[c][s * 25 + j * 5 + k];
+            uj[c][s] = su[c][i * 25 + s * 5 + k];
+            uk[c][s] = su[c][i * 25 + j * 5 + s];
+        }
+    flux_core(ui, uj, uk, Di, Dj, Dk, g, lam, mu, F);
+}
+
+// One component of the weighted divergence from the point's flux rows
+// a[s] = F_c0(s,j,k), b[s] = F_c1(i,s,k), e[s] = F_c2(i,j,s):
+// -sum_s (D[s][i] a[s] + D[s][j] b[s] + D[s][k] e[s]), one fixed order.
+__device__ __forceinline__ float div_core(const float (&a)[5], const float (&b)[5], const float (&e)[5],
+                                         const float* Dti, const float* Dtj, const float* Dtk) {
+    float acc = 0.f;
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+        acc += Dti[s] * a[s];
+        acc += Dtj[s] * b[s];
+        acc += Dtk[s] * e[s];
+    }
+    return -acc;
+}
+
 // Weighted divergence of the staged fluxes at point (i,j,k):
 // r_c = -sum_s (D[s][i] F_c0(s,j,k) + D[s][j] F_c1(i,s,k) + D[s][k] F_c2(i,j,s))
 // (element_cpu.cpp:110-157 rewritten as a gather over the point's neighbours).
@@ -181,14 +215,14 @@ __device__ __forceinline__ void point_div(const float* __restrict__ sf, int stri
                                           float* rv) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        float acc = 0.f;
+        float a[5], b[5], e[5];
 #pragma unroll
         for (int s = 0; s < 5; ++s) {
-            acc += Dti[s] * sf[(c * 3 + 0) * stride + s * 25 + j * 5 + k];
-            acc += Dtj[s] * sf[(c * 3 + 1) * stride + i * 25 + s * 5 + k];
-            acc += Dtk[s] * sf[(c * 3 + 2) * stride + i * 25 + j * 5 + s];
+            a[s] = sf[(c * 3 + 0) * stride + s * 25 + j * 5 + k];
+            b[s] = sf[(c * 3 + 1) * stride + i * 25 + s * 5 + k];
+            e[s] = sf[(c * 3 + 2) * stride + i * 25 + j * 5 + s];
         }
-        rv[c] = -acc;
+        rv[c] = div_core(a, b, e, Dti, Dtj, Dtk);
     }
 }
 
@@ -266,7 +300,7 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
     }
     const int64_t node = int64_t(nodef & ~FIRST_TOUCH);
     const bool first = (nodef & FIRST_TOUCH) != 0;
-    const int64_t rslot = (MODE == ASM_GATHER && valid) ? A.slot[pt] : 0;
+    const int64_t rslot = pt;  // GATHER: the point's own residual entry (coalesced stores)
     const int pslot = (PML_ELEM && valid) ? A.pml_slot[e] : -1;
     int sb = 0, se = 0;
     if (valid) {
@@ -408,7 +442,7 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
     }
     const int64_t node = int64_t(nodef & ~FIRST_TOUCH);
     const bool first = (nodef & FIRST_TOUCH) != 0;
-    const int64_t rslot = (MODE == ASM_GATHER && valid) ? A.slot[pt] : 0;
+    const int64_t rslot = pt;  // GATHER: the point's own residual entry
     int sb = 0, se = 0;
     if (valid) {
         sb = A.src_off[e];
@@ -494,6 +528,166 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
     }
 }
 
+// Pencil kernel: ten elements per 256-lane workgroup (250 lanes busy); lane
+// (element, i, j) owns the five points (i, j, 0..4) of its element, a pencil
+// along k.  The k-direction contractions of the gradient and of the
+// divergence use the lane's own registers; the i and j directions read the
+// other pencils of the element through LDS.  Per point and field this stages
+// 9 floats in LDS and reads 60 (the per-point forms stage 12 and read 90), and
+// geometry, material and node ids stay in registers for all fields.  The
+// arithmetic per point is flux_core / div_core, so the residuals equal the
+// other forms' bit for bit.  Nodal C-PML only.
+constexpr int PENCIL_EPB = 10;
+template <int MODE>
+__global__ void __launch_bounds__(256) k_element_pencil(ElemArgs A, int e0, int ne) {
+    constexpr int EPB = PENCIL_EPB;
+    __shared__ float sD[25];
+    __shared__ float su[EPB][3][NP];  // displacement of the element's points
+    __shared__ float s0[EPB][3][NP];  // F_c0 (xi-direction flux) of the points
+    __shared__ float s1[EPB][3][NP];  // F_c1 (eta-direction flux)
+    const int tid = threadIdx.x;
+    if (tid < 25) sD[tid] = A.D[tid];
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int le = tid / 25;
+    const int q = tid - le * 25;  // pencil (i, j): q = i * 5 + j
+    const int eloc = blk * EPB + le;
+    const bool valid = (le < EPB) && (eloc < ne);
+    const int lel = valid ? le : 0;
+    const int e = e0 + (valid ? eloc : 0);
+    const int i = q / 5, j = q - (q / 5) * 5;
+    const int p0 = q * 5;  // first point of the pencil
+    const int64_t pt0 = int64_t(e) * NP + p0;
+
+    float g[5][9], lam[5], mu[5];
+    uint32_t nodef[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        if (valid) {
+#pragma unroll
+            for (int c = 0; c < 9; ++c) g[k][c] = __builtin_nontemporal_load(&A.G[c * A.P + pt0 + k]);
+            lam[k] = __builtin_nontemporal_load(&A.lam[pt0 + k]);
+            mu[k] = __builtin_nontemporal_load(&A.mu[pt0 + k]);
+            nodef[k] = __builtin_nontemporal_load(&A.ibool[pt0 + k]);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 9; ++c) g[k][c] = 0.f;
+            lam[k] = mu[k] = 0.f;
+            nodef[k] = 0;
+        }
+    }
+    int sb = 0, se = 0;
+    if (valid) {
+        sb = A.src_off[e];
+        se = A.src_off[e + 1];
+    }
+    const int step = A.ctr[0];
+    const int nf = A.nf;
+    float un[5][3];  // displacement of the pencil, loaded a field ahead
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) un[k][c] = 0.f;
+    if (valid) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            const float* ut = A.U + int64_t(nodef[k] & ~FIRST_TOUCH) * 3;
+            un[k][0] = ut[0];
+            un[k][1] = ut[1];
+            un[k][2] = ut[2];
+        }
+    }
+    __syncthreads();
+    float Di[5], Dj[5], Dti[5], Dtj[5];
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+        Di[s] = sD[i * 5 + s];
+        Dj[s] = sD[j * 5 + s];
+        Dti[s] = sD[s * 5 + i];
+        Dtj[s] = sD[s * 5 + j];
+    }
+
+    // Two barriers per field: (1) after the displacement is staged, (2) after
+    // the fluxes are staged.  The next field's displacement is staged after (2),
+    // when every lane has finished reading su; the next fluxes are written after
+    // the next (1), when every lane has finished reading s0 and s1.
+    for (int f = 0; f < nf; ++f) {
+        const int64_t fo = int64_t(f) * A.fstride;
+        if (valid) {
+#pragma unroll
+            for (int k = 0; k < 5; ++k)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) su[lel][c][p0 + k] = un[k][c];
+        }
+        __syncthreads();
+        float F2[5][3];  // zeta-direction fluxes stay in registers
+        if (valid) {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                float ui[3][5], uj[3][5], uk[3][5], Dk[5];
+#pragma unroll
+                for (int s = 0; s < 5; ++s) {
+                    Dk[s] = A.D[k * 5 + s];  // uniform: scalar operands
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        ui[c][s] = su[lel][c][s * 25 + j * 5 + k];
+                        uj[c][s] = su[lel][c][i * 25 + s * 5 + k];
+                        uk[c][s] = un[s][c];
+                    }
+                }
+                float F[9];
+                flux_core(ui, uj, uk, Di, Dj, Dk, g[k], lam[k], mu[k], F);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    s0[lel][c][p0 + k] = F[c * 3 + 0];
+                    s1[lel][c][p0 + k] = F[c * 3 + 1];
+                    F2[k][c] = F[c * 3 + 2];
+                }
+            }
+            // the next field's displacement, loaded while this one finishes
+            if (f + 1 < nf) {
+#pragma unroll
+                for (int k = 0; k < 5; ++k) {
+                    const float* ut = A.U + fo + A.fstride + int64_t(nodef[k] & ~FIRST_TOUCH) * 3;
+                    un[k][0] = ut[0];
+                    un[k][1] = ut[1];
+                    un[k][2] = ut[2];
+                }
+            }
+        }
+        __syncthreads();
+        if (valid) {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                float Dtk[5];
+#pragma unroll
+                for (int s = 0; s < 5; ++s) Dtk[s] = A.D[s * 5 + k];
+                float rv[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    float a[5], b[5], ez[5];
+#pragma unroll
+                    for (int s = 0; s < 5; ++s) {
+                        a[s] = s0[lel][c][s * 25 + j * 5 + k];
+                        b[s] = s1[lel][c][i * 25 + s * 5 + k];
+                        ez[s] = F2[s][c];
+                    }
+                    rv[c] = div_core(a, b, ez, Dti, Dtj, Dtk);
+                }
+                add_sources(A, step, f, sb, se, p0 + k, rv);
+                const int64_t node = int64_t(nodef[k] & ~FIRST_TOUCH);
+                float rold[3] = {0.f, 0.f, 0.f};
+                if (MODE == ASM_COLORED && !(nodef[k] & FIRST_TOUCH)) {
+                    const float* r = A.R + fo + node * 3;
+                    rold[0] = r[0];
+                    rold[1] = r[1];
+                    rold[2] = r[2];
+                }
+                emit_residual<MODE>(A, f, node, pt0 + k, rold, rv);
+            }
+        }
+    }
+}
+
 // One wavefront per element, lane l owns points l and l + 64 (l < 61).  All
 // exchange between the element's points stays inside one wavefront, so the
 // field loop has no workgroup barrier; WPB elements (wavefronts) per block.
@@ -536,7 +730,7 @@ __global__ void __launch_bounds__(64 * WPB) k_element_wave(ElemArgs A, int e0, i
         const uint32_t nf_ = vq[q] ? __builtin_nontemporal_load(&A.ibool[pt]) : 0u;
         node[q] = int64_t(nf_ & ~FIRST_TOUCH);
         first[q] = (nf_ & FIRST_TOUCH) != 0;
-        ptq[q] = (MODE == ASM_GATHER && vq[q]) ? A.slot[pt] : 0;
+        ptq[q] = pt;  // GATHER: the point's own residual entry
     }
     float un[2][3];
 #pragma unroll
@@ -601,7 +795,7 @@ __device__ __forceinline__ void gather_r(const NodeArgs& A, int f, int kb, int k
     r[0] = r[1] = r[2] = 0.f;
     const float* Rl = A.Rl + int64_t(f) * A.P * 3;
     for (int q = kb; q < ke; ++q) {
-        const float* c = Rl + int64_t(q) * 3;
+        const float* c = Rl + int64_t(A.pts[q]) * 3;
         r[0] += c[0];
         r[1] += c[1];
         r[2] += c[2];
@@ -609,11 +803,11 @@ __device__ __forceinline__ void gather_r(const NodeArgs& A, int f, int kb, int k
 }
 
 // LDS-staged fixed-order slot sums for the nodes of one workgroup: the
-// workgroup's slots [s0, s1) of field f come into LDS in chunks with coalesced
-// loads, then each lane adds its nodes' slots in increasing slot order -- the
-// order gather_r uses, so the sums are bit-identical.  Every lane of the
-// workgroup must call it (it holds workgroup barriers); kb[0..NN] are the
-// lane's node slot bounds.
+// residuals of the workgroup's slots [s0, s1) of field f come into LDS in
+// chunks (one 12-byte load per slot, spread over the workgroup), then each
+// lane adds its nodes' slots in increasing slot order -- the order gather_r
+// uses, so the sums are bit-identical.  Every lane of the workgroup must call
+// it (it holds workgroup barriers); kb[0..NN] are the lane's node slot bounds.
 constexpr int SLOT_CHUNK = 2048;
 template <int NN>
 __device__ __forceinline__ void staged_sums(const NodeArgs& A, int f, int64_t s0, int64_t s1, const int* kb,
@@ -623,9 +817,14 @@ __device__ __forceinline__ void staged_sums(const NodeArgs& A, int f, int64_t s0
     const float* Rl = A.Rl + int64_t(f) * A.P * 3;
     for (int64_t c0 = s0; c0 < s1; c0 += SLOT_CHUNK) {
         const int64_t c1 = min(s1, c0 + SLOT_CHUNK);
-        const int cnt = int(c1 - c0) * 3;
+        const int cnt = int(c1 - c0);
         __syncthreads();  // the previous chunk has been consumed
-        for (int q = threadIdx.x; q < cnt; q += blockDim.x) sl[q] = Rl[c0 * 3 + q];
+        for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+            const float* src = Rl + int64_t(A.pts[c0 + q]) * 3;
+            sl[3 * q + 0] = src[0];
+            sl[3 * q + 1] = src[1];
+            sl[3 * q + 2] = src[2];
+        }
         __syncthreads();
         if (active) {
 #pragma unroll

@@ -60,8 +60,10 @@ struct gfx_solver {
     gfx::DevBuf<uint8_t> pmask;  // nodal C-PML: memory directions in use (bits x, y, z)
     gfx::DevBuf<int32_t> pml_slot, src_off;
     gfx::DevBuf<float> U, W, R, Aacc, Rl;
-    gfx::DevBuf<int32_t> goff, gslot;  // GATHER assembly: node slot ranges, point -> slot
-    std::vector<int32_t> goff_host;
+    // GATHER assembly: node n owns slots [goff[n], goff[n+1]); slot q sums the
+    // residual of element point gpts[q] (increasing in solver order per node)
+    gfx::DevBuf<int32_t> goff, gpts;
+    std::vector<int32_t> goff_host, gpts_host;
     std::vector<gfx::FieldSource> sources;
     gfx::DevBuf<int4> src_ent;
     gfx::DevBuf<float> src_w, stf;
@@ -202,12 +204,27 @@ void launch_pair_t(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
     }
 }
 
+void launch_pencil(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
+    const int grid = (n + dev::PENCIL_EPB - 1) / dev::PENCIL_EPB;
+    if (grid == 0) return;
+    switch (S.assembly) {
+        case GFX_ASSEMBLY_ATOMIC:
+            dev::k_element_pencil<dev::ASM_ATOMIC><<<grid, 256, 0, S.stream>>>(a, b, n);
+            break;
+        case GFX_ASSEMBLY_GATHER:
+            dev::k_element_pencil<dev::ASM_GATHER><<<grid, 256, 0, S.stream>>>(a, b, n);
+            break;
+        default:
+            dev::k_element_pencil<dev::ASM_COLORED><<<grid, 256, 0, S.stream>>>(a, b, n);
+    }
+}
+
 // Element kernel form for a request: AUTO = wave when the nodal C-PML form
-// applies, else block; WAVE and PAIR need the nodal form.
+// applies, else block; WAVE, PAIR and PENCIL need the nodal form.
 int resolve_kernel(int requested, bool nodal_ok) {
-    require(requested >= GFX_ELEMENT_AUTO && requested <= GFX_ELEMENT_PAIR, "unknown element kernel");
-    if (requested == GFX_ELEMENT_WAVE || requested == GFX_ELEMENT_PAIR)
-        require(nodal_ok, "the wave and pair element kernels need the nodal C-PML form");
+    require(requested >= GFX_ELEMENT_AUTO && requested <= GFX_ELEMENT_PENCIL, "unknown element kernel");
+    if (requested == GFX_ELEMENT_WAVE || requested == GFX_ELEMENT_PAIR || requested == GFX_ELEMENT_PENCIL)
+        require(nodal_ok, "the wave, pair and pencil element kernels need the nodal C-PML form");
     if (requested == GFX_ELEMENT_AUTO) return nodal_ok ? GFX_ELEMENT_WAVE : GFX_ELEMENT_BLOCK;
     return requested;
 }
@@ -215,10 +232,20 @@ int resolve_kernel(int requested, bool nodal_ok) {
 void check_epb(int kernel, int epb) {
     require(epb == 0 || epb == 1 || epb == 2 || epb == 4 || epb == 8, "elems_per_block must be 1, 2, 4 or 8");
     require(!(kernel == GFX_ELEMENT_PAIR && epb == 8), "the pair element kernel takes 1, 2 or 4 elements per block");
+    require(!(kernel == GFX_ELEMENT_PENCIL && epb != 0),
+            "the pencil element kernel has a fixed ten elements per block (elems_per_block must be 0)");
+}
+
+// Elements per workgroup reported and used for a kernel form and request.
+int resolve_epb(int kernel, int epb) {
+    if (kernel == GFX_ELEMENT_PENCIL) return dev::PENCIL_EPB;
+    return epb ? epb : (kernel == GFX_ELEMENT_WAVE ? 4 : 2);
 }
 
 void launch_element(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
-    if (S.kernel == GFX_ELEMENT_PAIR) {
+    if (S.kernel == GFX_ELEMENT_PENCIL) {
+        launch_pencil(S, a, b, n);
+    } else if (S.kernel == GFX_ELEMENT_PAIR) {
         switch (S.epb) {
             case 1: launch_pair_t<1>(S, a, b, n); break;
             case 4: launch_pair_t<4>(S, a, b, n); break;
@@ -291,7 +318,6 @@ StepArgs make_step_args(gfx_solver& S) {
     ea.U = S.U.p;
     ea.R = S.R.p;
     ea.Rl = S.Rl.p;
-    ea.slot = S.gslot.p;
     ea.W = S.W.p;
     ea.Aacc = S.Aacc.p;
     ea.astride = S.astride;
@@ -324,6 +350,7 @@ StepArgs make_step_args(gfx_solver& S) {
     na.gather = (S.assembly == GFX_ASSEMBLY_GATHER) ? 1 : 0;
     na.Rl = S.Rl.p;
     na.off = S.goff.p;
+    na.pts = S.gpts.p;
     na.minv = S.minv.p;
     na.damp = S.has_cpml ? nullptr : S.damp.p;
     na.U = S.U.p;
@@ -525,7 +552,7 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
     require(d.assembly == GFX_ASSEMBLY_COLORED || d.assembly == GFX_ASSEMBLY_ATOMIC ||
                 d.assembly == GFX_ASSEMBLY_GATHER,
             "unknown assembly mode");
-    require(d.element_kernel >= GFX_ELEMENT_AUTO && d.element_kernel <= GFX_ELEMENT_PAIR,
+    require(d.element_kernel >= GFX_ELEMENT_AUTO && d.element_kernel <= GFX_ELEMENT_PENCIL,
             "unknown element kernel");
     int epb = d.elems_per_block;
     require(epb == 0 || epb == 1 || epb == 2 || epb == 4 || epb == 8,
@@ -818,7 +845,7 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
     const bool wave_ok = !(S.has_cpml && !S.pml_nodal);
     S.kernel = resolve_kernel(d.element_kernel, wave_ok);
     check_epb(S.kernel, epb);
-    S.epb = epb ? epb : (S.kernel == GFX_ELEMENT_WAVE ? 4 : 2);
+    S.epb = resolve_epb(S.kernel, epb);
 
     // ---- nodal state: U, W, R [nf][fstride]; a at special nodes [nf][astride] ----
     S.fstride = ((S.G4 * 3 + 63) / 64) * 64;
@@ -830,17 +857,19 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
     dv.alloc(S.R, fs);
     dv.alloc(S.Aacc, size_t(S.astride) * S.nf);
     if (S.assembly == GFX_ASSEMBLY_GATHER) {
-        // each node owns the contiguous slots [off[n], off[n+1]); the element points
-        // of a node take its slots in increasing solver order (fixed summation order)
-        std::vector<int32_t> off(S.G4 + 1, 0), slot(P);
+        // each node owns the contiguous slots [off[n], off[n+1]), filled with its
+        // element points in increasing solver order (the fixed summation order);
+        // the element kernels store each point's residual at the point itself
+        std::vector<int32_t> off(S.G4 + 1, 0), pts(P);
         for (int64_t q = 0; q < P; ++q) ++off[hib[q] + 1];
         for (int64_t n = 0; n < S.G4; ++n) off[n + 1] += off[n];
         require(off[S.G4] == P, "internal: gather slots do not cover every element point");
         std::vector<int32_t> fill(off.begin(), off.end() - 1);
-        for (int64_t q = 0; q < P; ++q) slot[q] = fill[hib[q]]++;
+        for (int64_t q = 0; q < P; ++q) pts[fill[hib[q]]++] = int32_t(q);
         S.goff_host = off;
+        S.gpts_host = pts;
         dv.upload(S.goff, std::move(off));
-        dv.upload(S.gslot, std::move(slot));
+        dv.upload(S.gpts, std::move(pts));
         dv.alloc(S.Rl, size_t(S.nf) * P * 3);
     }
     std::vector<uint32_t>().swap(hib);
@@ -994,11 +1023,12 @@ extern "C" gfx_status gfx_solver_get_state(const gfx_solver* s, int field, doubl
             // node kernels form it
             const int64_t P = int64_t(s->E) * NP;
             const std::vector<float> hl = fetch(s, s->Rl.p + int64_t(field) * P * 3, size_t(P) * 3);
-            const std::vector<int32_t> off = s->goff_host;
+            const std::vector<int32_t>& off = s->goff_host;
+            const std::vector<int32_t>& pts = s->gpts_host;
             hR.assign(size_t(s->G4) * 3, 0.f);
             for (int64_t n = 0; n < s->G; ++n)
                 for (int q = off[n]; q < off[n + 1]; ++q)
-                    for (int c = 0; c < 3; ++c) hR[n * 3 + c] += hl[int64_t(q) * 3 + c];
+                    for (int c = 0; c < 3; ++c) hR[n * 3 + c] += hl[int64_t(pts[q]) * 3 + c];
         } else {
             hR = fetch(s, s->R.p + int64_t(field) * s->fstride, n3);
         }
@@ -1052,12 +1082,13 @@ extern "C" gfx_status gfx_solver_set_state(gfx_solver* s, int field, const doubl
         if (!hA.empty())
             GFX_HIP(hipMemcpy(s->Aacc.p + int64_t(field) * s->astride, hA.data(), hA.size() * 4,
                               hipMemcpyHostToDevice));
-        if (s->assembly == GFX_ASSEMBLY_GATHER) {  // r = m a in the first slot of each node
+        if (s->assembly == GFX_ASSEMBLY_GATHER) {  // r = m a at the first point of each node
             const int64_t P = int64_t(s->E) * NP;
             std::vector<float> hl(size_t(P) * 3, 0.f);
             for (int64_t n = 0; n < s->G; ++n)
                 if (s->goff_host[n + 1] > s->goff_host[n])
-                    for (int c = 0; c < 3; ++c) hl[int64_t(s->goff_host[n]) * 3 + c] = hR[n * 3 + c];
+                    for (int c = 0; c < 3; ++c)
+                        hl[int64_t(s->gpts_host[s->goff_host[n]]) * 3 + c] = hR[n * 3 + c];
             GFX_HIP(hipMemcpy(s->Rl.p + int64_t(field) * P * 3, hl.data(), hl.size() * 4,
                               hipMemcpyHostToDevice));
         }
@@ -1170,7 +1201,7 @@ extern "C" gfx_status gfx_solver_checkpoint_save(const gfx_solver* s, void* host
                 float* r = p + int64_t(f) * s->fstride;
                 for (int64_t n = 0; n < s->G; ++n)
                     for (int q = s->goff_host[n]; q < s->goff_host[n + 1]; ++q)
-                        for (int c = 0; c < 3; ++c) r[n * 3 + c] += hl[int64_t(q) * 3 + c];
+                        for (int c = 0; c < 3; ++c) r[n * 3 + c] += hl[int64_t(s->gpts_host[q]) * 3 + c];
             }
         }
     });
@@ -1202,7 +1233,7 @@ extern "C" gfx_status gfx_solver_checkpoint_load(gfx_solver* s, const void* host
         if (s->assembly != GFX_ASSEMBLY_GATHER) {
             put(s->R.p, fl);
         } else {
-            // the sum in the first slot of each node, zeros elsewhere: get_state
+            // the sum at the first point of each node, zeros elsewhere: get_state
             // then forms the same r
             const int64_t P = int64_t(s->E) * NP;
             std::vector<float> hl(size_t(P) * 3);
@@ -1211,7 +1242,7 @@ extern "C" gfx_status gfx_solver_checkpoint_load(gfx_solver* s, const void* host
                 const float* r = p + int64_t(f) * s->fstride;
                 for (int64_t n = 0; n < s->G; ++n)
                     if (s->goff_host[n + 1] > s->goff_host[n])
-                        for (int c = 0; c < 3; ++c) hl[int64_t(s->goff_host[n]) * 3 + c] = r[n * 3 + c];
+                        for (int c = 0; c < 3; ++c) hl[int64_t(s->gpts_host[s->goff_host[n]]) * 3 + c] = r[n * 3 + c];
                 GFX_HIP(hipMemcpy(s->Rl.p + int64_t(f) * P * 3, hl.data(), hl.size() * 4, hipMemcpyHostToDevice));
             }
         }
@@ -1264,7 +1295,7 @@ extern "C" gfx_status gfx_solver_set_launch(gfx_solver* s, int element_kernel, i
         const int kern = resolve_kernel(element_kernel, wave_ok);
         check_epb(kern, elems_per_block);
         s->kernel = kern;
-        s->epb = elems_per_block ? elems_per_block : (s->kernel == GFX_ELEMENT_WAVE ? 4 : 2);
+        s->epb = resolve_epb(s->kernel, elems_per_block);
         drop_graph(*s);
     });
 }
@@ -1296,9 +1327,10 @@ extern "C" gfx_status gfx_autotune(const gfx_solver_desc* base, int trial_steps,
         double best_t = 1e300;
         gfx_solver_desc pick = *base;
         const int asms[2] = {GFX_ASSEMBLY_COLORED, GFX_ASSEMBLY_GATHER};
-        const int launch[8][2] = {{GFX_ELEMENT_BLOCK, 1}, {GFX_ELEMENT_BLOCK, 2}, {GFX_ELEMENT_BLOCK, 4},
-                                  {GFX_ELEMENT_WAVE, 2},  {GFX_ELEMENT_WAVE, 4},  {GFX_ELEMENT_WAVE, 8},
-                                  {GFX_ELEMENT_PAIR, 1},  {GFX_ELEMENT_PAIR, 2}};
+        constexpr int NL = 9;
+        const int launch[NL][2] = {{GFX_ELEMENT_BLOCK, 1}, {GFX_ELEMENT_BLOCK, 2}, {GFX_ELEMENT_BLOCK, 4},
+                                   {GFX_ELEMENT_WAVE, 2},  {GFX_ELEMENT_WAVE, 4},  {GFX_ELEMENT_WAVE, 8},
+                                   {GFX_ELEMENT_PAIR, 1},  {GFX_ELEMENT_PAIR, 2},  {GFX_ELEMENT_PENCIL, 0}};
         auto time_steps = [&](gfx_solver& S) {
             run_steps(S, 1);  // warm-up
             hipEvent_t e0, e1;
@@ -1329,12 +1361,12 @@ extern "C" gfx_status gfx_autotune(const gfx_solver_desc* base, int trial_steps,
             create(d, *S);
             double best_here = 1e300;
             int best_lc = -1;
-            for (int li = 0; li < 8; ++li) {
+            for (int li = 0; li < NL; ++li) {
                 const auto& lc = launch[li];
                 if (lc[0] != GFX_ELEMENT_BLOCK && S->has_cpml && !S->pml_nodal) continue;
                 if (lc[0] == GFX_ELEMENT_PAIR && S->nf < 2) continue;
                 S->kernel = lc[0];
-                S->epb = lc[1];
+                S->epb = resolve_epb(lc[0], lc[1]);
                 const double per = time_steps(*S);
                 if (per < best_here) {
                     best_here = per;
@@ -1343,7 +1375,7 @@ extern "C" gfx_status gfx_autotune(const gfx_solver_desc* base, int trial_steps,
             }
             // node-kernel field groups for this assembly's best element launch
             S->kernel = launch[best_lc][0];
-            S->epb = launch[best_lc][1];
+            S->epb = resolve_epb(launch[best_lc][0], launch[best_lc][1]);
             int best_fg = fg0;
             for (int fg : {1, 2, 4, 8}) {
                 if (fg == fg0 || fg > 2 * S->nf) continue;

@@ -21,6 +21,7 @@ GFX_ELEMENT_AUTO = 0
 GFX_ELEMENT_BLOCK = 1
 GFX_ELEMENT_WAVE = 2
 GFX_ELEMENT_PAIR = 3
+GFX_ELEMENT_PENCIL = 4
 MAX_LAYERS = 8
 
 _dp = C.POINTER(C.c_double)

@@ -81,7 +81,10 @@ typedef enum {
     GFX_ELEMENT_AUTO = 0,  /* wave kernel when applicable, else block kernel */
     GFX_ELEMENT_BLOCK = 1, /* one lane per GLL point, workgroup barriers between stages */
     GFX_ELEMENT_WAVE = 2,  /* one wavefront per element, two points per lane, no barriers */
-    GFX_ELEMENT_PAIR = 3   /* as BLOCK, two fields per pass (nodal C-PML only) */
+    GFX_ELEMENT_PAIR = 3,  /* as BLOCK, two fields per pass (nodal C-PML only) */
+    GFX_ELEMENT_PENCIL = 4 /* lane per 5-point pencil along k, ten elements per workgroup
+                              (fixed; elems_per_block must be 0), k contractions in
+                              registers (nodal C-PML only) */
 } gfx_element_kernel;
 
 typedef struct {

@@ -240,6 +240,38 @@ def test_pair_kernel_matches_block_kernel_and_oracle(assembly, nf):
     assert mc.rel_l2(got, ref["strain"][:a.n_snapshots]) <= TOL
 
 
+@pytest.mark.parametrize("assembly,nf", [(pygfx.GFX_ASSEMBLY_GATHER, 3), (pygfx.GFX_ASSEMBLY_COLORED, 2),
+                                         (pygfx.GFX_ASSEMBLY_ATOMIC, 1)])
+def test_pencil_kernel_matches_block_kernel_and_oracle(assembly, nf):
+    """Lane-per-pencil element kernel (k contractions in registers, ten
+    elements per workgroup, last workgroup partly empty) against the
+    lane-per-point block kernel, and field 0 against the oracle.  Both run the
+    same per-point operation sequence, so with deterministic assembly the
+    states agree to the last bit; atomic assembly (float atomics, order
+    varies from run to run) only to rounding: 1e-5 rel L2, the acceleration
+    being the most cancellation-prone of u, v, a."""
+    mesh = mc.small_layered(duration=0.8)
+    assert mesh.n_elem % 10 != 0  # a partial last workgroup is exercised
+    cells, w, stf = mc.buried_source(mesh)
+    c2, w2, stf2 = mc.buried_source(mesh, frac=(0.4, 0.6, 0.0), depth=900.0)
+    srcs = [(cells, w, stf, 2), (c2, w2, stf2, 0), (cells, w, stf, 1)][:nf]
+    a = _gpu(mesh, srcs, assembly=assembly, kernel=pygfx.GFX_ELEMENT_PENCIL)
+    b = _gpu(mesh, srcs, assembly=assembly, kernel=pygfx.GFX_ELEMENT_BLOCK, epb=2)
+    assert int(a.query("element_kernel")) == pygfx.GFX_ELEMENT_PENCIL
+    assert int(a.query("elems_per_block")) == 10
+    for f in range(nf):
+        for x, y in zip(a.get_state(f), b.get_state(f)):
+            if assembly == pygfx.GFX_ASSEMBLY_ATOMIC:
+                assert mc.rel_l2(x, y) <= 1e-5
+            else:
+                assert np.array_equal(x, y)
+    ref = _oracle(mesh, cells, w, stf, 2)
+    got = np.stack([a.get_record(0, k) for k in range(a.n_snapshots)])
+    err = mc.rel_l2(got, ref["strain"][:a.n_snapshots])
+    print(f"pencil assembly={assembly} strain rel L2 = {err:.3e}")
+    assert err <= TOL
+
+
 @pytest.mark.parametrize("assembly", [pygfx.GFX_ASSEMBLY_COLORED, pygfx.GFX_ASSEMBLY_GATHER])
 def test_node_field_groups_do_not_change_results(assembly):
     """Spreading the fields of a node block over 1, 2, 4 or 8 lane groups only

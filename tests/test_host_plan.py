@@ -39,6 +39,8 @@ def test_create_still_rejects_bad_arguments_first():
         pygfx.Solver(mesh, elems_per_block=3)
     with pytest.raises(pygfx.GfxError, match="pair element kernel"):
         pygfx.Solver(mesh, n_fields=2, element_kernel=pygfx.GFX_ELEMENT_PAIR, elems_per_block=8)
+    with pytest.raises(pygfx.GfxError, match="elems_per_block must be 0"):
+        pygfx.Solver(mesh, element_kernel=pygfx.GFX_ELEMENT_PENCIL, elems_per_block=2)
     good = mesh.arrays["ibool"]
     bad = np.array(good, copy=True)
     bad.reshape(-1)[7] = mesh.n_node
