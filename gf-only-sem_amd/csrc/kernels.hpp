@@ -809,9 +809,21 @@ __device__ __forceinline__ void gather_r(const NodeArgs& A, int f, int kb, int k
 // uses, so the sums are bit-identical.  Every lane of the workgroup must call
 // it (it holds workgroup barriers); kb[0..NN] are the lane's node slot bounds.
 constexpr int SLOT_CHUNK = 2048;
+
+// Point ids of the workgroup's slots [s0, s1) into LDS when they fit one
+// chunk and serve more than one field; staged_sums then reads them there for
+// every field.  Returns whether they were staged (uniform over the
+// workgroup); the ids become visible at the first barrier inside staged_sums.
+__device__ __forceinline__ bool stage_points(const NodeArgs& A, int64_t s0, int64_t s1, int32_t* __restrict__ sp) {
+    if (A.nf < 2 || s1 - s0 > SLOT_CHUNK) return false;
+    for (int q = threadIdx.x; q < int(s1 - s0); q += blockDim.x) sp[q] = A.pts[s0 + q];
+    return true;
+}
+
 template <int NN>
 __device__ __forceinline__ void staged_sums(const NodeArgs& A, int f, int64_t s0, int64_t s1, const int* kb,
-                                            bool active, float* __restrict__ sl, float* r) {
+                                            bool active, float* __restrict__ sl, const int32_t* __restrict__ sp,
+                                            bool ids_staged, float* r) {
 #pragma unroll
     for (int q = 0; q < 3 * NN; ++q) r[q] = 0.f;
     const float* Rl = A.Rl + int64_t(f) * A.P * 3;
@@ -820,7 +832,8 @@ __device__ __forceinline__ void staged_sums(const NodeArgs& A, int f, int64_t s0
         const int cnt = int(c1 - c0);
         __syncthreads();  // the previous chunk has been consumed
         for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
-            const float* src = Rl + int64_t(A.pts[c0 + q]) * 3;
+            const int32_t pid = ids_staged ? sp[q] : A.pts[c0 + q];
+            const float* src = Rl + int64_t(pid) * 3;
             sl[3 * q + 0] = src[0];
             sl[3 * q + 1] = src[1];
             sl[3 * q + 2] = src[2];
@@ -851,6 +864,7 @@ __global__ void __launch_bounds__(256) k_node_special(NodeArgs A) {
     // one field group: the workgroup's slot sums are staged through LDS
     constexpr bool STAGE = FG == 1;
     __shared__ float sl[STAGE ? SLOT_CHUNK * 3 : 1];
+    __shared__ int32_t sp[STAGE ? SLOT_CHUNK : 1];
     const int fy = threadIdx.x / NPB;
     const int64_t g0 = int64_t(blockIdx.x) * NPB + (threadIdx.x - fy * NPB);
     const bool active = g0 < A.Ga;
@@ -874,6 +888,7 @@ __global__ void __launch_bounds__(256) k_node_special(NodeArgs A) {
     }
     int kb[2] = {0, 0};
     int64_t s0 = 0, s1 = 0;
+    bool ids = false;
     if (A.gather) {
         kb[0] = A.off[g];
         kb[1] = A.off[g + 1];
@@ -881,38 +896,46 @@ __global__ void __launch_bounds__(256) k_node_special(NodeArgs A) {
             const int64_t gb = int64_t(blockIdx.x) * NPB;
             s0 = A.off[gb];
             s1 = A.off[min(A.Ga, gb + NPB)];
+            ids = stage_points(A, s0, s1, sp);
         }
     }
     for (int f = fy; f < A.nf; f += FG) {
         const int64_t o = int64_t(f) * A.fstride + g * 3;
         const int64_t oa = int64_t(f) * A.astride + g * 3;
+        float* pmem = A.pmem + int64_t(f) * 9 * A.Gpml;
+        float* pnew = A.pnew + int64_t(f) * 3 * A.Gpml;
         float U[3], W[3], R[3], an[3], un[3], vn[3];
-        if (staged) {
-            staged_sums<1>(A, f, s0, s1, kb, active, sl, R);
-            if (!active) continue;
-        } else if (A.gather) {
-            gather_r(A, f, kb[0], kb[1], R);
-        }
+        // this field's node state first, so that its loads overlap the slot sums;
+        // C-PML memory of unused directions is zero and stays zero (alpha = (1, 0, 0),
+        // C = 0): neither read nor written
+        float m[9], pn[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             U[c] = A.U[o + c];
             W[c] = A.W[o + c];
-            if (!A.gather) R[c] = A.R[o + c];
             an[c] = A.Aacc[oa + c];
+            pn[c] = pml ? pnew[c * A.Gpml + g] : 0.f;
+        }
+#pragma unroll
+        for (int q = 0; q < 9; ++q) m[q] = (pml && use[q % 3]) ? pmem[q * A.Gpml + g] : 0.f;
+        if (staged) {
+            staged_sums<1>(A, f, s0, s1, kb, active, sl, sp, ids, R);
+            if (!active) continue;
+        } else if (A.gather) {
+            gather_r(A, f, kb[0], kb[1], R);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) R[c] = A.R[o + c];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
             un[c] = U[c] - dt * W[c];
             vn[c] = W[c] - hdt * an[c];
         }
         if (pml) {
-            float* pmem = A.pmem + int64_t(f) * 9 * A.Gpml;
-            float* pnew = A.pnew + int64_t(f) * 3 * A.Gpml;
-            // memory of unused directions is zero and stays zero (alpha = (1, 0, 0),
-            // C = 0): neither read nor written
-            float m[9];
-#pragma unroll
-            for (int q = 0; q < 9; ++q) m[q] = use[q % 3] ? pmem[q * A.Gpml + g] : 0.f;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const float oldv = pnew[c * A.Gpml + g] + A.c2 * an[c];  // pml.cpp:91-109
+                const float oldv = pn[c] + A.c2 * an[c];  // pml.cpp:91-109
                 const float newv = un[c] + A.c1 * vn[c];                 // pml.cpp:72-89
                 R[c] += C[0] * vn[c] + C[1] * un[c] + C[2] * m[c * 3 + 0] + C[3] * m[c * 3 + 1] +
                         C[4] * m[c * 3 + 2];  // pml.cpp:209-212, assembled over PML elements
@@ -953,6 +976,7 @@ __global__ void __launch_bounds__(256) k_node_regular(NodeArgs A) {
     constexpr int NPB = 256 / FG;
     constexpr bool STAGE = GATHER && FG == 1;  // slot sums staged through LDS
     __shared__ float sl[STAGE ? SLOT_CHUNK * 3 : 1];
+    __shared__ int32_t sp[STAGE ? SLOT_CHUNK : 1];
     const int fy = threadIdx.x / NPB;
     const int64_t t = int64_t(blockIdx.x) * NPB + (threadIdx.x - fy * NPB);
     const int64_t g0 = A.Ga + 4 * t;
@@ -964,6 +988,7 @@ __global__ void __launch_bounds__(256) k_node_regular(NodeArgs A) {
     const float mi[4] = {mi4.x, mi4.y, mi4.z, mi4.w};
     int kb[5] = {0, 0, 0, 0, 0};
     int64_t s0 = 0, s1 = 0;
+    bool ids = false;
     if (GATHER) {
 #pragma unroll
         for (int q = 0; q < 5; ++q) kb[q] = A.off[g + q];
@@ -971,6 +996,7 @@ __global__ void __launch_bounds__(256) k_node_regular(NodeArgs A) {
             const int64_t gb = A.Ga + int64_t(blockIdx.x) * NPB * 4;
             s0 = A.off[gb];
             s1 = A.off[min(A.G4, gb + int64_t(NPB) * 4)];
+            ids = stage_points(A, s0, s1, sp);
         }
     }
     for (int f = fy; f < A.nf; f += FG) {
@@ -978,8 +1004,15 @@ __global__ void __launch_bounds__(256) k_node_regular(NodeArgs A) {
         float4* U4 = reinterpret_cast<float4*>(A.U + o);
         float4* W4 = reinterpret_cast<float4*>(A.W + o);
         float u[12], w[12], r[12];
+        // this field's U and W first, so that their loads overlap the slot sums
+        float4 ua[3], wa[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            ua[q] = U4[q];
+            wa[q] = W4[q];
+        }
         if (STAGE) {
-            staged_sums<4>(A, f, s0, s1, kb, active, sl, r);
+            staged_sums<4>(A, f, s0, s1, kb, active, sl, sp, ids, r);
             if (!active) continue;
         } else if (GATHER) {
 #pragma unroll
@@ -994,7 +1027,7 @@ __global__ void __launch_bounds__(256) k_node_regular(NodeArgs A) {
         }
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            const float4 a = U4[q], b = W4[q];
+            const float4 a = ua[q], b = wa[q];
             u[4 * q + 0] = a.x; u[4 * q + 1] = a.y; u[4 * q + 2] = a.z; u[4 * q + 3] = a.w;
             w[4 * q + 0] = b.x; w[4 * q + 1] = b.y; w[4 * q + 2] = b.z; w[4 * q + 3] = b.w;
         }
