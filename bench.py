@@ -319,6 +319,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-uneven", action="store_true", help="skip the 16-mesh uneven batch line")
+    ap.add_argument("--general-geometry", action="store_true",
+                    help="run every element on the general (full dxi/dx) form, no axis-aligned path")
     ap.add_argument("--surrogate", action="store_true",
                     help="instead of the step benchmark: Green tile of the layered mesh (x, y, z forces at the "
                          "surface centre, cells to 1.8 km, every snapshot), MLP training, 1M scored queries")
@@ -357,7 +359,8 @@ def main():
     def launch_config(n_fields):
         if args.assembly == "auto":
             t0 = time.perf_counter()
-            pick = pygfx.autotune(mesh, n_fields=n_fields, trial_steps=3, record_capacity=0)
+            pick = pygfx.autotune(mesh, n_fields=n_fields, trial_steps=3, record_capacity=0,
+                                  general_geometry=args.general_geometry)
             log(f"autotune F={n_fields}: {pick} ({time.perf_counter() - t0:.1f} s)")
             return dict(assembly=pick["assembly"], element_kernel=pick["element_kernel"],
                         elems_per_block=pick["elems_per_block"], node_field_groups=pick["node_field_groups"])
@@ -386,7 +389,7 @@ def main():
     if not args.no_plain:
         lc1 = launch_config(1)
         t0 = time.perf_counter()
-        s1 = pygfx.Solver(mesh, n_fields=1, record_capacity=2, **lc1)
+        s1 = pygfx.Solver(mesh, n_fields=1, record_capacity=2, general_geometry=args.general_geometry, **lc1)
         cells, w = mesh.locate_source(cx, cy, 0.0)
         s1.set_source(0, 2, cells, w, stf)
         log(f"plain solver ready ({time.perf_counter() - t0:.1f} s)")
@@ -409,7 +412,7 @@ def main():
     F = args.fields
     lcb = launch_config(F)
     t0 = time.perf_counter()
-    sb = pygfx.Solver(mesh, n_fields=F, record_capacity=2, **lcb)
+    sb = pygfx.Solver(mesh, n_fields=F, record_capacity=2, general_geometry=args.general_geometry, **lcb)
     st = stations(mesh, n=max(1, (F + 2) // 3))
     for f in range(F):
         x, y = st[f // 3]
@@ -455,7 +458,8 @@ def main():
                    "elements": E, "nodes": G, "fields": F, "pml_elements": mesh.n_pml_elem,
                    "recording_cells": int(len(mesh.rec_cell)), "snapshot_stride": mesh.snapshot_stride,
                    "dt_s": mesh.dt, "parallelism": "none (1 GPU)",
-                   "launch": launch},
+                   "launch": launch,
+                   "axis_aligned_elements_on_diagonal_form": int(sb.query("diag_elements"))},
         "node_updates_per_s": G * F * args.steps / tb["wall_s"],
         "plain": plain,
         "batched_vs_plain": (value / plain["value"]) if plain else None,

@@ -134,7 +134,11 @@ __device__ __forceinline__ void wave_sync() {
 // neighbours ui[c][s] = u_c(s,j,k), uj[c][s] = u_c(i,s,k), uk[c][s] = u_c(i,j,s)
 // and the scaled geometry g (dxi_a/dx_m sqrt(J w)): writes the 9 fluxes
 // F[c][a] = sum_m sigma[c][m] dxi_a/dx_m J w  (element_cpu.cpp:51-108).  Every
-// element kernel form goes through this one sequence of operations.
+// element kernel form goes through this one sequence of operations.  DIAG:
+// the element is axis-aligned (dxi_a/dx_m = 0 for a != m, up to FP64
+// round-off of the input), only g[0], g[4], g[8] are used and the products
+// with the zero entries are left out.
+template <bool DIAG>
 __device__ __forceinline__ void flux_core(const float (&ui)[3][5], const float (&uj)[3][5], const float (&uk)[3][5],
                                           const float* Di, const float* Dj, const float* Dk, const float* g,
                                           float lam, float mu, float* F) {
@@ -157,7 +161,8 @@ __device__ __forceinline__ void flux_core(const float (&ui)[3][5], const float (
     for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int m = 0; m < 3; ++m)
-            gr[c][m] = du[c][0] * g[m * 3 + 0] + du[c][1] * g[m * 3 + 1] + du[c][2] * g[m * 3 + 2];
+            gr[c][m] = DIAG ? du[c][m] * g[m * 4]
+                            : du[c][0] * g[m * 3 + 0] + du[c][1] * g[m * 3 + 1] + du[c][2] * g[m * 3 + 2];
     const float tr = gr[0][0] + gr[1][1] + gr[2][2];
     const float mu2 = 2.f * mu;
     float sig[3][3];
@@ -171,10 +176,21 @@ __device__ __forceinline__ void flux_core(const float (&ui)[3][5], const float (
     for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int a = 0; a < 3; ++a)
-            F[c * 3 + a] = sig[c][0] * g[0 * 3 + a] + sig[c][1] * g[1 * 3 + a] + sig[c][2] * g[2 * 3 + a];
+            F[c * 3 + a] = DIAG ? sig[c][a] * g[a * 4]
+                                : sig[c][0] * g[0 * 3 + a] + sig[c][1] * g[1 * 3 + a] + sig[c][2] * g[2 * 3 + a];
+}
+
+// Geometry of one element point: all 9 scaled entries, or (DIAG) the three
+// diagonal ones with the others 0.
+template <bool DIAG>
+__device__ __forceinline__ void load_geometry(const ElemArgs& A, int64_t pt, float* g) {
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+        g[c] = (DIAG && (c & 3) != 0) ? 0.f : __builtin_nontemporal_load(&A.G[c * A.P + pt]);
 }
 
 // point_flux with the neighbours read from a staged element (su_c[p], p = (i*5+j)*5+k).
+template <bool DIAG>
 __device__ __forceinline__ void point_flux(const float* __restrict__ su0, const float* __restrict__ su1,
                                            const float* __restrict__ su2, const float* Di,
                                            const float* Dj, const float* Dk, int i, int j, int k,
@@ -189,7 +205,7 @@ __device__ __forceinline__ void point_flux(const float* __restrict__ su0, const 
             uj[c][s] = su[c][i * 25 + s * 5 + k];
             uk[c][s] = su[c][i * 25 + j * 5 + s];
         }
-    flux_core(ui, uj, uk, Di, Dj, Dk, g, lam, mu, F);
+    flux_core<DIAG>(ui, uj, uk, Di, Dj, Dk, g, lam, mu, F);
 }
 
 // One component of the weighted divergence from the point's flux rows
@@ -267,7 +283,7 @@ __global__ void k_advance(int* ctr, int stride) {
 
 // One lane per GLL point, EPB elements per workgroup, workgroup barriers
 // between the stages.  Supports the element-local C-PML fallback.
-template <int EPB, int MODE, bool PML_ELEM>
+template <int EPB, int MODE, bool PML_ELEM, bool DIAG>
 __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
     k_element(ElemArgs A, int e0, int ne) {
     __shared__ float sD[25];
@@ -289,8 +305,7 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
     uint32_t nodef = 0;
     const int64_t pt = int64_t(e) * NP + p;
     if (valid) {
-#pragma unroll
-        for (int c = 0; c < 9; ++c) g[c] = __builtin_nontemporal_load(&A.G[c * A.P + pt]);
+        load_geometry<DIAG>(A, pt, g);
         lam = __builtin_nontemporal_load(&A.lam[pt]);
         mu = __builtin_nontemporal_load(&A.mu[pt]);
         nodef = __builtin_nontemporal_load(&A.ibool[pt]);
@@ -354,7 +369,7 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
         __syncthreads();
         if (valid) {
             float F[9];
-            point_flux(su[lel][0], su[lel][1], su[lel][2], Di, Dj, Dk, i, j, k, g, lam, mu, F);
+            point_flux<DIAG>(su[lel][0], su[lel][1], su[lel][2], Di, Dj, Dk, i, j, k, g, lam, mu, F);
 #pragma unroll
             for (int q = 0; q < 9; ++q) sf[le][q][p] = F[q];
         }
@@ -409,7 +424,7 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
 // (nodal C-PML only), with fields f and f+1 staged and computed together, so
 // each pass has two independent dependency chains per lane and one pair of
 // workgroup barriers serves two fields.
-template <int EPB, int MODE>
+template <int EPB, int MODE, bool DIAG>
 __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
     k_element_pair(ElemArgs A, int e0, int ne) {
     __shared__ float sD[25];
@@ -431,8 +446,7 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
     uint32_t nodef = 0;
     const int64_t pt = int64_t(e) * NP + p;
     if (valid) {
-#pragma unroll
-        for (int c = 0; c < 9; ++c) g[c] = __builtin_nontemporal_load(&A.G[c * A.P + pt]);
+        load_geometry<DIAG>(A, pt, g);
         lam = __builtin_nontemporal_load(&A.lam[pt]);
         mu = __builtin_nontemporal_load(&A.mu[pt]);
         nodef = __builtin_nontemporal_load(&A.ibool[pt]);
@@ -505,8 +519,8 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
         __syncthreads();
         if (valid) {
             float F0[9], F1[9];
-            point_flux(su[0][lel][0], su[0][lel][1], su[0][lel][2], Di, Dj, Dk, i, j, k, g, lam, mu, F0);
-            point_flux(su[1][lel][0], su[1][lel][1], su[1][lel][2], Di, Dj, Dk, i, j, k, g, lam, mu, F1);
+            point_flux<DIAG>(su[0][lel][0], su[0][lel][1], su[0][lel][2], Di, Dj, Dk, i, j, k, g, lam, mu, F0);
+            point_flux<DIAG>(su[1][lel][0], su[1][lel][1], su[1][lel][2], Di, Dj, Dk, i, j, k, g, lam, mu, F1);
 #pragma unroll
             for (int q = 0; q < 9; ++q) {
                 sf[0][le][q][p] = F0[q];
@@ -538,7 +552,7 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
 // arithmetic per point is flux_core / div_core, so the residuals equal the
 // other forms' bit for bit.  Nodal C-PML only.
 constexpr int PENCIL_EPB = 10;
-template <int MODE>
+template <int MODE, bool DIAG>
 __global__ void __launch_bounds__(256) k_element_pencil(ElemArgs A, int e0, int ne) {
     constexpr int EPB = PENCIL_EPB;
     __shared__ float sD[25];
@@ -563,8 +577,7 @@ __global__ void __launch_bounds__(256) k_element_pencil(ElemArgs A, int e0, int 
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
         if (valid) {
-#pragma unroll
-            for (int c = 0; c < 9; ++c) g[k][c] = __builtin_nontemporal_load(&A.G[c * A.P + pt0 + k]);
+            load_geometry<DIAG>(A, pt0 + k, g[k]);
             lam[k] = __builtin_nontemporal_load(&A.lam[pt0 + k]);
             mu[k] = __builtin_nontemporal_load(&A.mu[pt0 + k]);
             nodef[k] = __builtin_nontemporal_load(&A.ibool[pt0 + k]);
@@ -635,7 +648,7 @@ __global__ void __launch_bounds__(256) k_element_pencil(ElemArgs A, int e0, int 
                     }
                 }
                 float F[9];
-                flux_core(ui, uj, uk, Di, Dj, Dk, g[k], lam[k], mu[k], F);
+                flux_core<DIAG>(ui, uj, uk, Di, Dj, Dk, g[k], lam[k], mu[k], F);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     s0[lel][c][p0 + k] = F[c * 3 + 0];
@@ -691,7 +704,7 @@ __global__ void __launch_bounds__(256) k_element_pencil(ElemArgs A, int e0, int 
 // One wavefront per element, lane l owns points l and l + 64 (l < 61).  All
 // exchange between the element's points stays inside one wavefront, so the
 // field loop has no workgroup barrier; WPB elements (wavefronts) per block.
-template <int WPB, int MODE>
+template <int WPB, int MODE, bool DIAG>
 __global__ void __launch_bounds__(64 * WPB) k_element_wave(ElemArgs A, int e0, int ne) {
     __shared__ __align__(16) float sDr[5][8];  // rows of D, padded
     __shared__ __align__(16) float sDc[5][8];  // columns of D, padded
@@ -724,7 +737,8 @@ __global__ void __launch_bounds__(64 * WPB) k_element_wave(ElemArgs A, int e0, i
         kq[q] = p % 5;
         const int64_t pt = int64_t(e) * NP + p;
 #pragma unroll
-        for (int c = 0; c < 9; ++c) g[q][c] = vq[q] ? __builtin_nontemporal_load(&A.G[c * A.P + pt]) : 0.f;
+        for (int c = 0; c < 9; ++c) g[q][c] = 0.f;
+        if (vq[q]) load_geometry<DIAG>(A, pt, g[q]);
         lam[q] = vq[q] ? __builtin_nontemporal_load(&A.lam[pt]) : 0.f;
         mu[q] = vq[q] ? __builtin_nontemporal_load(&A.mu[pt]) : 0.f;
         const uint32_t nf_ = vq[q] ? __builtin_nontemporal_load(&A.ibool[pt]) : 0u;
@@ -767,7 +781,7 @@ __global__ void __launch_bounds__(64 * WPB) k_element_wave(ElemArgs A, int e0, i
             const float Dj[5] = {dj.x, dj.y, dj.z, dj.w, sDr[j][4]};
             const float Dk[5] = {dk.x, dk.y, dk.z, dk.w, sDr[k][4]};
             float F[9];
-            point_flux(su[wv][0], su[wv][1], su[wv][2], Di, Dj, Dk, i, j, k, g[q], lam[q], mu[q], F);
+            point_flux<DIAG>(su[wv][0], su[wv][1], su[wv][2], Di, Dj, Dk, i, j, k, g[q], lam[q], mu[q], F);
 #pragma unroll
             for (int c = 0; c < 9; ++c) sf[wv][c][pq[q]] = F[c];
         }

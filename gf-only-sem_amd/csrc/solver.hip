@@ -24,6 +24,13 @@
 namespace gfx {
 
 namespace {
+// Solver elements [b, e) launched together: one colour (COLORED) or all
+// elements, split into the axis-aligned part (diag) and the rest.
+struct ElemRange {
+    int b, e;
+    bool diag;
+};
+
 struct FieldSource {
     int direction = -1;
     std::vector<int32_t> cells;  // caller element indices
@@ -52,7 +59,9 @@ struct gfx_solver {
     std::vector<int32_t> int_of_elem;        // caller element -> solver element
     std::vector<int32_t> node_new_of_old;    // caller node -> solver node
     std::vector<double> mass_node;           // assembled mass, solver numbering
-    std::vector<std::pair<int, int>> ranges;  // solver element ranges launched together
+    std::vector<gfx::ElemRange> ranges;      // solver element ranges launched together
+    int n_colors = 1;
+    int64_t n_diag = 0;                       // elements on the axis-aligned (diagonal) path
     gfx::DevBuf<float> G9, lam, mu, minv, damp;
     gfx::DevBuf<uint32_t> ibool;
     // C-PML: nodal (pc, palpha, pmem, pnew over [0,Gpml)) or element-local (Ppml points)
@@ -144,78 +153,81 @@ void init_state(gfx_solver& S) {
     S.n_snap = 0;
 }
 
-template <int EPB, bool PE>
+// Launchers.  DIAG selects the axis-aligned geometry form for a range whose
+// elements all have diagonal dxi/dx.
+template <int EPB, bool PE, bool DIAG>
 void launch_element_t(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
     constexpr int NT = ((EPB * NP + 63) / 64) * 64;
     const int grid = (n + EPB - 1) / EPB;
     if (grid == 0) return;
     switch (S.assembly) {
         case GFX_ASSEMBLY_ATOMIC:
-            dev::k_element<EPB, dev::ASM_ATOMIC, PE><<<grid, NT, 0, S.stream>>>(a, b, n);
+            dev::k_element<EPB, dev::ASM_ATOMIC, PE, DIAG><<<grid, NT, 0, S.stream>>>(a, b, n);
             break;
         case GFX_ASSEMBLY_GATHER:
-            dev::k_element<EPB, dev::ASM_GATHER, PE><<<grid, NT, 0, S.stream>>>(a, b, n);
+            dev::k_element<EPB, dev::ASM_GATHER, PE, DIAG><<<grid, NT, 0, S.stream>>>(a, b, n);
             break;
         default:
-            dev::k_element<EPB, dev::ASM_COLORED, PE><<<grid, NT, 0, S.stream>>>(a, b, n);
+            dev::k_element<EPB, dev::ASM_COLORED, PE, DIAG><<<grid, NT, 0, S.stream>>>(a, b, n);
     }
 }
 
-template <bool PE>
+template <bool PE, bool DIAG>
 void launch_element_p(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
     switch (S.epb) {
-        case 1: launch_element_t<1, PE>(S, a, b, n); break;
-        case 2: launch_element_t<2, PE>(S, a, b, n); break;
-        case 4: launch_element_t<4, PE>(S, a, b, n); break;
-        default: launch_element_t<8, PE>(S, a, b, n); break;
+        case 1: launch_element_t<1, PE, DIAG>(S, a, b, n); break;
+        case 2: launch_element_t<2, PE, DIAG>(S, a, b, n); break;
+        case 4: launch_element_t<4, PE, DIAG>(S, a, b, n); break;
+        default: launch_element_t<8, PE, DIAG>(S, a, b, n); break;
     }
 }
 
-template <int WPB>
+template <int WPB, bool DIAG>
 void launch_wave_t(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
     const int grid = (n + WPB - 1) / WPB;
     if (grid == 0) return;
     switch (S.assembly) {
         case GFX_ASSEMBLY_ATOMIC:
-            dev::k_element_wave<WPB, dev::ASM_ATOMIC><<<grid, 64 * WPB, 0, S.stream>>>(a, b, n);
+            dev::k_element_wave<WPB, dev::ASM_ATOMIC, DIAG><<<grid, 64 * WPB, 0, S.stream>>>(a, b, n);
             break;
         case GFX_ASSEMBLY_GATHER:
-            dev::k_element_wave<WPB, dev::ASM_GATHER><<<grid, 64 * WPB, 0, S.stream>>>(a, b, n);
+            dev::k_element_wave<WPB, dev::ASM_GATHER, DIAG><<<grid, 64 * WPB, 0, S.stream>>>(a, b, n);
             break;
         default:
-            dev::k_element_wave<WPB, dev::ASM_COLORED><<<grid, 64 * WPB, 0, S.stream>>>(a, b, n);
+            dev::k_element_wave<WPB, dev::ASM_COLORED, DIAG><<<grid, 64 * WPB, 0, S.stream>>>(a, b, n);
     }
 }
 
-template <int EPB>
+template <int EPB, bool DIAG>
 void launch_pair_t(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
     constexpr int NT = ((EPB * NP + 63) / 64) * 64;
     const int grid = (n + EPB - 1) / EPB;
     if (grid == 0) return;
     switch (S.assembly) {
         case GFX_ASSEMBLY_ATOMIC:
-            dev::k_element_pair<EPB, dev::ASM_ATOMIC><<<grid, NT, 0, S.stream>>>(a, b, n);
+            dev::k_element_pair<EPB, dev::ASM_ATOMIC, DIAG><<<grid, NT, 0, S.stream>>>(a, b, n);
             break;
         case GFX_ASSEMBLY_GATHER:
-            dev::k_element_pair<EPB, dev::ASM_GATHER><<<grid, NT, 0, S.stream>>>(a, b, n);
+            dev::k_element_pair<EPB, dev::ASM_GATHER, DIAG><<<grid, NT, 0, S.stream>>>(a, b, n);
             break;
         default:
-            dev::k_element_pair<EPB, dev::ASM_COLORED><<<grid, NT, 0, S.stream>>>(a, b, n);
+            dev::k_element_pair<EPB, dev::ASM_COLORED, DIAG><<<grid, NT, 0, S.stream>>>(a, b, n);
     }
 }
 
+template <bool DIAG>
 void launch_pencil(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
     const int grid = (n + dev::PENCIL_EPB - 1) / dev::PENCIL_EPB;
     if (grid == 0) return;
     switch (S.assembly) {
         case GFX_ASSEMBLY_ATOMIC:
-            dev::k_element_pencil<dev::ASM_ATOMIC><<<grid, 256, 0, S.stream>>>(a, b, n);
+            dev::k_element_pencil<dev::ASM_ATOMIC, DIAG><<<grid, 256, 0, S.stream>>>(a, b, n);
             break;
         case GFX_ASSEMBLY_GATHER:
-            dev::k_element_pencil<dev::ASM_GATHER><<<grid, 256, 0, S.stream>>>(a, b, n);
+            dev::k_element_pencil<dev::ASM_GATHER, DIAG><<<grid, 256, 0, S.stream>>>(a, b, n);
             break;
         default:
-            dev::k_element_pencil<dev::ASM_COLORED><<<grid, 256, 0, S.stream>>>(a, b, n);
+            dev::k_element_pencil<dev::ASM_COLORED, DIAG><<<grid, 256, 0, S.stream>>>(a, b, n);
     }
 }
 
@@ -242,26 +254,36 @@ int resolve_epb(int kernel, int epb) {
     return epb ? epb : (kernel == GFX_ELEMENT_WAVE ? 4 : 2);
 }
 
-void launch_element(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
+template <bool DIAG>
+void launch_element_d(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
     if (S.kernel == GFX_ELEMENT_PENCIL) {
-        launch_pencil(S, a, b, n);
+        launch_pencil<DIAG>(S, a, b, n);
     } else if (S.kernel == GFX_ELEMENT_PAIR) {
         switch (S.epb) {
-            case 1: launch_pair_t<1>(S, a, b, n); break;
-            case 4: launch_pair_t<4>(S, a, b, n); break;
-            default: launch_pair_t<2>(S, a, b, n); break;
+            case 1: launch_pair_t<1, DIAG>(S, a, b, n); break;
+            case 4: launch_pair_t<4, DIAG>(S, a, b, n); break;
+            default: launch_pair_t<2, DIAG>(S, a, b, n); break;
         }
     } else if (S.kernel == GFX_ELEMENT_WAVE) {
         switch (S.epb) {
-            case 1: launch_wave_t<1>(S, a, b, n); break;
-            case 2: launch_wave_t<2>(S, a, b, n); break;
-            case 4: launch_wave_t<4>(S, a, b, n); break;
-            default: launch_wave_t<8>(S, a, b, n); break;
+            case 1: launch_wave_t<1, DIAG>(S, a, b, n); break;
+            case 2: launch_wave_t<2, DIAG>(S, a, b, n); break;
+            case 4: launch_wave_t<4, DIAG>(S, a, b, n); break;
+            default: launch_wave_t<8, DIAG>(S, a, b, n); break;
         }
-    } else if (S.has_cpml && !S.pml_nodal) {
-        launch_element_p<true>(S, a, b, n);
     } else {
-        launch_element_p<false>(S, a, b, n);
+        launch_element_p<false, DIAG>(S, a, b, n);
+    }
+}
+
+void launch_element(gfx_solver& S, const dev::ElemArgs& a, const ElemRange& rg) {
+    const int b = rg.b, n = rg.e - rg.b;
+    if (S.has_cpml && !S.pml_nodal) {
+        launch_element_p<true, false>(S, a, b, n);  // element-local C-PML: block kernel, general form
+    } else if (rg.diag) {
+        launch_element_d<true>(S, a, b, n);
+    } else {
+        launch_element_d<false>(S, a, b, n);
     }
 }
 
@@ -403,7 +425,7 @@ StepArgs make_step_args(gfx_solver& S) {
 void enqueue_step(gfx_solver& S, const StepArgs& a, bool record, hipEvent_t* ev) {
     if (S.assembly == GFX_ASSEMBLY_ATOMIC)
         dev::k_zero<<<a.zero_grid, 256, 0, S.stream>>>(reinterpret_cast<float4*>(S.R.p), a.r4);
-    for (const auto& rg : S.ranges) launch_element(S, a.ea, rg.first, rg.second - rg.first);
+    for (const auto& rg : S.ranges) launch_element(S, a.ea, rg);
     GFX_HIP(hipGetLastError());
     if (ev) GFX_HIP(hipEventRecord(ev[0], S.stream));
     if (a.spec_grid > 0) launch_node_special(S, a.na, a.spec_grid);
@@ -603,22 +625,41 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
             key[e] = m;
         }
     }
+    // axis-aligned elements: dxi_a/dx_m = 0 for a != m at every point, up to FP64
+    // round-off of the input (|off-diagonal| <= 1e-12 x the largest diagonal
+    // entry); they run the diagonal geometry form of the element kernels
+    std::vector<char> diag_el(E, 0);
+    if (!d.general_geometry) {
+        for (int e = 0; e < E; ++e) {
+            bool ok = true;
+            for (int p = 0; p < NP && ok; ++p) {
+                const double* x = d.dxi_dx + (int64_t(e) * NP + p) * 9;
+                const double dmax = std::max(std::fabs(x[0]), std::max(std::fabs(x[4]), std::fabs(x[8])));
+                for (int c : {1, 2, 3, 5, 6, 7}) ok = ok && std::fabs(x[c]) <= 1e-12 * dmax;
+            }
+            diag_el[e] = ok;
+        }
+    }
     std::vector<int32_t> elem_of_int(E);
     std::iota(elem_of_int.begin(), elem_of_int.end(), 0);
     std::vector<int> col(E, 0);
     if (S.assembly == GFX_ASSEMBLY_COLORED) col = colour_elements(E, d.ibool, G);
     std::stable_sort(elem_of_int.begin(), elem_of_int.end(), [&](int a, int b) {
-        return col[a] != col[b] ? col[a] < col[b] : key[a] < key[b];
+        if (col[a] != col[b]) return col[a] < col[b];
+        if (diag_el[a] != diag_el[b]) return diag_el[a] > diag_el[b];
+        return key[a] < key[b];
     });
-    if (S.assembly == GFX_ASSEMBLY_COLORED) {
+    {
         int b = 0;
-        for (int i = 1; i <= E; ++i)
-            if (i == E || col[elem_of_int[i]] != col[elem_of_int[b]]) {
-                S.ranges.push_back({b, i});
+        for (int i = 1; i <= E; ++i) {
+            const int e0 = elem_of_int[b];
+            if (i == E || col[elem_of_int[i]] != col[e0] || diag_el[elem_of_int[i]] != diag_el[e0]) {
+                S.ranges.push_back({b, i, diag_el[e0] != 0});
                 b = i;
             }
-    } else {
-        S.ranges.push_back({0, E});
+        }
+        S.n_colors = S.assembly == GFX_ASSEMBLY_COLORED ? (*std::max_element(col.begin(), col.end()) + 1) : 1;
+        S.n_diag = std::count(diag_el.begin(), diag_el.end(), char(1));
     }
     S.int_of_elem.assign(E, 0);
     for (int i = 0; i < E; ++i) S.int_of_elem[elem_of_int[i]] = i;
@@ -841,6 +882,11 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
     }
     dv.upload(S.pml_slot, std::move(hslot));
 
+    // element-local C-PML runs the block kernel's general form only
+    if (S.has_cpml && !S.pml_nodal) {
+        for (auto& rg : S.ranges) rg.diag = false;
+        S.n_diag = 0;
+    }
     // element kernel: the wavefront-per-element form needs no element-local C-PML state
     const bool wave_ok = !(S.has_cpml && !S.pml_nodal);
     S.kernel = resolve_kernel(d.element_kernel, wave_ok);
@@ -1405,7 +1451,8 @@ extern "C" gfx_status gfx_solver_query(const gfx_solver* s, const char* key, dou
     return guarded([&] {
         require(s && key && value, "null argument");
         const std::string k(key);
-        if (k == "colors") *value = double(s->ranges.size());
+        if (k == "colors") *value = double(s->n_colors);
+        else if (k == "diag_elements") *value = double(s->n_diag);
         else if (k == "element_kernel") *value = s->kernel;
         else if (k == "elems_per_block") *value = s->epb;
         else if (k == "node_field_groups") *value = s->node_fg;
@@ -1425,9 +1472,11 @@ extern "C" double gfx_solver_algorithmic_bytes(const gfx_solver* s, int which) {
     if (!s) return 0.0;
     const double P = double(s->E) * NP, G = s->G, F = s->nf, Ga = double(s->Ga),
                  Gp = double(s->Gpml), Q = double(s->Ppml);
-    // element stage: geometry (9 scaled dxi_dx + lambda + mu + node id) per point;
-    // per field the predicted displacement read and the residual written once per node.
-    double elem = 48.0 * P + F * 24.0 * G;
+    // element stage: geometry (9 scaled dxi_dx + lambda + mu + node id) per point,
+    // 3 + 3 values at points of axis-aligned elements on the diagonal form; per
+    // field the predicted displacement read and the residual written once per node.
+    const double Pd = double(s->n_diag) * NP;
+    double elem = 48.0 * (P - Pd) + 24.0 * Pd + F * 24.0 * G;
     if (s->has_cpml && !s->pml_nodal) elem += 56.0 * Q + F * (96.0 * Q + 36.0 * Gp);
     // node stage: 1/m per node; per field r, u~, w read and u~, w written;
     // special nodes also read and write a; C-PML nodes read 5 + 9 coefficients

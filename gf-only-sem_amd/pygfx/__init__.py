@@ -91,6 +91,7 @@ class SolverDesc(C.Structure):
         ("element_kernel", C.c_int),
         ("record_capacity", C.c_int), ("record_uva", C.c_int),
         ("node_field_groups", C.c_int),
+        ("general_geometry", C.c_int),
     ]
 
 
@@ -503,7 +504,7 @@ def h5_attr(path, obj, name, string=False):
 def make_desc(mesh: Mesh, keep: list, n_fields=1, assembly=GFX_ASSEMBLY_COLORED, elems_per_block=0,
               record_capacity=None, record_uva=False, has_cpml=True, dt=None,
               snapshot_stride=None, rec_cell=None, element_kernel=GFX_ELEMENT_AUTO, morton=True,
-              node_field_groups=0):
+              node_field_groups=0, general_geometry=False):
     """Solver descriptor over mesh arrays; arrays it points to are appended to keep."""
     a = mesh.arrays
 
@@ -541,6 +542,7 @@ def make_desc(mesh: Mesh, keep: list, n_fields=1, assembly=GFX_ASSEMBLY_COLORED,
     d.record_capacity = record_capacity
     d.record_uva = 1 if record_uva else 0
     d.node_field_groups = node_field_groups
+    d.general_geometry = 1 if general_geometry else 0
     return d
 
 
@@ -562,14 +564,14 @@ class Solver:
     def __init__(self, mesh: Mesh, n_fields=1, assembly=GFX_ASSEMBLY_COLORED, elems_per_block=0,
                  record_capacity=None, record_uva=False, has_cpml=True, dt=None,
                  snapshot_stride=None, rec_cell=None, element_kernel=GFX_ELEMENT_AUTO,
-                 morton=True, node_field_groups=0):
+                 morton=True, node_field_groups=0, general_geometry=False):
         self._keep = []
         d = make_desc(mesh, self._keep, n_fields=n_fields, assembly=assembly,
                       elems_per_block=elems_per_block, record_capacity=record_capacity,
                       record_uva=record_uva, has_cpml=has_cpml, dt=dt,
                       snapshot_stride=snapshot_stride, rec_cell=rec_cell,
                       element_kernel=element_kernel, morton=morton,
-                      node_field_groups=node_field_groups)
+                      node_field_groups=node_field_groups, general_geometry=general_geometry)
         rk_size = d.n_rec_cell
         h = C.c_void_p()
         _check(lib.gfx_solver_create(C.byref(d), C.byref(h)))

@@ -124,6 +124,12 @@ typedef struct {
     /* node kernels: field groups per workgroup (1, 2, 4 or 8; 0 = 1): the
        fields of one node block are spread over that many lane groups */
     int node_field_groups;
+    /* 0: elements whose dxi_dx is diagonal at every point (off-diagonal
+       entries at most 1e-12 of the largest diagonal one, i.e. axis-aligned up
+       to round-off) run the element kernels' diagonal form, which leaves out
+       the products with the zero entries; 1: every element runs the general
+       form */
+    int general_geometry;
 } gfx_solver_desc;
 
 gfx_status gfx_solver_create(const gfx_solver_desc* desc, gfx_solver** out);
