@@ -367,7 +367,8 @@ def test_autotuned_configuration_matches_oracle():
     cells, w, stf = mc.buried_source(mesh)
     pick = pygfx.autotune(mesh, n_fields=2, trial_steps=2)
     assert pick["assembly"] in (pygfx.GFX_ASSEMBLY_COLORED, pygfx.GFX_ASSEMBLY_GATHER)
-    assert pick["element_kernel"] in (pygfx.GFX_ELEMENT_BLOCK, pygfx.GFX_ELEMENT_WAVE)
+    assert pick["element_kernel"] in (pygfx.GFX_ELEMENT_BLOCK, pygfx.GFX_ELEMENT_WAVE, pygfx.GFX_ELEMENT_PAIR,
+                                      pygfx.GFX_ELEMENT_PENCIL)
     assert pick["elems_per_block"] in (1, 2, 4, 8) and pick["ms_per_step"] > 0
     ref = _oracle(mesh, cells, w, stf, 1)
     s = pygfx.Solver(mesh, n_fields=2, assembly=pick["assembly"], element_kernel=pick["element_kernel"],
