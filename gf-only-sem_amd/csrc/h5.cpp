@@ -53,6 +53,8 @@ struct Api {
     hid_t (*pcreate)(hid_t);
     herr_t (*pset_chunk)(hid_t, int, const hsize_t*);
     herr_t (*pset_deflate)(hid_t, unsigned);
+    herr_t (*pset_shuffle)(hid_t);
+    herr_t (*sselect_hyperslab)(hid_t, int, const hsize_t*, const hsize_t*, const hsize_t*, const hsize_t*);
     herr_t (*pclose)(hid_t);
     htri_t (*lexists)(hid_t, const char*, hid_t);
     herr_t (*ldelete)(hid_t, const char*, hid_t);
@@ -125,6 +127,8 @@ Api* load() {
               bind(api.so, "H5Sget_simple_extent_dims", api.sget_dims, w) &&
               bind(api.so, "H5Pcreate", api.pcreate, w) && bind(api.so, "H5Pset_chunk", api.pset_chunk, w) &&
               bind(api.so, "H5Pset_deflate", api.pset_deflate, w) && bind(api.so, "H5Pclose", api.pclose, w) &&
+              bind(api.so, "H5Pset_shuffle", api.pset_shuffle, w) &&
+              bind(api.so, "H5Sselect_hyperslab", api.sselect_hyperslab, w) &&
               bind(api.so, "H5Lexists", api.lexists, w) && bind(api.so, "H5Ldelete", api.ldelete, w) &&
               bind(api.so, "H5Lget_name_by_idx", api.lget_name_by_idx, w) &&
               bind(api.so, "H5Aexists", api.aexists, w) && bind(api.so, "H5Aopen", api.aopen, w) &&
@@ -411,12 +415,27 @@ template void write_vec<int64_t>(hid_t, const std::string&, const std::vector<in
 template void write_vec<uint8_t>(hid_t, const std::string&, const std::vector<uint8_t>&, std::vector<hsize_t>,
                                  const WriteOpts&);
 
+namespace {
+// Number of elements of an attribute (1 for a scalar).
+hsize_t attr_points(Api& a, hid_t at) {
+    Obj sp(a.aget_space(at), Obj::SPACE);
+    if (!sp.valid()) return 0;
+    const int nd = a.sget_ndims(sp.id());
+    if (nd <= 0) return nd == 0 ? 1 : 0;
+    std::vector<hsize_t> d(nd);
+    a.sget_dims(sp.id(), d.data(), nullptr);
+    hsize_t n = 1;
+    for (hsize_t v : d) n *= v;
+    return n;
+}
+}  // namespace
+
 bool read_attr_int(hid_t loc, const std::string& name, int& out) {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
     Api& a = A();
     if (a.aexists(loc, name.c_str()) <= 0) return false;
     Obj at(a.aopen(loc, name.c_str(), P_DEFAULT), Obj::ATTR);
-    if (!at.valid()) return false;
+    if (!at.valid() || attr_points(a, at.id()) != 1) return false;
     return a.aread(at.id(), *a.g_int, &out) >= 0;
 }
 
@@ -425,8 +444,18 @@ bool read_attr_double(hid_t loc, const std::string& name, double& out) {
     Api& a = A();
     if (a.aexists(loc, name.c_str()) <= 0) return false;
     Obj at(a.aopen(loc, name.c_str(), P_DEFAULT), Obj::ATTR);
-    if (!at.valid()) return false;
+    if (!at.valid() || attr_points(a, at.id()) != 1) return false;
     return a.aread(at.id(), *a.g_double, &out) >= 0;
+}
+
+bool read_attr_doubles(hid_t loc, const std::string& name, std::vector<double>& out) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    Api& a = A();
+    if (a.aexists(loc, name.c_str()) <= 0) return false;
+    Obj at(a.aopen(loc, name.c_str(), P_DEFAULT), Obj::ATTR);
+    if (!at.valid()) return false;
+    out.assign(size_t(attr_points(a, at.id())), 0.0);
+    return out.empty() || a.aread(at.id(), *a.g_double, out.data()) >= 0;
 }
 
 bool read_attr_string(hid_t loc, const std::string& name, std::string& out) {
@@ -462,19 +491,69 @@ void write_attr(hid_t loc, const std::string& name, T type, const void* value, b
     if (a.awrite(at.id(), native(type), value) < 0) fail("H5Awrite failed for attr: " + name);
 }
 
-void write_attr_string(hid_t loc, const std::string& name, const std::string& value, bool overwrite) {
+void write_attr_string(hid_t loc, const std::string& name, const std::string& value, bool overwrite,
+                       size_t fixed_size) {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
     Api& a = A();
     if (overwrite && a.aexists(loc, name.c_str()) > 0) a.adelete(loc, name.c_str());
-    // fixed-length C string of the value's size, as record.cpp:37-57 writes
+    // fixed-length C string of the value's size (record.cpp:37-57), or of
+    // fixed_size bytes, zero padded (writer.hh:131-141 uses 64)
+    const size_t size = fixed_size ? fixed_size : (value.empty() ? 1 : value.size());
+    if (value.size() > size) fail("string attr longer than its fixed size: " + name);
     Obj ty(a.tcopy(*a.g_c_s1), Obj::TYPE);
-    if (!ty.valid() || a.tset_size(ty.id(), value.empty() ? 1 : value.size()) < 0)
-        fail("string type failed for attr: " + name);
+    if (!ty.valid() || a.tset_size(ty.id(), size) < 0) fail("string type failed for attr: " + name);
     Obj sp(a.screate(S_SCALAR), Obj::SPACE);
     Obj at(a.acreate2(loc, name.c_str(), ty.id(), sp.id(), P_DEFAULT, P_DEFAULT), Obj::ATTR);
     if (!at.valid()) fail("H5Acreate2 failed for string attr: " + name);
-    std::string v = value.empty() ? std::string(1, '\0') : value;
+    std::string v = value;
+    v.resize(size, '\0');
     if (a.awrite(at.id(), ty.id(), v.c_str()) < 0) fail("H5Awrite failed for string attr: " + name);
+}
+
+void write_attr_array(hid_t loc, const std::string& name, T type, const void* values, hsize_t n, bool overwrite) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    Api& a = A();
+    if (overwrite && a.aexists(loc, name.c_str()) > 0) a.adelete(loc, name.c_str());
+    Obj sp(a.screate_simple(1, &n, nullptr), Obj::SPACE);
+    if (!sp.valid()) fail("H5Screate_simple failed for attr: " + name);
+    Obj at(a.acreate2(loc, name.c_str(), native(type), sp.id(), P_DEFAULT, P_DEFAULT), Obj::ATTR);
+    if (!at.valid()) fail("H5Acreate2 failed for attr: " + name);
+    if (a.awrite(at.id(), native(type), values) < 0) fail("H5Awrite failed for attr: " + name);
+}
+
+Obj create_dataset(hid_t loc, const std::string& path, T file_type, const std::vector<hsize_t>& dims,
+                   const std::vector<hsize_t>& chunk, bool shuffle, int deflate) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    Api& a = A();
+    const size_t slash = path.find_last_of('/');
+    if (slash != std::string::npos && slash > 0) require_group(loc, path.substr(0, slash));
+    Obj space(a.screate_simple(int(dims.size()), dims.data(), nullptr), Obj::SPACE);
+    if (!space.valid()) fail("H5Screate_simple failed for " + path);
+    Obj plist(a.pcreate(*a.g_dcpl), Obj::PLIST);
+    if (!plist.valid()) fail("H5Pcreate failed for " + path);
+    if (!chunk.empty()) {
+        if (chunk.size() != dims.size()) fail("chunk rank differs from dataset rank: " + path);
+        if (shuffle && a.pset_shuffle(plist.id()) < 0) fail("H5Pset_shuffle failed for " + path);
+        if (a.pset_chunk(plist.id(), int(chunk.size()), chunk.data()) < 0) fail("H5Pset_chunk failed for " + path);
+        if (deflate > 0 && a.pset_deflate(plist.id(), unsigned(deflate)) < 0) fail("H5Pset_deflate failed for " + path);
+    }
+    Obj d(a.dcreate2(loc, path.c_str(), native(file_type), space.id(), P_DEFAULT, plist.id(), P_DEFAULT),
+          Obj::DSET);
+    if (!d.valid()) fail("H5Dcreate2 failed for " + path);
+    return d;
+}
+
+void write_slab(hid_t dset, T mem_type, const void* data, const std::vector<hsize_t>& start,
+                const std::vector<hsize_t>& count) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    Api& a = A();
+    Obj fs(a.dget_space(dset), Obj::SPACE);
+    if (!fs.valid()) fail("H5Dget_space failed");
+    if (a.sselect_hyperslab(fs.id(), 0 /* H5S_SELECT_SET */, start.data(), nullptr, count.data(), nullptr) < 0)
+        fail("H5Sselect_hyperslab failed");
+    Obj ms(a.screate_simple(int(count.size()), count.data(), nullptr), Obj::SPACE);
+    if (!ms.valid()) fail("H5Screate_simple failed for a slab");
+    if (a.dwrite(dset, native(mem_type), ms.id(), fs.id(), P_DEFAULT, data) < 0) fail("H5Dwrite of a slab failed");
 }
 
 std::vector<std::string> list(hid_t loc, const std::string& group) {

@@ -86,13 +86,28 @@ template <typename V>
 void write_vec(hid_t loc, const std::string& path, const std::vector<V>& v,
                std::vector<hsize_t> dims = {}, const WriteOpts& o = {});
 
-// Scalar attributes (create, or overwrite when `overwrite`).
+// Scalar attributes (create, or overwrite when `overwrite`); the scalar
+// readers return false for an attribute of more than one element.
 bool read_attr_int(hid_t loc, const std::string& name, int& out);
 bool read_attr_double(hid_t loc, const std::string& name, double& out);
 bool read_attr_string(hid_t loc, const std::string& name, std::string& out);
+// Every element of a numeric attribute (scalar or 1-D array) as FP64.
+bool read_attr_doubles(hid_t loc, const std::string& name, std::vector<double>& out);
 void write_attr(hid_t loc, const std::string& name, T type, const void* value, bool overwrite = false);
+// fixed_size = 0: the value's own length; else that many bytes, zero padded.
 void write_attr_string(hid_t loc, const std::string& name, const std::string& value,
-                       bool overwrite = false);
+                       bool overwrite = false, size_t fixed_size = 0);
+// 1-D array attribute of n values.
+void write_attr_array(hid_t loc, const std::string& name, T type, const void* values, hsize_t n,
+                      bool overwrite = false);
+
+// Dataset created empty (chunked when chunk is given, with optional shuffle
+// and gzip level) for hyperslab writes; parent groups are created.
+Obj create_dataset(hid_t loc, const std::string& path, T file_type, const std::vector<hsize_t>& dims,
+                   const std::vector<hsize_t>& chunk = {}, bool shuffle = false, int deflate = 0);
+// Writes the block [start, start + count) of an open dataset from memory of mem_type.
+void write_slab(hid_t dset, T mem_type, const void* data, const std::vector<hsize_t>& start,
+                const std::vector<hsize_t>& count);
 
 // Names of the links directly under a group, in native order.
 std::vector<std::string> list(hid_t loc, const std::string& group);

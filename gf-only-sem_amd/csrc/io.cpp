@@ -1,5 +1,6 @@
 // io.cpp -- reference file formats (see io.hpp), on the run-time HDF5 binding.
 #include "io.hpp"
+#include "mesh.hpp"
 
 #include <algorithm>
 #include <filesystem>
@@ -470,8 +471,14 @@ void write_inputs(const gfx_mesh& m, const std::string& dir, int n_parts, const 
         h5::write_attr(s, "nz_elements", T::I32, &m.nz);
         h5::write_attr(s, "log_stride", T::I32, &log_stride);
         h5::write_attr(s, "restart_stride", T::I32, &restart_stride);
-        h5::write_vec<int64_t>(s, "tilex_elements", {int64_t(m.nx)});
-        h5::write_vec<int64_t>(s, "tiley_elements", {int64_t(m.ny)});
+        // C-PML thickness and one tile over the interior (config_loader.py:153-186:
+        // the tile sizes sum to the element count between the x / y C-PML layers)
+        const gfx_box_spec& sp = reinterpret_cast<const MeshHolder&>(m).spec;
+        const int pml[6] = {sp.pml_xmin, sp.pml_xmax, sp.pml_ymin, sp.pml_ymax, 0, sp.pml_zmax};
+        const char* pname[6] = {"pml_xmin", "pml_xmax", "pml_ymin", "pml_ymax", "pml_zmin", "pml_zmax"};
+        for (int k = 0; k < 6; ++k) h5::write_attr(s, pname[k], T::I32, &pml[k]);
+        h5::write_vec<int64_t>(s, "tilex_elements", {int64_t(m.nx - sp.pml_xmin - sp.pml_xmax)});
+        h5::write_vec<int64_t>(s, "tiley_elements", {int64_t(m.ny - sp.pml_ymin - sp.pml_ymax)});
     }
     {
         h5::Obj g = h5::require_group(id, "/domain");

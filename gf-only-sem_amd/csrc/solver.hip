@@ -9,6 +9,7 @@
 // Host-side setup (colouring, node renumbering, C-PML assembly, FP64 -> FP32
 // packing) happens once in gfx_solver_create.
 #include <atomic>
+#include <cmath>
 #include <cstring>
 #include <functional>
 #include <memory>
@@ -1682,3 +1683,171 @@ extern "C" gfx_status gfx_green_capture(gfx_green* g, const gfx_solver* s, int s
     });
 }
 
+
+// ---------------------------------------------------------------------------
+// Exact tile slabs: the post-processor's merge (postprocess/cpp/main.cpp:
+// 306-426) and Green's tensor layout (main.cpp:554-575, 607-618, 800-811) on
+// the device, from one snapshot of the solver's record ring.
+// ---------------------------------------------------------------------------
+struct gfx_green_slab {
+    int n_node = 0;
+    int64_t n_slab = 0;
+    bool f64 = false, disp = false;
+    int64_t map_max = -1;
+    gfx::DevBuf<int64_t> off;
+    gfx::DevBuf<int32_t> pt, slab_node;
+    gfx::DevBuf<double> w, inv;
+    gfx::DevBuf<uint8_t> out_g, out_d;
+    hipStream_t stream = nullptr;
+    ~gfx_green_slab() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace gfx {
+namespace dev {
+// One thread per slab entry j (a node of one tile; nodes on tile borders
+// appear once per tile).  The sums run over the node's entries in the order
+// given, each term rounded as the FP64 host loop rounds it (product, then
+// sum; no fused multiply-add), then scale by inv = 1 / sum of weights.
+// Strain: dst[d*6 + c] (force direction d major, main.cpp:556-575);
+// displacement: plain sums of the records scaled by the same inv,
+// dst[c*3 + d] (main.cpp:382-399, 414-423, 581-598).
+template <typename V>
+__global__ void k_green_slab(const float* __restrict__ strain, const float* __restrict__ uva, int64_t nrp, int f0,
+                             int f1, int f2, const int64_t* __restrict__ off, const int32_t* __restrict__ pt,
+                             const double* __restrict__ w, const double* __restrict__ inv,
+                             const int32_t* __restrict__ slab_node, int64_t n_slab, V* __restrict__ out_g,
+                             V* __restrict__ out_d) {
+    // products and sums are rounded one by one, as the host loop rounds them:
+    // no fused multiply-add
+#pragma clang fp contract(off)
+    const int64_t j = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= n_slab) return;
+    const int n = slab_node[j];
+    const int fd[3] = {f0, f1, f2};
+    double g[18], u[9];
+#pragma unroll
+    for (int k = 0; k < 18; ++k) g[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) u[k] = 0.0;
+    for (int64_t e = off[n]; e < off[n + 1]; ++e) {
+        const double we = w[e];
+        const int64_t q = pt[e];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (fd[d] < 0) continue;
+            const float* s6 = strain + (int64_t(fd[d]) * nrp + q) * 6;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) g[d * 6 + c] += double(s6[c]) * we;
+            if (out_d) {
+                const float* u3 = uva + (int64_t(fd[d]) * nrp + q) * 9;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) u[c * 3 + d] += double(u3[c]);
+            }
+        }
+    }
+    const double s = inv[n];
+    V* og = out_g + j * 18;
+#pragma unroll
+    for (int k = 0; k < 18; ++k) og[k] = V(g[k] * s);
+    if (out_d) {
+        V* od = out_d + j * 9;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) od[k] = V(u[k] * s);
+    }
+}
+}  // namespace dev
+}  // namespace gfx
+
+extern "C" gfx_status gfx_green_slab_create(int n_node, const int64_t* off, const int32_t* rec_point,
+                                            const double* weight, int64_t n_slab, const int32_t* slab_node,
+                                            int precision_f64, int with_displacement, gfx_green_slab** out) {
+    return guarded([&] {
+        require(out && off && slab_node && (off[n_node] == 0 || (rec_point && weight)), "null argument");
+        *out = nullptr;
+        require(n_node >= 1 && n_slab >= 1, "empty slab");
+        require(off[0] == 0, "off[0] must be 0");
+        auto T = std::make_unique<gfx_green_slab>();
+        T->n_node = n_node;
+        T->n_slab = n_slab;
+        T->f64 = precision_f64 != 0;
+        T->disp = with_displacement != 0;
+        const int64_t ne = off[n_node];
+        std::vector<int64_t> ho(off, off + n_node + 1);
+        std::vector<double> inv(n_node, 1.0);
+        for (int n = 0; n < n_node; ++n) {
+            require(ho[n + 1] >= ho[n], "off must be non-decreasing");
+            // node_weight_sum in entry order; no scaling when it is not positive (main.cpp:414-416)
+            double sw = 0.0;
+            for (int64_t e = ho[n]; e < ho[n + 1]; ++e) {
+                require(std::isfinite(weight[e]), "weights must be finite");
+                sw += weight[e];
+            }
+            if (sw > 0.0) inv[n] = 1.0 / sw;
+        }
+        std::vector<int32_t> hp(rec_point, rec_point + ne);
+        for (int32_t q : hp) {
+            require(q >= 0, "negative recording point");
+            T->map_max = std::max<int64_t>(T->map_max, q);
+        }
+        std::vector<int32_t> hs(slab_node, slab_node + n_slab);
+        for (int32_t n : hs) require(n >= 0 && n < n_node, "slab node out of range");
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+            (void)hipGetLastError();
+            throw gfx::Error(GFX_ERR_DEVICE, "no HIP device available");
+        }
+        GFX_HIP(hipStreamCreateWithFlags(&T->stream, hipStreamNonBlocking));
+        T->off.upload(ho);
+        T->pt.upload(hp.empty() ? std::vector<int32_t>{0} : hp);
+        T->w.upload(ne ? std::vector<double>(weight, weight + ne) : std::vector<double>{0.0});
+        T->inv.upload(inv);
+        T->slab_node.upload(hs);
+        const size_t vb = T->f64 ? 8 : 4;
+        T->out_g.alloc(size_t(n_slab) * 18 * vb);
+        if (T->disp) T->out_d.alloc(size_t(n_slab) * 9 * vb);
+        *out = T.release();
+    });
+}
+
+extern "C" gfx_status gfx_green_slab_destroy(gfx_green_slab* t) {
+    return guarded([&] { delete t; });
+}
+
+extern "C" gfx_status gfx_green_slab_capture(gfx_green_slab* t, const gfx_solver* s, int snap,
+                                             const int32_t* field_of_dir, void* greens, void* displacement) {
+    return guarded([&] {
+        require(t && s && field_of_dir && greens, "null argument");
+        require(!t->disp || displacement, "the slab was made with displacement: pass its buffer");
+        require(t->disp || !displacement, "the slab was made without displacement");
+        require(s->rec_cap > 0 && s->n_rec > 0, "the solver records nothing");
+        require(!t->disp || s->rec_uva_on, "displacement needs the solver's u, v, a records");
+        require(snap >= 0 && snap < s->n_snap && snap >= s->n_snap - s->rec_cap, "snapshot not held in the record ring");
+        for (int d = 0; d < 3; ++d) require(field_of_dir[d] >= -1 && field_of_dir[d] < s->nf, "bad field index");
+        const int64_t nrp = int64_t(s->n_rec) * NP;
+        require(t->map_max < nrp, "slab map addresses recording points the solver does not have");
+        const int slot = snap % s->rec_cap;
+        const float* strain = s->rec_strain.p + int64_t(slot) * s->nf * nrp * 6;
+        const float* uva = t->disp ? s->rec_uva.p + int64_t(slot) * s->nf * nrp * 9 : nullptr;
+        GFX_HIP(hipStreamSynchronize(s->stream));
+        const int64_t blocks = (t->n_slab + 255) / 256;
+        const size_t vb = t->f64 ? 8 : 4;
+        if (t->f64)
+            dev::k_green_slab<double><<<dim3(unsigned(blocks)), 256, 0, t->stream>>>(
+                strain, uva, nrp, field_of_dir[0], field_of_dir[1], field_of_dir[2], t->off.p, t->pt.p, t->w.p,
+                t->inv.p, t->slab_node.p, t->n_slab, reinterpret_cast<double*>(t->out_g.p),
+                t->disp ? reinterpret_cast<double*>(t->out_d.p) : nullptr);
+        else
+            dev::k_green_slab<float><<<dim3(unsigned(blocks)), 256, 0, t->stream>>>(
+                strain, uva, nrp, field_of_dir[0], field_of_dir[1], field_of_dir[2], t->off.p, t->pt.p, t->w.p,
+                t->inv.p, t->slab_node.p, t->n_slab, reinterpret_cast<float*>(t->out_g.p),
+                t->disp ? reinterpret_cast<float*>(t->out_d.p) : nullptr);
+        GFX_HIP(hipGetLastError());
+        GFX_HIP(hipMemcpyAsync(greens, t->out_g.p, size_t(t->n_slab) * 18 * vb, hipMemcpyDeviceToHost, t->stream));
+        if (t->disp)
+            GFX_HIP(hipMemcpyAsync(displacement, t->out_d.p, size_t(t->n_slab) * 9 * vb, hipMemcpyDeviceToHost,
+                                   t->stream));
+        GFX_HIP(hipStreamSynchronize(t->stream));
+    });
+}

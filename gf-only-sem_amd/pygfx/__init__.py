@@ -104,6 +104,14 @@ class ForwardOptions(C.Structure):
     ]
 
 
+class GreenTileOptions(C.Structure):
+    _fields_ = [
+        ("workdir", C.c_char_p), ("output_dir", C.c_char_p), ("model_path", C.c_char_p),
+        ("with_uva", C.c_int), ("assembly", C.c_int), ("element_kernel", C.c_int),
+        ("elems_per_block", C.c_int), ("ring", C.c_int), ("verbose", C.c_int),
+    ]
+
+
 class MlpConfig(C.Structure):
     _fields_ = [("n_freq", C.c_int), ("width", C.c_int), ("depth", C.c_int), ("n_out", C.c_int),
                 ("sigma", C.c_float), ("seed", C.c_uint64), ("lr", C.c_float), ("beta1", C.c_float),
@@ -193,6 +201,14 @@ def _load():
         "gfx_h5_write_attr": (st, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, vp]),
         "gfx_config_summary": (st, [C.c_char_p, _dp, C.c_int, C.c_char_p, C.c_char_p, C.c_size_t]),
         "gfx_partition_summary": (st, [C.c_char_p, C.c_int, _dp, C.c_int]),
+        "gfx_h5_read_attr_f64_array": (st, [C.c_char_p, C.c_char_p, C.c_char_p, _dp, C.c_size_t,
+                                            C.POINTER(C.c_size_t)]),
+        "gfx_green_tile_options_default": (st, [C.POINTER(GreenTileOptions)]),
+        "gfx_run_green_tiles": (st, [C.POINTER(GreenTileOptions), C.POINTER(C.c_int)]),
+        "gfx_green_slab_create": (st, [C.c_int, C.POINTER(C.c_int64), _ip, _dp, C.c_int64, _ip, C.c_int, C.c_int,
+                                       C.POINTER(vp)]),
+        "gfx_green_slab_destroy": (st, [vp]),
+        "gfx_green_slab_capture": (st, [vp, vp, C.c_int, _ip, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -420,6 +436,24 @@ def run_forward(workdir=None, directions="z", stations=None, resume=False, recor
     _check(lib.gfx_run_forward_ex(C.byref(o)))
 
 
+def run_green_tiles(workdir=None, output_dir=None, model_path=None, with_uva=True, assembly=-1,
+                    element_kernel=GFX_ELEMENT_AUTO, elems_per_block=0, ring=8, verbose=False) -> int:
+    """Green's function tiles straight from the device (gfx_run_green_tiles); returns the tile count."""
+    o = GreenTileOptions()
+    _check(lib.gfx_green_tile_options_default(C.byref(o)))
+    keep = [None if x is None else os.fsencode(x) for x in (workdir, output_dir, model_path)]
+    o.workdir, o.output_dir, o.model_path = keep
+    o.with_uva = 1 if with_uva else 0
+    o.assembly = assembly
+    o.element_kernel = element_kernel
+    o.elems_per_block = elems_per_block
+    o.ring = ring
+    o.verbose = 1 if verbose else 0
+    n = C.c_int(0)
+    _check(lib.gfx_run_green_tiles(C.byref(o), C.byref(n)))
+    return n.value
+
+
 def h5_available() -> bool:
     return bool(lib.gfx_h5_available())
 
@@ -499,6 +533,27 @@ def h5_attr(path, obj, name, string=False):
     v = C.c_double(0)
     _check(lib.gfx_h5_read_attr_f64(os.fsencode(path), obj.encode(), name.encode(), C.byref(v)))
     return v.value
+
+
+def h5_attr_array(path, obj, name) -> np.ndarray:
+    """Every element of a numeric attribute as float64."""
+    n = C.c_size_t(0)
+    _check(lib.gfx_h5_read_attr_f64_array(os.fsencode(path), obj.encode(), name.encode(), _ptr(None), 0,
+                                          C.byref(n)))
+    out = np.zeros(n.value)
+    _check(lib.gfx_h5_read_attr_f64_array(os.fsencode(path), obj.encode(), name.encode(), _ptr(out), out.size,
+                                          C.byref(n)))
+    return out
+
+
+def h5_layout(path) -> dict:
+    """{path: (type, shape)} for every dataset and {path@attr: (type, shape)} for every attribute."""
+    out = {}
+    for line in h5_describe(path).splitlines():
+        name, typ, dims = line.rsplit(" ", 2)
+        shape = tuple(int(v) for v in dims.strip("[]").split(",") if v)
+        out[name] = (typ, shape)
+    return out
 
 
 def make_desc(mesh: Mesh, keep: list, n_fields=1, assembly=GFX_ASSEMBLY_COLORED, elems_per_block=0,
@@ -786,6 +841,43 @@ def green_capture_map(mesh, rec_cell, tile_nodes):
     off = np.zeros(len(tile_nodes) + 1, np.int64)
     np.add.at(off, tn + 1, 1)
     return np.cumsum(off), rp.astype(np.int32), w
+
+
+class GreenSlab:
+    """Exact per-tile node merge of one snapshot on the device (gfx_green_slab_*)."""
+
+    def __init__(self, off, rec_point, weight, slab_node, f64=False, displacement=False):
+        self._keep = [np.ascontiguousarray(off, np.int64), np.ascontiguousarray(rec_point, np.int32),
+                      np.ascontiguousarray(weight, np.float64), np.ascontiguousarray(slab_node, np.int32)]
+        o, r, w, sn = self._keep
+        self.n_slab = len(sn)
+        self.dtype = np.float64 if f64 else np.float32
+        self.displacement = displacement
+        h = C.c_void_p()
+        _check(lib.gfx_green_slab_create(len(o) - 1, o.ctypes.data_as(C.POINTER(C.c_int64)), _ptr(r, C.c_int32),
+                                         _ptr(w), self.n_slab, _ptr(sn, C.c_int32), 1 if f64 else 0,
+                                         1 if displacement else 0, C.byref(h)))
+        self.h = h
+
+    def capture(self, solver, snap, field_of_dir=(0, 1, 2)):
+        g = np.zeros((self.n_slab, 18), self.dtype)
+        d = np.zeros((self.n_slab, 9), self.dtype) if self.displacement else None
+        fd = np.asarray(field_of_dir, np.int32)
+        _check(lib.gfx_green_slab_capture(self.h, solver.h, snap, _ptr(fd, C.c_int32),
+                                          g.ctypes.data_as(C.c_void_p),
+                                          None if d is None else d.ctypes.data_as(C.c_void_p)))
+        return g, d
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.gfx_green_slab_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class MLP:

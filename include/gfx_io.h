@@ -47,6 +47,36 @@ gfx_status gfx_forward_options_default(gfx_forward_options* o);
 /* Batched / configurable forward run; same files as gfx_run_forward. */
 gfx_status gfx_run_forward_ex(const gfx_forward_options* o);
 
+/* Green's function tiles straight from the device: the reference's three
+ * forward runs (x, y, z) plus its post-processor
+ * (postprocess/cpp/main.cpp:446-875, writer.hh:105-310) in one pass.  The
+ * three force directions run as three fields of one solver over config.h5 +
+ * partitions/ in workdir; every snapshot is merged to the recorded GLL nodes
+ * (mass-weighted strain, displacement scaled the same way) and laid out per
+ * tile on the GPU (gfx_green_slab_capture), then appended to
+ * output_dir/tile_x{tx:03d}_y{ty:03d}.h5, whose attributes, datasets, types,
+ * chunking and filters follow writer.hh.  Each tile holds the values the
+ * post-processor computes from the record files a gfx_run_forward over the
+ * same inputs writes.  No record or restart file is written. */
+typedef struct {
+    const char* workdir;    /* config.h5 + partitions/; NULL or "" = current directory */
+    const char* output_dir; /* NULL or "" = workdir/greenfun (the post-processor's default -o) */
+    const char* model_path; /* NULL: bounds from config.h5 /domain, mass from the partitions;
+                               else model.h5 /domain and /field/cell/mass (main.cpp:458-483) */
+    int with_uva;           /* displacement_tensor and the velocity / acceleration tensors, which the
+                               post-processor leaves at zero (main.cpp:401); default 1, as the
+                               forward run records u, v, a by default */
+    int assembly;           /* -1 = autotune (default), else gfx_assembly */
+    int element_kernel;     /* gfx_element_kernel (ignored when autotuning) */
+    int elems_per_block;    /* 0 = default (ignored when autotuning) */
+    int ring;               /* snapshots the device holds between drains (0 = 8) */
+    int verbose;            /* echo log lines (log/forward_greens_0.log) to stdout */
+} gfx_green_tile_options;
+
+gfx_status gfx_green_tile_options_default(gfx_green_tile_options* o);
+/* *n_tiles (may be NULL) receives the number of tile files written. */
+gfx_status gfx_run_green_tiles(const gfx_green_tile_options* o, int* n_tiles);
+
 /* Write the inputs the reference preprocessor would produce for a generated
  * box (preprocess/model_writer.py:304-446, preprocess/config_writer.py:57-143
  * schemas): dir/partitions/partition_{r}.h5 for n_parts contiguous element
@@ -80,6 +110,9 @@ gfx_status gfx_h5_describe(const char* path, char* buf, size_t len, size_t* need
 gfx_status gfx_h5_read_f64(const char* path, const char* dataset, double* out, size_t n, size_t* count);
 /* Read a numeric or string attribute of a group or dataset ("/" = root). */
 gfx_status gfx_h5_read_attr_f64(const char* path, const char* object, const char* name, double* out);
+/* Every element of a numeric attribute; out may be NULL to query *count. */
+gfx_status gfx_h5_read_attr_f64_array(const char* path, const char* object, const char* name, double* out,
+                                      size_t n, size_t* count);
 gfx_status gfx_h5_read_attr_str(const char* path, const char* object, const char* name, char* buf,
                                 size_t len);
 

@@ -36,6 +36,34 @@ gfx_status gfx_green_set_capture_map(gfx_green* g, const int64_t* off, const int
 gfx_status gfx_green_capture(gfx_green* g, const gfx_solver* s, int snap, const int32_t* field_of_dir,
                              int time_index);
 
+/* ---- Exact tile slabs ----------------------------------------------------
+ * The post-processor's node merge and tensor layout for one snapshot, on the
+ * device (postprocess/cpp/main.cpp:306-426 merge; 554-575, 581-598 layout;
+ * 800-811 per-tile copy).  n_node merged nodes; node n is formed from the
+ * entries [off[n], off[n+1]) of (rec_point, weight), rec_point = position of
+ * the cell in the solver's recording-cell list * 125 + local point, in the
+ * order the post-processor visits them (record cell, then point):
+ *   strain  = (sum_e weight_e * strain_e) * (1 / sum_e weight_e)
+ *   displ.  = (sum_e displacement_e)      * (1 / sum_e weight_e)
+ * (no scaling where the weight sum is not positive), each product and sum
+ * rounded in FP64 as the host loop rounds it, then stored as FP32 or FP64.
+ * The slab lists n_slab node indices slab_node[] (the tiles' local node
+ * lists, concatenated); a capture writes greens[n_slab*18] with the 18 values
+ * of an entry force-direction major (index d*6 + c, the byte order of the
+ * reference's /field/greens_tensor [nt, n, 6, 3]) and, when made with
+ * displacement, displacement[n_slab*9] at index c*3 + d. */
+typedef struct gfx_green_slab gfx_green_slab;
+gfx_status gfx_green_slab_create(int n_node, const int64_t* off, const int32_t* rec_point, const double* weight,
+                                 int64_t n_slab, const int32_t* slab_node, int precision_f64,
+                                 int with_displacement, gfx_green_slab** out);
+gfx_status gfx_green_slab_destroy(gfx_green_slab* t);
+/* Snapshot `snap` of solver s (still in its record ring; field_of_dir[d] =
+ * the solver field driven in direction d, or -1 for zeros) into the host
+ * buffers greens (float or double per precision_f64) and displacement (NULL
+ * unless made with_displacement, which needs a solver recording u, v, a). */
+gfx_status gfx_green_slab_capture(gfx_green_slab* t, const gfx_solver* s, int snap, const int32_t* field_of_dir,
+                                  void* greens, void* displacement);
+
 /* Judge: GLLInterpolator.interpolate (greenfun/gll_interpolator.py:117-227)
  * on the device for n points xyz[n*3] at tile time indices tix[n]; out[n*18]
  * (NaN outside all cells); cell[n] receives the cell used (-1 outside), may
