@@ -17,11 +17,16 @@
 // lexsort in preprocess/partition.py:192-254 yields some other permutation;
 // solver results do not depend on it).
 #include <cstring>
+#include <functional>
+#include <limits>
 #include <memory>
+#include <numeric>
 #include <set>
 
 #include "common.hpp"
+#include "gfx_io.h"
 #include "mesh.hpp"
+#include "topology.hpp"
 
 namespace gfx {
 namespace {
@@ -161,9 +166,16 @@ void abar(int region, const double K[3], const double d[3], const double al[3], 
     }
 }
 
-void validate(const gfx_box_spec& s) {
-    require(s.nx >= 1 && s.ny >= 1 && s.nz >= 1, "element counts must be >= 1");
-    require(s.lx > 0 && s.ly > 0 && s.lz > 0, "box lengths must be > 0");
+// General cells: corner coordinates per cell in GMSH order and the domain
+// bounds (the vertex extent, preprocess/cli.py:409-417).
+struct GeneralCells {
+    std::vector<double> cv;  // [E][8][3]
+    double bnd[6];
+};
+
+void validate(const gfx_box_spec& s, bool general) {
+    require(s.nx >= 1 && s.ny >= 1 && (general || s.nz >= 1), "element counts must be >= 1");
+    require(general || (s.lx > 0 && s.ly > 0 && s.lz > 0), "box lengths must be > 0");
     require(s.order >= 1 && s.order <= 7, "polynomial order must be in [1, 7]");
     require(s.n_layers >= 1 && s.n_layers <= GFX_MAX_LAYERS, "n_layers must be in [1, 8]");
     for (int l = 0; l < s.n_layers; ++l)
@@ -172,105 +184,206 @@ void validate(const gfx_box_spec& s) {
                 s.pml_zmax >= 0,
             "PML thickness must be >= 0");
     require(s.pml_xmin + s.pml_xmax <= s.nx && s.pml_ymin + s.pml_ymax <= s.ny &&
-                s.pml_zmax <= s.nz,
+                (general || s.pml_zmax <= s.nz),
             "PML thicker than the box");
     require(s.cfl_safety > 0 && s.output_dt > 0 && s.duration > 0,
             "cfl_safety, output_dt and duration must be > 0");
-    require(s.jitter >= 0 && s.jitter < 0.25, "jitter must be in [0, 0.25)");
-    const double n_nodes = double(s.nx * s.order + 1) * (s.ny * s.order + 1) * (s.nz * s.order + 1);
-    require(n_nodes < 2.0e9, "too many global nodes for int32 ids");
+    require(general || (s.jitter >= 0 && s.jitter < 0.25), "jitter must be in [0, 0.25)");
+    if (!general) {
+        const double n_nodes = double(s.nx * s.order + 1) * (s.ny * s.order + 1) * (s.nz * s.order + 1);
+        require(n_nodes < 2.0e9, "too many global nodes for int32 ids");
+    }
 }
 
-void build(MeshHolder& H) {
+// Node numbering of general cells (preprocess/partition.py:192-254): every
+// element point's coordinates quantised to 1e-12 of the domain extent, points
+// sorted by (x, y, z) of the quantised values, and a new node wherever any of
+// them changes.  ibool[p] receives the node; node_xyz the coordinates of the
+// node's first point in that order.
+int64_t number_by_coordinates(const std::vector<double>& xyz, std::vector<int32_t>& ibool,
+                              std::vector<double>& node_xyz) {
+    const int64_t P = int64_t(xyz.size() / 3);
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int64_t p = 0; p < P; ++p)
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = std::min(lo[c], xyz[size_t(p) * 3 + c]);
+            hi[c] = std::max(hi[c], xyz[size_t(p) * 3 + c]);
+        }
+    double extent = std::numeric_limits<double>::epsilon();
+    for (int c = 0; c < 3; ++c) extent = std::max(extent, hi[c] - lo[c]);
+    const double scale = 1.0 / (1e-12 * extent);
+    std::vector<int64_t> q(size_t(P) * 3);
+    for (int64_t p = 0; p < P * 3; ++p) q[size_t(p)] = int64_t(std::nearbyint(xyz[size_t(p)] * scale));
+    std::vector<int64_t> order(static_cast<size_t>(P));
+    std::iota(order.begin(), order.end(), int64_t(0));
+    // lexsort is stable: equal keys keep element-point order
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+        for (int c = 0; c < 3; ++c)
+            if (q[size_t(a) * 3 + c] != q[size_t(b) * 3 + c]) return q[size_t(a) * 3 + c] < q[size_t(b) * 3 + c];
+        return false;
+    });
+    ibool.assign(size_t(P), 0);
+    node_xyz.clear();
+    int64_t id = -1;
+    for (int64_t r = 0; r < P; ++r) {
+        const int64_t p = order[size_t(r)];
+        bool fresh = r == 0;
+        if (!fresh) {
+            const int64_t pp = order[size_t(r - 1)];
+            for (int c = 0; c < 3; ++c) fresh = fresh || q[size_t(p) * 3 + c] != q[size_t(pp) * 3 + c];
+        }
+        if (fresh) {
+            ++id;
+            node_xyz.insert(node_xyz.end(), &xyz[size_t(p) * 3], &xyz[size_t(p) * 3] + 3);
+        }
+        ibool[size_t(p)] = int32_t(id);
+    }
+    return id + 1;
+}
+
+void build(MeshHolder& H, const GeneralCells* gen = nullptr) {
     const gfx_box_spec& s = H.spec;
-    validate(s);
+    const bool general = gen != nullptr;
+    validate(s, general);
     const GLL gll(s.order);
     const int N = s.order, ng = N + 1, nn = ng * ng * ng;
-    const int nx = s.nx, ny = s.ny, nz = s.nz;
-    const int64_t E = int64_t(nx) * ny * nz;
+    const int nx = s.nx, ny = s.ny;
+    int64_t E = 0;
+    int nz = s.nz;
+    if (general) {
+        E = int64_t(gen->cv.size() / 24);
+        require(E >= 1, "no cells");
+        // the index-based C-PML flags need nx * ny * nz = n_cell (preprocess/cli.py:241-247)
+        require(E % (int64_t(nx) * ny) == 0,
+                "n_cell is not a multiple of nx_elements * ny_elements (x-fastest structured cell order "
+                "is required for the C-PML flags)");
+        nz = int(E / (int64_t(nx) * ny));
+        require(s.pml_zmax <= nz, "PML thicker than the box");
+    } else {
+        E = int64_t(nx) * ny * nz;
+    }
     require(E * nn < (int64_t(1) << 31), "too many element points for int32 indexing");
     const int64_t P = E * nn;
     const int NX = N * nx + 1, NY = N * ny + 1, NZ = N * nz + 1;
-    const int64_t G = int64_t(NX) * NY * NZ;
-    const double hx = s.lx / nx, hy = s.ly / ny, hz = s.lz / nz;
+    const double bnd[6] = {general ? gen->bnd[0] : 0.0, general ? gen->bnd[1] : s.lx,
+                           general ? gen->bnd[2] : 0.0, general ? gen->bnd[3] : s.ly,
+                           general ? gen->bnd[4] : 0.0, general ? gen->bnd[5] : s.lz};
+    // element sizes from the bounds (preprocess/cli.py:464-469)
+    const double hx = (bnd[1] - bnd[0]) / nx, hy = (bnd[3] - bnd[2]) / ny, hz = (bnd[5] - bnd[4]) / nz;
 
     gfx_mesh& m = H.m;
     m.n_elem = int(E);
     m.ngll = ng;
-    m.n_node_glob = int(G);
     m.nx = nx;
     m.ny = ny;
     m.nz = nz;
-    m.xmin = 0;
-    m.xmax = s.lx;
-    m.ymin = 0;
-    m.ymax = s.ly;
-    m.zmin = 0;
-    m.zmax = s.lz;
+    m.xmin = bnd[0];
+    m.xmax = bnd[1];
+    m.ymin = bnd[2];
+    m.ymax = bnd[3];
+    m.zmin = bnd[4];
+    m.zmax = bnd[5];
 
-    // vertex lattice, interior vertices optionally jittered
-    const int VX = nx + 1, VY = ny + 1, VZ = nz + 1;
-    std::vector<double> vert(size_t(VX) * VY * VZ * 3);
-    {
-        Rng rng(s.jitter_seed ? s.jitter_seed : 7);
-        for (int vz = 0; vz < VZ; ++vz)
-            for (int vy = 0; vy < VY; ++vy)
-                for (int vx = 0; vx < VX; ++vx) {
-                    double* p = &vert[((size_t(vz) * VY + vy) * VX + vx) * 3];
-                    p[0] = (vx == nx) ? s.lx : vx * hx;
-                    p[1] = (vy == ny) ? s.ly : vy * hy;
-                    p[2] = (vz == nz) ? s.lz : vz * hz;
-                    const bool interior =
-                        vx > 0 && vx < nx && vy > 0 && vy < ny && vz > 0 && vz < nz;
-                    if (s.jitter > 0 && interior) {
-                        p[0] += (2 * rng.uniform() - 1) * s.jitter * hx;
-                        p[1] += (2 * rng.uniform() - 1) * s.jitter * hy;
-                        p[2] += (2 * rng.uniform() - 1) * s.jitter * hz;
+    // per element corners; global nodes and connectivity
+    std::vector<double> vert;
+    std::function<void(int64_t, double (*)[3])> corners;
+    int64_t G = 0;
+    if (general) {
+        corners = [&](int64_t e, double cv[8][3]) {
+            for (int a = 0; a < 8; ++a)
+                for (int c = 0; c < 3; ++c) cv[a][c] = gen->cv[(size_t(e) * 8 + a) * 3 + c];
+        };
+        // every element point's coordinates (preprocess/gll_geometry.py:170-175), then the
+        // reference's coordinate-sort numbering
+        std::vector<double> pxyz(size_t(P) * 3);
+        parallel_for(E, [&](int64_t e) {
+            double cv[8][3];
+            corners(e, cv);
+            for (int i = 0; i < ng; ++i)
+                for (int j = 0; j < ng; ++j)
+                    for (int k = 0; k < ng; ++k) {
+                        double S[8], dS[8][3];
+                        shape(gll.x[i], gll.x[j], gll.x[k], S, dS);
+                        double* X = &pxyz[(size_t(e) * nn + (i * ng + j) * ng + k) * 3];
+                        for (int c = 0; c < 3; ++c) {
+                            double v = 0;
+                            for (int a = 0; a < 8; ++a) v += S[a] * cv[a][c];
+                            X[c] = v;
+                        }
                     }
-                }
-    }
-    auto corners = [&](int ex, int ey, int ez, double cv[8][3]) {
-        for (int a = 0; a < 8; ++a) {
-            const int vx = ex + (kCorner[a][0] > 0), vy = ey + (kCorner[a][1] > 0),
-                      vz = ez + (kCorner[a][2] > 0);
-            const double* p = &vert[((size_t(vz) * VY + vy) * VX + vx) * 3];
-            cv[a][0] = p[0];
-            cv[a][1] = p[1];
-            cv[a][2] = p[2];
+        });
+        G = number_by_coordinates(pxyz, H.ibool, H.node_xyz);
+        require(G < (int64_t(1) << 31) - 8, "too many global nodes for int32 ids");
+    } else {
+        // vertex lattice, interior vertices optionally jittered
+        const int VX = nx + 1, VY = ny + 1, VZ = nz + 1;
+        G = int64_t(NX) * NY * NZ;
+        vert.resize(size_t(VX) * VY * VZ * 3);
+        {
+            Rng rng(s.jitter_seed ? s.jitter_seed : 7);
+            for (int vz = 0; vz < VZ; ++vz)
+                for (int vy = 0; vy < VY; ++vy)
+                    for (int vx = 0; vx < VX; ++vx) {
+                        double* p = &vert[((size_t(vz) * VY + vy) * VX + vx) * 3];
+                        p[0] = (vx == nx) ? s.lx : vx * hx;
+                        p[1] = (vy == ny) ? s.ly : vy * hy;
+                        p[2] = (vz == nz) ? s.lz : vz * hz;
+                        const bool interior =
+                            vx > 0 && vx < nx && vy > 0 && vy < ny && vz > 0 && vz < nz;
+                        if (s.jitter > 0 && interior) {
+                            p[0] += (2 * rng.uniform() - 1) * s.jitter * hx;
+                            p[1] += (2 * rng.uniform() - 1) * s.jitter * hy;
+                            p[2] += (2 * rng.uniform() - 1) * s.jitter * hz;
+                        }
+                    }
         }
-    };
-
-    // global node coordinates from one canonical element per node
-    H.node_xyz.resize(G * 3);
-    parallel_for(int64_t(NZ) * NY, [&](int64_t row) {
-        const int gz = int(row / NY), gy = int(row % NY);
-        const int ez = std::min(gz / N, nz - 1), k = gz - ez * N;
-        const int ey = std::min(gy / N, ny - 1), j = gy - ey * N;
-        for (int gx = 0; gx < NX; ++gx) {
-            const int ex = std::min(gx / N, nx - 1), i = gx - ex * N;
-            double cv[8][3], S[8], dS[8][3];
-            corners(ex, ey, ez, cv);
-            shape(gll.x[i], gll.x[j], gll.x[k], S, dS);
-            double* X = &H.node_xyz[((size_t(gz) * NY + gy) * NX + gx) * 3];
-            for (int c = 0; c < 3; ++c) {
-                double v = 0;
-                for (int a = 0; a < 8; ++a) v += S[a] * cv[a][c];
-                X[c] = v;
+        auto lattice_corners = [&vert, VX, VY](int ex, int ey, int ez, double cv[8][3]) {
+            for (int a = 0; a < 8; ++a) {
+                const int vx = ex + (kCorner[a][0] > 0), vy = ey + (kCorner[a][1] > 0),
+                          vz = ez + (kCorner[a][2] > 0);
+                const double* p = &vert[((size_t(vz) * VY + vy) * VX + vx) * 3];
+                cv[a][0] = p[0];
+                cv[a][1] = p[1];
+                cv[a][2] = p[2];
             }
-        }
-    });
+        };
+        corners = [lattice_corners, nx, ny](int64_t e, double cv[8][3]) {
+            lattice_corners(int(e % nx), int((e / nx) % ny), int(e / (int64_t(nx) * ny)), cv);
+        };
 
-    // connectivity (element x fastest, as preprocess/cli.py:252-254 assumes)
-    H.ibool.resize(P);
-    parallel_for(E, [&](int64_t e) {
-        const int ex = int(e % nx), ey = int((e / nx) % ny), ez = int(e / (int64_t(nx) * ny));
-        for (int i = 0; i < ng; ++i)
-            for (int j = 0; j < ng; ++j)
-                for (int k = 0; k < ng; ++k) {
-                    const int64_t gx = ex * N + i, gy = ey * N + j, gz = ez * N + k;
-                    H.ibool[e * nn + (i * ng + j) * ng + k] = int32_t((gz * NY + gy) * NX + gx);
+        // global node coordinates from one canonical element per node
+        H.node_xyz.resize(G * 3);
+        parallel_for(int64_t(NZ) * NY, [&](int64_t row) {
+            const int gz = int(row / NY), gy = int(row % NY);
+            const int ez = std::min(gz / N, nz - 1), k = gz - ez * N;
+            const int ey = std::min(gy / N, ny - 1), j = gy - ey * N;
+            for (int gx = 0; gx < NX; ++gx) {
+                const int ex = std::min(gx / N, nx - 1), i = gx - ex * N;
+                double cv[8][3], S[8], dS[8][3];
+                lattice_corners(ex, ey, ez, cv);
+                shape(gll.x[i], gll.x[j], gll.x[k], S, dS);
+                double* X = &H.node_xyz[((size_t(gz) * NY + gy) * NX + gx) * 3];
+                for (int c = 0; c < 3; ++c) {
+                    double v = 0;
+                    for (int a = 0; a < 8; ++a) v += S[a] * cv[a][c];
+                    X[c] = v;
                 }
-    });
+            }
+        });
+
+        // connectivity (element x fastest, as preprocess/cli.py:252-254 assumes)
+        H.ibool.resize(P);
+        parallel_for(E, [&](int64_t e) {
+            const int ex = int(e % nx), ey = int((e / nx) % ny), ez = int(e / (int64_t(nx) * ny));
+            for (int i = 0; i < ng; ++i)
+                for (int j = 0; j < ng; ++j)
+                    for (int k = 0; k < ng; ++k) {
+                        const int64_t gx = ex * N + i, gy = ey * N + j, gz = ez * N + k;
+                        H.ibool[e * nn + (i * ng + j) * ng + k] = int32_t((gz * NY + gy) * NX + gx);
+                    }
+        });
+    }
+    m.n_node_glob = int(G);
 
     // material at global nodes
     std::vector<double> nvp(G), nvs(G), nrho(G);
@@ -294,7 +407,7 @@ void build(MeshHolder& H) {
         const double amp_norm = (K > 0) ? std::sqrt(2.0 / K) : 0.0;
         // Regular boxes: x depends on gx only (likewise y, z), so
         // cos(kx x + ky y + kz z + ph) = Re(e^{i kx x} * e^{i(ky y + kz z + ph)}).
-        const bool separable = (K > 0) && s.jitter == 0.0;
+        const bool separable = (K > 0) && !general && s.jitter == 0.0;
         std::vector<double> cxr, cxi;
         if (separable) {
             cxr.resize(size_t(K) * NX);
@@ -306,9 +419,12 @@ void build(MeshHolder& H) {
                     cxi[size_t(q) * NX + gx] = std::sin(a);
                 }
         }
-        parallel_for(int64_t(NZ) * NY, [&](int64_t row) {
-            const int64_t g0 = row * NX;
-            std::vector<double> f(NX, 0.0);
+        // rows of the node lattice, or runs of 256 nodes of a general mesh
+        const int64_t RW = general ? 256 : NX;
+        parallel_for((G + RW - 1) / RW, [&](int64_t row) {
+            const int64_t g0 = row * RW;
+            const int nrow = int(std::min<int64_t>(RW, G - g0));
+            std::vector<double> f(nrow, 0.0);
             if (separable) {
                 const double y = H.node_xyz[g0 * 3 + 1], z = H.node_xyz[g0 * 3 + 2];
                 for (int q = 0; q < K; ++q) {
@@ -319,7 +435,7 @@ void build(MeshHolder& H) {
                     for (int gx = 0; gx < NX; ++gx) f[gx] += wr * xr[gx] - wi * xi[gx];
                 }
             } else if (K > 0) {
-                for (int gx = 0; gx < NX; ++gx) {
+                for (int gx = 0; gx < nrow; ++gx) {
                     const double* X = &H.node_xyz[(g0 + gx) * 3];
                     double acc = 0;
                     for (int q = 0; q < K; ++q)
@@ -327,7 +443,7 @@ void build(MeshHolder& H) {
                     f[gx] = acc;
                 }
             }
-            for (int gx = 0; gx < NX; ++gx) {
+            for (int gx = 0; gx < nrow; ++gx) {
                 const int64_t g = g0 + gx;
                 const double* X = &H.node_xyz[g * 3];
                 int l = 0;  // layer l holds layer_zbot[l-1] <= z < layer_zbot[l]; last is halfspace
@@ -358,9 +474,8 @@ void build(MeshHolder& H) {
     }
     std::vector<double> hmin_e(E, 1e300), vpmax_e(E, 0.0);
     parallel_for(E, [&](int64_t e) {
-        const int ex = int(e % nx), ey = int((e / nx) % ny), ez = int(e / (int64_t(nx) * ny));
         double cv[8][3];
-        corners(ex, ey, ez, cv);
+        corners(e, cv);
         double hmin = 1e300, vpm = 0;
         for (int i = 0; i < ng; ++i)
             for (int j = 0; j < ng; ++j)
@@ -448,7 +563,6 @@ void build(MeshHolder& H) {
     }
     const double wid[6] = {s.pml_xmin * hx, s.pml_xmax * hx, s.pml_ymin * hy,
                            s.pml_ymax * hy, 0.0,             s.pml_zmax * hz};
-    const double bnd[6] = {0.0, s.lx, 0.0, s.ly, 0.0, s.lz};
     H.pml_region.assign(E, 0);
     H.coef_alpha.resize(P * 9);
     H.coef_abar.assign(P * 5, 0.0);
@@ -679,6 +793,84 @@ extern "C" gfx_status gfx_mesh_build_box(const gfx_box_spec* spec, gfx_mesh** ou
         H->spec = *spec;
         build(*H);
         *out = &H.release()->m;
+    });
+}
+
+namespace {
+// A mesh over a topology: corners of every cell in GMSH order
+// (preprocess/gll_geometry.py:70-129), bounds from the vertices
+// (preprocess/cli.py:409-417), then the same pipeline as a box.
+gfx_mesh* build_from_topology(const topo::Topology& T, const gfx_box_spec& spec) {
+    GeneralCells gc;
+    gc.cv.resize(size_t(T.n_cell) * 24);
+    for (int64_t c = 0; c < T.n_cell; ++c) {
+        const auto v = topo::cell_vertices(T, c);
+        for (int a = 0; a < 8; ++a) {
+            require(v[a] >= 1 && v[a] <= T.n_vertex, "vertex id out of range");
+            for (int q = 0; q < 3; ++q) gc.cv[(size_t(c) * 8 + a) * 3 + q] = T.vertex_to_coord[size_t(v[a] - 1) * 3 + q];
+        }
+    }
+    for (int q = 0; q < 3; ++q) {
+        gc.bnd[2 * q] = 1e300;
+        gc.bnd[2 * q + 1] = -1e300;
+    }
+    for (int64_t v = 0; v < T.n_vertex; ++v)
+        for (int q = 0; q < 3; ++q) {
+            gc.bnd[2 * q] = std::min(gc.bnd[2 * q], T.vertex_to_coord[size_t(v) * 3 + q]);
+            gc.bnd[2 * q + 1] = std::max(gc.bnd[2 * q + 1], T.vertex_to_coord[size_t(v) * 3 + q]);
+        }
+    auto H = std::make_unique<MeshHolder>();
+    H->spec = spec;
+    build(*H, &gc);
+    return &H.release()->m;
+}
+}  // namespace
+
+extern "C" gfx_status gfx_gmsh_to_topology(const char* msh_path, const char* model_path, const char* aux_path) {
+    return guarded([&] {
+        require(msh_path && model_path, "null argument");
+        const topo::Topology T = topo::extract(topo::read_gmsh(msh_path));
+        topo::write_topology(model_path, T);
+        if (aux_path && aux_path[0]) topo::write_auxiliary(aux_path, T);
+    });
+}
+
+extern "C" gfx_status gfx_mesh_build_topology(const char* model_path, const gfx_box_spec* spec, gfx_mesh** out) {
+    return guarded([&] {
+        require(model_path && spec && out, "null argument");
+        *out = nullptr;
+        *out = build_from_topology(topo::read_topology(model_path), *spec);
+    });
+}
+
+extern "C" gfx_status gfx_mesh_build_hex(int64_t n_point, const double* xyz, int64_t n_cell, const int64_t* hex,
+                                         const gfx_box_spec* spec, gfx_mesh** out) {
+    return guarded([&] {
+        require(xyz && hex && spec && out && n_point > 0 && n_cell > 0, "null or empty argument");
+        *out = nullptr;
+        topo::HexMesh M;
+        M.xyz.assign(xyz, xyz + n_point * 3);
+        M.hex.assign(hex, hex + n_cell * 8);
+        *out = build_from_topology(topo::extract(M), *spec);
+    });
+}
+
+extern "C" gfx_status gfx_topology_summary(const char* model_path, int64_t* counts, int64_t* cell_corners,
+                                           int64_t n_cell_corners) {
+    return guarded([&] {
+        require(model_path && counts, "null argument");
+        const topo::Topology T = topo::read_topology(model_path);
+        counts[0] = T.n_vertex;
+        counts[1] = T.n_edge;
+        counts[2] = T.n_surface;
+        counts[3] = T.n_cell;
+        if (cell_corners) {
+            require(n_cell_corners >= T.n_cell, "cell_corners holds fewer cells than the topology");
+            for (int64_t c = 0; c < T.n_cell; ++c) {
+                const auto v = topo::cell_vertices(T, c);
+                for (int a = 0; a < 8; ++a) cell_corners[c * 8 + a] = v[a];
+            }
+        }
     });
 }
 

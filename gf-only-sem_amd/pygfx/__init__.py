@@ -209,6 +209,11 @@ def _load():
                                        C.POINTER(vp)]),
         "gfx_green_slab_destroy": (st, [vp]),
         "gfx_green_slab_capture": (st, [vp, vp, C.c_int, _ip, vp, vp]),
+        "gfx_gmsh_to_topology": (st, [C.c_char_p, C.c_char_p, C.c_char_p]),
+        "gfx_mesh_build_topology": (st, [C.c_char_p, C.POINTER(BoxSpec), C.POINTER(C.POINTER(_Mesh))]),
+        "gfx_mesh_build_hex": (st, [C.c_int64, _dp, C.c_int64, C.POINTER(C.c_int64), C.POINTER(BoxSpec),
+                                    C.POINTER(C.POINTER(_Mesh))]),
+        "gfx_topology_summary": (st, [C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.c_int64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -338,6 +343,46 @@ def build_box(spec: BoxSpec | None = None, keep_handle=True, copy=True, **kw) ->
     h = C.POINTER(_Mesh)()
     _check(lib.gfx_mesh_build_box(C.byref(spec), C.byref(h)))
     return _wrap_mesh(h, keep_handle, copy)
+
+
+def gmsh_to_topology(msh_path, model_path, aux_path=None):
+    """tools/gmsh_to_hdf5.py: an ASCII GMSH hexahedral mesh to model.h5 /topology."""
+    _check(lib.gfx_gmsh_to_topology(os.fsencode(msh_path), os.fsencode(model_path),
+                                    None if aux_path is None else os.fsencode(aux_path)))
+
+
+def build_topology(model_path, spec: BoxSpec | None = None, keep_handle=True, copy=True, **kw) -> Mesh:
+    """The preprocessor's pipeline on model.h5 /topology (general hexahedra in
+    x-fastest structured order; spec nx, ny give the C-PML flag grid)."""
+    if spec is None:
+        spec = make_spec(**kw)
+    h = C.POINTER(_Mesh)()
+    _check(lib.gfx_mesh_build_topology(os.fsencode(model_path), C.byref(spec), C.byref(h)))
+    return _wrap_mesh(h, keep_handle, copy)
+
+
+def build_hex(xyz, hexes, spec: BoxSpec | None = None, keep_handle=True, copy=True, **kw) -> Mesh:
+    """The same from points [n, 3] and hexahedra [n_cell, 8] (0-based, GMSH corner order)."""
+    if spec is None:
+        spec = make_spec(**kw)
+    xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+    hexes = np.ascontiguousarray(hexes, np.int64).reshape(-1, 8)
+    h = C.POINTER(_Mesh)()
+    _check(lib.gfx_mesh_build_hex(len(xyz), _ptr(xyz), len(hexes), hexes.ctypes.data_as(C.POINTER(C.c_int64)),
+                                  C.byref(spec), C.byref(h)))
+    return _wrap_mesh(h, keep_handle, copy)
+
+
+def topology_summary(model_path):
+    """(n_vertex, n_edge, n_surface, n_cell) of model.h5 /topology and each cell's
+    1-based corner vertex ids [n_cell, 8] in GMSH order."""
+    counts = np.zeros(4, np.int64)
+    _check(lib.gfx_topology_summary(os.fsencode(model_path), counts.ctypes.data_as(C.POINTER(C.c_int64)),
+                                    None, 0))
+    corners = np.zeros((int(counts[3]), 8), np.int64)
+    _check(lib.gfx_topology_summary(os.fsencode(model_path), counts.ctypes.data_as(C.POINTER(C.c_int64)),
+                                    corners.ctypes.data_as(C.POINTER(C.c_int64)), len(corners)))
+    return tuple(int(c) for c in counts), corners
 
 
 def read_inputs(directory, keep_handle=True, copy=True) -> Mesh:

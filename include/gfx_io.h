@@ -77,6 +77,35 @@ gfx_status gfx_green_tile_options_default(gfx_green_tile_options* o);
 /* *n_tiles (may be NULL) receives the number of tile files written. */
 gfx_status gfx_run_green_tiles(const gfx_green_tile_options* o, int* n_tiles);
 
+/* ---- General hexahedral meshes ------------------------------------------
+ * Drop-in for tools/gmsh_to_hdf5.py (`gmsh_to_hdf5.py input.msh -o model.h5
+ * [--aux aux.h5]`): reads the 8-node hexahedra of an ASCII GMSH file (format
+ * 4.1 or 2.2; points in file order, as meshio keeps them) and writes
+ * model.h5 /topology (vertex_to_coord, edge_to_vertex, surface_to_edge,
+ * cell_to_surface with signed 1-based ids; gmsh_to_hdf5.py:45-170) and,
+ * when aux_path is given, the CSR adjacency file (gmsh_to_hdf5.py:190-268). */
+gfx_status gfx_gmsh_to_topology(const char* msh_path, const char* model_path, const char* aux_path);
+/* Inputs for a general hexahedral mesh: the reference preprocessor's pipeline
+ * (preprocess/cli.py:396-619) on model.h5 /topology, with the spec's material
+ * layers and random field, C-PML thicknesses (in elements), f0, CFL safety,
+ * output dt, duration and recording depth.  Cell corners come from the
+ * topology (gll_geometry.py:70-129), the domain bounds from the vertices, the
+ * node numbering from the quantised coordinate sort of partition.py:192-254,
+ * and the C-PML flags from the cell index with nx = spec->nx, ny = spec->ny and
+ * nz = n_cell / (nx ny) (cli.py:241-264: cells in x-fastest structured order);
+ * the spec's box lengths, nz and jitter are not used.  Write the result with
+ * gfx_mesh_write_inputs. */
+gfx_status gfx_mesh_build_topology(const char* model_path, const gfx_box_spec* spec, gfx_mesh** out);
+/* The same from points [n_point*3] and hexahedra [n_cell*8] (0-based point
+ * indices, GMSH corner order), through the same topology extraction. */
+gfx_status gfx_mesh_build_hex(int64_t n_point, const double* xyz, int64_t n_cell, const int64_t* hex,
+                              const gfx_box_spec* spec, gfx_mesh** out);
+/* counts[4] = n_vertex, n_edge, n_surface, n_cell of model.h5 /topology; when
+ * cell_corners is given, [n_cell*8] 1-based vertex ids of each cell in GMSH
+ * corner order. */
+gfx_status gfx_topology_summary(const char* model_path, int64_t* counts, int64_t* cell_corners,
+                                int64_t n_cell_corners);
+
 /* Write the inputs the reference preprocessor would produce for a generated
  * box (preprocess/model_writer.py:304-446, preprocess/config_writer.py:57-143
  * schemas): dir/partitions/partition_{r}.h5 for n_parts contiguous element
