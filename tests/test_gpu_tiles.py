@@ -10,6 +10,7 @@ attributes, types and shapes, and the same values bit for bit (both runs use
 one fixed deterministic launch, so the solver's records are identical and
 the device merge rounds as the FP64 host loop does)."""
 import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -166,3 +167,15 @@ def test_slab_capture_rejects_what_it_cannot_serve():
             far.capture(s, 0)
     finally:
         s.close()
+
+
+def test_cli_writes_the_tiles(tmp_path):
+    d = str(tmp_path)
+    make_inputs(d, "float32", tilex=(2, 2), tiley=(4,))
+    cli = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gf-only-sem_amd", "bin",
+                       "gf_greens_hip")
+    r = subprocess.run([cli, "-o", "out", "--assembly", "gather", "--quiet"], cwd=d, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "STAT_NTILES=2" in r.stdout
+    assert sorted(os.listdir(os.path.join(d, "out"))) == ["tile_x000_y000.h5", "tile_x001_y000.h5"]

@@ -9,6 +9,7 @@ cells must produce the same geometry, material, C-PML coefficients, time step
 and recording cells, with a node numbering that is the same partition of the
 element points into nodes, ordered by (x, y, z)."""
 import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -281,3 +282,18 @@ def test_cell_order_must_fill_the_pml_grid():
     xyz, hexes = box_cells(3, 2, 2)
     with pytest.raises(pygfx.GfxError, match="multiple of nx_elements"):
         pygfx.build_hex(xyz, hexes, **_flat_spec(nx=5, ny=1))
+
+
+def test_cli_converts_gmsh(tmp_path):
+    cli = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gf-only-sem_amd", "bin",
+                       "gf_greens_hip")
+    xyz, hexes = box_cells(2, 2, 1)
+    write_msh41(str(tmp_path / "in.msh"), xyz, hexes)
+    r = subprocess.run([cli, "--gmsh", "in.msh", "--aux", "aux.h5"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    counts, corners = pygfx.topology_summary(str(tmp_path / "model.h5"))
+    assert counts[3] == 4 and np.array_equal(corners - 1, hexes)
+    assert (tmp_path / "aux.h5").exists()
+    bad = subprocess.run([cli, "--gmsh", "missing.msh"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert bad.returncode == 1 and "cannot open" in bad.stderr
