@@ -66,6 +66,16 @@ struct ElemArgs {
     const float* __restrict__ stf;        // [nf][nstf]
     int nstf;
     const int* __restrict__ ctr;          // device counters: [0] step index, [1] snapshots taken
+    // shear attenuation (n_sls > 0): lam / mu hold the unrelaxed moduli; per
+    // mechanism l and point: att_a = exp(-dt/tau_sigma), att_b = Y b, att_c = Y c
+    // ([n_sls][P]); memory zeta [nf][n_sls*5][P] and previous deviatoric
+    // strain devp [nf][5][P], both in the scaled strain units of the kernels
+    int n_sls;
+    const float* __restrict__ att_a;
+    const float* __restrict__ att_b;
+    const float* __restrict__ att_c;
+    float* __restrict__ zeta;
+    float* __restrict__ devp;
     float D[25];
 };
 
@@ -417,6 +427,158 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
             add_sources(A, step, f, sb, se, p, rv);
             emit_residual<MODE>(A, f, node, rslot, rold, rv);
         }
+    }
+}
+
+// Stress with shear attenuation at one point of field f (general geometry):
+// strain from the gradient, the memory update of every mechanism
+// z <- a z + (Y b) dev_old + (Y c) dev_new, sigma = lam_U tr I + 2 mu_U eps -
+// sum z, and the fluxes F = sigma . g.
+__device__ __forceinline__ void flux_att(const float (&ui)[3][5], const float (&uj)[3][5], const float (&uk)[3][5],
+                                         const float* Di, const float* Dj, const float* Dk, const float* g,
+                                         float lam, float mu, const ElemArgs& A, int f, int64_t pt, float* F) {
+    float du[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+#pragma unroll
+        for (int s = 0; s < 5; ++s) {
+            a0 += Di[s] * ui[c][s];
+            a1 += Dj[s] * uj[c][s];
+            a2 += Dk[s] * uk[c][s];
+        }
+        du[c][0] = a0;
+        du[c][1] = a1;
+        du[c][2] = a2;
+    }
+    float gr[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int m = 0; m < 3; ++m) gr[c][m] = du[c][0] * g[m * 3 + 0] + du[c][1] * g[m * 3 + 1] + du[c][2] * g[m * 3 + 2];
+    const float tr = gr[0][0] + gr[1][1] + gr[2][2];
+    const float dev[5] = {gr[0][0] - tr * (1.f / 3.f), gr[1][1] - tr * (1.f / 3.f), 0.5f * (gr[0][1] + gr[1][0]),
+                          0.5f * (gr[0][2] + gr[2][0]), 0.5f * (gr[1][2] + gr[2][1])};
+    float* dp = A.devp + int64_t(f) * 5 * A.P + pt;
+    float old[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) old[q] = dp[int64_t(q) * A.P];
+    float zs[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    float* zf = A.zeta + int64_t(f) * A.n_sls * 5 * A.P + pt;
+    for (int l = 0; l < A.n_sls; ++l) {
+        const float a = A.att_a[int64_t(l) * A.P + pt], b = A.att_b[int64_t(l) * A.P + pt],
+                    c = A.att_c[int64_t(l) * A.P + pt];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            float* zq = zf + int64_t(l * 5 + q) * A.P;
+            const float z = a * zq[0] + b * old[q] + c * dev[q];
+            zq[0] = z;
+            zs[q] += z;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 5; ++q) dp[int64_t(q) * A.P] = dev[q];
+    const float mu2 = 2.f * mu;
+    float sig[3][3];
+    sig[0][0] = mu2 * gr[0][0] + lam * tr - zs[0];
+    sig[1][1] = mu2 * gr[1][1] + lam * tr - zs[1];
+    sig[2][2] = mu2 * gr[2][2] + lam * tr + zs[0] + zs[1];
+    sig[0][1] = sig[1][0] = mu * (gr[0][1] + gr[1][0]) - zs[2];
+    sig[0][2] = sig[2][0] = mu * (gr[0][2] + gr[2][0]) - zs[3];
+    sig[1][2] = sig[2][1] = mu * (gr[1][2] + gr[2][1]) - zs[4];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            F[c * 3 + a] = sig[c][0] * g[0 * 3 + a] + sig[c][1] * g[1 * 3 + a] + sig[c][2] * g[2 * 3 + a];
+}
+
+// Block kernel with shear attenuation: one element per workgroup, one lane per
+// GLL point, general geometry, nodal C-PML only.
+template <int MODE>
+__global__ void __launch_bounds__(128) k_element_att(ElemArgs A, int e0, int ne) {
+    __shared__ float sD[25];
+    __shared__ float su[3][NP];
+    __shared__ float sf[9][NP];
+    const int tid = threadIdx.x;
+    if (tid < 25) sD[tid] = A.D[tid];
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int p = tid;
+    const bool valid = p < NP && blk < ne;
+    const int e = e0 + (valid ? blk : 0);
+    const int pc = valid ? p : 0;
+    const int i = pc / 25, j = (pc / 5) % 5, k = pc % 5;
+    float g[9];
+    float lam = 0.f, mu = 0.f;
+    uint32_t nodef = 0;
+    const int64_t pt = int64_t(e) * NP + pc;
+    if (valid) {
+        load_geometry<false>(A, pt, g);
+        lam = A.lam[pt];
+        mu = A.mu[pt];
+        nodef = A.ibool[pt];
+    } else {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) g[c] = 0.f;
+    }
+    const int64_t node = int64_t(nodef & ~FIRST_TOUCH);
+    const bool first = (nodef & FIRST_TOUCH) != 0;
+    int sb = 0, se = 0;
+    if (valid) {
+        sb = A.src_off[e];
+        se = A.src_off[e + 1];
+    }
+    const int step = A.ctr[0];
+    __syncthreads();
+    float Di[5], Dj[5], Dk[5], Dti[5], Dtj[5], Dtk[5];
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+        Di[s] = sD[i * 5 + s];
+        Dj[s] = sD[j * 5 + s];
+        Dk[s] = sD[k * 5 + s];
+        Dti[s] = sD[s * 5 + i];
+        Dtj[s] = sD[s * 5 + j];
+        Dtk[s] = sD[s * 5 + k];
+    }
+    for (int f = 0; f < A.nf; ++f) {
+        const int64_t fo = int64_t(f) * A.fstride;
+        if (valid) {
+            const float* ut = A.U + fo + node * 3;
+            su[0][p] = ut[0];
+            su[1][p] = ut[1];
+            su[2][p] = ut[2];
+        }
+        float rold[3] = {0.f, 0.f, 0.f};
+        if (MODE == ASM_COLORED && valid && !first) {
+            const float* r = A.R + fo + node * 3;
+            rold[0] = r[0];
+            rold[1] = r[1];
+            rold[2] = r[2];
+        }
+        __syncthreads();
+        if (valid) {
+            float ui[3][5], uj[3][5], uk[3][5];
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int s = 0; s < 5; ++s) {
+                    ui[c][s] = su[c][s * 25 + j * 5 + k];
+                    uj[c][s] = su[c][i * 25 + s * 5 + k];
+                    uk[c][s] = su[c][i * 25 + j * 5 + s];
+                }
+            float F[9];
+            flux_att(ui, uj, uk, Di, Dj, Dk, g, lam, mu, A, f, pt, F);
+#pragma unroll
+            for (int q = 0; q < 9; ++q) sf[q][p] = F[q];
+        }
+        __syncthreads();
+        if (valid) {
+            float rv[3];
+            point_div(&sf[0][0], NP, Dti, Dtj, Dtk, i, j, k, rv);
+            add_sources(A, step, f, sb, se, p, rv);
+            emit_residual<MODE>(A, f, node, pt, rold, rv);
+        }
+        __syncthreads();  // su and sf are rewritten by the next field
     }
 }
 

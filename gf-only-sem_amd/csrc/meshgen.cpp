@@ -166,6 +166,59 @@ void abar(int region, const double K[3], const double d[3], const double al[3], 
     }
 }
 
+// Constant-Q standard linear solids (Emmerich and Korn 1987, linearised least
+// squares on 1/Q; see gfx_sls_fit).
+void sls_fit(double q, double f_min, double f_max, int L, double* ts, double* te) {
+    require(L >= 1 && L <= 8, "n_sls must be in [1, 8]");
+    require(f_min > 0 && f_max >= f_min, "attenuation band must have 0 < f_min <= f_max");
+    for (int l = 0; l < L; ++l) {
+        const double f = L == 1 ? std::sqrt(f_min * f_max) : f_min * std::pow(f_max / f_min, double(l) / (L - 1));
+        ts[l] = 1.0 / (2.0 * M_PI * f);
+    }
+    if (!(q > 0) || std::isinf(q)) {
+        for (int l = 0; l < L; ++l) te[l] = ts[l];
+        return;
+    }
+    const double qi = 1.0 / q;
+    const int K = 10 * L;
+    std::vector<double> ata(size_t(L) * L, 0.0), atb(L, 0.0);
+    for (int k = 0; k < K; ++k) {
+        const double f = K == 1 ? f_min : f_min * std::pow(f_max / f_min, double(k) / (K - 1));
+        const double w = 2.0 * M_PI * f;
+        std::vector<double> row(L);
+        for (int l = 0; l < L; ++l) row[l] = (w * ts[l] + qi) / (1.0 + w * w * ts[l] * ts[l]);
+        for (int a = 0; a < L; ++a) {
+            atb[a] += row[a] * qi;
+            for (int b = 0; b < L; ++b) ata[size_t(a) * L + b] += row[a] * row[b];
+        }
+    }
+    // Gaussian elimination with partial pivoting
+    std::vector<double> y = atb;
+    for (int c = 0; c < L; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < L; ++r)
+            if (std::abs(ata[size_t(r) * L + c]) > std::abs(ata[size_t(piv) * L + c])) piv = r;
+        require(std::abs(ata[size_t(piv) * L + c]) > 0, "attenuation fit is singular");
+        if (piv != c) {
+            for (int k = 0; k < L; ++k) std::swap(ata[size_t(c) * L + k], ata[size_t(piv) * L + k]);
+            std::swap(y[c], y[piv]);
+        }
+        for (int r = c + 1; r < L; ++r) {
+            const double m = ata[size_t(r) * L + c] / ata[size_t(c) * L + c];
+            for (int k = c; k < L; ++k) ata[size_t(r) * L + k] -= m * ata[size_t(c) * L + k];
+            y[r] -= m * y[c];
+        }
+    }
+    for (int c = L - 1; c >= 0; --c) {
+        for (int k = c + 1; k < L; ++k) y[c] -= ata[size_t(c) * L + k] * y[k];
+        y[c] /= ata[size_t(c) * L + c];
+    }
+    double sy = 0;
+    for (int l = 0; l < L; ++l) sy += y[l];
+    require(sy < 1.0, "quality factor too low for the standard linear solids");
+    for (int l = 0; l < L; ++l) te[l] = ts[l] * (1.0 + y[l] / (1.0 - sy));
+}
+
 // General cells: corner coordinates per cell in GMSH order and the domain
 // bounds (the vertex extent, preprocess/cli.py:409-417).
 struct GeneralCells {
@@ -456,6 +509,28 @@ void build(MeshHolder& H, const GeneralCells* gen = nullptr) {
                 nvp[g] = vp;
                 nvs[g] = vs;
                 nrho[g] = s.rho[l];
+            }
+        });
+    }
+
+    // shear attenuation per layer, at the element points' nodes
+    m.n_sls = s.n_sls;
+    if (s.n_sls > 0) {
+        require(s.n_sls <= 8, "n_sls must be in [0, 8]");
+        const int L = s.n_sls;
+        std::vector<double> lts(size_t(s.n_layers) * L), lte(size_t(s.n_layers) * L);
+        for (int l = 0; l < s.n_layers; ++l) sls_fit(s.q_mu[l], s.att_fmin, s.att_fmax, L, &lts[l * L], &lte[l * L]);
+        H.tau_sigma.resize(size_t(P) * L);
+        H.tau_epsilon.resize(size_t(P) * L);
+        parallel_for(E, [&](int64_t e) {
+            for (int n = 0; n < nn; ++n) {
+                const double z = H.node_xyz[int64_t(H.ibool[e * nn + n]) * 3 + 2];
+                int l = 0;
+                while (l < s.n_layers - 1 && z >= s.layer_zbot[l]) ++l;
+                for (int q = 0; q < L; ++q) {
+                    H.tau_sigma[(size_t(e) * nn + n) * L + q] = lts[size_t(l) * L + q];
+                    H.tau_epsilon[(size_t(e) * nn + n) * L + q] = lte[size_t(l) * L + q];
+                }
             }
         });
     }
@@ -962,6 +1037,14 @@ extern "C" gfx_status gfx_mesh_locate_source(const gfx_mesh* m, double x, double
     });
 }
 
+extern "C" gfx_status gfx_sls_fit(double q, double f_min, double f_max, int n_sls, double* tau_sigma,
+                                   double* tau_epsilon) {
+    return guarded([&] {
+        require(tau_sigma && tau_epsilon, "null argument");
+        sls_fit(q, f_min, f_max, n_sls, tau_sigma, tau_epsilon);
+    });
+}
+
 extern "C" gfx_status gfx_ricker(double f0, double t0, double amplitude, double dt, int n,
                                  double* out) {
     return guarded([&] {
@@ -996,6 +1079,8 @@ void wire_mesh(MeshHolder& H) {
     m.coef_abar = H.coef_abar.empty() ? nullptr : H.coef_abar.data();
     m.n_rec_cell = int(H.rec_cell.size());
     m.rec_cell = H.rec_cell.empty() ? nullptr : H.rec_cell.data();
+    m.tau_sigma = H.tau_sigma.empty() ? nullptr : H.tau_sigma.data();
+    m.tau_epsilon = H.tau_epsilon.empty() ? nullptr : H.tau_epsilon.data();
 }
 
 }  // namespace gfx

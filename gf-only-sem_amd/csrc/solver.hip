@@ -64,6 +64,8 @@ struct gfx_solver {
     int n_colors = 1;
     int64_t n_diag = 0;                       // elements on the axis-aligned (diagonal) path
     gfx::DevBuf<float> G9, lam, mu, minv, damp;
+    int n_sls = 0;                                     // shear attenuation mechanisms
+    gfx::DevBuf<float> att_a, att_b, att_c, zeta, devp;
     gfx::DevBuf<uint32_t> ibool;
     // C-PML: nodal (pc, palpha, pmem, pnew over [0,Gpml)) or element-local (Ppml points)
     gfx::DevBuf<float> pc, palpha, pabar, pmem, pnew;
@@ -148,6 +150,8 @@ void init_state(gfx_solver& S) {
     S.Rl.zero(S.stream);
     S.pmem.zero(S.stream);
     S.pnew.zero(S.stream);
+    S.zeta.zero(S.stream);
+    S.devp.zero(S.stream);
     S.ctr.zero(S.stream);
     GFX_HIP(hipStreamSynchronize(S.stream));
     S.step = 0;
@@ -279,6 +283,15 @@ void launch_element_d(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
 
 void launch_element(gfx_solver& S, const dev::ElemArgs& a, const ElemRange& rg) {
     const int b = rg.b, n = rg.e - rg.b;
+    if (S.n_sls > 0) {  // shear attenuation: one kernel form for every range
+        if (n <= 0) return;
+        switch (S.assembly) {
+            case GFX_ASSEMBLY_GATHER: dev::k_element_att<dev::ASM_GATHER><<<n, 128, 0, S.stream>>>(a, b, n); break;
+            case GFX_ASSEMBLY_ATOMIC: dev::k_element_att<dev::ASM_ATOMIC><<<n, 128, 0, S.stream>>>(a, b, n); break;
+            default: dev::k_element_att<dev::ASM_COLORED><<<n, 128, 0, S.stream>>>(a, b, n); break;
+        }
+        return;
+    }
     if (S.has_cpml && !S.pml_nodal) {
         launch_element_p<true, false>(S, a, b, n);  // element-local C-PML: block kernel, general form
     } else if (rg.diag) {
@@ -359,6 +372,12 @@ StepArgs make_step_args(gfx_solver& S) {
     ea.stf = S.stf.p;
     ea.nstf = S.nstf;
     ea.ctr = S.ctr.p;
+    ea.n_sls = S.n_sls;
+    ea.att_a = S.att_a.p;
+    ea.att_b = S.att_b.p;
+    ea.att_c = S.att_c.p;
+    ea.zeta = S.zeta.p;
+    ea.devp = S.devp.p;
     std::memcpy(ea.D, S.D, sizeof(S.D));
 
     dev::NodeArgs& na = a.na;
@@ -779,6 +798,46 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
             }
         }
     }
+    // ---- shear attenuation: unrelaxed moduli and per-mechanism step coefficients ----
+    S.n_sls = d.n_sls;
+    require(d.n_sls >= 0 && d.n_sls <= 8, "n_sls must be in [0, 8]");
+    if (S.n_sls > 0) {
+        require(d.tau_sigma && d.tau_epsilon, "n_sls > 0 needs tau_sigma and tau_epsilon");
+        const int L = S.n_sls;
+        std::vector<float> ha(size_t(L) * P), hb(size_t(L) * P), hc(size_t(L) * P);
+        std::atomic<bool> bad_tau{false};
+        parallel_for(E, [&](int64_t ie) {
+            const int64_t e = elem_of_int[ie];
+            for (int p = 0; p < NP; ++p) {
+                const int64_t src = e * NP + p, dst = ie * NP + p;
+                const double mu_r = d.mu_[src];
+                double s_u = 1.0;
+                for (int l = 0; l < L; ++l) {
+                    const double ts = d.tau_sigma[src * L + l], te = d.tau_epsilon[src * L + l];
+                    if (!(ts > 0) || !(te > 0) || !std::isfinite(ts) || !std::isfinite(te)) bad_tau = true;
+                    s_u += te / ts - 1.0;
+                    // exact step for strain linear in time: z <- a z + Y (b dev_old + c dev_new)
+                    const double a = std::exp(-S.dt / ts), q = ts * (1.0 - a) / S.dt;
+                    const double Y = 2.0 * mu_r * (te / ts - 1.0);
+                    const bool solid = mu_r > 0.0;
+                    ha[size_t(l) * P + dst] = solid ? float(a) : 0.f;
+                    hb[size_t(l) * P + dst] = solid ? float(Y * (q - a)) : 0.f;
+                    hc[size_t(l) * P + dst] = solid ? float(Y * (1.0 - q)) : 0.f;
+                }
+                if (mu_r > 0.0) {
+                    const double mu_u = mu_r * s_u;
+                    hmu[dst] = float(mu_u);
+                    hlam[dst] = float(d.lambda_[src] + 2.0 * (mu_r - mu_u) / 3.0);
+                }
+            }
+        });
+        require(!bad_tau, "tau_sigma and tau_epsilon must be positive and finite");
+        dv.upload(S.att_a, std::move(ha));
+        dv.upload(S.att_b, std::move(hb));
+        dv.upload(S.att_c, std::move(hc));
+        dv.alloc(S.zeta, size_t(S.nf) * L * 5 * P);
+        dv.alloc(S.devp, size_t(S.nf) * 5 * P);
+    }
     dv.upload(S.G9, std::move(hG));
     dv.upload(S.lam, std::move(hlam));
     dv.upload(S.mu, std::move(hmu));
@@ -850,6 +909,9 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
             dv.alloc(S.pnew, size_t(S.nf) * 3 * std::max<int64_t>(Gp, 1));
         } else {
             S.pml_nodal = false;
+            require(S.n_sls == 0,
+                    "shear attenuation needs the nodal C-PML form (every element using a memory variable at a "
+                    "node must carry the same alpha for that direction)");
             int npml = 0;
             for (int ie = 0; ie < E; ++ie)
                 if (d.pml_region[elem_of_int[ie]] != 0) hslot[ie] = npml++;
@@ -1195,7 +1257,7 @@ uint64_t layout_signature(const gfx_solver& S) {
 
 size_t checkpoint_bytes(const gfx_solver& S) {
     const size_t fl = size_t(S.nf) * S.fstride;
-    return CKPT_WORDS * 8 + 4 * (3 * fl + size_t(S.nf) * S.astride + S.pmem.n + S.pnew.n);
+    return CKPT_WORDS * 8 + 4 * (3 * fl + size_t(S.nf) * S.astride + S.pmem.n + S.pnew.n + S.zeta.n + S.devp.n);
 }
 
 }  // namespace
@@ -1225,6 +1287,8 @@ extern "C" gfx_status gfx_solver_checkpoint_save(const gfx_solver* s, void* host
         hdr[7] = int64_t(layout_signature(*s));
         hdr[8] = int64_t(s->pmem.n);
         hdr[9] = int64_t(s->pnew.n);
+        hdr[10] = int64_t(s->zeta.n);
+        hdr[11] = int64_t(s->devp.n);
         float* p = reinterpret_cast<float*>(static_cast<char*>(host) + CKPT_WORDS * 8);
         const size_t fl = size_t(s->nf) * s->fstride, al = size_t(s->nf) * s->astride;
         auto get = [&](const float* d, size_t n) {
@@ -1236,6 +1300,8 @@ extern "C" gfx_status gfx_solver_checkpoint_save(const gfx_solver* s, void* host
         get(s->Aacc.p, al);
         get(s->pmem.p, s->pmem.n);
         get(s->pnew.p, s->pnew.n);
+        get(s->zeta.p, s->zeta.n);
+        get(s->devp.p, s->devp.n);
         // assembled residual of the last step (what get_state reads for a at
         // regular nodes): R itself, or the fixed-order slot sums
         if (s->assembly != GFX_ASSEMBLY_GATHER) {
@@ -1262,8 +1328,8 @@ extern "C" gfx_status gfx_solver_checkpoint_load(gfx_solver* s, const void* host
         require(hdr[0] == CKPT_MAGIC, "not a solver checkpoint image");
         require(hdr[1] == s->E && hdr[2] == s->G && hdr[3] == s->nf && hdr[4] == s->assembly &&
                     hdr[7] == int64_t(layout_signature(*s)) && hdr[8] == int64_t(s->pmem.n) &&
-                    hdr[9] == int64_t(s->pnew.n),
-                "checkpoint was taken from a solver with another mesh, field count or assembly");
+                    hdr[9] == int64_t(s->pnew.n) && hdr[10] == int64_t(s->zeta.n) && hdr[11] == int64_t(s->devp.n),
+                "checkpoint was taken from a solver with another mesh, field count, assembly or attenuation");
         require(bytes >= checkpoint_bytes(*s), "checkpoint image truncated");
         const float* p = reinterpret_cast<const float*>(static_cast<const char*>(host) + CKPT_WORDS * 8);
         const size_t fl = size_t(s->nf) * s->fstride, al = size_t(s->nf) * s->astride;
@@ -1277,6 +1343,8 @@ extern "C" gfx_status gfx_solver_checkpoint_load(gfx_solver* s, const void* host
         put(s->Aacc.p, al);
         put(s->pmem.p, s->pmem.n);
         put(s->pnew.p, s->pnew.n);
+        put(s->zeta.p, s->zeta.n);
+        put(s->devp.p, s->devp.n);
         if (s->assembly != GFX_ASSEMBLY_GATHER) {
             put(s->R.p, fl);
         } else {
@@ -1457,6 +1525,7 @@ extern "C" gfx_status gfx_solver_query(const gfx_solver* s, const char* key, dou
         else if (k == "element_kernel") *value = s->kernel;
         else if (k == "elems_per_block") *value = s->epb;
         else if (k == "node_field_groups") *value = s->node_fg;
+        else if (k == "n_sls") *value = s->n_sls;
         else if (k == "pml_nodal") *value = (s->has_cpml && s->pml_nodal) ? 1.0 : 0.0;
         else if (k == "pml_elements") *value = s->n_pml_elem;
         else if (k == "special_nodes") *value = double(s->Ga);

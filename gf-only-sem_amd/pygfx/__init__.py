@@ -56,6 +56,9 @@ class BoxSpec(C.Structure):
         ("record_depth_max", C.c_double),
         ("want_coords", C.c_int),
         ("want_beta", C.c_int),
+        ("n_sls", C.c_int),
+        ("q_mu", C.c_double * MAX_LAYERS),
+        ("att_fmin", C.c_double), ("att_fmax", C.c_double),
     ]
 
 
@@ -75,6 +78,9 @@ class _Mesh(C.Structure):
         ("record_depth_actual", C.c_double),
         ("n_rec_cell", C.c_int),
         ("rec_cell", _ip),
+        ("n_sls", C.c_int),
+        ("tau_sigma", _dp),
+        ("tau_epsilon", _dp),
     ]
 
 
@@ -92,6 +98,7 @@ class SolverDesc(C.Structure):
         ("record_capacity", C.c_int), ("record_uva", C.c_int),
         ("node_field_groups", C.c_int),
         ("general_geometry", C.c_int),
+        ("n_sls", C.c_int), ("tau_sigma", _dp), ("tau_epsilon", _dp),
     ]
 
 
@@ -180,6 +187,7 @@ def _load():
         "gfx_mesh_locate_source": (st, [C.POINTER(_Mesh), C.c_double, C.c_double, C.c_double,
                                         C.c_int, _ip, _dp, C.POINTER(C.c_int)]),
         "gfx_ricker": (st, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, _dp]),
+        "gfx_sls_fit": (st, [C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp]),
         "gfx_solver_set_step": (st, [vp, C.c_int]),
         "gfx_solver_checkpoint_size": (st, [vp, C.POINTER(C.c_size_t)]),
         "gfx_solver_checkpoint_save": (st, [vp, vp, C.c_size_t]),
@@ -249,6 +257,13 @@ def element_residual(dxi_dx, jacobian, lambda_, mu_, D, weights, ngll, u, r, fp3
     _check(fn(n_elem, *[_ptr(a) for a in arrs[:6]], ngll, _ptr(arrs[6]), _ptr(r)))
 
 
+def sls_fit(q, f_min, f_max, n_sls):
+    """(tau_sigma, tau_epsilon) of n_sls standard linear solids for a constant q."""
+    ts, te = np.zeros(n_sls), np.zeros(n_sls)
+    _check(lib.gfx_sls_fit(q, f_min, f_max, n_sls, _ptr(ts), _ptr(te)))
+    return ts, te
+
+
 def ricker(f0, t0, amplitude, dt, n):
     out = np.zeros(n, dtype=np.float64)
     _check(lib.gfx_ricker(f0, t0, amplitude, dt, n, _ptr(out)))
@@ -275,6 +290,7 @@ class Mesh:
     n_pml_elem: int
     record_depth_actual: float
     rec_cell: np.ndarray
+    n_sls: int = 0
     _handle: object = field(default=None, repr=False)
 
     def __getattr__(self, k):
@@ -321,6 +337,10 @@ def make_spec(**kw) -> BoxSpec:
     """Box spec from defaults; `layers` = [(zbot, vp, vs, rho), ...] (last zbot ignored)."""
     s = default_spec()
     layers = kw.pop("layers", None)
+    q_mu = kw.pop("q_mu", None)
+    if q_mu is not None:
+        for l, q in enumerate(q_mu):
+            s.q_mu[l] = q
     for k, v in kw.items():
         if not hasattr(s, k):
             raise KeyError(k)
@@ -424,6 +444,8 @@ def _wrap_mesh(h, keep_handle, copy) -> Mesh:
         "coef_alpha": arr(m.coef_alpha, P * 9),
         "coef_beta": arr(m.coef_beta, P * 9),
         "coef_abar": arr(m.coef_abar, P * 5),
+        "tau_sigma": arr(m.tau_sigma, P * m.n_sls),
+        "tau_epsilon": arr(m.tau_epsilon, P * m.n_sls),
     }
     rec = arr(m.rec_cell, m.n_rec_cell, np.int32)
     rec = None if rec is None else rec.copy()
@@ -431,7 +453,7 @@ def _wrap_mesh(h, keep_handle, copy) -> Mesh:
                 bounds=(m.xmin, m.xmax, m.ymin, m.ymax, m.zmin, m.zmax), arrays=arrays,
                 dt=m.dt, snapshot_stride=m.snapshot_stride, nsteps=m.nsteps, h_min=m.h_min,
                 vp_max=m.vp_max, cfl_dt=m.cfl_dt, n_pml_elem=m.n_pml_elem,
-                record_depth_actual=m.record_depth_actual,
+                record_depth_actual=m.record_depth_actual, n_sls=m.n_sls,
                 rec_cell=rec if rec is not None else np.zeros(0, np.int32), _handle=h)
     if not keep_handle:
         if not copy:
@@ -604,7 +626,7 @@ def h5_layout(path) -> dict:
 def make_desc(mesh: Mesh, keep: list, n_fields=1, assembly=GFX_ASSEMBLY_COLORED, elems_per_block=0,
               record_capacity=None, record_uva=False, has_cpml=True, dt=None,
               snapshot_stride=None, rec_cell=None, element_kernel=GFX_ELEMENT_AUTO, morton=True,
-              node_field_groups=0, general_geometry=False):
+              node_field_groups=0, general_geometry=False, attenuation=True):
     """Solver descriptor over mesh arrays; arrays it points to are appended to keep."""
     a = mesh.arrays
 
@@ -643,6 +665,10 @@ def make_desc(mesh: Mesh, keep: list, n_fields=1, assembly=GFX_ASSEMBLY_COLORED,
     d.record_uva = 1 if record_uva else 0
     d.node_field_groups = node_field_groups
     d.general_geometry = 1 if general_geometry else 0
+    if attenuation and getattr(mesh, "n_sls", 0) > 0:
+        d.n_sls = mesh.n_sls
+        d.tau_sigma = _ptr(kp(a["tau_sigma"]))
+        d.tau_epsilon = _ptr(kp(a["tau_epsilon"]))
     return d
 
 
@@ -664,14 +690,15 @@ class Solver:
     def __init__(self, mesh: Mesh, n_fields=1, assembly=GFX_ASSEMBLY_COLORED, elems_per_block=0,
                  record_capacity=None, record_uva=False, has_cpml=True, dt=None,
                  snapshot_stride=None, rec_cell=None, element_kernel=GFX_ELEMENT_AUTO,
-                 morton=True, node_field_groups=0, general_geometry=False):
+                 morton=True, node_field_groups=0, general_geometry=False, attenuation=True):
         self._keep = []
         d = make_desc(mesh, self._keep, n_fields=n_fields, assembly=assembly,
                       elems_per_block=elems_per_block, record_capacity=record_capacity,
                       record_uva=record_uva, has_cpml=has_cpml, dt=dt,
                       snapshot_stride=snapshot_stride, rec_cell=rec_cell,
                       element_kernel=element_kernel, morton=morton,
-                      node_field_groups=node_field_groups, general_geometry=general_geometry)
+                      node_field_groups=node_field_groups, general_geometry=general_geometry,
+                      attenuation=attenuation)
         rk_size = d.n_rec_cell
         h = C.c_void_p()
         _check(lib.gfx_solver_create(C.byref(d), C.byref(h)))

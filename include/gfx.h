@@ -130,6 +130,15 @@ typedef struct {
        the products with the zero entries; 1: every element runs the general
        form */
     int general_geometry;
+    /* shear attenuation by n_sls standard linear solids (0 = elastic; at most
+       8): per point and mechanism the stress and strain relaxation times
+       [n_elem*ngll^3*n_sls] (caller order); mu_ is the relaxed modulus.  The
+       elements then run the attenuating block kernel (general geometry; the
+       C-PML must take the nodal form).  Not in the reference, whose solver is
+       elastic (its planned inputs: /field/cell/tau_sigma, tau_epsilon). */
+    int n_sls;
+    const double* tau_sigma;
+    const double* tau_epsilon;
 } gfx_solver_desc;
 
 gfx_status gfx_solver_create(const gfx_solver_desc* desc, gfx_solver** out);
@@ -270,6 +279,12 @@ typedef struct {
     double record_depth_max;  /* m; 0 = no recording */
     int want_coords;          /* produce coords/vp/vs (x3 memory) */
     int want_beta;            /* produce pml_coef_beta */
+    /* shear attenuation: n_sls standard linear solids fitted to a constant
+       q_mu[layer] over [att_fmin, att_fmax] Hz (gfx_sls_fit); 0 = elastic;
+       q_mu <= 0 leaves that layer elastic */
+    int n_sls;
+    double q_mu[GFX_MAX_LAYERS];
+    double att_fmin, att_fmax;
 } gfx_box_spec;
 
 typedef struct {
@@ -298,6 +313,9 @@ typedef struct {
     double record_depth_actual;
     int n_rec_cell;
     int32_t* rec_cell;  /* [n_rec_cell] */
+    int n_sls;            /* shear attenuation mechanisms (0 = elastic) */
+    double* tau_sigma;    /* [E,n,n_sls] or NULL */
+    double* tau_epsilon;  /* [E,n,n_sls] or NULL */
 } gfx_mesh;
 
 gfx_status gfx_box_spec_default(gfx_box_spec* spec);
@@ -310,6 +328,14 @@ gfx_status gfx_mesh_free(gfx_mesh* m);
  * over all containing cells.  *n_src receives the count. */
 gfx_status gfx_mesh_locate_source(const gfx_mesh* m, double x, double y, double z, int max_cells,
                                   int32_t* cells, double* weights, int* n_src);
+
+/* Standard linear solids for a constant shear quality factor q over
+ * [f_min, f_max] Hz: relaxation frequencies log-spaced over the band
+ * (tau_sigma_l = 1 / (2 pi f_l)) and strengths from the linear least-squares
+ * fit of Emmerich and Korn (1987) to 1/Q at 10 n_sls log-spaced frequencies,
+ * returned as tau_epsilon_l.  The modulus of the solids is
+ * M(w) = M_R (1 + sum_l (tau_eps_l / tau_sig_l - 1) i w tau_sig_l / (1 + i w tau_sig_l)). */
+gfx_status gfx_sls_fit(double q, double f_min, double f_max, int n_sls, double* tau_sigma, double* tau_epsilon);
 
 /* Ricker wavelet A (1 - 2a^2) exp(-a^2), a = pi f0 (t - t0), sampled at
  * t = i*dt for i < n (preprocess/stf_evaluator.py:19-45). */

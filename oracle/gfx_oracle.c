@@ -196,6 +196,137 @@ void gfo_element_residual(int n_elem, const double* dxi_dx, const double* jacobi
 }
 
 /* ------------------------------------------------------------------ */
+/* Shear attenuation (standard linear solids); no reference counterpart */
+/* ------------------------------------------------------------------ */
+
+void gfo_sls_coefficients(double tau_sigma, double tau_epsilon, double mu_relaxed, double dt, double* abcY) {
+    const double a = exp(-dt / tau_sigma);
+    const double q = tau_sigma * (1.0 - a) / dt;
+    abcY[0] = a;
+    abcY[1] = q - a;
+    abcY[2] = 1.0 - q;
+    abcY[3] = 2.0 * mu_relaxed * (tau_epsilon / tau_sigma - 1.0);
+}
+
+/* z^{n+1} = a z^n + Y (b dev^n + c dev^{n+1}) for the 5 stored components of
+ * every mechanism; returns sum_l z_l in zs[5]. */
+static void sls_update(int n_sls, const double* tsg, const double* teps, double mu_r, double dt,
+                       const double* dev_old, const double* dev_new, double* z, double* zs) {
+    for (int q = 0; q < 5; ++q) zs[q] = 0.0;
+    for (int l = 0; l < n_sls; ++l) {
+        double c[4];
+        gfo_sls_coefficients(tsg[l], teps[l], mu_r, dt, c);
+        for (int q = 0; q < 5; ++q) {
+            z[l * 5 + q] = c[0] * z[l * 5 + q] + c[3] * (c[1] * dev_old[q] + c[2] * dev_new[q]);
+            zs[q] += z[l * 5 + q];
+        }
+    }
+}
+
+static double sls_unrelaxed(int n_sls, const double* tsg, const double* teps, double mu_r) {
+    double s = 1.0;
+    for (int l = 0; l < n_sls; ++l) s += teps[l] / tsg[l] - 1.0;
+    return mu_r * s;
+}
+
+void gfo_sls_point_history(int n_sls, const double* tau_sigma, const double* tau_epsilon, double mu_relaxed,
+                           double dt, int n, const double* dev, double* sig) {
+    double z[8 * 5] = {0}, zs[5];
+    const double mu_u = sls_unrelaxed(n_sls, tau_sigma, tau_epsilon, mu_relaxed);
+    for (int t = 1; t < n; ++t) {
+        sls_update(n_sls, tau_sigma, tau_epsilon, mu_relaxed, dt, dev + (t - 1) * 5, dev + t * 5, z, zs);
+        for (int q = 0; q < 5; ++q) sig[(t - 1) * 5 + q] = 2.0 * mu_u * dev[t * 5 + q] - zs[q];
+    }
+}
+
+void gfo_element_residual_att(int n_elem, const double* dxi_dx, const double* jacobian, const double* lambda_,
+                              const double* mu_, int n_sls, const double* tau_sigma, const double* tau_epsilon,
+                              double dt, const double* D, const double* weights, int ngll, const double* u,
+                              double* zeta, double* dev_prev, double* r) {
+    const int n_node = ngll * ngll * ngll;
+#pragma omp parallel for schedule(static)
+    for (long e = 0; e < n_elem; ++e) {
+        const double* edd = dxi_dx + e * n_node * 9;
+        const double* ejac = jacobian + e * n_node;
+        const double* eu = u + e * n_node * 3;
+        double* er = r + e * n_node * 3;
+        for (int i = 0; i < ngll; ++i)
+            for (int j = 0; j < ngll; ++j)
+                for (int k = 0; k < ngll; ++k) {
+                    const int n = nidx(i, j, k, ngll);
+                    const long pt = e * n_node + n;
+                    const double lam_r = lambda_[pt], mu_r = mu_[pt];
+                    if (mu_r <= 0.0) continue; /* element_cpu.cpp:42-43 */
+                    const double* tsg = tau_sigma + pt * n_sls;
+                    const double* teps = tau_epsilon + pt * n_sls;
+                    const double* dd = edd + 9 * n;
+                    double gx[3] = {0, 0, 0}, ge[3] = {0, 0, 0}, gz[3] = {0, 0, 0};
+                    for (int s = 0; s < ngll; ++s) {
+                        const double di = D[i * ngll + s], dj = D[j * ngll + s], dk = D[k * ngll + s];
+                        const int a = nidx(s, j, k, ngll), b = nidx(i, s, k, ngll), c = nidx(i, j, s, ngll);
+                        for (int d = 0; d < 3; ++d) {
+                            gx[d] += di * eu[3 * a + d];
+                            ge[d] += dj * eu[3 * b + d];
+                            gz[d] += dk * eu[3 * c + d];
+                        }
+                    }
+                    double g[3][3];
+                    for (int c = 0; c < 3; ++c) {
+                        g[c][0] = gx[c] * dd[0] + ge[c] * dd[1] + gz[c] * dd[2];
+                        g[c][1] = gx[c] * dd[3] + ge[c] * dd[4] + gz[c] * dd[5];
+                        g[c][2] = gx[c] * dd[6] + ge[c] * dd[7] + gz[c] * dd[8];
+                    }
+                    double eps[3][3];
+                    for (int l = 0; l < 3; ++l)
+                        for (int m = 0; m < 3; ++m) eps[l][m] = 0.5 * (g[l][m] + g[m][l]);
+                    const double tr = eps[0][0] + eps[1][1] + eps[2][2];
+                    const double dev[5] = {eps[0][0] - tr / 3.0, eps[1][1] - tr / 3.0, eps[0][1], eps[0][2],
+                                           eps[1][2]};
+                    double zs[5];
+                    sls_update(n_sls, tsg, teps, mu_r, dt, dev_prev + pt * 5, dev, zeta + pt * 5 * n_sls, zs);
+                    for (int q = 0; q < 5; ++q) dev_prev[pt * 5 + q] = dev[q];
+                    const double mu_u = sls_unrelaxed(n_sls, tsg, teps, mu_r);
+                    const double lam_u = lam_r + 2.0 * (mu_r - mu_u) / 3.0;
+                    double sig[3][3];
+                    for (int l = 0; l < 3; ++l) {
+                        for (int m = 0; m < 3; ++m) sig[l][m] = 2.0 * mu_u * eps[l][m];
+                        sig[l][l] += lam_u * tr;
+                    }
+                    sig[0][0] -= zs[0];
+                    sig[1][1] -= zs[1];
+                    sig[2][2] += zs[0] + zs[1];
+                    sig[0][1] -= zs[2];
+                    sig[1][0] -= zs[2];
+                    sig[0][2] -= zs[3];
+                    sig[2][0] -= zs[3];
+                    sig[1][2] -= zs[4];
+                    sig[2][1] -= zs[4];
+                    const double fac = ejac[n] * weights[i] * weights[j] * weights[k];
+                    for (int dir = 0; dir < 3; ++dir) {
+                        for (int s = 0; s < ngll; ++s) {
+                            double Dq;
+                            int ns;
+                            if (dir == 0) {
+                                Dq = D[i * ngll + s];
+                                ns = nidx(s, j, k, ngll);
+                            } else if (dir == 1) {
+                                Dq = D[j * ngll + s];
+                                ns = nidx(i, s, k, ngll);
+                            } else {
+                                Dq = D[k * ngll + s];
+                                ns = nidx(i, j, s, ngll);
+                            }
+                            const double gn[3] = {Dq * dd[dir], Dq * dd[3 + dir], Dq * dd[6 + dir]};
+                            for (int c = 0; c < 3; ++c)
+                                er[3 * ns + c] -=
+                                    (sig[c][0] * gn[0] + sig[c][1] * gn[1] + sig[c][2] * gn[2]) * fac;
+                        }
+                    }
+                }
+    }
+}
+
+/* ------------------------------------------------------------------ */
 /* Assembly: forward/share/src/assembly.cpp:10-41                      */
 /* ------------------------------------------------------------------ */
 
@@ -385,6 +516,7 @@ void gfo_full_strain(int n_cell, const int32_t* cells, const double* dxi_dx, con
 typedef struct {
     double *u, *v, *a, *r, *ut, *uloc, *rloc, *mass_g, *damp_g;
     double *pold, *pnew, *rmem;
+    double *zeta, *devp;
     double gll_x[16], gll_w[16], D[256];
     gfo_cpml pml;
 } run_ws;
@@ -393,6 +525,7 @@ static void ws_free(run_ws* w) {
     free(w->u); free(w->v); free(w->a); free(w->r); free(w->ut);
     free(w->uloc); free(w->rloc); free(w->mass_g); free(w->damp_g);
     free(w->pold); free(w->pnew); free(w->rmem);
+    free(w->zeta); free(w->devp);
 }
 
 static int ws_init(run_ws* w, const gfo_problem* pb) {
@@ -422,6 +555,12 @@ static int ws_init(run_ws* w, const gfo_problem* pb) {
         const long g = pb->ibool[p];
         w->mass_g[g] += pb->mass[p];
         if (pb->damping) w->damp_g[g] = pb->damping[p];
+    }
+    if (pb->n_sls > 0) {
+        if (pb->n_sls > 8 || !pb->tau_sigma || !pb->tau_epsilon) return -1;
+        w->zeta = calloc(P * 5 * pb->n_sls, sizeof(double));
+        w->devp = calloc(P * 5, sizeof(double));
+        if (!w->zeta || !w->devp) return -2;
     }
     if (pb->has_cpml) { /* cpml_initialize, pml.cpp:41-53 */
         w->pold = calloc(P * 3, sizeof(double));
@@ -458,8 +597,13 @@ static void run_one_step(run_ws* w, const gfo_problem* pb, int step) {
     if (pb->has_cpml) gfo_cpml_update_displ_fields(&w->pml, w->u, w->v, w->a, dt);
     /* 3b. zero + element kernel */
     memset(w->rloc, 0, sizeof(double) * 3 * P);
-    gfo_element_residual(pb->n_elem, pb->dxi_dx, pb->jacobian, pb->lambda_, pb->mu_, w->D,
-                         w->gll_w, ngll, w->uloc, w->rloc);
+    if (pb->n_sls > 0)
+        gfo_element_residual_att(pb->n_elem, pb->dxi_dx, pb->jacobian, pb->lambda_, pb->mu_, pb->n_sls,
+                                 pb->tau_sigma, pb->tau_epsilon, dt, w->D, w->gll_w, ngll, w->uloc, w->zeta,
+                                 w->devp, w->rloc);
+    else
+        gfo_element_residual(pb->n_elem, pb->dxi_dx, pb->jacobian, pb->lambda_, pb->mu_, w->D,
+                             w->gll_w, ngll, w->uloc, w->rloc);
     /* 4. C-PML acceleration contribution, else legacy damping on v (solver.cpp:557-574) */
     if (pb->has_cpml) {
         gfo_cpml_accel_contribution(&w->pml, w->u, w->v, w->gll_w, w->rloc);

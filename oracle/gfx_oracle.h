@@ -111,7 +111,35 @@ typedef struct {
     int snapshot_stride;
     int n_rec_cell;
     const int32_t* rec_cell;    /* [n_rec_cell] local element index */
+    /* shear attenuation by standard linear solids (0 = elastic) */
+    int n_sls;
+    const double* tau_sigma;    /* [E*n_node*n_sls] stress relaxation times (s) */
+    const double* tau_epsilon;  /* [E*n_node*n_sls] strain relaxation times (s) */
 } gfo_problem;
+
+/* ---- shear attenuation (standard linear solids) ----
+ * Not in the reference (its forward/ solver is elastic; the planned inputs are
+ * per-point tau_sigma, tau_epsilon per mechanism).  mu_ is the relaxed shear
+ * modulus mu_R.  Per mechanism l:
+ *   Y_l = 2 mu_R (tau_eps_l / tau_sig_l - 1),  mu_U = mu_R (1 + sum_l (tau_eps_l/tau_sig_l - 1)),
+ *   lambda_U = lambda_R + 2 (mu_R - mu_U) / 3   (bulk modulus unchanged),
+ *   dz_l/dt = (Y_l dev(eps) - z_l) / tau_sig_l   (z_l: traceless symmetric memory tensor),
+ *   sigma = lambda_U tr(eps) I + 2 mu_U eps - sum_l z_l.
+ * Over a step the ODE is integrated exactly for strain linear in time between
+ * the two levels: z^{n+1} = a z^n + Y (b dev^n + c dev^{n+1}),
+ *   a = exp(-dt/tau), b = tau (1 - a)/dt - a, c = 1 - tau (1 - a)/dt.
+ * Memory per point: z [n_sls][5] (xx yy xy xz yz; zz = -xx - yy) and the
+ * previous deviatoric strain [5]. */
+void gfo_sls_coefficients(double tau_sigma, double tau_epsilon, double mu_relaxed, double dt, double* abcY);
+void gfo_element_residual_att(int n_elem, const double* dxi_dx, const double* jacobian, const double* lambda_,
+                              const double* mu_, int n_sls, const double* tau_sigma, const double* tau_epsilon,
+                              double dt, const double* D, const double* weights, int ngll, const double* u,
+                              double* zeta, double* dev_prev, double* r);
+/* Stress response of one point to a deviatoric strain history dev[n][5]
+ * (dev[0] is the level before the first step): sig[n][5] deviatoric stress
+ * after each of the n-1 steps, by the same update as the element residual. */
+void gfo_sls_point_history(int n_sls, const double* tau_sigma, const double* tau_epsilon, double mu_relaxed,
+                           double dt, int n, const double* dev, double* sig);
 
 /* Runs nsteps steps.  Outputs (any may be NULL):
  *   rec_strain [n_snap][n_rec_cell][n_node][6], rec_disp/vel/acc [n_snap][n_rec_cell][n_node][3]

@@ -27,6 +27,7 @@ class Problem(C.Structure):
         ("direction", C.c_int), ("nsteps", C.c_int), ("dt", C.c_double),
         ("n_stf", C.c_int), ("stf", _dp), ("snapshot_stride", C.c_int),
         ("n_rec_cell", C.c_int), ("rec_cell", _ip),
+        ("n_sls", C.c_int), ("tau_sigma", _dp), ("tau_epsilon", _dp),
     ]
 
 
@@ -59,6 +60,8 @@ def _load():
         "gfo_full_strain": (None, [C.c_int, _ip, _dp, _dp, C.c_int, _dp, _dp]),
         "gfo_run": (C.c_int, [C.POINTER(Problem), _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
         "gfo_run_state": (C.c_int, [C.POINTER(Problem), _dp, _dp, _dp]),
+        "gfo_sls_coefficients": (None, [C.c_double, C.c_double, C.c_double, C.c_double, _dp]),
+        "gfo_sls_point_history": (None, [C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_int, _dp, _dp]),
         "gfo_random_fill": (None, [_dp, C.c_int, C.c_uint]),
         "gfo_set_threads": (C.c_int, [C.c_int]),
     }
@@ -121,7 +124,7 @@ class Run:
     """One forward run of the oracle on mesh arrays (reference layout)."""
 
     def __init__(self, arrays, n_elem, ngll, n_node, dt, nsteps, snapshot_stride, direction,
-                 src_cells, src_weights, stf, rec_cell, has_cpml=True):
+                 src_cells, src_weights, stf, rec_cell, has_cpml=True, tau_sigma=None, tau_epsilon=None):
         k = self._k = {}
 
         def keep(name, x, dt_=np.float64):
@@ -153,6 +156,11 @@ class Run:
         rc = keep("rec", rec_cell, np.int32)
         pb.n_rec_cell = 0 if rc is None else rc.size
         pb.rec_cell = P(rc, C.c_int32)
+        if tau_sigma is not None:
+            ts = keep("tau_sigma", tau_sigma)
+            pb.n_sls = ts.size // (n_elem * ngll ** 3)
+            pb.tau_sigma = P(ts)
+            pb.tau_epsilon = P(keep("tau_epsilon", tau_epsilon))
         self.pb = pb
         self.nn = ngll ** 3
         self.n_node = n_node
@@ -183,3 +191,20 @@ def set_threads(n):
     """Threads of the oracle's element loop; returns the count in effect."""
     return int(lib.gfo_set_threads(int(n)))
 
+
+
+def sls_coefficients(tau_sigma, tau_epsilon, mu_relaxed, dt):
+    """(a, b, c, Y) of one mechanism's exact step for strain linear in time."""
+    out = np.zeros(4)
+    lib.gfo_sls_coefficients(tau_sigma, tau_epsilon, mu_relaxed, dt, P(out))
+    return out
+
+
+def sls_point_history(tau_sigma, tau_epsilon, mu_relaxed, dt, dev):
+    """Deviatoric stress [n-1, 5] of one point for the strain history dev [n, 5]."""
+    ts = np.ascontiguousarray(tau_sigma, np.float64)
+    te = np.ascontiguousarray(tau_epsilon, np.float64)
+    dev = np.ascontiguousarray(dev, np.float64)
+    sig = np.zeros((len(dev) - 1, 5))
+    lib.gfo_sls_point_history(ts.size, P(ts), P(te), mu_relaxed, dt, len(dev), P(dev), P(sig))
+    return sig
