@@ -37,8 +37,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cfg2_spec(pygfx, nx=100, ny=100, nz=50, h=600.0, duration=30.0):
+def cfg2_spec(pygfx, nx=100, ny=100, nz=50, h=600.0, duration=30.0, n_sls=0):
+    """The layered 100 x 100 x 50 benchmark box; n_sls > 0 adds shear attenuation
+    (Q_mu 60 / 120 / 250 by layer over 0.05-2 Hz)."""
     return pygfx.make_spec(
+        n_sls=n_sls, q_mu=[60.0, 120.0, 250.0], att_fmin=0.05, att_fmax=2.0,
         nx=nx, ny=ny, nz=nz, lx=h * nx, ly=h * ny, lz=h * nz,
         layers=[(3000.0, 5000.0, 2900.0, 2500.0), (20000.0, 6200.0, 3600.0, 2800.0),
                 (0.0, 8000.0, 4600.0, 3300.0)],
@@ -330,6 +333,8 @@ def main():
     ap.add_argument("--no-uneven", action="store_true", help="skip the 16-mesh uneven batch line")
     ap.add_argument("--general-geometry", action="store_true",
                     help="run every element on the general (full dxi/dx) form, no axis-aligned path")
+    ap.add_argument("--attenuation", type=int, default=0, metavar="N_SLS",
+                    help="shear attenuation with this many standard linear solids (step benchmark only)")
     ap.add_argument("--surrogate", action="store_true",
                     help="instead of the step benchmark: Green tile of the layered mesh (x, y, z forces at the "
                          "surface centre, cells to 1.8 km, every snapshot), MLP training, 1M scored queries")
@@ -383,7 +388,7 @@ def main():
 
     t0 = time.perf_counter()
     dims = (40, 40, 20) if args.small else (100, 100, 50)
-    mesh = pygfx.build_box(cfg2_spec(pygfx, *dims), copy=False)
+    mesh = pygfx.build_box(cfg2_spec(pygfx, *dims, n_sls=args.attenuation), copy=False)
     log(f"mesh: {mesh.n_elem} elements, {mesh.n_node} nodes, {mesh.n_pml_elem} PML elements, "
         f"{len(mesh.rec_cell)} recording cells, dt={mesh.dt:.5g} s, stride={mesh.snapshot_stride}, "
         f"nsteps={mesh.nsteps} ({time.perf_counter() - t0:.1f} s)")
@@ -468,7 +473,8 @@ def main():
                    "recording_cells": int(len(mesh.rec_cell)), "snapshot_stride": mesh.snapshot_stride,
                    "dt_s": mesh.dt, "parallelism": "none (1 GPU)",
                    "launch": launch,
-                   "axis_aligned_elements_on_diagonal_form": int(sb.query("diag_elements"))},
+                   "axis_aligned_elements_on_diagonal_form": int(sb.query("diag_elements")),
+                   "attenuation_solids": int(sb.query("n_sls"))},
         "node_updates_per_s": G * F * args.steps / tb["wall_s"],
         "plain": plain,
         "batched_vs_plain": (value / plain["value"]) if plain else None,

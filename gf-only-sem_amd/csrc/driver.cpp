@@ -271,6 +271,13 @@ gfx_solver_desc solver_desc(RunInputs& in, int F, int record_capacity, bool reco
     d.assembly = assembly < 0 ? GFX_ASSEMBLY_COLORED : assembly;
     d.element_kernel = element_kernel;
     d.elems_per_block = elems_per_block;
+    if (part.n_sls > 0) {  // shear attenuation solids in the partitions
+        require(part.tau_sigma.size() == in.P * size_t(part.n_sls) && part.tau_epsilon.size() == part.tau_sigma.size(),
+                "attenuation arrays do not match n_elem x ngll^3 x n_sls");
+        d.n_sls = part.n_sls;
+        d.tau_sigma = part.tau_sigma.data();
+        d.tau_epsilon = part.tau_epsilon.data();
+    }
     if (assembly < 0) {
         gfx_solver_desc best{};
         double ms = 0;
@@ -1158,6 +1165,8 @@ extern "C" gfx_status gfx_mesh_read_inputs(const char* dir, gfx_mesh** out) {
             H->coef_abar = std::move(p.coef_abar);
         }
         H->rec_cell = std::move(p.recording.rec_cell_local);
+        H->tau_sigma = std::move(p.tau_sigma);
+        H->tau_epsilon = std::move(p.tau_epsilon);
         H->node_xyz.assign(size_t(p.n_rank_node) * 3, 0.0);
         for (size_t q = 0; q < P; ++q)
             for (int d = 0; d < 3; ++d) H->node_xyz[size_t(H->ibool[q]) * 3 + d] = H->coords[q * 3 + d];
@@ -1184,6 +1193,7 @@ extern "C" gfx_status gfx_mesh_read_inputs(const char* dir, gfx_mesh** out) {
         m.nsteps = cfg.nsteps;
         m.snapshot_stride = cfg.snapshot_stride;
         m.record_depth_actual = cfg.record_depth_actual_m;
+        m.n_sls = H->tau_sigma.empty() ? 0 : p.n_sls;
         m.n_pml_elem = 0;
         for (int32_t r : H->pml_region) m.n_pml_elem += r != 0;
         wire_mesh(*H);

@@ -58,6 +58,15 @@ Partition read_partition(const std::string& path, const ReadOpts& o) {
     }
     p.coef_abar = h5::try_read<double>(id, "/field/cell/pml_coef_abar");
     p.has_cpml = !p.coef_abar.empty();  // io.cpp:237
+    if (h5::exists(id, "/field/cell/tau_sigma")) {
+        std::vector<h5::hsize_t> dims;
+        p.tau_sigma = h5::read<double>(id, "/field/cell/tau_sigma", &dims);
+        if (dims.size() != 5) throw Error(GFX_ERR_IO, path + ": /field/cell/tau_sigma must be [n_cell, n, n, n, n_sls]");
+        p.n_sls = int(dims[4]);
+        p.tau_epsilon = h5::read<double>(id, "/field/cell/tau_epsilon");
+        if (p.tau_epsilon.size() != p.tau_sigma.size())
+            throw Error(GFX_ERR_IO, path + ": tau_epsilon differs in size from tau_sigma");
+    }
     p.local_cell2rank_node = h5::try_read<int32_t>(id, "/field/cell/local_cell2rank_node");
     p.local_cell2global_node = h5::try_read<int32_t>(id, "/field/cell/local_cell2global_node");
     if (h5::exists(id, "/field/cell")) {
@@ -121,6 +130,9 @@ Partition read_partition_all(const std::string& dir, const ReadOpts& o) {
         cat(m.coef_beta, part.coef_beta);
         cat(m.coef_abar, part.coef_abar);
         cat(m.coef_strain, part.coef_strain);
+        if (part.n_sls != m.n_sls) throw Error(GFX_ERR_IO, "partitions differ in their number of attenuation solids");
+        cat(m.tau_sigma, part.tau_sigma);
+        cat(m.tau_epsilon, part.tau_epsilon);
         m.has_cpml = m.has_cpml || part.has_cpml;
         cat(m.local_cell2rank_node, part.local_cell2global_node);
         if (part.recording.has_recording) {
@@ -393,6 +405,11 @@ void write_inputs(const gfx_mesh& m, const std::string& dir, int n_parts, const 
             if (m.coef_beta)
                 h5::write_vec(id, "/field/cell/pml_coef_beta", slice(m.coef_beta, 9), {hn, hsize_t(NP), 9}, gz);
             h5::write_vec(id, "/field/cell/pml_coef_abar", slice(m.coef_abar, 5), {hn, hsize_t(NP), 5}, gz);
+        }
+        if (m.n_sls > 0 && m.tau_sigma && m.tau_epsilon) {
+            const hsize_t L = hsize_t(m.n_sls);
+            h5::write_vec(id, "/field/cell/tau_sigma", slice(m.tau_sigma, m.n_sls), {hn, hg, hg, hg, L}, gz);
+            h5::write_vec(id, "/field/cell/tau_epsilon", slice(m.tau_epsilon, m.n_sls), {hn, hg, hg, hg, L}, gz);
         }
         // node numbering: global ids, and a compact per-partition numbering
         std::vector<int32_t> gib(m.ibool + int64_t(b) * NP, m.ibool + int64_t(e) * NP), rib(gib.size());

@@ -33,6 +33,10 @@ struct Partition {
     std::vector<double> coords, jacobian, dxi_dx, mass, vp, vs, density, lambda_, mu_, damping;
     std::vector<int32_t> pml_region;
     std::vector<double> coef_alpha, coef_beta, coef_abar, coef_strain;
+    // shear attenuation (planned in the reference, not read by its solver):
+    // /field/cell/tau_sigma, tau_epsilon [n_cell, ngll, ngll, ngll, n_sls]
+    int n_sls = 0;
+    std::vector<double> tau_sigma, tau_epsilon;
     bool has_cpml = false;
     std::vector<int32_t> local_cell2rank_node;    // after the merge: global node ids
     std::vector<int32_t> local_cell2global_node;  // cleared by the merge (moved into the above)

@@ -1548,6 +1548,13 @@ extern "C" double gfx_solver_algorithmic_bytes(const gfx_solver* s, int which) {
     const double Pd = double(s->n_diag) * NP;
     double elem = 48.0 * (P - Pd) + 24.0 * Pd + F * 24.0 * G;
     if (s->has_cpml && !s->pml_nodal) elem += 56.0 * Q + F * (96.0 * Q + 36.0 * Gp);
+    // shear attenuation: the general form (48 B) at every point, 3 step
+    // coefficients per solid per point, and per field the memory (5 per solid)
+    // and previous deviatoric strain (5) read and written at every point
+    if (s->n_sls > 0) {
+        const double L = s->n_sls;
+        elem = 48.0 * P + F * 24.0 * G + 12.0 * L * P + F * 8.0 * (5.0 * L + 5.0) * P;
+    }
     // node stage: 1/m per node; per field r, u~, w read and u~, w written;
     // special nodes also read and write a; C-PML nodes read 5 + 9 coefficients
     // and per field read and write the memory (9) and "new" (3) values.

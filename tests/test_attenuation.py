@@ -93,3 +93,17 @@ def test_box_generator_assigns_solids_per_layer():
     fit30 = pygfx.sls_fit(30.0, 0.1, 5.0, 3)
     assert np.allclose(ts[top], fit30[0]) and np.allclose(te[top], fit30[1])
     assert np.array_equal(ts[deep], te[deep])
+
+
+@pytest.mark.skipif(not pygfx.h5_available(), reason="no HDF5 library to bind")
+def test_solids_round_trip_through_the_partitions(tmp_path):
+    m = mc.small_halfspace(nx=6, ny=6, nz=4, pml=1, duration=0.5, n_sls=3, q_mu=[30.0], att_fmin=0.1,
+                           att_fmax=4.0)
+    stf = pygfx.ricker(1.0, 0.5, 1e20, m.dt, m.nsteps)
+    d = str(tmp_path)
+    pygfx.write_inputs(m, d, (3000.0, 3000.0, 0.0), stf, n_parts=2)
+    lay = pygfx.h5_layout(str(tmp_path / "partitions" / "partition_0.h5"))
+    assert lay["/field/cell/tau_sigma"] == ("f64", (72, 5, 5, 5, 3))
+    back = pygfx.read_inputs(d)
+    assert back.n_sls == 3
+    assert np.array_equal(back.tau_sigma, m.tau_sigma) and np.array_equal(back.tau_epsilon, m.tau_epsilon)

@@ -92,3 +92,23 @@ def test_checkpoint_carries_the_memory():
     e = gpu_run(mesh, (cells, w, stf, 1), nsteps=0, attenuation=False)
     with pytest.raises(pygfx.GfxError, match="attenuation"):
         e.restore(image)
+
+
+@pytest.mark.skipif(not pygfx.h5_available(), reason="no HDF5 library to bind")
+def test_forward_driver_reads_the_solids(tmp_path):
+    mesh = mc.small_halfspace(nx=8, ny=8, nz=5, duration=1.0, n_sls=3, q_mu=[25.0], att_fmin=0.1, att_fmax=4.0)
+    xyz = (0.5 * (mesh.bounds[0] + mesh.bounds[1]), 0.5 * (mesh.bounds[2] + mesh.bounds[3]), 0.0)
+    stf = pygfx.ricker(1.0, 0.6, 1e20, mesh.dt, mesh.nsteps)
+    d = str(tmp_path)
+    pygfx.write_inputs(mesh, d, xyz, stf, precision="float64")
+    pygfx.run_forward(d, directions="z", assembly=pygfx.GFX_ASSEMBLY_GATHER, restart_stride=0)
+    back = pygfx.read_inputs(d)
+    cells, w = back.locate_source(*xyz)
+    s = gpu_run(back, (cells, w, stf, 2))
+    steps = [k for k in range(mesh.nsteps) if k % mesh.snapshot_stride == 0]
+    rec = np.stack([pygfx.h5_read(str(tmp_path / "wavefields" / "z" / f"record_0_{k}.h5"), "/strain")
+                    for k in steps])
+    mine = records(s).reshape(rec.shape)
+    assert mc.rel_l2(rec, mine) <= 1e-6
+    ela = gpu_run(back, (cells, w, stf, 2), attenuation=False)
+    assert mc.rel_l2(records(ela).reshape(rec.shape), rec) > 1e-3
