@@ -433,10 +433,13 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
 // Stress with shear attenuation at one point of field f (general geometry):
 // strain from the gradient, the memory update of every mechanism
 // z <- a z + (Y b) dev_old + (Y c) dev_new, sigma = lam_U tr I + 2 mu_U eps -
-// sum z, and the fluxes F = sigma . g.
+// sum z, and the fluxes F = sigma . g.  L > 0: L solids whose coefficients
+// the caller holds in ca, cb, cc; L = 0: A.n_sls solids read from memory.
+template <int L>
 __device__ __forceinline__ void flux_att(const float (&ui)[3][5], const float (&uj)[3][5], const float (&uk)[3][5],
                                          const float* Di, const float* Dj, const float* Dk, const float* g,
-                                         float lam, float mu, const ElemArgs& A, int f, int64_t pt, float* F) {
+                                         float lam, float mu, const ElemArgs& A, int f, int64_t pt, const float* ca,
+                                         const float* cb, const float* cc, float* F) {
     float du[3][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -464,16 +467,31 @@ __device__ __forceinline__ void flux_att(const float (&ui)[3][5], const float (&
 #pragma unroll
     for (int q = 0; q < 5; ++q) old[q] = dp[int64_t(q) * A.P];
     float zs[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
-    float* zf = A.zeta + int64_t(f) * A.n_sls * 5 * A.P + pt;
-    for (int l = 0; l < A.n_sls; ++l) {
-        const float a = A.att_a[int64_t(l) * A.P + pt], b = A.att_b[int64_t(l) * A.P + pt],
-                    c = A.att_c[int64_t(l) * A.P + pt];
+    const int nl = L > 0 ? L : A.n_sls;
+    float* zf = A.zeta + int64_t(f) * nl * 5 * A.P + pt;
+    if constexpr (L > 0) {
+        float zo[L * 5];
 #pragma unroll
-        for (int q = 0; q < 5; ++q) {
-            float* zq = zf + int64_t(l * 5 + q) * A.P;
-            const float z = a * zq[0] + b * old[q] + c * dev[q];
-            zq[0] = z;
-            zs[q] += z;
+        for (int q = 0; q < L * 5; ++q) zo[q] = zf[int64_t(q) * A.P];  // all loads first
+#pragma unroll
+        for (int l = 0; l < L; ++l)
+#pragma unroll
+            for (int q = 0; q < 5; ++q) {
+                const float z = ca[l] * zo[l * 5 + q] + cb[l] * old[q] + cc[l] * dev[q];
+                zf[int64_t(l * 5 + q) * A.P] = z;
+                zs[q] += z;
+            }
+    } else {
+        for (int l = 0; l < nl; ++l) {
+            const float a = A.att_a[int64_t(l) * A.P + pt], b = A.att_b[int64_t(l) * A.P + pt],
+                        c = A.att_c[int64_t(l) * A.P + pt];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) {
+                float* zq = zf + int64_t(l * 5 + q) * A.P;
+                const float z = a * zq[0] + b * old[q] + c * dev[q];
+                zq[0] = z;
+                zs[q] += z;
+            }
         }
     }
 #pragma unroll
@@ -494,8 +512,9 @@ __device__ __forceinline__ void flux_att(const float (&ui)[3][5], const float (&
 }
 
 // Block kernel with shear attenuation: one element per workgroup, one lane per
-// GLL point, general geometry, nodal C-PML only.
-template <int MODE>
+// GLL point, general geometry, nodal C-PML only.  L > 0 solids keep their step
+// coefficients in registers for all fields; L = 0 reads A.n_sls solids' from memory.
+template <int MODE, int L>
 __global__ void __launch_bounds__(128) k_element_att(ElemArgs A, int e0, int ne) {
     __shared__ float sD[25];
     __shared__ float su[3][NP];
@@ -529,6 +548,15 @@ __global__ void __launch_bounds__(128) k_element_att(ElemArgs A, int e0, int ne)
         se = A.src_off[e + 1];
     }
     const int step = A.ctr[0];
+    constexpr int LC = L > 0 ? L : 1;
+    float ca[LC], cb[LC], cc[LC];
+#pragma unroll
+    for (int l = 0; l < LC; ++l) {
+        const bool have = L > 0 && valid;
+        ca[l] = have ? A.att_a[int64_t(l) * A.P + pt] : 0.f;
+        cb[l] = have ? A.att_b[int64_t(l) * A.P + pt] : 0.f;
+        cc[l] = have ? A.att_c[int64_t(l) * A.P + pt] : 0.f;
+    }
     __syncthreads();
     float Di[5], Dj[5], Dk[5], Dti[5], Dtj[5], Dtk[5];
 #pragma unroll
@@ -567,7 +595,7 @@ __global__ void __launch_bounds__(128) k_element_att(ElemArgs A, int e0, int ne)
                     uk[c][s] = su[c][i * 25 + j * 5 + s];
                 }
             float F[9];
-            flux_att(ui, uj, uk, Di, Dj, Dk, g, lam, mu, A, f, pt, F);
+            flux_att<L>(ui, uj, uk, Di, Dj, Dk, g, lam, mu, A, f, pt, ca, cb, cc, F);
 #pragma unroll
             for (int q = 0; q < 9; ++q) sf[q][p] = F[q];
         }

@@ -281,14 +281,26 @@ void launch_element_d(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
     }
 }
 
+template <int MODE>
+void launch_att(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
+    switch (S.n_sls) {
+        case 1: dev::k_element_att<MODE, 1><<<n, 128, 0, S.stream>>>(a, b, n); break;
+        case 2: dev::k_element_att<MODE, 2><<<n, 128, 0, S.stream>>>(a, b, n); break;
+        case 3: dev::k_element_att<MODE, 3><<<n, 128, 0, S.stream>>>(a, b, n); break;
+        case 4: dev::k_element_att<MODE, 4><<<n, 128, 0, S.stream>>>(a, b, n); break;
+        default: dev::k_element_att<MODE, 0><<<n, 128, 0, S.stream>>>(a, b, n); break;
+    }
+}
+
 void launch_element(gfx_solver& S, const dev::ElemArgs& a, const ElemRange& rg) {
     const int b = rg.b, n = rg.e - rg.b;
     if (S.n_sls > 0) {  // shear attenuation: one kernel form for every range
-        if (n <= 0) return;
-        switch (S.assembly) {
-            case GFX_ASSEMBLY_GATHER: dev::k_element_att<dev::ASM_GATHER><<<n, 128, 0, S.stream>>>(a, b, n); break;
-            case GFX_ASSEMBLY_ATOMIC: dev::k_element_att<dev::ASM_ATOMIC><<<n, 128, 0, S.stream>>>(a, b, n); break;
-            default: dev::k_element_att<dev::ASM_COLORED><<<n, 128, 0, S.stream>>>(a, b, n); break;
+        if (n > 0) {
+            switch (S.assembly) {
+                case GFX_ASSEMBLY_GATHER: launch_att<dev::ASM_GATHER>(S, a, b, n); break;
+                case GFX_ASSEMBLY_ATOMIC: launch_att<dev::ASM_ATOMIC>(S, a, b, n); break;
+                default: launch_att<dev::ASM_COLORED>(S, a, b, n); break;
+            }
         }
         return;
     }

@@ -37,13 +37,15 @@ def records(s):
     return np.stack([s.get_record(0, k) for k in range(s.n_snapshots)])
 
 
-@pytest.mark.parametrize("assembly", [pygfx.GFX_ASSEMBLY_GATHER, pygfx.GFX_ASSEMBLY_COLORED,
-                                      pygfx.GFX_ASSEMBLY_ATOMIC])
-def test_attenuated_solver_matches_oracle(assembly):
-    mesh = att_mesh(20.0)
+@pytest.mark.parametrize("assembly,n_sls", [(pygfx.GFX_ASSEMBLY_GATHER, 3), (pygfx.GFX_ASSEMBLY_COLORED, 3),
+                                            (pygfx.GFX_ASSEMBLY_ATOMIC, 3), (pygfx.GFX_ASSEMBLY_GATHER, 1),
+                                            (pygfx.GFX_ASSEMBLY_GATHER, 5)])
+def test_attenuated_solver_matches_oracle(assembly, n_sls):
+    """n_sls 1-4 hold the solids' coefficients in registers, 5-8 read them per field."""
+    mesh = att_mesh(20.0, n_sls=n_sls)
     cells, w, stf = mc.buried_source(mesh)
     s = gpu_run(mesh, (cells, w, stf, 2), assembly=assembly)
-    assert int(s.query("n_sls")) == 3
+    assert int(s.query("n_sls")) == n_sls
     ref = pyoracle.Run(mesh.arrays, mesh.n_elem, 5, mesh.n_node, mesh.dt, mesh.nsteps, mesh.snapshot_stride, 2,
                        cells, w, stf, mesh.rec_cell, tau_sigma=mesh.tau_sigma,
                        tau_epsilon=mesh.tau_epsilon).run()
