@@ -163,15 +163,15 @@ def traffic_from_profiles(kernel_prefix, launch, fields, dims):
     return same or other or (None, None)
 
 
-def uneven_specs(pygfx, n=16, seed=4, h=600.0):
-    """16 independent boxes (seeded): n_target log-uniform in [8k, 200k] elements,
+def uneven_specs(pygfx, n=16, seed=4, h=600.0, lo=8e3, hi=2e5):
+    """n independent boxes (seeded): n_target log-uniform in [lo, hi] elements,
     nx = ny = round((2n)^(1/3)), nz = round(nx/2), 0.6 km hexes; 2-4 layers with
     Vs ~ U(1500, 4500) increasing with depth, Vp = 1.73 Vs, rho ~ U(2300, 3300);
     C-PML U{3..6}; run length U(5, 30) s; Ricker f0 = 1.5 Hz."""
     rng = np.random.default_rng(seed)
     specs = []
     for _ in range(n):
-        nt = float(np.exp(rng.uniform(np.log(8e3), np.log(2e5))))
+        nt = float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
         nx = int(round((2 * nt) ** (1 / 3)))
         nz = max(4, int(round(nx / 2)))
         nl = int(rng.integers(2, 5))
@@ -190,10 +190,13 @@ def uneven_specs(pygfx, n=16, seed=4, h=600.0):
     return specs
 
 
-def uneven_workload(pygfx, steps, warmup):
+def uneven_workload(pygfx, steps, warmup, group_counts=(0, 1, 2, 4)):
     """x, y and z forces at the surface centre of each of 16 uneven meshes:
-    the meshes advanced together (one captured graph with a branch per mesh)
-    against the same solvers advanced one after another, kernel by kernel."""
+    the meshes advanced together as one captured graph per step, for each
+    number of launch groups in group_counts (0: one branch of launches per
+    mesh; k: k branches, each sharing one launch per stage among its meshes),
+    against the same solvers advanced one after another, kernel by kernel.
+    value is the fastest batched form."""
     t0 = time.perf_counter()
     meshes = [pygfx.build_box(sp, copy=False) for sp in uneven_specs(pygfx)]
     solvers = []
@@ -216,18 +219,24 @@ def uneven_workload(pygfx, steps, warmup):
     for s in solvers:
         s.run(steps)
     t_seq = time.perf_counter() - t0
-    batch = pygfx.Batch(solvers)
-    batch.run(warmup)
-    t0 = time.perf_counter()
-    batch.run(steps)
-    t_bat = time.perf_counter() - t0
-    batch.close()
+    times = {}
+    for groups in group_counts:
+        batch = pygfx.Batch(solvers, groups=groups)
+        batch.run(warmup)
+        t0 = time.perf_counter()
+        batch.run(steps)
+        times[groups] = time.perf_counter() - t0
+        assert batch.last_groups == groups
+        batch.close()
+    best = min(times, key=times.get)
+    t_bat = times[best]
     pts = E * 125 * 3 * steps
     out = {"meshes": len(meshes), "elements": E, "fields_per_mesh": 3, "steps": steps,
            "elements_min": min(m.n_elem for m in meshes), "elements_max": max(m.n_elem for m in meshes),
-           "value": pts / t_bat, "unit": "GLL-point updates/s",
-           "ms_per_step_batched": 1e3 * t_bat / steps, "ms_per_step_sequential": 1e3 * t_seq / steps,
-           "speedup_vs_sequential": t_seq / t_bat}
+           "value": pts / t_bat, "unit": "GLL-point updates/s", "launch_groups": best,
+           "ms_per_step_batched": 1e3 * t_bat / steps,
+           "ms_per_step_by_launch_groups": {str(g): 1e3 * t / steps for g, t in times.items()},
+           "ms_per_step_sequential": 1e3 * t_seq / steps, "speedup_vs_sequential": t_seq / t_bat}
     for s in solvers:
         s.close()
     for m in meshes:

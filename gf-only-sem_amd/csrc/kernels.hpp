@@ -291,17 +291,25 @@ __global__ void k_advance(int* ctr, int stride) {
     }
 }
 
+// k_advance for several solvers in one launch (lane i: solver i).
+__global__ void k_advance_grp(int* const* __restrict__ ctrs, const int* __restrict__ strides, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int* ctr = ctrs[i];
+    const int st = ctr[0];
+    if (strides[i] > 0 && st % strides[i] == 0) ctr[1] = ctr[1] + 1;
+    ctr[0] = st + 1;
+}
+
 // One lane per GLL point, EPB elements per workgroup, workgroup barriers
 // between the stages.  Supports the element-local C-PML fallback.
 template <int EPB, int MODE, bool PML_ELEM, bool DIAG>
-__global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
-    k_element(ElemArgs A, int e0, int ne) {
+__device__ __forceinline__ void element_body(const ElemArgs& A, int e0, int ne, int blk) {
     __shared__ float sD[25];
     __shared__ float su[EPB][3][NP];
     __shared__ float sf[EPB][9][NP];
     const int tid = threadIdx.x;
     if (tid < 25) sD[tid] = A.D[tid];
-    const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int le = tid / NP;
     const int p = tid - le * NP;
     const int eloc = blk * EPB + le;
@@ -428,6 +436,36 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
             emit_residual<MODE>(A, f, node, rslot, rold, rv);
         }
     }
+}
+
+template <int EPB, int MODE, bool PML_ELEM, bool DIAG>
+__global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
+    k_element(ElemArgs A, int e0, int ne) {
+    element_body<EPB, MODE, PML_ELEM, DIAG>(A, e0, ne, xcd_block(blockIdx.x, gridDim.x));
+}
+
+// Grouped launches cover several solvers: workgroup b of such a launch reads
+// its work from map[b] = (solver, first element of its range, elements in the
+// range, workgroup within the range), or (solver, -, -, workgroup within the
+// solver's node launch) for the node kernels, and the solver's arguments from
+// tab[solver].  One table read per workgroup, no search.
+
+// Row i of an argument table, read through the constant address space: the
+// table does not change during a launch, and pointers read from constant
+// memory are known to address global memory, so the kernel bodies issue the
+// same global (not flat) loads and stores as with by-value arguments.
+template <class T>
+__device__ __forceinline__ const T& const_row(const T* tab, int i) {
+    using CT = const __attribute__((address_space(4))) T;
+    return *(const T*)((CT*)tab + i);
+}
+
+// One launch for the element ranges of several solvers (nodal C-PML only).
+template <int EPB, int MODE, bool DIAG>
+__global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
+    k_element_grp(const ElemArgs* __restrict__ tab, const int4* __restrict__ map) {
+    const int4 E = map[xcd_block(blockIdx.x, gridDim.x)];
+    element_body<EPB, MODE, false, DIAG>(const_row(tab, E.x), E.y, E.z, E.w);
 }
 
 // Stress with shear attenuation at one point of field f (general geometry):
@@ -895,7 +933,7 @@ __global__ void __launch_bounds__(256) k_element_pencil(ElemArgs A, int e0, int 
 // exchange between the element's points stays inside one wavefront, so the
 // field loop has no workgroup barrier; WPB elements (wavefronts) per block.
 template <int WPB, int MODE, bool DIAG>
-__global__ void __launch_bounds__(64 * WPB) k_element_wave(ElemArgs A, int e0, int ne) {
+__device__ __forceinline__ void wave_body(const ElemArgs& A, int e0, int ne, int blk) {
     __shared__ __align__(16) float sDr[5][8];  // rows of D, padded
     __shared__ __align__(16) float sDc[5][8];  // columns of D, padded
     __shared__ float su[WPB][3][128];
@@ -907,7 +945,6 @@ __global__ void __launch_bounds__(64 * WPB) k_element_wave(ElemArgs A, int e0, i
     }
     __syncthreads();  // the only workgroup barrier
     const int lane = tid & 63, wv = tid >> 6;
-    const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int eloc = blk * WPB + wv;
     if (eloc >= ne) return;  // wave-uniform
     const int e = e0 + eloc;
@@ -994,6 +1031,19 @@ __global__ void __launch_bounds__(64 * WPB) k_element_wave(ElemArgs A, int e0, i
     }
 }
 
+template <int WPB, int MODE, bool DIAG>
+__global__ void __launch_bounds__(64 * WPB) k_element_wave(ElemArgs A, int e0, int ne) {
+    wave_body<WPB, MODE, DIAG>(A, e0, ne, xcd_block(blockIdx.x, gridDim.x));
+}
+
+// One launch for the element ranges of several solvers, wavefront form.
+template <int WPB, int MODE, bool DIAG>
+__global__ void __launch_bounds__(64 * WPB)
+    k_element_wave_grp(const ElemArgs* __restrict__ tab, const int4* __restrict__ map) {
+    const int4 E = map[xcd_block(blockIdx.x, gridDim.x)];
+    wave_body<WPB, MODE, DIAG>(const_row(tab, E.x), E.y, E.z, E.w);
+}
+
 // Fixed-order sum of a node's element contributions (GATHER assembly).
 __device__ __forceinline__ void gather_r(const NodeArgs& A, int f, int kb, int ke, float* r) {
     r[0] = r[1] = r[2] = 0.f;
@@ -1063,14 +1113,14 @@ __device__ __forceinline__ void staged_sums(const NodeArgs& A, int f, int64_t s0
 // group fy handles fields fy, fy + FG, ... (each wavefront stays on one field
 // group, so its loads remain contiguous over nodes).
 template <int FG>
-__global__ void __launch_bounds__(256) k_node_special(NodeArgs A) {
+__device__ __forceinline__ void node_special_body(const NodeArgs& A, int blk) {
     constexpr int NPB = 256 / FG;
     // one field group: the workgroup's slot sums are staged through LDS
     constexpr bool STAGE = FG == 1;
     __shared__ float sl[STAGE ? SLOT_CHUNK * 3 : 1];
     __shared__ int32_t sp[STAGE ? SLOT_CHUNK : 1];
     const int fy = threadIdx.x / NPB;
-    const int64_t g0 = int64_t(blockIdx.x) * NPB + (threadIdx.x - fy * NPB);
+    const int64_t g0 = int64_t(blk) * NPB + (threadIdx.x - fy * NPB);
     const bool active = g0 < A.Ga;
     const bool staged = STAGE && A.gather;  // uniform over the workgroup
     if (!active && !staged) return;
@@ -1097,7 +1147,7 @@ __global__ void __launch_bounds__(256) k_node_special(NodeArgs A) {
         kb[0] = A.off[g];
         kb[1] = A.off[g + 1];
         if (staged) {
-            const int64_t gb = int64_t(blockIdx.x) * NPB;
+            const int64_t gb = int64_t(blk) * NPB;
             s0 = A.off[gb];
             s1 = A.off[min(A.Ga, gb + NPB)];
             ids = stage_points(A, s0, s1, sp);
@@ -1173,16 +1223,26 @@ __global__ void __launch_bounds__(256) k_node_special(NodeArgs A) {
     }
 }
 
+template <int FG>
+__global__ void __launch_bounds__(256) k_node_special(NodeArgs A) { node_special_body<FG>(A, blockIdx.x); }
+
+template <int FG>
+__global__ void __launch_bounds__(256)
+    k_node_special_grp(const NodeArgs* __restrict__ tab, const int4* __restrict__ map) {
+    const int4 E = map[blockIdx.x];
+    node_special_body<FG>(const_row(tab, E.x), E.w);
+}
+
 // Regular nodes [Ga, G4): a = r/m, w += dt a, u~ += dt w.  4 nodes per lane,
 // 16-byte accesses (Ga and the field stride are multiples of 4 nodes).
 template <bool GATHER, int FG>
-__global__ void __launch_bounds__(256) k_node_regular(NodeArgs A) {
+__device__ __forceinline__ void node_regular_body(const NodeArgs& A, int blk) {
     constexpr int NPB = 256 / FG;
     constexpr bool STAGE = GATHER && FG == 1;  // slot sums staged through LDS
     __shared__ float sl[STAGE ? SLOT_CHUNK * 3 : 1];
     __shared__ int32_t sp[STAGE ? SLOT_CHUNK : 1];
     const int fy = threadIdx.x / NPB;
-    const int64_t t = int64_t(blockIdx.x) * NPB + (threadIdx.x - fy * NPB);
+    const int64_t t = int64_t(blk) * NPB + (threadIdx.x - fy * NPB);
     const int64_t g0 = A.Ga + 4 * t;
     const bool active = g0 < A.G4;
     if (!active && !STAGE) return;
@@ -1197,7 +1257,7 @@ __global__ void __launch_bounds__(256) k_node_regular(NodeArgs A) {
 #pragma unroll
         for (int q = 0; q < 5; ++q) kb[q] = A.off[g + q];
         if (STAGE) {
-            const int64_t gb = A.Ga + int64_t(blockIdx.x) * NPB * 4;
+            const int64_t gb = A.Ga + int64_t(blk) * NPB * 4;
             s0 = A.off[gb];
             s1 = A.off[min(A.G4, gb + int64_t(NPB) * 4)];
             ids = stage_points(A, s0, s1, sp);
@@ -1247,6 +1307,18 @@ __global__ void __launch_bounds__(256) k_node_regular(NodeArgs A) {
             W4[q] = make_float4(w[4 * q + 0], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
         }
     }
+}
+
+template <bool GATHER, int FG>
+__global__ void __launch_bounds__(256) k_node_regular(NodeArgs A) {
+    node_regular_body<GATHER, FG>(A, blockIdx.x);
+}
+
+template <bool GATHER, int FG>
+__global__ void __launch_bounds__(256)
+    k_node_regular_grp(const NodeArgs* __restrict__ tab, const int4* __restrict__ map) {
+    const int4 E = map[blockIdx.x];
+    node_regular_body<GATHER, FG>(const_row(tab, E.x), E.w);
 }
 
 // Fill kernel (used before atomic assembly; no memset nodes in a step).

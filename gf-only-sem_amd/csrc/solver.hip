@@ -1591,18 +1591,40 @@ struct gfx_batch {
     hipStream_t stream = nullptr;
     hipEvent_t fork = nullptr;
     std::vector<hipEvent_t> join;
+    // launch groups when the solvers allow shared launches: 0 = one branch of
+    // launches per solver (default: the branches overlap one mesh's element
+    // stage with another's node stages), k > 0 = k branches of launches shared
+    // by their solvers (fewer graph nodes for many small meshes)
+    int want_groups = 0;
+    int last_groups = 0;
+    // One branch of shared launches: argument tables and entries its kernels read.
+    struct Group {
+        std::vector<int> members;  // solver indices
+        gfx::DevBuf<gfx::dev::ElemArgs> etab;
+        gfx::DevBuf<gfx::dev::NodeArgs> ntab;
+        // per-workgroup work of each launch (general and diagonal element
+        // forms, special and regular nodes)
+        gfx::DevBuf<int4> map_gen, map_diag, map_spec, map_reg;
+        gfx::DevBuf<int*> ctrs;
+        gfx::DevBuf<int> strides;
+        // launch shape: workgroups per launch, kernel form
+        int bg = 0, bd = 0, bs = 0, br = 0, n_ctr = 0;
+        int fg = 1, kernel = 0, epb = 1;
+    };
     struct Cached {
         std::vector<char> active;
         std::vector<int> gens;
+        int groups = 0;
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
+        std::vector<std::unique_ptr<Group>> grp;
     };
-    std::vector<Cached> cache;
+    std::vector<std::unique_ptr<Cached>> cache;
     ~gfx_batch() {
         if (stream) (void)hipStreamSynchronize(stream);
         for (auto& c : cache) {
-            if (c.exec) (void)hipGraphExecDestroy(c.exec);
-            if (c.graph) (void)hipGraphDestroy(c.graph);
+            if (c->exec) (void)hipGraphExecDestroy(c->exec);
+            if (c->graph) (void)hipGraphDestroy(c->graph);
         }
         for (auto e : join)
             if (e) (void)hipEventDestroy(e);
@@ -1614,17 +1636,180 @@ struct gfx_batch {
 namespace gfx {
 namespace {
 
+// Can the active solvers share one launch per stage?  GATHER assembly (no
+// colours, no zero fill), nodal C-PML or none, no attenuation, and one launch
+// shape: the same element kernel form (block or wavefront), elements per
+// workgroup and node field grouping, so that every solver's arithmetic is the
+// one of its own launches.
+bool groupable(const gfx_batch& B, const std::vector<char>& active) {
+    const gfx_solver* first = nullptr;
+    for (size_t i = 0; i < B.s.size(); ++i) {
+        if (!active[i]) continue;
+        const gfx_solver& S = *B.s[i];
+        if (S.assembly != GFX_ASSEMBLY_GATHER || (S.has_cpml && !S.pml_nodal) || S.n_sls > 0) return false;
+        if (S.kernel != GFX_ELEMENT_BLOCK && S.kernel != GFX_ELEMENT_WAVE) return false;
+        if (first && (S.kernel != first->kernel || S.epb != first->epb || S.node_fg != first->node_fg)) return false;
+        if (!first) first = &S;
+    }
+    return first != nullptr;
+}
+
+template <bool DIAG>
+void launch_element_grp(int kernel, int epb, gfx_batch::Group& c, const gfx::DevBuf<int4>& map, int grid,
+                        hipStream_t st) {
+    constexpr int M = dev::ASM_GATHER;
+    if (kernel == GFX_ELEMENT_WAVE) {
+        switch (epb) {
+            case 1: dev::k_element_wave_grp<1, M, DIAG><<<grid, 64, 0, st>>>(c.etab.p, map.p); break;
+            case 2: dev::k_element_wave_grp<2, M, DIAG><<<grid, 128, 0, st>>>(c.etab.p, map.p); break;
+            case 4: dev::k_element_wave_grp<4, M, DIAG><<<grid, 256, 0, st>>>(c.etab.p, map.p); break;
+            default: dev::k_element_wave_grp<8, M, DIAG><<<grid, 512, 0, st>>>(c.etab.p, map.p); break;
+        }
+    } else {
+        constexpr auto T = [](int e) { return ((e * NP + 63) / 64) * 64; };
+        switch (epb) {
+            case 1: dev::k_element_grp<1, M, DIAG><<<grid, T(1), 0, st>>>(c.etab.p, map.p); break;
+            case 2: dev::k_element_grp<2, M, DIAG><<<grid, T(2), 0, st>>>(c.etab.p, map.p); break;
+            case 4: dev::k_element_grp<4, M, DIAG><<<grid, T(4), 0, st>>>(c.etab.p, map.p); break;
+            default: dev::k_element_grp<8, M, DIAG><<<grid, T(8), 0, st>>>(c.etab.p, map.p); break;
+        }
+    }
+}
+
+template <int FG>
+void launch_nodes_grp(gfx_batch::Group& c, hipStream_t st) {
+    if (c.bs > 0) dev::k_node_special_grp<FG><<<c.bs, 256, 0, st>>>(c.ntab.p, c.map_spec.p);
+    if (c.br > 0) dev::k_node_regular_grp<true, FG><<<c.br, 256, 0, st>>>(c.ntab.p, c.map_reg.p);
+}
+
+// Argument tables and entries of a group of solvers, uploaded before the
+// capture that records the group's launches.
+void build_group(gfx_batch& B, gfx_batch::Group& c, const std::vector<StepArgs>& args) {
+    std::vector<dev::ElemArgs> et;
+    std::vector<dev::NodeArgs> nt;
+    std::vector<int4> eg, ed, es, er;  // one entry per workgroup
+    std::vector<int*> ctr;
+    std::vector<int> strides;
+    int fg = 1, kernel = GFX_ELEMENT_BLOCK, epb = 1;
+    for (const int i : c.members) {
+        gfx_solver& S = *B.s[i];
+        const int t = int(et.size());
+        et.push_back(args[i].ea);
+        nt.push_back(args[i].na);
+        fg = S.node_fg;
+        kernel = S.kernel;
+        epb = S.epb;
+        for (const auto& rg : S.ranges) {
+            const int n = rg.e - rg.b;
+            auto& m = rg.diag ? ed : eg;
+            for (int w = 0; w < (n + S.epb - 1) / S.epb; ++w) m.push_back(make_int4(t, rg.b, n, w));
+        }
+        for (int w = 0; w < args[i].spec_grid; ++w) es.push_back(make_int4(t, 0, 0, w));
+        for (int w = 0; w < args[i].reg_grid; ++w) er.push_back(make_int4(t, 0, 0, w));
+        ctr.push_back(S.ctr.p);
+        strides.push_back(S.stride);
+    }
+    auto put = [](auto& buf, const auto& v) {
+        buf.alloc(std::max<size_t>(v.size(), 1));
+        if (!v.empty()) GFX_HIP(hipMemcpy(buf.p, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice));
+    };
+    put(c.etab, et);
+    put(c.ntab, nt);
+    put(c.map_gen, eg);
+    put(c.map_diag, ed);
+    put(c.map_spec, es);
+    put(c.map_reg, er);
+    put(c.ctrs, ctr);
+    put(c.strides, strides);
+    c.bg = int(eg.size());
+    c.bd = int(ed.size());
+    c.bs = int(es.size());
+    c.br = int(er.size());
+    c.n_ctr = int(ctr.size());
+    c.fg = fg;
+    c.kernel = kernel;
+    c.epb = epb;
+}
+
+// One step of a group of solvers as shared launches on stream st: the element
+// ranges of every member (general form, then diagonal form), all special nodes,
+// all regular nodes, then each member's recording and one counter update for all.
+void enqueue_group(gfx_batch::Group& c, const std::vector<StepArgs>& args, hipStream_t st) {
+    if (c.bg > 0) launch_element_grp<false>(c.kernel, c.epb, c, c.map_gen, c.bg, st);
+    if (c.bd > 0) launch_element_grp<true>(c.kernel, c.epb, c, c.map_diag, c.bd, st);
+    switch (c.fg) {
+        case 2: launch_nodes_grp<2>(c, st); break;
+        case 4: launch_nodes_grp<4>(c, st); break;
+        case 8: launch_nodes_grp<8>(c, st); break;
+        default: launch_nodes_grp<1>(c, st); break;
+    }
+    for (const int i : c.members)
+        if (args[i].recording && args[i].rec_grid > 0)
+            dev::k_record<REPB><<<args[i].rec_grid, args[i].rec_threads, 0, st>>>(args[i].ra);
+    dev::k_advance_grp<<<(c.n_ctr + 63) / 64, 64, 0, st>>>(c.ctrs.p, c.strides.p, c.n_ctr);
+    GFX_HIP(hipGetLastError());
+}
+
+// Splits the active solvers into k groups of about equal work (element-field
+// updates per step): largest first, each to the least loaded group.
+std::vector<std::vector<int>> split_groups(const gfx_batch& B, const std::vector<char>& active, int k) {
+    std::vector<int> idx;
+    for (size_t i = 0; i < B.s.size(); ++i)
+        if (active[i]) idx.push_back(int(i));
+    auto work = [&](int i) { return double(B.s[i]->E) * B.s[i]->nf; };
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return work(a) > work(b); });
+    k = std::max(1, std::min<int>(k, int(idx.size())));
+    std::vector<std::vector<int>> g(k);
+    std::vector<double> load(k, 0.0);
+    for (const int i : idx) {
+        const int j = int(std::min_element(load.begin(), load.end()) - load.begin());
+        g[j].push_back(i);
+        load[j] += work(i);
+    }
+    for (auto& m : g) std::sort(m.begin(), m.end());
+    return g;
+}
+
 hipGraphExec_t batch_graph(gfx_batch& B, const std::vector<char>& active) {
     std::vector<int> gens(B.s.size());
     for (size_t i = 0; i < B.s.size(); ++i) gens[i] = B.s[i]->gen;
+    const int groups = (B.want_groups > 0 && groupable(B, active)) ? B.want_groups : 0;
     for (auto& c : B.cache)
-        if (c.active == active && c.gens == gens) return c.exec;
-    gfx_batch::Cached c;
+        if (c->active == active && c->gens == gens && c->groups == groups) {
+            B.last_groups = int(c->grp.size());
+            return c->exec;
+        }
+    auto cp = std::make_unique<gfx_batch::Cached>();
+    gfx_batch::Cached& c = *cp;
     c.active = active;
     c.gens = gens;
+    c.groups = groups;
     std::vector<StepArgs> args(B.s.size());
     for (size_t i = 0; i < B.s.size(); ++i)
         if (active[i]) args[i] = make_step_args(*B.s[i]);
+    if (groups > 0) {
+        for (auto& m : split_groups(B, active, groups)) {
+            c.grp.push_back(std::make_unique<gfx_batch::Group>());
+            c.grp.back()->members = std::move(m);
+            build_group(B, *c.grp.back(), args);
+        }
+        // one branch per group, on the stream of the group's first solver
+        GFX_HIP(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
+        GFX_HIP(hipEventRecord(B.fork, B.stream));
+        for (auto& g : c.grp) {
+            const int lead = g->members.front();
+            hipStream_t st = B.s[lead]->stream;
+            GFX_HIP(hipStreamWaitEvent(st, B.fork, 0));
+            enqueue_group(*g, args, st);
+            GFX_HIP(hipEventRecord(B.join[lead], st));
+            GFX_HIP(hipStreamWaitEvent(B.stream, B.join[lead], 0));
+        }
+        GFX_HIP(hipStreamEndCapture(B.stream, &c.graph));
+        GFX_HIP(hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0));
+        B.last_groups = int(c.grp.size());
+        B.cache.push_back(std::move(cp));
+        return c.exec;
+    }
     GFX_HIP(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
     GFX_HIP(hipEventRecord(B.fork, B.stream));
     for (size_t i = 0; i < B.s.size(); ++i) {
@@ -1637,7 +1822,8 @@ hipGraphExec_t batch_graph(gfx_batch& B, const std::vector<char>& active) {
     }
     GFX_HIP(hipStreamEndCapture(B.stream, &c.graph));
     GFX_HIP(hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0));
-    B.cache.push_back(c);
+    B.cache.push_back(std::move(cp));
+    B.last_groups = 0;
     return c.exec;
 }
 
@@ -1682,6 +1868,16 @@ extern "C" gfx_status gfx_batch_run(gfx_batch* b, int n_steps, const int* until)
         GFX_HIP(hipStreamSynchronize(b->stream));
     });
 }
+
+extern "C" gfx_status gfx_batch_set_groups(gfx_batch* b, int n_groups) {
+    return guarded([&] {
+        require(b != nullptr, "null batch");
+        require(n_groups >= 0, "n_groups must be >= 0");
+        b->want_groups = n_groups;
+    });
+}
+
+extern "C" int gfx_batch_groups(const gfx_batch* b) { return b ? b->last_groups : 0; }
 
 extern "C" gfx_status gfx_batch_destroy(gfx_batch* b) {
     return guarded([&] { delete b; });

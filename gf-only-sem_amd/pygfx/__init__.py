@@ -157,6 +157,8 @@ def _load():
         "gfx_solver_set_graph": (st, [vp, C.c_int]),
         "gfx_batch_create": (st, [C.c_int, C.POINTER(vp), C.POINTER(vp)]),
         "gfx_batch_run": (st, [vp, C.c_int, _ip]),
+        "gfx_batch_set_groups": (st, [vp, C.c_int]),
+        "gfx_batch_groups": (C.c_int, [vp]),
         "gfx_batch_destroy": (st, [vp]),
         "gfx_green_create": (st, [C.c_int, C.c_int, _dp, C.c_int, _ip, C.POINTER(vp)]),
         "gfx_green_destroy": (st, [vp]),
@@ -804,14 +806,24 @@ class Solver:
 
 class Batch:
     """Independent solvers (uneven meshes) advanced together, one captured
-    graph replay per step with one branch per instance."""
+    graph replay per step.  groups=k > 0 splits the solvers into k launch
+    groups, each sharing one launch per stage, where every active instance
+    allows it; groups=0 (the library default) or solvers that cannot share
+    keep one branch of launches per instance."""
 
-    def __init__(self, solvers):
+    def __init__(self, solvers, groups=None):
         self.solvers = list(solvers)
         arr = (C.c_void_p * len(self.solvers))(*[s.h.value for s in self.solvers])
         h = C.c_void_p()
         _check(lib.gfx_batch_create(len(self.solvers), arr, C.byref(h)))
         self.h = h
+        if groups is not None:
+            _check(lib.gfx_batch_set_groups(self.h, int(groups)))
+
+    @property
+    def last_groups(self):
+        """Launch groups of the last step run (0: one branch per instance)."""
+        return int(lib.gfx_batch_groups(self.h))
 
     def run(self, n_steps, until=None):
         u = None if until is None else np.ascontiguousarray(until, np.int32)

@@ -244,9 +244,23 @@ typedef struct gfx_batch gfx_batch;
 gfx_status gfx_batch_create(int n, gfx_solver* const* solvers, gfx_batch** out);
 /* Advance every instance by up to n_steps steps, instance i only while its
  * step counter is below until[i] (until may be NULL).  Each step of all active
- * instances is one replay of a captured graph with one branch per instance;
- * graphs are cached per active set.  Returns when all work has finished. */
+ * instances is one replay of a captured graph, cached per active set.  With
+ * launch groups enabled (gfx_batch_set_groups) and every active instance
+ * using GATHER assembly, nodal C-PML (or none), no attenuation and the same
+ * launch shape (element kernel form, elements per workgroup, node field
+ * groups), the instances are split into launch groups of about equal work and
+ * each group shares one launch per stage (element stage, special nodes,
+ * regular nodes, counter update; results bitwise equal to running each
+ * instance alone); the groups are independent branches of the graph.
+ * Otherwise the graph holds one branch of launches per instance.  Returns
+ * when all work has finished. */
 gfx_status gfx_batch_run(gfx_batch* b, int n_steps, const int* until);
+/* Number of launch groups where launches can be shared (default 0: one
+ * branch per instance, whose launches the device overlaps across instances). */
+gfx_status gfx_batch_set_groups(gfx_batch* b, int n_groups);
+/* Launch groups the last step gfx_batch_run took used (0: one branch per
+ * instance). */
+int gfx_batch_groups(const gfx_batch* b);
 gfx_status gfx_batch_destroy(gfx_batch* b);
 
 /* -------------------------------------------------------------------------

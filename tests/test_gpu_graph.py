@@ -4,8 +4,9 @@ A captured step is replayed many times with the program's own eager work
 (state reads, record reads, synchronisations) between the replays; every
 replay must equal the kernel-by-kernel run bit for bit.  A batch of meshes of
 different size, material, C-PML thickness, time step and run length, advanced
-as one graph with a branch per mesh, must reproduce each mesh's single run bit
-for bit, and one of them is checked against the FP64 oracle."""
+as one graph (one branch per mesh, or launches shared by all meshes), must
+reproduce each mesh's single run bit for bit, and one of them is checked
+against the FP64 oracle."""
 import numpy as np
 import pytest
 
@@ -87,8 +88,11 @@ def _uneven_meshes():
     return [m0, m1, m2]
 
 
-@pytest.mark.parametrize("assembly", [pygfx.GFX_ASSEMBLY_COLORED, pygfx.GFX_ASSEMBLY_GATHER])
-def test_batch_of_uneven_meshes_equals_single_runs(assembly):
+@pytest.mark.parametrize("assembly,groups", [(pygfx.GFX_ASSEMBLY_COLORED, 2), (pygfx.GFX_ASSEMBLY_GATHER, 0),
+                                             (pygfx.GFX_ASSEMBLY_GATHER, 1), (pygfx.GFX_ASSEMBLY_GATHER, 2)])
+def test_batch_of_uneven_meshes_equals_single_runs(assembly, groups):
+    """GATHER batches share one launch per stage among the meshes of a launch
+    group; COLORED ones, and GATHER with groups=0, keep one branch per mesh."""
     meshes = _uneven_meshes()
     assert len({m.dt for m in meshes}) > 1 and len({m.nsteps for m in meshes}) == 3
     srcs = []
@@ -97,10 +101,11 @@ def test_batch_of_uneven_meshes_equals_single_runs(assembly):
         srcs.append([(cells, w, stf, 2), (cells, w, stf, 0)])
     batched = [_solver(m, s, assembly) for m, s in zip(meshes, srcs)]
     single = [_solver(m, s, assembly) for m, s in zip(meshes, srcs)]
-    batch = pygfx.Batch(batched)
+    batch = pygfx.Batch(batched, groups=groups)
     until = [m.nsteps for m in meshes]
     # in pieces, with eager reads in between
     batch.run(7, until)
+    assert batch.last_groups == (groups if assembly == pygfx.GFX_ASSEMBLY_GATHER else 0)
     for s in batched:
         s.get_state(0)
     batch.run(10 ** 6, until)
