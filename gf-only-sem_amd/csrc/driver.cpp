@@ -1162,6 +1162,7 @@ extern "C" gfx_status gfx_mesh_read_inputs(const char* dir, gfx_mesh** out) {
             H->pml_region = std::move(p.pml_region);
             H->coef_alpha = std::move(p.coef_alpha);
             H->coef_beta = std::move(p.coef_beta);
+            H->coef_strain = std::move(p.coef_strain);
             H->coef_abar = std::move(p.coef_abar);
         }
         H->rec_cell = std::move(p.recording.rec_cell_local);

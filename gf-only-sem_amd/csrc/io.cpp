@@ -404,6 +404,9 @@ void write_inputs(const gfx_mesh& m, const std::string& dir, int n_parts, const 
             h5::write_vec(id, "/field/cell/pml_coef_alpha", slice(m.coef_alpha, 9), {hn, hsize_t(NP), 9}, gz);
             if (m.coef_beta)
                 h5::write_vec(id, "/field/cell/pml_coef_beta", slice(m.coef_beta, 9), {hn, hsize_t(NP), 9}, gz);
+            if (m.coef_strain)
+                h5::write_vec(id, "/field/cell/pml_coef_strain", slice(m.coef_strain, 18), {hn, hsize_t(NP), 18},
+                              gz);
             h5::write_vec(id, "/field/cell/pml_coef_abar", slice(m.coef_abar, 5), {hn, hsize_t(NP), 5}, gz);
         }
         if (m.n_sls > 0 && m.tau_sigma && m.tau_epsilon) {

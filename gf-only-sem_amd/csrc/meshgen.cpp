@@ -96,6 +96,79 @@ void conv_coef(double b, double dt, double c[3]) {
     }
 }
 
+// Partial fractions of s_1 s_2 / s_3 (pml_cpml.py:600-692, _lijk_parameter,
+// always in its 1-2-3 form; callers permute the axes): A0 and the residues at
+// alpha_1, alpha_2 and beta_3.  Directions the region does not damp drop out.
+void lijk(int region, const double k1, const double d1, const double a1, const double k2, const double d2,
+          const double a2, const double k3, const double d3, const double a3, double* A) {
+    const double b1 = a1 + d1 / std::max(k1, 1.0), b2 = a2 + d2 / std::max(k2, 1.0),
+                 b3 = a3 + d3 / std::max(k3, 1.0);
+    A[0] = A[1] = A[2] = A[3] = 0.0;
+    switch (region) {
+        case R_XYZ: {
+            A[0] = k1 * k2 / std::max(k3, 1.0);
+            A[1] = -A[0] * (a1 - a3) * (a1 - b1) * (a1 - b2) / (safe_den(a1 - a2) * safe_den(a1 - b3));
+            A[2] = -A[0] * (a2 - a3) * (a2 - b1) * (a2 - b2) / (safe_den(a2 - a1) * safe_den(a2 - b3));
+            A[3] = -A[0] * (b3 - a3) * (b3 - b1) * (b3 - b2) / (safe_den(b3 - a1) * safe_den(b3 - a2));
+            break;
+        }
+        case R_X:
+            A[0] = k1;
+            A[1] = -A[0] * (a1 - b1);
+            break;
+        case R_Y:
+            A[0] = k2;
+            A[2] = -A[0] * (a2 - b2);
+            break;
+        case R_Z:
+            A[0] = 1.0 / std::max(k3, 1.0);
+            A[3] = -A[0] * (b3 - a3);
+            break;
+        case R_XY:
+            A[0] = k1 * k2;
+            A[1] = -A[0] * (a1 - b1) * (a1 - b2) / safe_den(a1 - a2);
+            A[2] = -A[0] * (a2 - b1) * (a2 - b2) / safe_den(a2 - a1);
+            break;
+        case R_XZ:
+            A[0] = k1 / std::max(k3, 1.0);
+            A[1] = -A[0] * (a1 - a3) * (a1 - b1) / safe_den(a1 - b3);
+            A[3] = -A[0] * (b3 - a3) * (b3 - b1) / safe_den(b3 - a1);
+            break;
+        case R_YZ:
+            A[0] = k2 / std::max(k3, 1.0);
+            A[2] = -A[0] * (a2 - a3) * (a2 - b2) / safe_den(a2 - b3);
+            A[3] = -A[0] * (b3 - a3) * (b3 - b2) / safe_den(b3 - a2);
+            break;
+        default:
+            break;
+    }
+}
+
+// (K, d) of the stretch along one axis where the region damps it, (1, 0)
+// elsewhere (pml_cpml.py:695-734, _lx/_ly/_lz_parameter).
+void lone(int region, int axis, double k, double d, double a, double* A) {
+    if (region != 0 && axis_active(region, axis)) {
+        A[0] = k;
+        A[1] = -k * (a - (a + d / std::max(k, 1.0)));
+    } else {
+        A[0] = 1.0;
+        A[1] = 0.0;
+    }
+}
+
+// The 18 strain coefficients of one point (pml_cpml.py:523-597).  The
+// region's index permutations 231, 132, 123 map (z, y, x), (x, z, y), (x, y, z)
+// onto lijk's (1, 2, 3); the region code itself is passed unchanged, as the
+// reference does.
+void strain_coef(int region, const double K[3], const double d[3], const double al[3], double S[18]) {
+    for (int q = 0; q < 18; ++q) S[q] = 0.0;
+    if (region == 0) return;
+    lijk(region, K[2], d[2], al[2], K[1], d[1], al[1], K[0], d[0], al[0], S + 0);
+    lijk(region, K[0], d[0], al[0], K[2], d[2], al[2], K[1], d[1], al[1], S + 4);
+    lijk(region, K[0], d[0], al[0], K[1], d[1], al[1], K[2], d[2], al[2], S + 8);
+    for (int ax = 0; ax < 3; ++ax) lone(region, ax, K[ax], d[ax], al[ax], S + 12 + 2 * ax);
+}
+
 // A1..A5 by region (pml_cpml.py:388-513, _l_parameter).
 void abar(int region, const double K[3], const double d[3], const double al[3], double A[5]) {
     const double kx = K[0], ky = K[1], kz = K[2];
@@ -642,6 +715,7 @@ void build(MeshHolder& H, const GeneralCells* gen = nullptr) {
     H.coef_alpha.resize(P * 9);
     H.coef_abar.assign(P * 5, 0.0);
     if (s.want_beta) H.coef_beta.resize(P * 9);
+    if (s.want_strain) H.coef_strain.assign(P * 18, 0.0);
     H.damping.assign(P, 0.0);
     const double alpha_max[3] = {M_PI * s.f0_pml * 0.9, M_PI * s.f0_pml * 1.0,
                                  M_PI * s.f0_pml * 1.1};
@@ -708,6 +782,7 @@ void build(MeshHolder& H, const GeneralCells* gen = nullptr) {
                 }
             }
             if (region != 0) abar(region, K, d, al, &H.coef_abar[p * 5]);
+            if (region != 0 && s.want_strain) strain_coef(region, K, d, al, &H.coef_strain[p * 18]);
         }
         // legacy linear ramp (preprocess/pml.py:55-122)
         if (is_pml[e]) {
@@ -857,6 +932,7 @@ extern "C" gfx_status gfx_box_spec_default(gfx_box_spec* s) {
         s->record_depth_max = 2000.0;
         s->want_coords = 1;
         s->want_beta = 0;
+        s->want_strain = 0;
     });
 }
 
@@ -1037,6 +1113,15 @@ extern "C" gfx_status gfx_mesh_locate_source(const gfx_mesh* m, double x, double
     });
 }
 
+extern "C" gfx_status gfx_pml_strain_coefficients(int region, const double* K, const double* d,
+                                                   const double* alpha, double* out18) {
+    return guarded([&] {
+        require(K && d && alpha && out18, "K, d, alpha and out18 are required");
+        require(region >= 0 && region <= R_XYZ, "region must be 0..7");
+        strain_coef(region, K, d, alpha, out18);
+    });
+}
+
 extern "C" gfx_status gfx_sls_fit(double q, double f_min, double f_max, int n_sls, double* tau_sigma,
                                    double* tau_epsilon) {
     return guarded([&] {
@@ -1076,6 +1161,7 @@ void wire_mesh(MeshHolder& H) {
     m.pml_region = H.pml_region.empty() ? nullptr : H.pml_region.data();
     m.coef_alpha = H.coef_alpha.empty() ? nullptr : H.coef_alpha.data();
     m.coef_beta = H.coef_beta.empty() ? nullptr : H.coef_beta.data();
+    m.coef_strain = H.coef_strain.empty() ? nullptr : H.coef_strain.data();
     m.coef_abar = H.coef_abar.empty() ? nullptr : H.coef_abar.data();
     m.n_rec_cell = int(H.rec_cell.size());
     m.rec_cell = H.rec_cell.empty() ? nullptr : H.rec_cell.data();

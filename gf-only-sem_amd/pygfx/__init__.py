@@ -56,6 +56,7 @@ class BoxSpec(C.Structure):
         ("record_depth_max", C.c_double),
         ("want_coords", C.c_int),
         ("want_beta", C.c_int),
+        ("want_strain", C.c_int),
         ("n_sls", C.c_int),
         ("q_mu", C.c_double * MAX_LAYERS),
         ("att_fmin", C.c_double), ("att_fmax", C.c_double),
@@ -71,7 +72,7 @@ class _Mesh(C.Structure):
         ("coords", _dp), ("jacobian", _dp), ("dxi_dx", _dp), ("mass", _dp),
         ("vp", _dp), ("vs", _dp), ("density", _dp), ("lambda_", _dp), ("mu_", _dp),
         ("damping", _dp), ("ibool", _ip), ("node_xyz", _dp),
-        ("pml_region", _ip), ("coef_alpha", _dp), ("coef_beta", _dp), ("coef_abar", _dp),
+        ("pml_region", _ip), ("coef_alpha", _dp), ("coef_beta", _dp), ("coef_strain", _dp), ("coef_abar", _dp),
         ("n_pml_elem", C.c_int),
         ("h_min", C.c_double), ("vp_max", C.c_double), ("cfl_dt", C.c_double), ("dt", C.c_double),
         ("snapshot_stride", C.c_int), ("nsteps", C.c_int),
@@ -190,6 +191,7 @@ def _load():
                                         C.c_int, _ip, _dp, C.POINTER(C.c_int)]),
         "gfx_ricker": (st, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, _dp]),
         "gfx_sls_fit": (st, [C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp]),
+        "gfx_pml_strain_coefficients": (st, [C.c_int, _dp, _dp, _dp, _dp]),
         "gfx_solver_set_step": (st, [vp, C.c_int]),
         "gfx_solver_checkpoint_size": (st, [vp, C.POINTER(C.c_size_t)]),
         "gfx_solver_checkpoint_save": (st, [vp, vp, C.c_size_t]),
@@ -264,6 +266,14 @@ def sls_fit(q, f_min, f_max, n_sls):
     ts, te = np.zeros(n_sls), np.zeros(n_sls)
     _check(lib.gfx_sls_fit(q, f_min, f_max, n_sls, _ptr(ts), _ptr(te)))
     return ts, te
+
+
+def pml_strain_coefficients(region, K, d, alpha):
+    """The 18 strain-based C-PML coefficients of one point (gfx_pml_strain_coefficients)."""
+    k, dd, a = (np.ascontiguousarray(x, np.float64) for x in (K, d, alpha))
+    out = np.zeros(18)
+    _check(lib.gfx_pml_strain_coefficients(int(region), _ptr(k), _ptr(dd), _ptr(a), _ptr(out)))
+    return out
 
 
 def ricker(f0, t0, amplitude, dt, n):
@@ -445,6 +455,7 @@ def _wrap_mesh(h, keep_handle, copy) -> Mesh:
         "pml_region": arr(m.pml_region, E, np.int32),
         "coef_alpha": arr(m.coef_alpha, P * 9),
         "coef_beta": arr(m.coef_beta, P * 9),
+        "coef_strain": arr(m.coef_strain, P * 18),
         "coef_abar": arr(m.coef_abar, P * 5),
         "tau_sigma": arr(m.tau_sigma, P * m.n_sls),
         "tau_epsilon": arr(m.tau_epsilon, P * m.n_sls),

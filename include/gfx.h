@@ -293,6 +293,7 @@ typedef struct {
     double record_depth_max;  /* m; 0 = no recording */
     int want_coords;          /* produce coords/vp/vs (x3 memory) */
     int want_beta;            /* produce pml_coef_beta */
+    int want_strain;          /* produce pml_coef_strain (A6..A23 of the strain-based C-PML) */
     /* shear attenuation: n_sls standard linear solids fitted to a constant
        q_mu[layer] over [att_fmin, att_fmax] Hz (gfx_sls_fit); 0 = elastic;
        q_mu <= 0 leaves that layer elastic */
@@ -320,6 +321,8 @@ typedef struct {
     int32_t* pml_region;  /* [E] */
     double* coef_alpha;   /* [E,n,9] */
     double* coef_beta;    /* [E,n,9] or NULL */
+    double* coef_strain;  /* [E,n,18] or NULL: per point A0 A6 A7 A8 | A0 A10 A11 A12 | A0 A14 A15 A16 |
+                             A18 A19 | A20 A21 | A22 A23 */
     double* coef_abar;    /* [E,n,5] */
     int n_pml_elem;
     double h_min, vp_max, cfl_dt, dt;
@@ -350,6 +353,24 @@ gfx_status gfx_mesh_locate_source(const gfx_mesh* m, double x, double y, double 
  * returned as tau_epsilon_l.  The modulus of the solids is
  * M(w) = M_R (1 + sum_l (tau_eps_l / tau_sig_l - 1) i w tau_sig_l / (1 + i w tau_sig_l)). */
 gfx_status gfx_sls_fit(double q, double f_min, double f_max, int n_sls, double* tau_sigma, double* tau_epsilon);
+
+/* Strain-based C-PML coefficients of one point (preprocess/pml_cpml.py:523-734,
+ * compute_strain_coefficients): K, d, alpha per direction (x, y, z) and the
+ * region code (1 X, 2 Y, 3 Z, 4 XY, 5 XZ, 6 YZ, 7 XYZ; 0 gives zeros) to the
+ * 18 values of pml_coef_strain.  With s_a = K_a + d_a / (alpha_a + i w) and
+ * beta_a = alpha_a + d_a / max(K_a, 1), each group of four is the partial
+ * fraction A0 + A1/(alpha_1 + i w) + A2/(alpha_2 + i w) + A3/(beta_3 + i w) of
+ * the region's form G(s_1, s_2, s_3) -- XYZ: s_1 s_2 / s_3, X: s_1, Y: s_2,
+ * Z: 1 / s_3, XY: s_1 s_2, XZ: s_1 / s_3, YZ: s_2 / s_3 -- with directions
+ * (1, 2, 3) = (z, y, x) for entries 0-3, (x, z, y) for 4-7 and (x, y, z) for
+ * 8-11.  (The reference keeps the region code when it permutes the
+ * directions, so outside XYZ the first two groups are not s_z s_y / s_x and
+ * s_x s_z / s_y; the values are kept as it writes them.)  Entries 12-17 are
+ * (K_a, K_a d_a / max(K_a, 1)) of s_x, s_y, s_z in the directions the region
+ * damps, (1, 0) in the others.  Poles closer than 1e-6 are separated as the
+ * reference does. */
+gfx_status gfx_pml_strain_coefficients(int region, const double* K, const double* d, const double* alpha,
+                                       double* out18);
 
 /* Ricker wavelet A (1 - 2a^2) exp(-a^2), a = pi f0 (t - t0), sampled at
  * t = i*dt for i < n (preprocess/stf_evaluator.py:19-45). */
