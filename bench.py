@@ -163,6 +163,25 @@ def traffic_from_profiles(kernel_prefix, launch, fields, dims):
     return same or other or (None, None)
 
 
+# LDS reads per element point and field of the block element kernel's field
+# loop (k_element<EPB, GATHER, no element-local C-PML, diagonal or general form),
+# counted in its gfx950 code: 39 ds_read2_b32 + 12 ds_read_b32 = 90 dwords
+# (45 displacement values for the gradient, 45 flux values for the divergence).
+# The measured rate of 32-bit and read2 LDS reads on this part is about 104
+# bytes per clock per CU: 256 CUs x 2.4 GHz x 104 B = 63.9 TB/s.
+BLOCK_LDS_DWORDS = 90
+LDS_MEASURED_TBS = 63.9
+
+
+def element_lds(launch, n_elem, n_fields, ms_elem):
+    """LDS read rate of the block element kernel's field loop (its binding limit)."""
+    if launch["element_kernel"] != "block" or ms_elem <= 0:
+        return None
+    tbs = n_elem * 125 * n_fields * BLOCK_LDS_DWORDS * 4 / (ms_elem * 1e-3) / 1e12
+    return {"dwords_per_point_field": BLOCK_LDS_DWORDS, "achieved_tbs": tbs, "measured_rate_tbs": LDS_MEASURED_TBS,
+            "frac_of_measured_rate": tbs / LDS_MEASURED_TBS}
+
+
 def uneven_specs(pygfx, n=16, seed=4, h=600.0, lo=8e3, hi=2e5):
     """n independent boxes (seeded): n_target log-uniform in [lo, hi] elements,
     nx = ny = round((2n)^(1/3)), nz = round(nx/2), 0.6 km hexes; 2-4 layers with
@@ -492,6 +511,7 @@ def main():
                      "traffic_pass_launch": traffic_launch,
                      "kernel": kname, "algorithmic_bytes_per_launch": alg_launch,
                      "ms_per_launch": ms_launch, "launches_per_step": launches_per_step},
+        "element_lds": element_lds(launch, E, F, tb["element"][0] / args.steps),
         "whole_step_hbm_gbs": sb.algorithmic_bytes(0) / (ms_step * 1e-3) / 1e9,
         "ms_per_step_by_kernel": {k: tb[k][0] / args.steps for k in ("element", "node_special",
                                                                       "node_regular", "record")},
