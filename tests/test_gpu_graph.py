@@ -131,3 +131,28 @@ def test_batch_of_uneven_meshes_equals_single_runs(assembly, groups):
     print(f"batched instance strain rel L2 vs oracle = {err:.3e}")
     assert err <= 1e-4
     batch.close()
+
+
+def test_launch_groups_fall_back_to_branches_when_a_mesh_cannot_share():
+    """An attenuating mesh runs its own element kernel form, so a batch holding
+    one keeps one branch per mesh even with launch groups requested; dropping it
+    from the active set (its run is over) lets the rest share launches."""
+    m0 = mc.small_halfspace(nx=6, ny=6, nz=4, pml=1, duration=0.3)
+    m1 = mc.small_layered(duration=0.8, n_sls=2, q_mu=[30.0, 30.0, 30.0], att_fmin=0.1, att_fmax=4.0)
+    m2 = mc.small_halfspace(nx=7, ny=6, nz=4, pml=1, duration=0.6)
+    meshes = [m0, m1, m2]
+    srcs = [[mc.surface_centre_source(m) + (2,)] for m in meshes]
+    batched = [_solver(m, s, pygfx.GFX_ASSEMBLY_GATHER) for m, s in zip(meshes, srcs)]
+    single = [_solver(m, s, pygfx.GFX_ASSEMBLY_GATHER) for m, s in zip(meshes, srcs)]
+    assert int(batched[1].query("n_sls")) == 2
+    until = [40, 20, 40]
+    batch = pygfx.Batch(batched, groups=1)
+    batch.run(10, until)
+    assert batch.last_groups == 0  # the attenuating mesh is active
+    batch.run(10 ** 6, until)
+    assert batch.last_groups == 1  # meshes 0 and 2 alone
+    for b, s, n in zip(batched, single, until):
+        s.run(n)
+        for x, y in zip(b.get_state(0), s.get_state(0)):
+            assert np.array_equal(x, y)
+    batch.close()
