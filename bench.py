@@ -38,25 +38,13 @@ def log(*a):
 
 
 def cfg2_spec(pygfx, nx=100, ny=100, nz=50, h=600.0, duration=30.0, n_sls=0):
-    """The layered 100 x 100 x 50 benchmark box; n_sls > 0 adds shear attenuation
-    (Q_mu 60 / 120 / 250 by layer over 0.05-2 Hz)."""
-    return pygfx.make_spec(
-        n_sls=n_sls, q_mu=[60.0, 120.0, 250.0], att_fmin=0.05, att_fmax=2.0,
-        nx=nx, ny=ny, nz=nz, lx=h * nx, ly=h * ny, lz=h * nz,
-        layers=[(3000.0, 5000.0, 2900.0, 2500.0), (20000.0, 6200.0, 3600.0, 2800.0),
-                (0.0, 8000.0, 4600.0, 3300.0)],
-        rf_amp=0.05, rf_corr_len=2000.0, rf_modes=256, rf_seed=2,
-        pml_xmin=5, pml_xmax=5, pml_ymin=5, pml_ymax=5, pml_zmax=5, f0_pml=1.5,
-        cfl_safety=0.5, output_dt=0.05, duration=duration, record_depth_max=2000.0,
-        want_coords=0, want_beta=0)
+    from pygfx import workloads
+    return workloads.layered_spec(nx, ny, nz, h=h, duration=duration, n_sls=n_sls)
 
 
 def stations(mesh, n=8, seed=3):
-    rng = np.random.default_rng(seed)
-    cx = 0.5 * (mesh.bounds[0] + mesh.bounds[1])
-    cy = 0.5 * (mesh.bounds[2] + mesh.bounds[3])
-    half = 20000.0
-    return [(cx + rng.uniform(-half, half), cy + rng.uniform(-half, half)) for _ in range(n)]
+    from pygfx import workloads
+    return workloads.stations(mesh, n=n, seed=seed)
 
 
 def time_run(pygfx, solver, steps, warmup):
@@ -182,31 +170,9 @@ def element_lds(launch, n_elem, n_fields, ms_elem):
             "frac_of_measured_rate": tbs / LDS_MEASURED_TBS}
 
 
-def uneven_specs(pygfx, n=16, seed=4, h=600.0, lo=8e3, hi=2e5):
-    """n independent boxes (seeded): n_target log-uniform in [lo, hi] elements,
-    nx = ny = round((2n)^(1/3)), nz = round(nx/2), 0.6 km hexes; 2-4 layers with
-    Vs ~ U(1500, 4500) increasing with depth, Vp = 1.73 Vs, rho ~ U(2300, 3300);
-    C-PML U{3..6}; run length U(5, 30) s; Ricker f0 = 1.5 Hz."""
-    rng = np.random.default_rng(seed)
-    specs = []
-    for _ in range(n):
-        nt = float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
-        nx = int(round((2 * nt) ** (1 / 3)))
-        nz = max(4, int(round(nx / 2)))
-        nl = int(rng.integers(2, 5))
-        vs = np.sort(rng.uniform(1500.0, 4500.0, nl))
-        rho = rng.uniform(2300.0, 3300.0, nl)
-        lz = h * nz
-        tops = np.sort(rng.uniform(0.1, 0.7, nl - 1)) * lz
-        layers = [(float(tops[i]), 1.73 * vs[i], vs[i], rho[i]) for i in range(nl - 1)]
-        layers.append((0.0, 1.73 * vs[-1], vs[-1], rho[-1]))
-        pml = int(rng.integers(3, 7))
-        dur = float(rng.uniform(5.0, 30.0))
-        specs.append(pygfx.make_spec(nx=nx, ny=nx, nz=nz, lx=h * nx, ly=h * nx, lz=lz, layers=layers,
-                                     pml_xmin=pml, pml_xmax=pml, pml_ymin=pml, pml_ymax=pml, pml_zmax=pml,
-                                     f0_pml=1.5, cfl_safety=0.5, output_dt=0.05, duration=dur,
-                                     record_depth_max=2000.0, want_coords=0, want_beta=0))
-    return specs
+def uneven_specs(pygfx, n=16, seed=4):
+    from pygfx import workloads
+    return workloads.uneven_specs(n=n, seed=seed)
 
 
 def uneven_workload(pygfx, steps, warmup, group_counts=(0, 1, 2, 4)):

@@ -6,6 +6,7 @@ module fails loudly when it is missing (there is no CPU fallback).
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 import os
 from dataclasses import dataclass, field
 
@@ -304,6 +305,7 @@ class Mesh:
     rec_cell: np.ndarray
     n_sls: int = 0
     _handle: object = field(default=None, repr=False)
+    _parent: object = field(default=None, repr=False)
 
     def __getattr__(self, k):
         arrays = self.__dict__.get("arrays", {})
@@ -318,7 +320,20 @@ class Mesh:
         xyz = self.arrays["node_xyz"].reshape(-1, 3)
         return np.ascontiguousarray(xyz[ib].mean(axis=1))
 
+    def derive(self, rec_cell=None, **arrays):
+        """Same mesh with some arrays (same shapes) or the recording cells replaced.
+        Source location uses the geometry, which is unchanged, through this mesh."""
+        a = dict(self.arrays)
+        for k, v in arrays.items():
+            if k not in a or a[k] is None or np.asarray(v).size != a[k].size:
+                raise KeyError(f"{k}: not an array of this mesh, or a different size")
+            a[k] = np.ascontiguousarray(v, dtype=a[k].dtype).reshape(a[k].shape)
+        rec = self.rec_cell if rec_cell is None else np.ascontiguousarray(rec_cell, np.int32)
+        return dataclasses.replace(self, arrays=a, rec_cell=rec, _handle=None, _parent=self)
+
     def locate_source(self, x, y, z, max_cells=8):
+        if self._handle is None and self._parent is not None:
+            return self._parent.locate_source(x, y, z, max_cells)
         nn = self.ngll ** 3
         cells = np.zeros(max_cells, dtype=np.int32)
         w = np.zeros(max_cells * nn, dtype=np.float64)

@@ -382,6 +382,15 @@ void gfo_newmark_correct(int n_dof, double dt, double beta, double gamma, const 
 /* C-PML: forward/share/src/pml.cpp                                    */
 /* ------------------------------------------------------------------ */
 
+/* pml.cpp:13-29 (apply_pml_damping): v[i] -= d[i/3] v[i] where d > 0; the
+ * solver's own loop over rank nodes (solver.cpp:565-573) is the same rule. */
+void gfo_apply_pml_damping(const double* damping_node, double* v, long n_dof) {
+    for (long i = 0; i < n_dof; ++i) {
+        const double d = damping_node[i / 3];
+        if (d > 0.0) v[i] -= d * v[i];
+    }
+}
+
 #define GFO_THETA (1.0 / 8.0) /* pml.cpp:35 */
 
 static inline int region_of(const gfo_cpml* p, int e) {
@@ -608,11 +617,7 @@ static void run_one_step(run_ws* w, const gfo_problem* pb, int step) {
     if (pb->has_cpml) {
         gfo_cpml_accel_contribution(&w->pml, w->u, w->v, w->gll_w, w->rloc);
     } else {
-        for (long g = 0; g < pb->n_rank_node; ++g) {
-            const double d = w->damp_g[g];
-            if (d > 0.0)
-                for (int c = 0; c < 3; ++c) w->v[g * 3 + c] -= d * w->v[g * 3 + c];
-        }
+        gfo_apply_pml_damping(w->damp_g, w->v, (long)pb->n_rank_node * 3);
     }
     /* 5. source: stf_values[step] = f(t_n) (solver.cpp:576-605) */
     {
@@ -641,21 +646,29 @@ static void run_one_step(run_ws* w, const gfo_problem* pb, int step) {
     if (pb->has_cpml) gfo_cpml_update_displ_memory(&w->pml);
 }
 
-int gfo_run(const gfo_problem* pb, double* rec_strain, double* rec_disp, double* rec_vel,
-            double* rec_acc, double* u_out, double* v_out, double* a_out) {
+/* Steps [start_step, nsteps) of solver.cpp:389-688 from (u0, v0, a0) (zero when
+ * NULL) with the C-PML state at zero: start_step > 0 is the reference's resume
+ * (solver.cpp:369-388: state of the restart file, start_step = its step + 1). */
+static int run_core(const gfo_problem* pb, int start_step, const double* u0, const double* v0,
+                    const double* a0, double* rec_strain, double* rec_disp, double* rec_vel,
+                    double* rec_acc, double* u_out, double* v_out, double* a_out) {
     run_ws w;
-    if (!pb || pb->direction < 0 || pb->direction > 2) return -1;
+    if (!pb || pb->direction < 0 || pb->direction > 2 || start_step < 0) return -1;
     if (ws_init(&w, pb) != 0) {
         ws_free(&w);
         return -1;
     }
+    const long n_dof = (long)pb->n_rank_node * 3;
+    if (u0) memcpy(w.u, u0, sizeof(double) * n_dof);
+    if (v0) memcpy(w.v, v0, sizeof(double) * n_dof);
+    if (a0) memcpy(w.a, a0, sizeof(double) * n_dof);
     const int ngll = pb->ngll;
     const int n_node = ngll * ngll * ngll;
     const long nrec = pb->n_rec_cell;
     int snap = 0;
     double* strain_tmp = NULL;
     if (rec_strain && nrec > 0) strain_tmp = malloc(sizeof(double) * nrec * n_node * 6);
-    for (int step = 0; step < pb->nsteps; ++step) {
+    for (int step = start_step; step < pb->nsteps; ++step) {
         run_one_step(&w, pb, step);
         if (pb->snapshot_stride > 0 && step % pb->snapshot_stride == 0) {
             /* solver.cpp:640-675: strain from gathered u after the corrector */
@@ -690,6 +703,16 @@ int gfo_run(const gfo_problem* pb, double* rec_strain, double* rec_disp, double*
     free(strain_tmp);
     ws_free(&w);
     return snap;
+}
+
+int gfo_run(const gfo_problem* pb, double* rec_strain, double* rec_disp, double* rec_vel,
+            double* rec_acc, double* u_out, double* v_out, double* a_out) {
+    return run_core(pb, 0, NULL, NULL, NULL, rec_strain, rec_disp, rec_vel, rec_acc, u_out, v_out, a_out);
+}
+
+int gfo_run_from(const gfo_problem* pb, int start_step, double* u, double* v, double* a,
+                 double* rec_strain, double* rec_disp, double* rec_vel, double* rec_acc) {
+    return run_core(pb, start_step, u, v, a, rec_strain, rec_disp, rec_vel, rec_acc, u, v, a);
 }
 
 int gfo_run_state(const gfo_problem* pb, double* u, double* v, double* a) {

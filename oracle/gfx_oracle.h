@@ -62,6 +62,10 @@ void gfo_newmark_predict(int n_dof, double dt, double beta, const double* u, con
 void gfo_newmark_correct(int n_dof, double dt, double beta, double gamma, const double* mass,
                          double* u, double* v, double* a, const double* r);
 
+/* ---- legacy linear-ramp damping (forward/share/src/pml.cpp:13-29) ----
+ * v[i] -= damping_node[i/3] * v[i] wherever the node's damping is > 0. */
+void gfo_apply_pml_damping(const double* damping_node, double* v, long n_dof);
+
 /* ---- C-PML (forward/share/src/pml.cpp:55-216) ---- */
 typedef struct {
     int n_elem, ngll;
@@ -148,6 +152,13 @@ void gfo_sls_point_history(int n_sls, const double* tau_sigma, const double* tau
  * Returns the number of snapshots written, or -1 on bad input. */
 int gfo_run(const gfo_problem* pb, double* rec_strain, double* rec_disp, double* rec_vel,
             double* rec_acc, double* u_out, double* v_out, double* a_out);
+
+/* Reference resume (solver.cpp:369-388): runs steps [start_step, nsteps) from
+ * the state u, v, a (in/out) with the C-PML state at zero, recording the
+ * snapshot steps among them (outputs as gfo_run, first snapshot first).
+ * Returns the number of snapshots written, or -1 on bad input. */
+int gfo_run_from(const gfo_problem* pb, int start_step, double* u, double* v, double* a,
+                 double* rec_strain, double* rec_disp, double* rec_vel, double* rec_acc);
 
 /* Runs nsteps starting from a given state (u,v,a in/out, C-PML state zero at
  * start); no recording.  Used by step-level parity tests. */

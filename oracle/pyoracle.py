@@ -54,11 +54,13 @@ def _load():
         "gfo_gather_from_rank": (None, [_dp, _ip, C.c_int, C.c_int, _dp]),
         "gfo_newmark_predict": (None, [C.c_int, C.c_double, C.c_double, _dp, _dp, _dp, _dp]),
         "gfo_newmark_correct": (None, [C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp]),
+        "gfo_apply_pml_damping": (None, [_dp, _dp, C.c_long]),
         "gfo_cpml_update_displ_fields": (None, [C.POINTER(Cpml), _dp, _dp, _dp, C.c_double]),
         "gfo_cpml_update_displ_memory": (None, [C.POINTER(Cpml)]),
         "gfo_cpml_accel_contribution": (None, [C.POINTER(Cpml), _dp, _dp, _dp, _dp]),
         "gfo_full_strain": (None, [C.c_int, _ip, _dp, _dp, C.c_int, _dp, _dp]),
         "gfo_run": (C.c_int, [C.POINTER(Problem), _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+        "gfo_run_from": (C.c_int, [C.POINTER(Problem), C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
         "gfo_run_state": (C.c_int, [C.POINTER(Problem), _dp, _dp, _dp]),
         "gfo_sls_coefficients": (None, [C.c_double, C.c_double, C.c_double, C.c_double, _dp]),
         "gfo_sls_point_history": (None, [C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_int, _dp, _dp]),
@@ -179,6 +181,25 @@ class Run:
         if uva:
             for name, x in (("disp", d), ("vel", v), ("acc", a)):
                 out[name] = x.reshape(self.n_snap, nrec, self.nn, 3)
+        return out
+
+    def run_from(self, start_step, u, v, a, uva=False):
+        """The reference's resume: steps [start_step, nsteps) from (u, v, a), C-PML
+        state zero; records of the snapshot steps among them."""
+        u, v, a = (np.ascontiguousarray(x, np.float64).copy() for x in (u, v, a))
+        nrec = self.pb.n_rec_cell
+        st = self.pb.snapshot_stride
+        n_snap = sum(1 for s in range(start_step, self.pb.nsteps) if st > 0 and s % st == 0)
+        strain = np.zeros(n_snap * nrec * self.nn * 6)
+        d = vel = acc = None
+        if uva:
+            d, vel, acc = (np.zeros(n_snap * nrec * self.nn * 3) for _ in range(3))
+        ns = lib.gfo_run_from(C.byref(self.pb), int(start_step), P(u), P(v), P(a), P(strain), P(d), P(vel), P(acc))
+        assert ns == n_snap, (ns, n_snap)
+        out = {"strain": strain.reshape(n_snap, nrec, self.nn, 6), "u": u, "v": v, "a": a}
+        if uva:
+            for name, x in (("disp", d), ("vel", vel), ("acc", acc)):
+                out[name] = x.reshape(n_snap, nrec, self.nn, 3)
         return out
 
     def run_state(self, u, v, a):

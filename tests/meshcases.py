@@ -63,3 +63,48 @@ def per_trace_rel_l2(s, ref, floor=1e-3):
     nr = np.linalg.norm(r, axis=0)
     keep = nr >= floor * nr.max()
     return np.linalg.norm(s[:, keep] - r[:, keep], axis=0) / nr[keep]
+
+
+def amplified_cpml(mesh, damp=0.1, spring=0.01, memory=20.0):
+    """Copy of `mesh` whose C-PML inputs make the reference's C-PML step dominate the
+    PML, recording the surface-layer PML cells.
+
+    The reference adds w J / rho (A1 v + A2 u + A3 m_x + A4 m_y + A5 m_z) to the residual
+    (forward/share/src/pml.cpp:197-212) and divides by the mass rho J w, so the term
+    reaches the acceleration as A / rho^2.  With the generated coefficients and rho = 2700
+    it moves the records by ~1e-8, below any parity gate.  Here A' = rho^2 (...):
+      A1' = -damp rho^2 A1          velocity damping over the generated ramp A1;
+      A2' = -spring rho^2 |A2|      a spring;
+      A3+d' = -memory rho^2 b_d A1 / max A1   memory coupling (m_d ~ u / b_d for slowly
+                                    varying u, so a lagged spring of strength `memory`);
+    and coef_alpha from poles b_d (1/s) that are linear in the node position, different
+    per direction, through the reference's convolution coefficients
+    (preprocess/pml_cpml.py:261-304): (exp(-b dt), (1 - exp(-b dt/2)) / b, c1 exp(-b dt/2)).
+    Being functions of the node position, the poles agree between the elements at a node,
+    so the nodal C-PML form applies.  The oracle runs every part of the C-PML step on
+    these inputs; on the 8x8x6 halfspace over 4 s each part moves the recorded strain by
+    1e-3 to 1 (tests/test_cpml_sensitivity.py), and the wave field decays."""
+    a = mesh.arrays
+    rho = a["density"]
+    ab0 = a["coef_abar"].reshape(-1, 5)
+    xyz = a["node_xyz"].reshape(-1, 3)[a["ibool"]]
+    lx, ly, lz = mesh.bounds[1], mesh.bounds[3], mesh.bounds[5]
+    b = np.stack([6.0 + 10.0 * xyz[:, 0] / lx, 12.0 + 8.0 * xyz[:, 1] / ly,
+                  20.0 + 15.0 * xyz[:, 2] / lz], axis=1)
+    t = np.exp(-0.5 * b * mesh.dt)
+    c1 = (1.0 - t) / b
+    alpha = np.stack([t * t, c1, c1 * t], axis=2).reshape(-1)
+    r2 = rho * rho
+    prof = ab0[:, 0] / ab0[:, 0].max()
+    ab = np.zeros_like(ab0)
+    ab[:, 0] = -damp * r2 * ab0[:, 0]
+    ab[:, 1] = -spring * r2 * np.abs(ab0[:, 1])
+    ab[:, 2:] = -memory * (r2 * prof)[:, None] * b
+    top = mesh.bounds[4] + (mesh.bounds[5] - mesh.bounds[4]) / mesh.nz
+    rec = np.nonzero((mesh.centroids()[:, 2] < top) & (mesh.pml_region != 0))[0]
+    return mesh.derive(rec_cell=rec, coef_alpha=alpha, coef_abar=ab.reshape(-1))
+
+
+def pml_nodes(mesh):
+    """Global ids of the nodes of the C-PML elements."""
+    return np.unique(mesh.ibool.reshape(mesh.n_elem, -1)[mesh.pml_region != 0])

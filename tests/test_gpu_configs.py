@@ -1,0 +1,175 @@
+"""Parity at the benchmark configurations' own sizes and lengths.
+
+  * halfspace: the 20 x 20 x 10 km case of 1 km hexes (4,000 elements, C-PML 3,
+    800 steps of 12.5 ms, strain recorded at 392 cells to 2 km), z force at the
+    surface centre, against the FP64 oracle (the reference's CPU algorithm on
+    the same arrays);
+  * stations: the same mesh with x, y and z forces at 8 seeded surface stations
+    in the central 12 x 12 km (inside the non-PML region), 24 fields advanced by
+    one solver, each field against its own oracle run;
+  * horizon: a 14 x 14 x 8 box of the 0.6 km layered discretisation (three
+    layers compressed into its 4.8 km depth, the same random field, C-PML 5, dt
+    5.56 ms) run for 6,000 steps, the step count of the 500k-element workload:
+    FP32 drift against the oracle;
+  * uneven: the 16 seeded meshes of uneven size (1.25 M elements, x/y/z forces
+    each) advanced as one batch equal each mesh run alone, bit for bit.
+
+Gates: aggregate rel L2 of the recorded strain <= 1e-4 (over all recorded
+cells, points, components, snapshots); per-trace rel L2 (one trace = one
+component at one point over time, traces below 1e-3 of the largest trace's norm
+skipped) is reported, with its 99th percentile held to 1e-3."""
+import hashlib
+
+import numpy as np
+import pytest
+
+import meshcases as mc
+import pygfx
+import pyoracle
+from pygfx import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+TOL, TOL_TRACE99 = 1e-4, 1e-3
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if pygfx.device_count() < 1:
+        pytest.fail("no HIP device visible; GPU tests must run on an MI355X box")
+
+
+def _records(s, field, n_snap):
+    return np.stack([s.get_record(field, k) for k in range(n_snap)]).reshape(n_snap, s.n_rec, 125, 6)
+
+
+def _oracle(mesh, cells, w, stf, direction, nsteps=None):
+    n = mesh.nsteps if nsteps is None else nsteps
+    return pyoracle.Run(mesh.arrays, mesh.n_elem, 5, mesh.n_node, mesh.dt, n, mesh.snapshot_stride, direction,
+                        cells, w, stf, mesh.rec_cell).run()
+
+
+def _report(tag, got, ref, windows=4):
+    agg = mc.rel_l2(got, ref)
+    tr = mc.per_trace_rel_l2(got, ref)
+    q = np.array_split(np.arange(ref.shape[0]), windows)
+    growth = [mc.rel_l2(got[i], ref[i]) for i in q]
+    print(f"{tag}: aggregate {agg:.3e}; per-trace ({tr.size} traces) median {np.median(tr):.3e} "
+          f"p99 {np.quantile(tr, 0.99):.3e} max {tr.max():.3e}; by time window "
+          + " ".join(f"{g:.2e}" for g in growth), flush=True)
+    return agg, tr
+
+
+@pytest.fixture(scope="module")
+def halfspace():
+    mesh = pygfx.build_box(W.halfspace_spec())
+    assert (mesh.n_elem, mesh.nsteps, len(mesh.rec_cell)) == (4000, 800, 392)
+    return mesh
+
+
+def test_halfspace_full_run_matches_oracle(halfspace):
+    m = halfspace
+    cells, w = m.locate_source(10000.0, 10000.0, 0.0)
+    stf = pygfx.ricker(1.0, 1.5, 1e20, m.dt, m.nsteps)
+    print(f"case: halfspace 20x20x10, {m.nsteps} steps, z force", flush=True)
+    s = pygfx.Solver(m, n_fields=1, assembly=pygfx.GFX_ASSEMBLY_GATHER)
+    s.set_source(0, 2, cells, w, stf)
+    s.run(m.nsteps)
+    got = _records(s, 0, s.n_snapshots)
+    s.close()
+    ref = _oracle(m, cells, w, stf, 2)["strain"]
+    assert got.shape == ref.shape == (200, 392, 125, 6)
+    agg, tr = _report("halfspace z", got, ref)
+    assert agg <= TOL and np.quantile(tr, 0.99) <= TOL_TRACE99
+
+
+@pytest.fixture(scope="module")
+def stations_run(halfspace):
+    m = halfspace
+    st = W.stations(m, n=8, seed=3, half=6000.0)
+    stf = pygfx.ricker(1.0, 1.5, 1e20, m.dt, m.nsteps)
+    s = pygfx.Solver(m, n_fields=24, assembly=pygfx.GFX_ASSEMBLY_GATHER)
+    srcs = []
+    for f in range(24):
+        cells, w = m.locate_source(st[f // 3][0], st[f // 3][1], 0.0)
+        s.set_source(f, f % 3, cells, w, stf)
+        srcs.append((cells, w))
+    s.run(m.nsteps)
+    recs = [_records(s, f, s.n_snapshots) for f in range(24)]
+    s.close()
+    return st, stf, srcs, recs
+
+
+@pytest.mark.parametrize("station", range(8))
+def test_halfspace_24_fields_each_match_oracle(halfspace, stations_run, station):
+    m = halfspace
+    st, stf, srcs, recs = stations_run
+    x, y = st[station]
+    assert 4000.0 <= x <= 16000.0 and 4000.0 <= y <= 16000.0  # central 12 x 12 km, outside the PML
+    for d in range(3):
+        f = 3 * station + d
+        cells, w = srcs[f]
+        ref = _oracle(m, cells, w, stf, d)["strain"]
+        agg, tr = _report(f"station {station} ({x:.0f}, {y:.0f}) dir {'xyz'[d]} (field {f} of 24)", recs[f], ref)
+        assert agg <= TOL and np.quantile(tr, 0.99) <= TOL_TRACE99
+
+
+def test_layered_box_6000_steps_drift():
+    sp = W.layered_spec(14, 14, 8, duration=33.34)
+    sp.layer_zbot[0], sp.layer_zbot[1] = 1200.0, 3000.0
+    m = pygfx.build_box(sp)
+    assert m.nsteps == 6000
+    cells, w = m.locate_source(0.5 * m.bounds[1], 0.5 * m.bounds[3], 0.0)
+    stf = pygfx.ricker(1.5, 1.0, 1e20, m.dt, m.nsteps)
+    print(f"case: layered 14x14x8 (dt {m.dt:.4g} s), {m.nsteps} steps, z force", flush=True)
+    s = pygfx.Solver(m, n_fields=1, assembly=pygfx.GFX_ASSEMBLY_GATHER)
+    s.set_source(0, 2, cells, w, stf)
+    s.run(m.nsteps)
+    got = _records(s, 0, s.n_snapshots)
+    s.close()
+    ref = _oracle(m, cells, w, stf, 2)["strain"]
+    agg, tr = _report("layered 6000 steps", got, ref, windows=6)
+    assert agg <= TOL and np.quantile(tr, 0.99) <= TOL_TRACE99
+
+
+def test_uneven_16_meshes_batched_equal_single_runs():
+    specs = W.uneven_specs(n=16, seed=4)
+    meshes = [pygfx.build_box(sp, copy=False) for sp in specs]
+    E = sum(m.n_elem for m in meshes)
+    print(f"case: 16 uneven meshes, {E} elements", flush=True)
+    assert E > 1_000_000
+    steps = 30
+    solvers = []
+    for m in meshes:
+        s = pygfx.Solver(m, n_fields=3, assembly=pygfx.GFX_ASSEMBLY_GATHER, record_capacity=8)
+        cells, w = m.locate_source(0.5 * m.bounds[1], 0.5 * m.bounds[3], 0.0)
+        stf = pygfx.ricker(1.5, 1.0, 1e20, m.dt, m.nsteps)
+        for f in range(3):
+            s.set_source(f, f, cells, w, stf)
+        solvers.append(s)
+
+    def snap(s):
+        """Digests of every field's u, v, a and records (equal digests: equal bits)."""
+        arrays = [x for f in range(3) for x in s.get_state(f)]
+        arrays += [s.get_record(f, k) for f in range(3) for k in range(s.n_snapshots)]
+        return [hashlib.blake2b(np.ascontiguousarray(x).tobytes(), digest_size=20).hexdigest() for x in arrays]
+
+    single = []
+    for s in solvers:
+        s.run(steps)
+        single.append(snap(s))
+    for groups in (0, 2):
+        for s in solvers:
+            s.reset()
+        b = pygfx.Batch(solvers, groups=groups)
+        b.run(steps)
+        assert b.last_groups == groups
+        b.close()
+        for i, s in enumerate(solvers):
+            got = snap(s)
+            assert len(got) == len(single[i])
+            assert got == single[i], (groups, i)
+    for s in solvers:
+        s.close()
+    for m in meshes:
+        m.close()

@@ -389,3 +389,233 @@ def test_cpml_accel_term_is_weighted_sum():
         pyoracle.lib.gfo_cpml_accel_contribution(C.byref(p), pyoracle.P(u), pyoracle.P(v), pyoracle.P(w), pyoracle.P(r))
         scale = 1.0 * 1.0 * 1.0 * 3.0 / 2.0  # N = 1: w = 1
         assert np.allclose(r.reshape(nn, 3), scale * np.asarray(expect)[None, :])
+
+
+# ------------------------------------- legacy ramp damping (forward/share/src/pml.cpp:13-29)
+
+
+def _damp(d, v):
+    v = np.ascontiguousarray(v, np.float64).copy()
+    pyoracle.lib.gfo_apply_pml_damping(pyoracle.P(np.ascontiguousarray(d, np.float64)), pyoracle.P(v), v.size)
+    return v
+
+
+def test_legacy_damping_reference_cases():
+    """The four cases of the reference's tests/test_pml.cpp:11-101 (NGLL 4, one or two
+    elements, damping per node shared by its 3 DOFs)."""
+    n_node = 2 * 4 ** 3  # test_pml.cpp:11-28: zero damping leaves v unchanged
+    assert np.all(_damp(np.zeros(n_node), np.ones(3 * n_node)) == 1.0)
+    n_node = 4 ** 3  # :30-56: d = 0.5 on the first half halves v there, the rest stays 2
+    d = np.zeros(n_node)
+    d[: n_node // 2] = 0.5
+    v = _damp(d, np.full(3 * n_node, 2.0))
+    assert np.all(v[: 3 * (n_node // 2)] == 1.0) and np.all(v[3 * (n_node // 2):] == 2.0)
+    assert np.all(_damp(np.ones(n_node), np.full(3 * n_node, 3.0)) == 0.0)  # :58-75
+    d = np.zeros(n_node)  # :77-101: damping is per node, not per DOF
+    d[0] = 0.8
+    v = _damp(d, np.ones(3 * n_node))
+    assert np.allclose(v[:3], 0.2, atol=1e-15) and np.all(v[3:] == 1.0)
+
+
+def test_legacy_damping_whole_run_closed_form():
+    """Without C-PML, solver.cpp:563-574 damps v after the predictor and before the
+    corrector, whose u update then uses the damped v.  A rigid translation with v0 and a
+    uniform d has K u = 0, so v_n = (1-d)^n v0 and u_n = dt v0 sum_{k=1..n} (1-d)^k.
+    (Damping after the corrector would give u_1 = dt v0; damping before the predictor
+    would give a u that leads v by one factor.)"""
+    m = _free_mesh(2, 1, 1)
+    arrays = dict(m.arrays)
+    d = 0.05
+    arrays["damping"] = np.full_like(m.damping, d)
+    dt, n = 1e-3, 7
+    run = pyoracle.Run(arrays, m.n_elem, m.ngll, m.n_node, dt, n, 0, 2, None, None, None, None,
+                       has_cpml=False)
+    v0 = np.array([0.3, -1.2, 2.0])
+    u, v, a = run.run_state(np.zeros(3 * m.n_node), np.tile(v0, m.n_node), np.zeros(3 * m.n_node))
+    q = 1.0 - d
+    assert np.allclose(v.reshape(-1, 3), q ** n * v0, rtol=1e-12, atol=1e-12)
+    want_u = dt * v0 * sum(q ** k for k in range(1, n + 1))
+    assert np.allclose(u.reshape(-1, 3), want_u, rtol=1e-11, atol=1e-15)
+    assert np.abs(a).max() < 1e-9 * np.abs(v0).max() / dt
+
+
+# ------------------------------------- C-PML step by hand (forward/share/src/pml.cpp:55-216)
+
+
+def _cpml_state(ng, n_elem, region, alpha, abar, rho=None, jac=None):
+    import ctypes as C
+    nn = ng ** 3
+    keep = dict(region=np.asarray(region, np.int32), alpha=np.ascontiguousarray(alpha, np.float64).ravel(),
+                abar=np.ascontiguousarray(abar, np.float64).ravel(),
+                rho=np.ones(n_elem * nn) if rho is None else rho, jac=np.ones(n_elem * nn) if jac is None else jac,
+                ib=np.arange(n_elem * nn, dtype=np.int32), old=np.zeros(3 * n_elem * nn),
+                new=np.zeros(3 * n_elem * nn), mem=np.zeros(9 * n_elem * nn))
+    p = pyoracle.Cpml(n_elem, ng, pyoracle.P(keep["region"], C.c_int32), pyoracle.P(keep["alpha"]),
+                      pyoracle.P(keep["abar"]), pyoracle.P(keep["rho"]), pyoracle.P(keep["jac"]),
+                      pyoracle.P(keep["ib"], C.c_int32), pyoracle.P(keep["old"]), pyoracle.P(keep["new"]),
+                      pyoracle.P(keep["mem"]))
+    return p, keep
+
+
+def test_cpml_displ_fields_three_steps_known_answer():
+    """pml.cpp:55-110 over three calls with v, a != 0: old <- previous new (the swap),
+    new = u + c1 v, old += c2 a, with c1 = (1 - 2 theta) dt / 2 and c2 = (1 - theta) dt^2 / 2,
+    theta = 1/8.  dt = 0.1 gives c1 = 0.0375, c2 = 0.004375.  Expected values worked by hand:
+      call 1  u (1, 2, 3)     v (10, 20, 30)  a (100, 200, 300)
+              new = (1.375, 2.75, 4.125), old = 0 + c2 a = (0.4375, 0.875, 1.3125)
+      call 2  u (-1, .5, 2)   v (4, -8, 0)    a (-40, 16, 8)
+              old = (1.375, 2.75, 4.125) + c2 a = (1.2, 2.82, 4.16), new = (-0.85, 0.2, 2)
+      call 3  u (2, -2, 1)    v (-16, 16, 8)  a (80, -160, 0)
+              old = (-0.85, 0.2, 2) + c2 a = (-0.5, -0.5, 2), new = (1.4, -1.4, 1.3)
+    Element 1 is interior (region 0) and is never touched."""
+    import ctypes as C
+    ng, nn = 2, 8
+    p, keep = _cpml_state(ng, 2, [3, 0], np.zeros(2 * nn * 9), np.zeros(2 * nn * 5))
+    calls = [((1, 2, 3), (10, 20, 30), (100, 200, 300), (1.375, 2.75, 4.125), (0.4375, 0.875, 1.3125)),
+             ((-1, .5, 2), (4, -8, 0), (-40, 16, 8), (-0.85, 0.2, 2), (1.2, 2.82, 4.16)),
+             ((2, -2, 1), (-16, 16, 8), (80, -160, 0), (1.4, -1.4, 1.3), (-0.5, -0.5, 2))]
+    for u, v, a, want_new, want_old in calls:
+        uu, vv, aa = (np.tile(np.asarray(x, np.float64), 2 * nn) for x in (u, v, a))
+        pyoracle.lib.gfo_cpml_update_displ_fields(C.byref(p), pyoracle.P(uu), pyoracle.P(vv), pyoracle.P(aa), 0.1)
+        new = np.ctypeslib.as_array(p.displ_new, shape=(2 * nn * 3,)).reshape(2, nn, 3)
+        old = np.ctypeslib.as_array(p.displ_old, shape=(2 * nn * 3,)).reshape(2, nn, 3)
+        assert np.allclose(new[0], want_new, rtol=0, atol=1e-14)
+        assert np.allclose(old[0], want_old, rtol=0, atol=1e-14)
+        assert np.all(new[1] == 0.0) and np.all(old[1] == 0.0)
+
+
+def test_cpml_memory_update_known_answer():
+    """pml.cpp:112-149: m[c*3+d] = alpha[d*3+0] m + alpha[d*3+1] new_c + alpha[d*3+2] old_c,
+    with distinct coefficients per direction, non-zero memory and distinct components.
+    alpha_x = (.5, .25, .125), alpha_y = (.9, .1, .05), alpha_z = (0, 1, -1);
+    m[c][d] = 10 c + d + 1; new = (1.4, -1.4, 1.3); old = (-0.5, -0.5, 2).  By hand:
+      c = 0: x .5*1 + .25*1.4 - .125*.5 = 0.7875,  y .9*2 + .1*1.4 - .05*.5 = 1.915,   z 1.4 + .5 = 1.9
+      c = 1: x 5.5 - .35 - .0625 = 5.0875,         y 10.8 - .14 - .025 = 10.635,      z -1.4 + .5 = -0.9
+      c = 2: x 10.5 + .325 + .25 = 11.075,         y 19.8 + .13 + .1 = 20.03,         z 1.3 - 2 = -0.7"""
+    import ctypes as C
+    ng, nn = 2, 8
+    alpha = np.tile([.5, .25, .125, .9, .1, .05, 0.0, 1.0, -1.0], nn)
+    p, keep = _cpml_state(ng, 1, [7], alpha, np.zeros(nn * 5))
+    keep["mem"][:] = np.tile([1, 2, 3, 11, 12, 13, 21, 22, 23], nn)
+    keep["new"][:] = np.tile([1.4, -1.4, 1.3], nn)
+    keep["old"][:] = np.tile([-0.5, -0.5, 2.0], nn)
+    pyoracle.lib.gfo_cpml_update_displ_memory(C.byref(p))
+    want = [0.7875, 1.915, 1.9, 5.0875, 10.635, -0.9, 11.075, 20.03, -0.7]
+    assert np.allclose(keep["mem"].reshape(nn, 9), want, rtol=0, atol=1e-13)
+
+
+def _one_element_n1(abar, alpha, region=7):
+    """One N = 1 element on [-1, 1]^3 with rho = J = 1 (GLL weights 1): every point is its
+    own node with mass 1, so a rigid translation has K u = 0 exactly and each node's C-PML
+    acceleration is abar . (v, u, m_x, m_y, m_z)."""
+    nn = 8
+    eye = np.tile(np.eye(3).ravel(), nn)
+    return dict(dxi_dx=eye, jacobian=np.ones(nn), lambda_=np.full(nn, 2.0), mu_=np.ones(nn),
+                mass=np.ones(nn), density=np.ones(nn), damping=np.zeros(nn),
+                ibool=np.arange(nn, dtype=np.int32), pml_region=np.array([region], np.int32),
+                coef_alpha=np.tile(alpha, nn), coef_abar=np.tile(abar, nn))
+
+
+def test_cpml_whole_step_three_steps_known_answer():
+    """Three steps of the CPU branch of solver.cpp:510-621 with C-PML on a rigid
+    translation, worked by hand from its call order: fields from (u_n, v_n, a_n) before the
+    element kernel (pml.cpp:55-110), a_{n+1} = A1 v_n + A2 u_n + A3 m_x + A4 m_y + A5 m_z with
+    the memory of the previous step, Newmark corrector, then the memory update from the new
+    and old fields (pml.cpp:112-149).  dt = 0.1, A = (1, -2, 4, 8, 16), alpha_x = (.5, .25,
+    .125), alpha_y = (.9, .1, .05), alpha_z = (0, 1, -1).  x starts from (u, v, a) = (0, 1, 0):
+      step 0: new 0.0375, old 0, a 1, u 0.1, v 1.05, m (0.009375, 0.00375, 0.0375)
+      step 1: new 0.139375, old 0.041875, a 1.05 - 0.2 + 0.0375 + 0.03 + 0.6 = 1.5175,
+              u 0.21, v 1.175875, m (0.044765625, 0.01940625, 0.0975)
+      step 2: a 1.175875 - 0.42 + 0.1790625 + 0.15525 + 1.56 = 2.6501875,
+              u 0.21 + 0.1175875 + 0.0075875 = 0.335175, v 1.175875 + 0.05 (1.5175 + 2.6501875)
+    y from (1, 0, 0) and z from (0, 0, 2) are worked the same way.  Using u~ instead of u_n
+    in the C-PML term, updating the memory before it is used, or taking 'old' from the
+    current 'new' each fails."""
+    alpha = [.5, .25, .125, .9, .1, .05, 0.0, 1.0, -1.0]
+    arrays = _one_element_n1([1.0, -2.0, 4.0, 8.0, 16.0], alpha)
+    run = pyoracle.Run(arrays, 1, 2, 8, 0.1, 3, 0, 2, None, None, None, None, has_cpml=True)
+    u0 = np.tile([0.0, 1.0, 0.0], 8)
+    v0 = np.tile([1.0, 0.0, 0.0], 8)
+    a0 = np.tile([0.0, 0.0, 2.0], 8)
+    u, v, a = run.run_state(u0, v0, a0)
+    want_u = [0.335175, 1.117, 0.02947875]
+    want_v = [1.384259375, 1.50051875, 0.1101615625]
+    want_a = [2.6501875, 2.610375, 0.30748125]
+    for got, want in ((u, want_u), (v, want_v), (a, want_a)):
+        assert np.allclose(got.reshape(8, 3), want, rtol=1e-13, atol=1e-15)
+    # an interior element (region 0) takes no C-PML term: plain rigid motion
+    arrays = _one_element_n1([1.0, -2.0, 4.0, 8.0, 16.0], alpha, region=0)
+    run = pyoracle.Run(arrays, 1, 2, 8, 0.1, 3, 0, 2, None, None, None, None, has_cpml=True)
+    u, v, a = run.run_state(u0, v0, a0)
+    assert np.allclose(u.reshape(8, 3), [0.3, 1.0, 0.03], rtol=0, atol=1e-14)
+    assert np.allclose(v.reshape(8, 3), [1.0, 0.0, 0.1], rtol=0, atol=1e-14)
+    assert np.all(a == 0.0)
+
+
+# ------------------------------------------ recorded strain (forward/share/src/solver.cpp:282-335)
+
+
+def test_full_strain_is_exact_for_affine_displacement():
+    """u = A x + b at the nodes of a jittered (non-affine) order-4 mesh: the element maps
+    are trilinear, so u is a polynomial of degree <= 1 per reference axis, the GLL
+    derivative is exact, and the strain at every point of every cell is sym(A) in the
+    reference's Voigt order (xx, yy, zz, (u_x,y + u_y,x)/2, (u_x,z + u_z,x)/2,
+    (u_y,z + u_z,y)/2).  A transposed dxi_dx, a wrong D index or a dropped chain-rule
+    term fails it (the jitter makes dxi_dx full and non-symmetric)."""
+    import ctypes as C
+
+    import pygfx
+    m = pygfx.build_box(nx=3, ny=3, nz=2, lx=2700.0, ly=2700.0, lz=1800.0, order=4,
+                        layers=[(0.0, 3000.0, 1700.0, 2400.0)], jitter=0.15, jitter_seed=9,
+                        pml_xmin=0, pml_xmax=0, pml_ymin=0, pml_ymax=0, pml_zmax=0,
+                        record_depth_max=0.0, duration=0.1)
+    dd = m.dxi_dx.reshape(-1, 3, 3)
+    assert np.abs(dd - dd.transpose(0, 2, 1)).max() > 1e-6 * np.abs(dd).max()  # non-symmetric
+    A = np.array([[0.3, -1.1, 0.7], [2.0, -0.4, 0.9], [-0.6, 1.3, 0.25]]) * 1e-3
+    xyz = m.node_xyz.reshape(-1, 3)
+    ug = (xyz @ A.T + np.array([5.0, -2.0, 1.0])).ravel()
+    nn = 125
+    uloc = pyoracle.gather(ug, m.ibool, m.n_elem, nn)
+    x, w, D = pyoracle.gll(4)
+    strain = np.zeros(m.n_elem * nn * 6)
+    pyoracle.lib.gfo_full_strain(m.n_elem, pyoracle.P(None, C.c_int32), pyoracle.P(m.dxi_dx),
+                                 pyoracle.P(D), 5, pyoracle.P(uloc), pyoracle.P(strain))
+    want = np.array([A[0, 0], A[1, 1], A[2, 2], 0.5 * (A[0, 1] + A[1, 0]), 0.5 * (A[0, 2] + A[2, 0]),
+                     0.5 * (A[1, 2] + A[2, 1])])
+    assert np.allclose(strain.reshape(-1, 6), want, rtol=0, atol=1e-12 * np.abs(A).max())
+    # the recording subset: the same values for a list of cells, in list order
+    cells = np.array([5, 0, 11], np.int32)
+    sub = np.zeros(len(cells) * nn * 6)
+    pyoracle.lib.gfo_full_strain(len(cells), pyoracle.P(cells, C.c_int32), pyoracle.P(m.dxi_dx),
+                                 pyoracle.P(D), 5, pyoracle.P(uloc), pyoracle.P(sub))
+    assert np.array_equal(sub.reshape(3, nn, 6), strain.reshape(-1, nn, 6)[cells])
+
+
+def test_run_from_is_the_reference_resume():
+    """gfo_run_from(k + 1, state after step k) is solver.cpp:369-388's resume.  Without
+    C-PML (whose memory the reference's restart does not keep) it continues the
+    uninterrupted run exactly: same records for the later snapshot steps, same final
+    state; started at step 0 from zeros it is gfo_run."""
+    import meshcases as mc
+    m = mc.small_halfspace(nx=4, ny=4, nz=3, pml=1, duration=1.0)
+    cells, w, stf = mc.surface_centre_source(m)
+    full = pyoracle.Run(m.arrays, m.n_elem, 5, m.n_node, m.dt, m.nsteps, m.snapshot_stride, 2, cells, w, stf,
+                        m.rec_cell, has_cpml=False).run(uva=True)
+    k = 37
+    head = pyoracle.Run(m.arrays, m.n_elem, 5, m.n_node, m.dt, k + 1, m.snapshot_stride, 2, cells, w, stf,
+                        m.rec_cell, has_cpml=False)
+    z = np.zeros(3 * m.n_node)
+    u, v, a = head.run_state(z, z, z)
+    tail = pyoracle.Run(m.arrays, m.n_elem, 5, m.n_node, m.dt, m.nsteps, m.snapshot_stride, 2, cells, w, stf,
+                        m.rec_cell, has_cpml=False).run_from(k + 1, u, v, a, uva=True)
+    later = [i for i, s in enumerate(range(0, m.nsteps, m.snapshot_stride)) if s > k]
+    assert tail["strain"].shape[0] == len(later)
+    for key in ("strain", "disp", "vel", "acc"):
+        assert np.array_equal(tail[key], full[key][later]), key
+    for key in ("u", "v", "a"):
+        assert np.array_equal(tail[key], full[key]), key
+    again = pyoracle.Run(m.arrays, m.n_elem, 5, m.n_node, m.dt, m.nsteps, m.snapshot_stride, 2, cells, w, stf,
+                         m.rec_cell).run_from(0, z, z, z)
+    ref = pyoracle.Run(m.arrays, m.n_elem, 5, m.n_node, m.dt, m.nsteps, m.snapshot_stride, 2, cells, w, stf,
+                       m.rec_cell).run()
+    assert np.array_equal(again["strain"], ref["strain"]) and np.array_equal(again["u"], ref["u"])
