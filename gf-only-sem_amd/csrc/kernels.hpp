@@ -140,6 +140,37 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The flux of one point from its reference-space gradient du[c][a] =
+// d u_c / d xi_a: chain rule with the scaled geometry g, small strain,
+// isotropic stress, then F[c][a] = sum_m sigma[c][m] g[m][a]
+// (element_cpu.cpp:51-108).
+template <bool DIAG>
+__device__ __forceinline__ void flux_from_du(const float (&du)[3][3], const float* g, float lam, float mu,
+                                             float* F) {
+    float gr[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int m = 0; m < 3; ++m)
+            gr[c][m] = DIAG ? du[c][m] * g[m * 4]
+                            : du[c][0] * g[m * 3 + 0] + du[c][1] * g[m * 3 + 1] + du[c][2] * g[m * 3 + 2];
+    const float tr = gr[0][0] + gr[1][1] + gr[2][2];
+    const float mu2 = 2.f * mu;
+    float sig[3][3];
+    sig[0][0] = mu2 * gr[0][0] + lam * tr;
+    sig[1][1] = mu2 * gr[1][1] + lam * tr;
+    sig[2][2] = mu2 * gr[2][2] + lam * tr;
+    sig[0][1] = sig[1][0] = mu * (gr[0][1] + gr[1][0]);
+    sig[0][2] = sig[2][0] = mu * (gr[0][2] + gr[2][0]);
+    sig[1][2] = sig[2][1] = mu * (gr[1][2] + gr[2][1]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            F[c * 3 + a] = DIAG ? sig[c][a] * g[a * 4]
+                                : sig[c][0] * g[0 * 3 + a] + sig[c][1] * g[1 * 3 + a] + sig[c][2] * g[2 * 3 + a];
+}
+
 // Element force of one GLL point (i,j,k) from its three rows of displacement
 // neighbours ui[c][s] = u_c(s,j,k), uj[c][s] = u_c(i,s,k), uk[c][s] = u_c(i,j,s)
 // and the scaled geometry g (dxi_a/dx_m sqrt(J w)): writes the 9 fluxes
@@ -166,28 +197,7 @@ __device__ __forceinline__ void flux_core(const float (&ui)[3][5], const float (
         du[c][1] = a1;
         du[c][2] = a2;
     }
-    float gr[3][3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int m = 0; m < 3; ++m)
-            gr[c][m] = DIAG ? du[c][m] * g[m * 4]
-                            : du[c][0] * g[m * 3 + 0] + du[c][1] * g[m * 3 + 1] + du[c][2] * g[m * 3 + 2];
-    const float tr = gr[0][0] + gr[1][1] + gr[2][2];
-    const float mu2 = 2.f * mu;
-    float sig[3][3];
-    sig[0][0] = mu2 * gr[0][0] + lam * tr;
-    sig[1][1] = mu2 * gr[1][1] + lam * tr;
-    sig[2][2] = mu2 * gr[2][2] + lam * tr;
-    sig[0][1] = sig[1][0] = mu * (gr[0][1] + gr[1][0]);
-    sig[0][2] = sig[2][0] = mu * (gr[0][2] + gr[2][0]);
-    sig[1][2] = sig[2][1] = mu * (gr[1][2] + gr[2][1]);
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-            F[c * 3 + a] = DIAG ? sig[c][a] * g[a * 4]
-                                : sig[c][0] * g[0 * 3 + a] + sig[c][1] * g[1 * 3 + a] + sig[c][2] * g[2 * 3 + a];
+    flux_from_du<DIAG>(du, g, lam, mu, F);
 }
 
 // Geometry of one element point: all 9 scaled entries, or (DIAG) the three
@@ -766,6 +776,180 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
                 add_sources(A, step, f + 1, sb, se, p, rv1);
                 emit_residual<MODE>(A, f + 1, node, rslot, rold[1], rv1);
             }
+        }
+    }
+}
+
+// Field-vector block kernel: as k_element, with FV fields per pass staged
+// together in LDS -- each LDS entry holds one value of FV consecutive fields
+// (8 or 16 bytes), so every neighbour read (45 for the gradient, 45 for the
+// divergence) is one wide read serving FV fields, and a pass needs two
+// barriers for FV fields.  The per-field operation sequence is flux_core's
+// and div_core's, so the residuals equal the other forms' bit for bit.  A
+// final pass with fewer than FV fields left computes the missing ones on
+// zeros and stores nothing for them.  Nodal C-PML only.
+template <int EPB, int MODE, bool DIAG, int FV>
+__global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
+    k_element_vec(ElemArgs A, int e0, int ne) {
+    using V = float __attribute__((ext_vector_type(FV)));
+    __shared__ float sD[25];
+    __shared__ V su[EPB][3][NP];
+    __shared__ V sf[EPB][9][NP];
+    const int tid = threadIdx.x;
+    if (tid < 25) sD[tid] = A.D[tid];
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int le = tid / NP;
+    const int p = tid - le * NP;
+    const int eloc = blk * EPB + le;
+    const bool valid = (le < EPB) && (eloc < ne);
+    const int e = e0 + (valid ? eloc : 0);
+    const int i = p / 25, j = (p / 5) % 5, k = p % 5;
+    const int lel = valid ? le : 0;
+
+    float g[9];
+    float lam = 0.f, mu = 0.f;
+    uint32_t nodef = 0;
+    const int64_t pt = int64_t(e) * NP + p;
+    if (valid) {
+        load_geometry<DIAG>(A, pt, g);
+        lam = __builtin_nontemporal_load(&A.lam[pt]);
+        mu = __builtin_nontemporal_load(&A.mu[pt]);
+        nodef = __builtin_nontemporal_load(&A.ibool[pt]);
+    } else {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) g[c] = 0.f;
+    }
+    const int64_t node = int64_t(nodef & ~FIRST_TOUCH);
+    const bool first = (nodef & FIRST_TOUCH) != 0;
+    int sb = 0, se = 0;
+    if (valid) {
+        sb = A.src_off[e];
+        se = A.src_off[e + 1];
+    }
+    const int step = A.ctr[0];
+    const int nf = A.nf;
+    float un[FV][3];  // the pass's fields, loaded a pass ahead
+#pragma unroll
+    for (int h = 0; h < FV; ++h) {
+        un[h][0] = un[h][1] = un[h][2] = 0.f;
+        if (valid && h < nf) {
+            const float* ut = A.U + int64_t(h) * A.fstride + node * 3;
+            un[h][0] = ut[0];
+            un[h][1] = ut[1];
+            un[h][2] = ut[2];
+        }
+    }
+    __syncthreads();
+    float Di[5], Dj[5], Dk[5], Dti[5], Dtj[5], Dtk[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        Di[q] = sD[i * 5 + q];
+        Dj[q] = sD[j * 5 + q];
+        Dk[q] = sD[k * 5 + q];
+        Dti[q] = sD[q * 5 + i];
+        Dtj[q] = sD[q * 5 + j];
+        Dtk[q] = sD[q * 5 + k];
+    }
+    const int ri = j * 5 + k, rj = i * 25 + k, rk = i * 25 + j * 5;  // row bases along i, j, k
+
+    for (int f = 0; f < nf; f += FV) {
+        if (valid) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                V t;
+#pragma unroll
+                for (int h = 0; h < FV; ++h) t[h] = un[h][c];
+                su[le][c][p] = t;
+            }
+#pragma unroll
+            for (int h = 0; h < FV; ++h) {
+                un[h][0] = un[h][1] = un[h][2] = 0.f;
+                if (f + FV + h < nf) {
+                    const float* ut = A.U + int64_t(f + FV + h) * A.fstride + node * 3;
+                    un[h][0] = ut[0];
+                    un[h][1] = ut[1];
+                    un[h][2] = ut[2];
+                }
+            }
+        }
+        float rold[FV][3];
+#pragma unroll
+        for (int h = 0; h < FV; ++h) {
+            rold[h][0] = rold[h][1] = rold[h][2] = 0.f;
+            if (MODE == ASM_COLORED && valid && !first && f + h < nf) {
+                const float* r = A.R + int64_t(f + h) * A.fstride + node * 3;
+                rold[h][0] = r[0];
+                rold[h][1] = r[1];
+                rold[h][2] = r[2];
+            }
+        }
+        __syncthreads();
+        if (valid) {
+            float du[FV][3][3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float a0[FV], a1[FV], a2[FV];
+#pragma unroll
+                for (int h = 0; h < FV; ++h) a0[h] = a1[h] = a2[h] = 0.f;
+#pragma unroll
+                for (int s = 0; s < 5; ++s) {
+                    const V x = su[lel][c][s * 25 + ri];
+                    const V y = su[lel][c][rj + s * 5];
+                    const V z = su[lel][c][rk + s];
+#pragma unroll
+                    for (int h = 0; h < FV; ++h) {
+                        a0[h] += Di[s] * x[h];
+                        a1[h] += Dj[s] * y[h];
+                        a2[h] += Dk[s] * z[h];
+                    }
+                }
+#pragma unroll
+                for (int h = 0; h < FV; ++h) {
+                    du[h][c][0] = a0[h];
+                    du[h][c][1] = a1[h];
+                    du[h][c][2] = a2[h];
+                }
+            }
+            float F[FV][9];
+#pragma unroll
+            for (int h = 0; h < FV; ++h) flux_from_du<DIAG>(du[h], g, lam, mu, F[h]);
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                V t;
+#pragma unroll
+                for (int h = 0; h < FV; ++h) t[h] = F[h][q];
+                sf[le][q][p] = t;
+            }
+        }
+        __syncthreads();
+        if (valid) {
+            float rv[FV][3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float acc[FV];
+#pragma unroll
+                for (int h = 0; h < FV; ++h) acc[h] = 0.f;
+#pragma unroll
+                for (int s = 0; s < 5; ++s) {
+                    const V x = sf[lel][c * 3 + 0][s * 25 + ri];
+                    const V y = sf[lel][c * 3 + 1][rj + s * 5];
+                    const V z = sf[lel][c * 3 + 2][rk + s];
+#pragma unroll
+                    for (int h = 0; h < FV; ++h) {
+                        acc[h] += Dti[s] * x[h];
+                        acc[h] += Dtj[s] * y[h];
+                        acc[h] += Dtk[s] * z[h];
+                    }
+                }
+#pragma unroll
+                for (int h = 0; h < FV; ++h) rv[h][c] = -acc[h];
+            }
+#pragma unroll
+            for (int h = 0; h < FV; ++h)
+                if (f + h < nf) {
+                    add_sources(A, step, f + h, sb, se, p, rv[h]);
+                    emit_residual<MODE>(A, f + h, node, pt, rold[h], rv[h]);
+                }
         }
     }
 }
@@ -1409,6 +1593,48 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64) k_record(RecArgs 
             }
         }
         if (f + 1 < A.nf) __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stage kernels of the per-stage entry points (the reference's device branch,
+// forward/share/src/solver.cpp:390-444, driven stage by stage).
+
+// Assembly of the element-local residuals into R (GATHER): R[f][n] = the
+// node's slots summed in increasing slot order, starting from 0 -- the sum the
+// node kernels form in registers, so a corrector reading R gives the same
+// bits.  One lane per node of [0, G4); nodes without slots get 0.
+__global__ void __launch_bounds__(256) k_scatter_slots(NodeArgs A) {
+    const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int f = blockIdx.y;
+    if (n >= A.G4) return;
+    float r[3];
+    gather_r(A, f, A.off[n], A.off[n + 1], r);
+    float* o = A.R + int64_t(f) * A.fstride + n * 3;
+    o[0] = r[0];
+    o[1] = r[1];
+    o[2] = r[2];
+}
+
+// Point source of one field added to the element stage's residuals:
+// r[cell point p][dir] += stf * w[c][p] (forward/share/src/cuda_step.cu:44-60).
+// One workgroup; lane p takes point p of each cell in turn, with a barrier
+// between cells, so points of different cells at one node never race (COLORED
+// and ATOMIC hold assembled R) and the additions follow the cell order, as
+// the element kernels' own source loop does.  GATHER: into Rl at the point.
+__global__ void __launch_bounds__(128)
+    k_inject(float* __restrict__ Rl, float* __restrict__ R, const uint32_t* __restrict__ ibool,
+             const int32_t* __restrict__ cells, const float* __restrict__ w, int n_cell, int dir, float stf,
+             int64_t P, int64_t fstride, int f, int gather) {
+    const int p = threadIdx.x;
+    for (int c = 0; c < n_cell; ++c) {
+        if (p < NP) {
+            const int64_t pt = int64_t(cells[c]) * NP + p;
+            float* o = gather ? Rl + (int64_t(f) * P + pt) * 3 + dir
+                              : R + int64_t(f) * fstride + int64_t(ibool[pt] & ~FIRST_TOUCH) * 3 + dir;
+            *o += stf * w[int64_t(c) * NP + p];
+        }
+        __syncthreads();
     }
 }
 

@@ -88,6 +88,17 @@ struct gfx_solver {
     bool rec_uva_on = false;
     float D[25];
     int step = 0;
+    // per-stage entry points: position inside the current step (0 = between
+    // steps) and whether the ATOMIC residual was cleared for it; the source
+    // cells (solver numbering) and weights of the last injection
+    int stage = 0;
+    bool stage_zeroed = false;
+    // GATHER: R (not the slot sums of Rl) holds the last step's assembled
+    // residual, as after a step driven stage by stage (the exchange may have
+    // changed it after the scatter)
+    bool r_in_R = false;
+    gfx::DevBuf<int32_t> inj_cells;
+    gfx::DevBuf<float> inj_w;
     // timing
     bool timing = false;
     double t_elem = 0, t_spec = 0, t_reg = 0, t_rec = 0, t_total = 0;
@@ -156,6 +167,9 @@ void init_state(gfx_solver& S) {
     GFX_HIP(hipStreamSynchronize(S.stream));
     S.step = 0;
     S.n_snap = 0;
+    S.stage = 0;
+    S.stage_zeroed = false;
+    S.r_in_R = false;
 }
 
 // Launchers.  DIAG selects the axis-aligned geometry form for a range whose
@@ -220,6 +234,23 @@ void launch_pair_t(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
     }
 }
 
+template <int EPB, int FV, bool DIAG>
+void launch_vec_t(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
+    constexpr int NT = ((EPB * NP + 63) / 64) * 64;
+    const int grid = (n + EPB - 1) / EPB;
+    if (grid == 0) return;
+    switch (S.assembly) {
+        case GFX_ASSEMBLY_ATOMIC:
+            dev::k_element_vec<EPB, dev::ASM_ATOMIC, DIAG, FV><<<grid, NT, 0, S.stream>>>(a, b, n);
+            break;
+        case GFX_ASSEMBLY_GATHER:
+            dev::k_element_vec<EPB, dev::ASM_GATHER, DIAG, FV><<<grid, NT, 0, S.stream>>>(a, b, n);
+            break;
+        default:
+            dev::k_element_vec<EPB, dev::ASM_COLORED, DIAG, FV><<<grid, NT, 0, S.stream>>>(a, b, n);
+    }
+}
+
 template <bool DIAG>
 void launch_pencil(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
     const int grid = (n + dev::PENCIL_EPB - 1) / dev::PENCIL_EPB;
@@ -239,9 +270,9 @@ void launch_pencil(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
 // Element kernel form for a request: AUTO = wave when the nodal C-PML form
 // applies, else block; WAVE, PAIR and PENCIL need the nodal form.
 int resolve_kernel(int requested, bool nodal_ok) {
-    require(requested >= GFX_ELEMENT_AUTO && requested <= GFX_ELEMENT_PENCIL, "unknown element kernel");
-    if (requested == GFX_ELEMENT_WAVE || requested == GFX_ELEMENT_PAIR || requested == GFX_ELEMENT_PENCIL)
-        require(nodal_ok, "the wave, pair and pencil element kernels need the nodal C-PML form");
+    require(requested >= GFX_ELEMENT_AUTO && requested <= GFX_ELEMENT_VEC4, "unknown element kernel");
+    if (requested != GFX_ELEMENT_AUTO && requested != GFX_ELEMENT_BLOCK)
+        require(nodal_ok, "the wave, pair, pencil and field-vector element kernels need the nodal C-PML form");
     if (requested == GFX_ELEMENT_AUTO) return nodal_ok ? GFX_ELEMENT_WAVE : GFX_ELEMENT_BLOCK;
     return requested;
 }
@@ -249,6 +280,8 @@ int resolve_kernel(int requested, bool nodal_ok) {
 void check_epb(int kernel, int epb) {
     require(epb == 0 || epb == 1 || epb == 2 || epb == 4 || epb == 8, "elems_per_block must be 1, 2, 4 or 8");
     require(!(kernel == GFX_ELEMENT_PAIR && epb == 8), "the pair element kernel takes 1, 2 or 4 elements per block");
+    require(!(kernel == GFX_ELEMENT_VEC2 && epb == 8), "the VEC2 element kernel takes 1, 2 or 4 elements per block");
+    require(!(kernel == GFX_ELEMENT_VEC4 && epb > 2), "the VEC4 element kernel takes 1 or 2 elements per block");
     require(!(kernel == GFX_ELEMENT_PENCIL && epb != 0),
             "the pencil element kernel has a fixed ten elements per block (elems_per_block must be 0)");
 }
@@ -263,6 +296,17 @@ template <bool DIAG>
 void launch_element_d(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
     if (S.kernel == GFX_ELEMENT_PENCIL) {
         launch_pencil<DIAG>(S, a, b, n);
+    } else if (S.kernel == GFX_ELEMENT_VEC2) {
+        switch (S.epb) {
+            case 1: launch_vec_t<1, 2, DIAG>(S, a, b, n); break;
+            case 4: launch_vec_t<4, 2, DIAG>(S, a, b, n); break;
+            default: launch_vec_t<2, 2, DIAG>(S, a, b, n); break;
+        }
+    } else if (S.kernel == GFX_ELEMENT_VEC4) {
+        switch (S.epb) {
+            case 1: launch_vec_t<1, 4, DIAG>(S, a, b, n); break;
+            default: launch_vec_t<2, 4, DIAG>(S, a, b, n); break;
+        }
     } else if (S.kernel == GFX_ELEMENT_PAIR) {
         switch (S.epb) {
             case 1: launch_pair_t<1, DIAG>(S, a, b, n); break;
@@ -498,6 +542,7 @@ void ensure_graph(gfx_solver& S) {
 }
 
 void run_steps(gfx_solver& S, int nsteps) {
+    if (nsteps > 0) S.r_in_R = false;
     if (S.use_graph && !S.timing) {
         ensure_graph(S);
         for (int t = 0; t < nsteps; ++t) {
@@ -606,7 +651,7 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
     require(d.assembly == GFX_ASSEMBLY_COLORED || d.assembly == GFX_ASSEMBLY_ATOMIC ||
                 d.assembly == GFX_ASSEMBLY_GATHER,
             "unknown assembly mode");
-    require(d.element_kernel >= GFX_ELEMENT_AUTO && d.element_kernel <= GFX_ELEMENT_PENCIL,
+    require(d.element_kernel >= GFX_ELEMENT_AUTO && d.element_kernel <= GFX_ELEMENT_VEC4,
             "unknown element kernel");
     int epb = d.elems_per_block;
     require(epb == 0 || epb == 1 || epb == 2 || epb == 4 || epb == 8,
@@ -1112,6 +1157,7 @@ extern "C" gfx_status gfx_solver_run(gfx_solver* s, int n_steps) {
     return guarded([&] {
         require(s != nullptr, "null solver");
         require(n_steps >= 0, "n_steps must be >= 0");
+        require(s->stage == 0, "a step driven stage by stage is not finished (gfx_step_newmark_correct)");
         if (n_steps > 0) run_steps(*s, n_steps);
     });
 }
@@ -1139,7 +1185,7 @@ extern "C" gfx_status gfx_solver_get_state(const gfx_solver* s, int field, doubl
         const std::vector<float> hU = fetch(s, s->U.p + int64_t(field) * s->fstride, n3);
         const std::vector<float> hW = fetch(s, s->W.p + int64_t(field) * s->fstride, n3);
         std::vector<float> hR;
-        if (s->assembly == GFX_ASSEMBLY_GATHER) {
+        if (s->assembly == GFX_ASSEMBLY_GATHER && !s->r_in_R) {
             // assembled r of the last step: fixed-order sum of the node's slots, as the
             // node kernels form it
             const int64_t P = int64_t(s->E) * NP;
@@ -1188,7 +1234,10 @@ extern "C" gfx_status gfx_solver_set_state(gfx_solver* s, int field, const doubl
             const double m = s->mass_node[n];
             for (int c = 0; c < 3; ++c) {
                 const int64_t o = int64_t(g) * 3 + c;
-                const double ag = (m > 0.0) ? a[o] : 0.0;  // massless nodes carry a = 0
+                // a massless node keeps the given a until its first corrector sets it to 0,
+                // so the first predictor uses it (solver.cpp:40-69); massless nodes are
+                // special nodes, whose a is stored
+                const double ag = a[o];
                 hU[n * 3 + c] = float(u[o] + dt * v[o] + 0.5 * dt * dt * ag);
                 hW[n * 3 + c] = float(v[o] + 0.5 * dt * ag);
                 hR[n * 3 + c] = float(ag * m);
@@ -1214,6 +1263,224 @@ extern "C" gfx_status gfx_solver_set_state(gfx_solver* s, int field, const doubl
                               hipMemcpyHostToDevice));
         }
     });
+}
+
+// ---------------------------------------------------------------------------
+// Per-stage entry points: the reference's device branch
+// (forward/share/src/solver.cpp:390-444) driven one stage at a time on the
+// solver's stream, with the multi-rank exchange copies between them.  The
+// stages are the fused step cut at the reference's boundaries; the state
+// between steps stays the leapfrog form (U = predicted displacement), so a
+// step driven stage by stage in the reference's order is the same step as
+// gfx_solver_run's.
+namespace gfx {
+namespace {
+
+enum { STAGE_IDLE = 0, STAGE_PREDICTED = 1, STAGE_ELEMENT = 2, STAGE_ASSEMBLED = 3 };
+
+void stage_require(const gfx_solver* s, int want, const char* fn, const char* after) {
+    require(s != nullptr, "null solver");
+    if (s->stage != want)
+        throw Error(GFX_ERR_INVALID, std::string(fn) + " called out of order: it follows " + after);
+}
+
+void check_newmark(const gfx_solver* s, double dt, double beta) {
+    require(beta == 0.0, "the solver's Newmark scheme is explicit: beta must be 0");
+    require(std::fabs(dt - s->dt) <= 1e-12 * std::fabs(s->dt), "dt differs from the solver's dt");
+}
+
+// Caller-numbered FP64 [G*3] <-> solver-numbered FP32 field of a nodal array.
+std::vector<double> nodal_to_caller(const gfx_solver* s, const float* dev_field) {
+    GFX_HIP(hipStreamSynchronize(s->stream));
+    const std::vector<float> h = fetch(s, dev_field, size_t(s->G) * 3);
+    std::vector<double> out(size_t(s->G) * 3);
+    for (int g = 0; g < s->G; ++g) {
+        const int64_t n = s->node_new_of_old[g];
+        for (int c = 0; c < 3; ++c) out[size_t(g) * 3 + c] = h[n * 3 + c];
+    }
+    return out;
+}
+
+void nodal_from_caller(gfx_solver* s, float* dev_field, const double* buf) {
+    std::vector<float> h(size_t(s->G) * 3);
+    for (int g = 0; g < s->G; ++g) {
+        const int64_t n = s->node_new_of_old[g];
+        for (int c = 0; c < 3; ++c) h[n * 3 + c] = float(buf[size_t(g) * 3 + c]);
+    }
+    GFX_HIP(hipStreamSynchronize(s->stream));
+    GFX_HIP(hipMemcpy(dev_field, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+}
+
+}  // namespace
+}  // namespace gfx
+
+extern "C" gfx_status gfx_step_newmark_predict(gfx_solver* s, double dt, double beta) {
+    return guarded([&] {
+        stage_require(s, STAGE_IDLE, "gfx_step_newmark_predict", "gfx_step_newmark_correct (or creation)");
+        check_newmark(s, dt, beta);
+        // U already holds u~_{n+1} = u_n + dt (v_n + dt/2 a_n): the corrector of
+        // the previous step wrote it (and set_state / creation for the first)
+        s->stage = STAGE_PREDICTED;
+        s->stage_zeroed = false;
+    });
+}
+
+extern "C" gfx_status gfx_step_gather_predicted(gfx_solver* s) {
+    return guarded([&] {
+        stage_require(s, STAGE_PREDICTED, "gfx_step_gather_predicted", "gfx_step_newmark_predict");
+        // the element kernels read u~ through the node ids themselves
+    });
+}
+
+extern "C" gfx_status gfx_step_gather_from_rank(gfx_solver* s) {
+    return guarded([&] {
+        require(s != nullptr, "null solver");
+        // no element-local displacement copy exists: every kernel reads the nodal arrays
+    });
+}
+
+extern "C" gfx_status gfx_step_zero_residual(gfx_solver* s) {
+    return guarded([&] {
+        stage_require(s, STAGE_PREDICTED, "gfx_step_zero_residual", "gfx_step_newmark_predict");
+        if (s->assembly == GFX_ASSEMBLY_ATOMIC) {
+            const int64_t r4 = int64_t(s->nf) * s->fstride / 4;
+            dev::k_zero<<<int((r4 + 255) / 256), 256, 0, s->stream>>>(reinterpret_cast<float4*>(s->R.p), r4);
+            GFX_HIP(hipGetLastError());
+        }
+        // COLORED stores at the first touch and GATHER stores per point: nothing to clear
+        s->stage_zeroed = true;
+    });
+}
+
+extern "C" gfx_status gfx_step_element_residual(gfx_solver* s, int ngll, int n_elem) {
+    return guarded([&] {
+        stage_require(s, STAGE_PREDICTED, "gfx_step_element_residual", "gfx_step_newmark_predict");
+        require(ngll == 5, "ngll must be 5");
+        require(n_elem == s->E, "n_elem differs from the solver's element count");
+        require(s->stage_zeroed || s->assembly != GFX_ASSEMBLY_ATOMIC,
+                "ATOMIC assembly adds into the residual: call gfx_step_zero_residual first");
+        StepArgs a = make_step_args(*s);
+        a.ea.nstf = 0;  // sources come from gfx_step_source_injection
+        for (const auto& rg : s->ranges) launch_element(*s, a.ea, rg);
+        GFX_HIP(hipGetLastError());
+        s->stage = STAGE_ELEMENT;
+    });
+}
+
+extern "C" gfx_status gfx_step_pml_damping(gfx_solver* s) {
+    return guarded([&] {
+        stage_require(s, STAGE_ELEMENT, "gfx_step_pml_damping", "gfx_step_element_residual");
+        // the legacy ramp v -= d v (without C-PML) acts on v_n inside the
+        // corrector, before the update, where the reference applies it
+    });
+}
+
+extern "C" gfx_status gfx_step_source_injection(gfx_solver* s, int field, int direction, double stf_val,
+                                                const double* src_weights, int n_src_cell) {
+    return guarded([&] {
+        stage_require(s, STAGE_ELEMENT, "gfx_step_source_injection", "gfx_step_element_residual");
+        check_field(s, field);
+        require(direction >= 0 && direction <= 2, "direction must be 0, 1 or 2");
+        const gfx::FieldSource& fs = s->sources[field];
+        require(n_src_cell == int(fs.cells.size()),
+                "n_src_cell differs from the cells given to gfx_solver_set_source for this field");
+        require(n_src_cell == 0 || src_weights, "missing source weights");
+        std::vector<int32_t> cells;
+        std::vector<float> w;
+        for (int c = 0; c < n_src_cell; ++c) {
+            if (fs.cells[c] < 0) continue;  // source cell not on this rank (solver.cpp:585-587)
+            cells.push_back(s->int_of_elem[fs.cells[c]]);
+            for (int p = 0; p < NP; ++p) w.push_back(float(src_weights[int64_t(c) * NP + p]));
+        }
+        if (cells.empty()) return;
+        GFX_HIP(hipStreamSynchronize(s->stream));  // the previous injection's tables are no longer read
+        if (s->inj_cells.n < cells.size()) s->inj_cells.alloc(cells.size());
+        if (s->inj_w.n < w.size()) s->inj_w.alloc(w.size());
+        GFX_HIP(hipMemcpy(s->inj_cells.p, cells.data(), cells.size() * 4, hipMemcpyHostToDevice));
+        GFX_HIP(hipMemcpy(s->inj_w.p, w.data(), w.size() * 4, hipMemcpyHostToDevice));
+        dev::k_inject<<<1, 128, 0, s->stream>>>(s->Rl.p, s->R.p, s->ibool.p, s->inj_cells.p, s->inj_w.p,
+                                               int(cells.size()), direction, float(stf_val),
+                                               int64_t(s->E) * NP, s->fstride, field,
+                                               s->assembly == GFX_ASSEMBLY_GATHER ? 1 : 0);
+        GFX_HIP(hipGetLastError());
+    });
+}
+
+extern "C" gfx_status gfx_step_scatter_to_rank(gfx_solver* s) {
+    return guarded([&] {
+        stage_require(s, STAGE_ELEMENT, "gfx_step_scatter_to_rank", "gfx_step_element_residual");
+        if (s->assembly == GFX_ASSEMBLY_GATHER) {
+            const StepArgs a = make_step_args(*s);
+            dim3 grid(unsigned((s->G4 + 255) / 256), unsigned(s->nf));
+            dev::k_scatter_slots<<<grid, 256, 0, s->stream>>>(a.na);
+            GFX_HIP(hipGetLastError());
+        }
+        // COLORED and ATOMIC assemble into R inside the element stage
+        s->stage = STAGE_ASSEMBLED;
+    });
+}
+
+extern "C" gfx_status gfx_step_newmark_correct(gfx_solver* s, double dt, double beta, double gamma) {
+    return guarded([&] {
+        stage_require(s, STAGE_ASSEMBLED, "gfx_step_newmark_correct", "gfx_step_scatter_to_rank");
+        check_newmark(s, dt, beta);
+        require(gamma == 0.5, "the solver's Newmark scheme has gamma = 1/2");
+        StepArgs a = make_step_args(*s);
+        a.na.gather = 0;  // R holds the assembled residual (k_scatter_slots, or the exchange)
+        if (a.spec_grid > 0) launch_node_special(*s, a.na, a.spec_grid);
+        if (a.reg_grid > 0) launch_node_regular(*s, a.na, a.reg_grid);
+        if (a.recording && s->step % s->stride == 0 && a.rec_grid > 0)
+            dev::k_record<REPB><<<a.rec_grid, a.rec_threads, 0, s->stream>>>(a.ra);
+        dev::k_advance<<<1, 64, 0, s->stream>>>(s->ctr.p, s->stride);
+        GFX_HIP(hipGetLastError());
+        advance_host(*s);
+        s->stage = STAGE_IDLE;
+        s->r_in_R = true;
+    });
+}
+
+extern "C" gfx_status gfx_step_copy_utilde_to_host(const gfx_solver* s, int field, double* buf) {
+    return guarded([&] {
+        check_field(s, field);
+        require(buf != nullptr, "null buffer");
+        const std::vector<double> h = nodal_to_caller(s, s->U.p + int64_t(field) * s->fstride);
+        std::copy(h.begin(), h.end(), buf);
+    });
+}
+
+extern "C" gfx_status gfx_step_copy_utilde_from_host(gfx_solver* s, int field, const double* buf) {
+    return guarded([&] {
+        stage_require(s, STAGE_PREDICTED, "gfx_step_copy_utilde_from_host", "gfx_step_newmark_predict");
+        check_field(s, field);
+        require(buf != nullptr, "null buffer");
+        nodal_from_caller(s, s->U.p + int64_t(field) * s->fstride, buf);
+    });
+}
+
+extern "C" gfx_status gfx_step_copy_residual_to_host(const gfx_solver* s, int field, double* buf) {
+    return guarded([&] {
+        stage_require(s, STAGE_ASSEMBLED, "gfx_step_copy_residual_to_host", "gfx_step_scatter_to_rank");
+        check_field(s, field);
+        require(buf != nullptr, "null buffer");
+        const std::vector<double> h = nodal_to_caller(s, s->R.p + int64_t(field) * s->fstride);
+        std::copy(h.begin(), h.end(), buf);
+    });
+}
+
+extern "C" gfx_status gfx_step_copy_residual_from_host(gfx_solver* s, int field, const double* buf) {
+    return guarded([&] {
+        stage_require(s, STAGE_ASSEMBLED, "gfx_step_copy_residual_from_host", "gfx_step_scatter_to_rank");
+        check_field(s, field);
+        require(buf != nullptr, "null buffer");
+        nodal_from_caller(s, s->R.p + int64_t(field) * s->fstride, buf);
+    });
+}
+
+extern "C" gfx_status gfx_step_copy_state_to_host(const gfx_solver* s, int field, double* u, double* v,
+                                                  double* a) {
+    return guarded([&] {
+        stage_require(s, STAGE_IDLE, "gfx_step_copy_state_to_host", "gfx_step_newmark_correct");
+    }) == GFX_OK ? gfx_solver_get_state(s, field, u, v, a) : GFX_ERR_INVALID;
 }
 
 extern "C" gfx_status gfx_solver_get_record(const gfx_solver* s, int field, int snap,
@@ -1316,7 +1583,7 @@ extern "C" gfx_status gfx_solver_checkpoint_save(const gfx_solver* s, void* host
         get(s->devp.p, s->devp.n);
         // assembled residual of the last step (what get_state reads for a at
         // regular nodes): R itself, or the fixed-order slot sums
-        if (s->assembly != GFX_ASSEMBLY_GATHER) {
+        if (s->assembly != GFX_ASSEMBLY_GATHER || s->r_in_R) {
             get(s->R.p, fl);
         } else {
             const int64_t P = int64_t(s->E) * NP;
@@ -1375,6 +1642,8 @@ extern "C" gfx_status gfx_solver_checkpoint_load(gfx_solver* s, const void* host
         }
         s->step = int(hdr[5]);
         s->n_snap = int(hdr[6]);
+        s->r_in_R = false;
+        s->stage = 0;
         upload_counters(*s);
     });
 }
@@ -1454,10 +1723,12 @@ extern "C" gfx_status gfx_autotune(const gfx_solver_desc* base, int trial_steps,
         double best_t = 1e300;
         gfx_solver_desc pick = *base;
         const int asms[2] = {GFX_ASSEMBLY_COLORED, GFX_ASSEMBLY_GATHER};
-        constexpr int NL = 9;
+        constexpr int NL = 13;
         const int launch[NL][2] = {{GFX_ELEMENT_BLOCK, 1}, {GFX_ELEMENT_BLOCK, 2}, {GFX_ELEMENT_BLOCK, 4},
                                    {GFX_ELEMENT_WAVE, 2},  {GFX_ELEMENT_WAVE, 4},  {GFX_ELEMENT_WAVE, 8},
-                                   {GFX_ELEMENT_PAIR, 1},  {GFX_ELEMENT_PAIR, 2},  {GFX_ELEMENT_PENCIL, 0}};
+                                   {GFX_ELEMENT_PAIR, 1},  {GFX_ELEMENT_PAIR, 2},  {GFX_ELEMENT_PENCIL, 0},
+                                   {GFX_ELEMENT_VEC2, 1},  {GFX_ELEMENT_VEC2, 2},  {GFX_ELEMENT_VEC4, 1},
+                                   {GFX_ELEMENT_VEC4, 2}};
         auto time_steps = [&](gfx_solver& S) {
             run_steps(S, 1);  // warm-up
             hipEvent_t e0, e1;
@@ -1491,7 +1762,8 @@ extern "C" gfx_status gfx_autotune(const gfx_solver_desc* base, int trial_steps,
             for (int li = 0; li < NL; ++li) {
                 const auto& lc = launch[li];
                 if (lc[0] != GFX_ELEMENT_BLOCK && S->has_cpml && !S->pml_nodal) continue;
-                if (lc[0] == GFX_ELEMENT_PAIR && S->nf < 2) continue;
+                if ((lc[0] == GFX_ELEMENT_PAIR || lc[0] == GFX_ELEMENT_VEC2) && S->nf < 2) continue;
+                if (lc[0] == GFX_ELEMENT_VEC4 && S->nf < 3) continue;
                 S->kernel = lc[0];
                 S->epb = resolve_epb(lc[0], lc[1]);
                 const double per = time_steps(*S);

@@ -13,7 +13,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "libgfx.so")
+# GFX_LIB selects another build of the library (the host-sanitizer build, for CPU checks)
+LIB_PATH = os.environ.get("GFX_LIB") or os.path.join(os.path.dirname(_HERE), "libgfx.so")
 
 GFX_ASSEMBLY_COLORED = 0
 GFX_ASSEMBLY_ATOMIC = 1
@@ -23,6 +24,8 @@ GFX_ELEMENT_BLOCK = 1
 GFX_ELEMENT_WAVE = 2
 GFX_ELEMENT_PAIR = 3
 GFX_ELEMENT_PENCIL = 4
+GFX_ELEMENT_VEC2 = 5
+GFX_ELEMENT_VEC4 = 6
 MAX_LAYERS = 8
 
 _dp = C.POINTER(C.c_double)
@@ -149,6 +152,20 @@ def _load():
         "gfx_solver_get_state": (st, [vp, C.c_int, _dp, _dp, _dp]),
         "gfx_solver_set_state": (st, [vp, C.c_int, _dp, _dp, _dp]),
         "gfx_solver_n_snapshots": (C.c_int, [vp]),
+        "gfx_step_newmark_predict": (st, [vp, C.c_double, C.c_double]),
+        "gfx_step_gather_predicted": (st, [vp]),
+        "gfx_step_gather_from_rank": (st, [vp]),
+        "gfx_step_zero_residual": (st, [vp]),
+        "gfx_step_element_residual": (st, [vp, C.c_int, C.c_int]),
+        "gfx_step_pml_damping": (st, [vp]),
+        "gfx_step_source_injection": (st, [vp, C.c_int, C.c_int, C.c_double, _dp, C.c_int]),
+        "gfx_step_scatter_to_rank": (st, [vp]),
+        "gfx_step_newmark_correct": (st, [vp, C.c_double, C.c_double, C.c_double]),
+        "gfx_step_copy_state_to_host": (st, [vp, C.c_int, _dp, _dp, _dp]),
+        "gfx_step_copy_utilde_to_host": (st, [vp, C.c_int, _dp]),
+        "gfx_step_copy_utilde_from_host": (st, [vp, C.c_int, _dp]),
+        "gfx_step_copy_residual_to_host": (st, [vp, C.c_int, _dp]),
+        "gfx_step_copy_residual_from_host": (st, [vp, C.c_int, _dp]),
         "gfx_solver_get_record": (st, [vp, C.c_int, C.c_int, _dp, _dp, _dp, _dp]),
         "gfx_solver_kernel_time": (st, [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
         "gfx_solver_set_timing": (st, [vp, C.c_int]),
@@ -788,6 +805,25 @@ class Solver:
             return s, u, v, a
         _check(lib.gfx_solver_get_record(self.h, field, snap, _ptr(s), _ptr(None), _ptr(None), _ptr(None)))
         return s
+
+    # Per-stage entry points (the reference's cuda_* step wrappers,
+    # forward/share/include/gf/cuda_step.hpp:81-127).
+    def stage(self, name, *args):
+        """Call gfx_step_<name>(handle, *args); arrays pass as float64 numpy."""
+        fn = getattr(lib, "gfx_step_" + name)
+        keep = [np.ascontiguousarray(a, np.float64) if isinstance(a, np.ndarray) else None for a in args]
+        conv = [_ptr(k) if k is not None else a for k, a in zip(keep, args)]
+        _check(fn(self.h, *conv))
+
+    def stage_status(self, name, *args):
+        """As stage(), returning the status code instead of raising."""
+        return int(getattr(lib, "gfx_step_" + name)(self.h, *args))
+
+    def nodal_copy(self, what, field):
+        """gfx_step_copy_{utilde,residual}_to_host: caller-numbered [n_node*3] float64."""
+        out = np.zeros(3 * self.mesh.n_node)
+        _check(getattr(lib, f"gfx_step_copy_{what}_to_host")(self.h, field, _ptr(out)))
+        return out
 
     def set_timing(self, on=True):
         _check(lib.gfx_solver_set_timing(self.h, 1 if on else 0))

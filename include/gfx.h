@@ -61,9 +61,10 @@ gfx_status gfx_compute_element_residual_f32(int n_elem, const double* dxi_dx,
 
 /* -------------------------------------------------------------------------
  * Device-resident solver state.
- * Replaces CudaDeviceState + cuda_allocate_state/cuda_free_state and the
- * per-step host wrappers (forward/share/include/gf/cuda_step.hpp:20-129),
- * and adds what the CUDA path lacks: the C-PML memory update of the CPU path
+ * Replaces CudaDeviceState + cuda_allocate_state/cuda_free_state
+ * (forward/share/include/gf/cuda_step.hpp:20-79); the per-step host wrappers
+ * (cuda_step.hpp:81-128) are the gfx_step_* entry points below, and
+ * gfx_solver_run is their whole step fused.  Adds what the CUDA path lacks: the C-PML memory update of the CPU path
  * (forward/share/src/pml.cpp:55-216), device strain recording
  * (forward/share/src/solver.cpp:282-335,640-675) and many right-hand sides
  * ("fields") advanced by one launch.
@@ -82,9 +83,13 @@ typedef enum {
     GFX_ELEMENT_BLOCK = 1, /* one lane per GLL point, workgroup barriers between stages */
     GFX_ELEMENT_WAVE = 2,  /* one wavefront per element, two points per lane, no barriers */
     GFX_ELEMENT_PAIR = 3,  /* as BLOCK, two fields per pass (nodal C-PML only) */
-    GFX_ELEMENT_PENCIL = 4 /* lane per 5-point pencil along k, ten elements per workgroup
+    GFX_ELEMENT_PENCIL = 4, /* lane per 5-point pencil along k, ten elements per workgroup
                               (fixed; elems_per_block must be 0), k contractions in
                               registers (nodal C-PML only) */
+    GFX_ELEMENT_VEC2 = 5,  /* as BLOCK, two fields per pass sharing each LDS entry (one
+                              8-byte LDS read per neighbour for both; nodal C-PML only) */
+    GFX_ELEMENT_VEC4 = 6   /* as VEC2 with four fields per pass (16-byte LDS entries;
+                              1 or 2 elements per block) */
 } gfx_element_kernel;
 
 typedef struct {
@@ -172,6 +177,73 @@ gfx_status gfx_solver_get_state(const gfx_solver* s, int field, double* u, doubl
 /* Load a field's nodal state from host FP64 [n_rank_node*3] (for restart). */
 gfx_status gfx_solver_set_state(gfx_solver* s, int field, const double* u, const double* v,
                                 const double* a);
+
+/* -------------------------------------------------------------------------
+ * Per-stage entry points: the reference's per-step device wrappers
+ * (forward/share/include/gf/cuda_step.hpp:81-127), for a driver that keeps the
+ * reference's device branch (forward/share/src/solver.cpp:390-444) and its
+ * multi-rank exchange hooks.  One step, in that order:
+ *   gfx_step_newmark_predict, [gfx_step_copy_utilde_to_host / _from_host],
+ *   gfx_step_gather_predicted, gfx_step_zero_residual,
+ *   gfx_step_element_residual, gfx_step_pml_damping,
+ *   gfx_step_source_injection (per field with a source this step),
+ *   gfx_step_scatter_to_rank, [gfx_step_copy_residual_to_host / _from_host],
+ *   gfx_step_newmark_correct.
+ * Calls out of this order fail with GFX_ERR_INVALID; gfx_solver_run refuses
+ * a solver in the middle of such a step.  The state between steps is the one
+ * gfx_solver_run keeps (the displacement array holds the next step's
+ * predicted displacement), so with GATHER assembly a step driven stage by
+ * stage equals a gfx_solver_run step bit for bit (sources injected with the
+ * values gfx_solver_set_source gave).  Work runs on the solver's stream; the
+ * copies to the host wait for it.  Arrays are caller-numbered FP64
+ * [n_rank_node*3].
+ * ---------------------------------------------------------------------- */
+/* cuda_newmark_predict (cuda_step.hpp:84): checks beta = 0 and dt; the
+ * predicted displacement u + dt v + dt^2/2 a was formed by the previous
+ * step's corrector (or by creation / gfx_solver_set_state), so no device work. */
+gfx_status gfx_step_newmark_predict(gfx_solver* s, double dt, double beta);
+/* cuda_gather_predicted / cuda_gather_from_rank (cuda_step.hpp:113-116): the
+ * element kernels read the nodal arrays through the node ids, so there is no
+ * element-local copy to fill (no device work). */
+gfx_status gfx_step_gather_predicted(gfx_solver* s);
+gfx_status gfx_step_gather_from_rank(gfx_solver* s);
+/* cuda_zero_residual (cuda_step.hpp:87): clears the assembled residual for
+ * ATOMIC assembly (a fill kernel); COLORED and GATHER need no clearing. */
+gfx_status gfx_step_zero_residual(gfx_solver* s);
+/* cuda_launch_element_residual (cuda_step.hpp:107): element internal force of
+ * every field (plus element-local C-PML or attenuation terms where active),
+ * without sources; ngll must be 5 and n_elem the solver's element count. */
+gfx_status gfx_step_element_residual(gfx_solver* s, int ngll, int n_elem);
+/* cuda_pml_damping (cuda_step.hpp:90): the legacy ramp v -= d v (solvers
+ * without C-PML) acts on v_n inside the corrector, where the reference's CPU
+ * step applies it (solver.cpp:565-573); no device work here. */
+gfx_status gfx_step_pml_damping(gfx_solver* s);
+/* cuda_source_injection (cuda_step.hpp:93-94): r[cell point][direction] +=
+ * stf_val * src_weights[c][point] for the n_src_cell cells given to
+ * gfx_solver_set_source for this field (n_src_cell must match; negative cell
+ * ids are skipped, as off-rank cells are). */
+gfx_status gfx_step_source_injection(gfx_solver* s, int field, int direction, double stf_val,
+                                     const double* src_weights, int n_src_cell);
+/* cuda_scatter_to_rank (cuda_step.hpp:110): GATHER assembly sums each node's
+ * element-point residuals in fixed order into the nodal residual; COLORED and
+ * ATOMIC assembled it in the element stage. */
+gfx_status gfx_step_scatter_to_rank(gfx_solver* s);
+/* cuda_newmark_correct (cuda_step.hpp:97): a = r/m (0 where m <= 0, u and v
+ * kept), v, C-PML memory and terms, legacy damping, then the next step's
+ * predicted displacement; records the snapshot when step % snapshot_stride
+ * == 0 and closes the step (beta must be 0, gamma 1/2). */
+gfx_status gfx_step_newmark_correct(gfx_solver* s, double dt, double beta, double gamma);
+/* cuda_copy_state_to_host (cuda_step.hpp:100-101): gfx_solver_get_state
+ * between steps. */
+gfx_status gfx_step_copy_state_to_host(const gfx_solver* s, int field, double* u, double* v, double* a);
+/* cuda_copy_utilde_{to,from}_host (cuda_step.hpp:119-122): the predicted
+ * displacement; "from" only between predict and the element stage. */
+gfx_status gfx_step_copy_utilde_to_host(const gfx_solver* s, int field, double* buf);
+gfx_status gfx_step_copy_utilde_from_host(gfx_solver* s, int field, const double* buf);
+/* cuda_copy_residual_{to,from}_host (cuda_step.hpp:125-128): the assembled
+ * residual, between scatter and correct. */
+gfx_status gfx_step_copy_residual_to_host(const gfx_solver* s, int field, double* buf);
+gfx_status gfx_step_copy_residual_from_host(gfx_solver* s, int field, const double* buf);
 
 /* Number of snapshots held so far. */
 int gfx_solver_n_snapshots(const gfx_solver* s);

@@ -69,6 +69,42 @@ private:
     std::unique_ptr<gfx_solver, Del> h_;
 };
 
+// Per-step stage wrappers, one for each of the reference's
+// (forward/share/include/gf/cuda_step.hpp:81-127), same arguments and order of
+// use (forward/share/src/solver.cpp:395-443); they act on every field of the
+// state except where a field is named.
+inline void newmark_predict(DeviceState& s, double dt, double beta) {
+    check(gfx_step_newmark_predict(s.get(), dt, beta), "HIP newmark predict");
+}
+inline void gather_predicted(DeviceState& s) { check(gfx_step_gather_predicted(s.get()), "HIP gather predicted"); }
+inline void gather_from_rank(DeviceState& s) { check(gfx_step_gather_from_rank(s.get()), "HIP gather"); }
+inline void zero_residual(DeviceState& s) { check(gfx_step_zero_residual(s.get()), "HIP zero residual"); }
+inline void launch_element_residual(DeviceState& s, int ngll, int n_elem) {
+    check(gfx_step_element_residual(s.get(), ngll, n_elem), "HIP element residual");
+}
+inline void pml_damping(DeviceState& s) { check(gfx_step_pml_damping(s.get()), "HIP pml damping"); }
+inline void source_injection(DeviceState& s, int direction, double stf_val, const double* h_src_weights,
+                             int n_src_cell, int field = 0) {
+    check(gfx_step_source_injection(s.get(), field, direction, stf_val, h_src_weights, n_src_cell),
+          "HIP source injection");
+}
+inline void scatter_to_rank(DeviceState& s) { check(gfx_step_scatter_to_rank(s.get()), "HIP scatter"); }
+inline void newmark_correct(DeviceState& s, double dt, double beta, double gamma) {
+    check(gfx_step_newmark_correct(s.get(), dt, beta, gamma), "HIP newmark correct");
+}
+inline void copy_utilde_to_host(const DeviceState& s, double* host_buf, int field = 0) {
+    check(gfx_step_copy_utilde_to_host(s.get(), field, host_buf), "HIP copy u~ to host");
+}
+inline void copy_utilde_from_host(DeviceState& s, const double* host_buf, int field = 0) {
+    check(gfx_step_copy_utilde_from_host(s.get(), field, host_buf), "HIP copy u~ from host");
+}
+inline void copy_residual_to_host(const DeviceState& s, double* host_buf, int field = 0) {
+    check(gfx_step_copy_residual_to_host(s.get(), field, host_buf), "HIP copy residual to host");
+}
+inline void copy_residual_from_host(DeviceState& s, const double* host_buf, int field = 0) {
+    check(gfx_step_copy_residual_from_host(s.get(), field, host_buf), "HIP copy residual from host");
+}
+
 // gf::run_forward (forward/share/include/gf/solver.hpp:21) on the GPU.
 inline int run_forward(const std::string& direction, bool resume_mode = false, int effective_nprocs = 0) {
     return gfx_run_forward(direction.c_str(), resume_mode ? 1 : 0, effective_nprocs);

@@ -11,24 +11,35 @@
  *   per element point arrays are element-major, node index n = (i*NGLL+j)*NGLL+k,
  *   dxi_dx[e][n][9] with [m*3+p] = d(xi_p)/d(x_m), vectors are xyz-interleaved.
  *
- * Pinning (what checks this restatement, by function):
- *   gfo_gll_*             reference tests/test_gll.cpp known answers + closed
- *                         forms (N=4 nodes/weights)
+ * Pinning (what checks this restatement, by function; tests/test_oracle.py):
+ *   gfo_gll_*             reference tests/test_gll.cpp known answers, sum of
+ *                         weights, D exact on polynomials, Lagrange cardinality
  *   gfo_element_residual  tests/test_element.cpp invariants (rigid translation,
- *                         rotation, force balance), brute-force weak form by an
- *                         independent numpy Voigt-matrix quadrature, the six
- *                         rigid-body null modes of K, the mu <= 0 skip
- *   gfo_scatter/gather    tests/test_assembly.cpp exact sums
- *   gfo_newmark_*         tests/test_newmark.cpp closed forms, discrete
- *                         eigenmode cos(n*theta) of the whole step
- *   gfo_source            momentum balance sum(m*a) = F(t_n) per step
- *   gfo_cpml_*            single-point closed-form recurrences with distinct
- *                         per-direction coefficients (index layout), the
- *                         old/new step-index mixing over hand-derived 3-step
- *                         sequences; no reference test exists for C-PML
- *                         (parity partially pinned: formula-level only)
- *   gfo_full_strain       exact strain of affine displacement fields
- * The checks live in tests/test_oracle.py (one test per line above).
+ *                         rotation, uniaxial force balance), an independent
+ *                         numpy Voigt-matrix stiffness, the six rigid-body null
+ *                         modes of K, the mu <= 0 skip
+ *   gfo_scatter/gather    tests/test_assembly.cpp exact sums, round trip
+ *   gfo_newmark_*         tests/test_newmark.cpp closed forms; the exact
+ *                         discrete eigenmode cos(n theta) of the whole step
+ *   source injection      momentum balance sum(m a) = F(t_n) at every step
+ *   gfo_apply_pml_damping the four cases of tests/test_pml.cpp:11-101; the
+ *                         legacy branch of the whole step (damping between
+ *                         predictor and corrector) against the closed form of
+ *                         a damped rigid translation
+ *   gfo_cpml_update_displ_fields  three hand-worked calls with v, a != 0 (the
+ *                         swap, c1, c2, interior elements untouched)
+ *   gfo_cpml_update_displ_memory  hand-worked update with distinct
+ *                         coefficients per direction and component
+ *   gfo_cpml_accel_contribution   weighted-sum closed form; steady state u/b
+ *   whole C-PML step      three hand-worked steps of one element (call order:
+ *                         fields, then the term from the old memory, corrector,
+ *                         memory update)
+ *   gfo_full_strain       exact strain sym(A) of affine fields u = A x + b on a
+ *                         jittered mesh, and the recording-cell subset
+ *   gfo_run_from          resume from (u, v, a) equals the uninterrupted run
+ *                         without C-PML
+ * No reference test covers C-PML; the hand-worked sequences pin the
+ * restatement of forward/share/src/pml.cpp:55-216 and its call order.
  */
 #ifndef GFX_ORACLE_H
 #define GFX_ORACLE_H

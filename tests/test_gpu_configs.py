@@ -115,7 +115,7 @@ def test_halfspace_24_fields_each_match_oracle(halfspace, stations_run, station)
 
 
 def test_layered_box_6000_steps_drift():
-    sp = W.layered_spec(14, 14, 8, duration=33.34)
+    sp = W.layered_spec(14, 14, 8, duration=33.33)  # 6,000 steps of 5.56 ms
     sp.layer_zbot[0], sp.layer_zbot[1] = 1200.0, 3000.0
     m = pygfx.build_box(sp)
     assert m.nsteps == 6000
