@@ -372,17 +372,52 @@ void write_inputs(const gfx_mesh& m, const std::string& dir, int n_parts, const 
     std::vector<char> is_rec(E, 0);
     for (int i = 0; i < m.n_rec_cell; ++i) is_rec[m.rec_cell[i]] = 1;
 
+    // cells of each rank: the reference's partitioner without METIS
+    // (preprocess/partition.py:75-110, 257-301): cell centroids (mean of the
+    // GLL points) sorted along the longest axis of their spread, cut into
+    // chunks of ceil(E / n_parts); each rank keeps its cells in increasing
+    // global order.  Equal centroids keep their cell order (stable sort).
+    std::vector<int> owner(E, 0);
+    if (n_parts > 1) {
+        std::vector<double> cen(size_t(E) * 3, 0.0);
+        for (int c = 0; c < E; ++c)
+            for (int q = 0; q < NP; ++q)
+                for (int d = 0; d < 3; ++d) cen[size_t(c) * 3 + d] += m.node_xyz[int64_t(m.ibool[int64_t(c) * NP + q]) * 3 + d];
+        for (double& v : cen) v /= NP;
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int c = 0; c < E; ++c)
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = std::min(lo[d], cen[size_t(c) * 3 + d]);
+                hi[d] = std::max(hi[d], cen[size_t(c) * 3 + d]);
+            }
+        int ax = 0;
+        for (int d = 1; d < 3; ++d)
+            if (hi[d] - lo[d] > hi[ax] - lo[ax]) ax = d;
+        std::vector<int> order(E);
+        for (int c = 0; c < E; ++c) order[c] = c;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return cen[size_t(a) * 3 + ax] < cen[size_t(b) * 3 + ax]; });
+        const int chunk = (E + n_parts - 1) / n_parts;
+        for (int i = 0; i < E; ++i) owner[order[i]] = std::min(i / chunk, n_parts - 1);
+    }
+
     for (int r = 0; r < n_parts; ++r) {
-        const int b = int(int64_t(E) * r / n_parts), e = int(int64_t(E) * (r + 1) / n_parts), ne = e - b;
+        std::vector<int64_t> ids;
+        for (int c = 0; c < E; ++c)
+            if (owner[c] == r) ids.push_back(c);
+        require(!ids.empty(), "n_parts = " + std::to_string(n_parts) + " leaves partition " + std::to_string(r) +
+                                  " without cells");
+        const int ne = int(ids.size());
         const hsize_t hn = hsize_t(ne), hg = hsize_t(n);
         const std::string path = dir + "/partitions/partition_" + std::to_string(r) + ".h5";
         h5::Obj f = h5::create_trunc(path);
         const h5::hid_t id = f.id();
         auto slice = [&](const double* a, int per) {
-            return std::vector<double>(a + int64_t(b) * NP * per, a + int64_t(e) * NP * per);
+            std::vector<double> out;
+            out.reserve(size_t(ne) * NP * per);
+            for (int64_t c : ids) out.insert(out.end(), a + c * NP * per, a + (c + 1) * NP * per);
+            return out;
         };
-        std::vector<int64_t> ids(ne);
-        for (int i = 0; i < ne; ++i) ids[i] = b + i;
         h5::write_vec(id, "/partition/local_cell_ids", ids);
         {
             h5::Obj g = h5::open_group(id, "/partition");
@@ -399,8 +434,9 @@ void write_inputs(const gfx_mesh& m, const std::string& dir, int n_parts, const 
         h5::write_vec(id, "/field/cell/mu", slice(m.mu_, 1), {hn, hg, hg, hg}, gz);
         h5::write_vec(id, "/field/cell/damping", slice(m.damping, 1), {hn, hg, hg, hg}, gz);
         if (m.pml_region) {
-            h5::write_vec(id, "/field/cell/pml_region", std::vector<int32_t>(m.pml_region + b, m.pml_region + e),
-                          {}, gz);
+            std::vector<int32_t> reg;
+            for (int64_t c : ids) reg.push_back(m.pml_region[c]);
+            h5::write_vec(id, "/field/cell/pml_region", reg, {}, gz);
             h5::write_vec(id, "/field/cell/pml_coef_alpha", slice(m.coef_alpha, 9), {hn, hsize_t(NP), 9}, gz);
             if (m.coef_beta)
                 h5::write_vec(id, "/field/cell/pml_coef_beta", slice(m.coef_beta, 9), {hn, hsize_t(NP), 9}, gz);
@@ -415,7 +451,10 @@ void write_inputs(const gfx_mesh& m, const std::string& dir, int n_parts, const 
             h5::write_vec(id, "/field/cell/tau_epsilon", slice(m.tau_epsilon, m.n_sls), {hn, hg, hg, hg, L}, gz);
         }
         // node numbering: global ids, and a compact per-partition numbering
-        std::vector<int32_t> gib(m.ibool + int64_t(b) * NP, m.ibool + int64_t(e) * NP), rib(gib.size());
+        std::vector<int32_t> gib;
+        gib.reserve(size_t(ne) * NP);
+        for (int64_t c : ids) gib.insert(gib.end(), m.ibool + c * NP, m.ibool + (c + 1) * NP);
+        std::vector<int32_t> rib(gib.size());
         std::unordered_map<int32_t, int32_t> compact;
         for (size_t q = 0; q < gib.size(); ++q) {
             auto it = compact.emplace(gib[q], int32_t(compact.size())).first;
@@ -431,10 +470,10 @@ void write_inputs(const gfx_mesh& m, const std::string& dir, int n_parts, const 
         // recording map of the cells in this partition (recording_map.py:52-90 rules)
         std::vector<int32_t> rloc;
         std::vector<int64_t> rglob;
-        for (int c = b; c < e; ++c)
-            if (is_rec[c]) {
-                rloc.push_back(c - b);
-                rglob.push_back(c);
+        for (int i = 0; i < ne; ++i)
+            if (is_rec[ids[i]]) {
+                rloc.push_back(i);
+                rglob.push_back(ids[i]);
             }
         if (!rloc.empty()) {
             std::vector<int64_t> nodes;

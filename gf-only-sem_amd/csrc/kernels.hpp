@@ -47,7 +47,9 @@ struct ElemArgs {
     int64_t fstride;                    // floats per field of the nodal arrays
     const float* __restrict__ U;        // [nf][fstride] predicted displacement
     float* __restrict__ R;              // [nf][fstride] assembled residual (COLORED, ATOMIC)
-    float* __restrict__ Rl;             // [nf][P][3] per element point, solver point order (GATHER)
+    float* __restrict__ Rl;             // [nf][PR][3] element-point residuals (GATHER): entry rdst[pt]
+    const int32_t* __restrict__ rdst;   // [P] a node's first point: the node id; others: G4 + rank
+    int64_t PR;
     // element-local C-PML (only when the nodal form is not applicable)
     const float* __restrict__ W;
     const float* __restrict__ Aacc;     // [nf][astride] a_n at special nodes
@@ -89,11 +91,12 @@ struct NodeArgs {
     float* __restrict__ W;
     float* __restrict__ R;
     float* __restrict__ Aacc;         // [nf][astride]
-    // GATHER assembly: node g owns the slots [off[g], off[g+1]); slot q holds
-    // the element point pts[q] (increasing in solver order), whose residual
-    // is Rl[f][pts[q]]
+    // GATHER assembly: node g's first element point (lowest solver order) has
+    // its residual at Rl[f][g]; its other points are the slots [off[g],
+    // off[g+1]), slot q holding the entry pts[q] (increasing in solver order)
     int gather;
-    const float* __restrict__ Rl;
+    const float* __restrict__ Rl;     // [nf][PR][3]
+    int64_t PR;
     const int32_t* __restrict__ off;  // [G4+1]
     const int32_t* __restrict__ pts;  // [P]
     // nodal C-PML
@@ -265,8 +268,8 @@ __device__ __forceinline__ void point_div(const float* __restrict__ sf, int stri
 template <int MODE>
 __device__ __forceinline__ void emit_residual(const ElemArgs& A, int f, int64_t node, int64_t slot,
                                               const float* rold, const float* rv) {
-    if (MODE == ASM_GATHER) {
-        float* o = A.Rl + (int64_t(f) * A.P + slot) * 3;
+    if (MODE == ASM_GATHER) {  // slot: the point's entry rdst[pt]
+        float* o = A.Rl + (int64_t(f) * A.PR + slot) * 3;
         o[0] = rv[0];
         o[1] = rv[1];
         o[2] = rv[2];
@@ -281,6 +284,13 @@ __device__ __forceinline__ void emit_residual(const ElemArgs& A, int f, int64_t 
         r[1] = rold[1] + rv[1];
         r[2] = rold[2] + rv[2];
     }
+}
+
+// Residual entry of element point pt: rdst[pt] for GATHER assembly (a node's
+// first point at the node id, later points after the nodes), unused otherwise.
+template <int MODE>
+__device__ __forceinline__ int64_t residual_entry(const ElemArgs& A, int64_t pt, bool valid) {
+    return (MODE == ASM_GATHER && valid) ? int64_t(A.rdst[pt]) : pt;
 }
 
 __device__ __forceinline__ void add_sources(const ElemArgs& A, int step, int f, int sb, int se, int p,
@@ -343,7 +353,7 @@ __device__ __forceinline__ void element_body(const ElemArgs& A, int e0, int ne, 
     }
     const int64_t node = int64_t(nodef & ~FIRST_TOUCH);
     const bool first = (nodef & FIRST_TOUCH) != 0;
-    const int64_t rslot = pt;  // GATHER: the point's own residual entry (coalesced stores)
+    const int64_t rslot = residual_entry<MODE>(A, pt, valid);
     const int pslot = (PML_ELEM && valid) ? A.pml_slot[e] : -1;
     int sb = 0, se = 0;
     if (valid) {
@@ -605,6 +615,7 @@ __global__ void __launch_bounds__(128) k_element_att(ElemArgs A, int e0, int ne)
         cb[l] = have ? A.att_b[int64_t(l) * A.P + pt] : 0.f;
         cc[l] = have ? A.att_c[int64_t(l) * A.P + pt] : 0.f;
     }
+    const int64_t rslot = residual_entry<MODE>(A, pt, valid);
     __syncthreads();
     float Di[5], Dj[5], Dk[5], Dti[5], Dtj[5], Dtk[5];
 #pragma unroll
@@ -652,7 +663,7 @@ __global__ void __launch_bounds__(128) k_element_att(ElemArgs A, int e0, int ne)
             float rv[3];
             point_div(&sf[0][0], NP, Dti, Dtj, Dtk, i, j, k, rv);
             add_sources(A, step, f, sb, se, p, rv);
-            emit_residual<MODE>(A, f, node, pt, rold, rv);
+            emit_residual<MODE>(A, f, node, rslot, rold, rv);
         }
         __syncthreads();  // su and sf are rewritten by the next field
     }
@@ -694,7 +705,7 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
     }
     const int64_t node = int64_t(nodef & ~FIRST_TOUCH);
     const bool first = (nodef & FIRST_TOUCH) != 0;
-    const int64_t rslot = pt;  // GATHER: the point's own residual entry
+    const int64_t rslot = residual_entry<MODE>(A, pt, valid);
     int sb = 0, se = 0;
     if (valid) {
         sb = A.src_off[e];
@@ -851,6 +862,7 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
         Dtk[q] = sD[q * 5 + k];
     }
     const int ri = j * 5 + k, rj = i * 25 + k, rk = i * 25 + j * 5;  // row bases along i, j, k
+    const int64_t rslot = residual_entry<MODE>(A, pt, valid);
 
     for (int f = 0; f < nf; f += FV) {
         if (valid) {
@@ -948,7 +960,7 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
             for (int h = 0; h < FV; ++h)
                 if (f + h < nf) {
                     add_sources(A, step, f + h, sb, se, p, rv[h]);
-                    emit_residual<MODE>(A, f + h, node, pt, rold[h], rv[h]);
+                    emit_residual<MODE>(A, f + h, node, rslot, rold[h], rv[h]);
                 }
         }
     }
@@ -986,6 +998,7 @@ __global__ void __launch_bounds__(256) k_element_pencil(ElemArgs A, int e0, int 
 
     float g[5][9], lam[5], mu[5];
     uint32_t nodef[5];
+    int64_t rs[5] = {0, 0, 0, 0, 0};
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
         if (valid) {
@@ -993,6 +1006,7 @@ __global__ void __launch_bounds__(256) k_element_pencil(ElemArgs A, int e0, int 
             lam[k] = __builtin_nontemporal_load(&A.lam[pt0 + k]);
             mu[k] = __builtin_nontemporal_load(&A.mu[pt0 + k]);
             nodef[k] = __builtin_nontemporal_load(&A.ibool[pt0 + k]);
+            rs[k] = residual_entry<MODE>(A, pt0 + k, true);
         } else {
 #pragma unroll
             for (int c = 0; c < 9; ++c) g[k][c] = 0.f;
@@ -1107,7 +1121,7 @@ __global__ void __launch_bounds__(256) k_element_pencil(ElemArgs A, int e0, int 
                     rold[1] = r[1];
                     rold[2] = r[2];
                 }
-                emit_residual<MODE>(A, f, node, pt0 + k, rold, rv);
+                emit_residual<MODE>(A, f, node, rs[k], rold, rv);
             }
         }
     }
@@ -1155,7 +1169,7 @@ __device__ __forceinline__ void wave_body(const ElemArgs& A, int e0, int ne, int
         const uint32_t nf_ = vq[q] ? __builtin_nontemporal_load(&A.ibool[pt]) : 0u;
         node[q] = int64_t(nf_ & ~FIRST_TOUCH);
         first[q] = (nf_ & FIRST_TOUCH) != 0;
-        ptq[q] = pt;  // GATHER: the point's own residual entry
+        ptq[q] = residual_entry<MODE>(A, pt, vq[q]);
     }
     float un[2][3];
 #pragma unroll
@@ -1228,10 +1242,14 @@ __global__ void __launch_bounds__(64 * WPB)
     wave_body<WPB, MODE, DIAG>(const_row(tab, E.x), E.y, E.z, E.w);
 }
 
-// Fixed-order sum of a node's element contributions (GATHER assembly).
-__device__ __forceinline__ void gather_r(const NodeArgs& A, int f, int kb, int ke, float* r) {
-    r[0] = r[1] = r[2] = 0.f;
-    const float* Rl = A.Rl + int64_t(f) * A.P * 3;
+// Fixed-order sum of a node's element contributions (GATHER assembly): the
+// node's first point (its entry at the node id) added to 0, then the other
+// points' entries (slots [kb, ke) of the node) in increasing solver order.
+__device__ __forceinline__ void gather_r(const NodeArgs& A, int f, int64_t n, int kb, int ke, float* r) {
+    const float* Rl = A.Rl + int64_t(f) * A.PR * 3;
+    r[0] = 0.f + Rl[n * 3 + 0];
+    r[1] = 0.f + Rl[n * 3 + 1];
+    r[2] = 0.f + Rl[n * 3 + 2];
     for (int q = kb; q < ke; ++q) {
         const float* c = Rl + int64_t(A.pts[q]) * 3;
         r[0] += c[0];
@@ -1258,13 +1276,15 @@ __device__ __forceinline__ bool stage_points(const NodeArgs& A, int64_t s0, int6
     return true;
 }
 
+// main[3 * NN]: the lane's nodes' first-point entries (read by the caller,
+// coalesced: they sit at the node ids); the slots hold the other points.
 template <int NN>
 __device__ __forceinline__ void staged_sums(const NodeArgs& A, int f, int64_t s0, int64_t s1, const int* kb,
                                             bool active, float* __restrict__ sl, const int32_t* __restrict__ sp,
-                                            bool ids_staged, float* r) {
+                                            bool ids_staged, const float* main, float* r) {
 #pragma unroll
-    for (int q = 0; q < 3 * NN; ++q) r[q] = 0.f;
-    const float* Rl = A.Rl + int64_t(f) * A.P * 3;
+    for (int q = 0; q < 3 * NN; ++q) r[q] = 0.f + main[q];
+    const float* Rl = A.Rl + int64_t(f) * A.PR * 3;
     for (int64_t c0 = s0; c0 < s1; c0 += SLOT_CHUNK) {
         const int64_t c1 = min(s1, c0 + SLOT_CHUNK);
         const int cnt = int(c1 - c0);
@@ -1357,10 +1377,12 @@ __device__ __forceinline__ void node_special_body(const NodeArgs& A, int blk) {
 #pragma unroll
         for (int q = 0; q < 9; ++q) m[q] = (pml && use[q % 3]) ? pmem[q * A.Gpml + g] : 0.f;
         if (staged) {
-            staged_sums<1>(A, f, s0, s1, kb, active, sl, sp, ids, R);
+            const float* rm = A.Rl + (int64_t(f) * A.PR + g) * 3;
+            const float main[3] = {rm[0], rm[1], rm[2]};
+            staged_sums<1>(A, f, s0, s1, kb, active, sl, sp, ids, main, R);
             if (!active) continue;
         } else if (A.gather) {
-            gather_r(A, f, kb[0], kb[1], R);
+            gather_r(A, f, g, kb[0], kb[1], R);
         } else {
 #pragma unroll
             for (int c = 0; c < 3; ++c) R[c] = A.R[o + c];
@@ -1460,11 +1482,19 @@ __device__ __forceinline__ void node_regular_body(const NodeArgs& A, int blk) {
             wa[q] = W4[q];
         }
         if (STAGE) {
-            staged_sums<4>(A, f, s0, s1, kb, active, sl, sp, ids, r);
+            // the 4 nodes' first-point entries: 12 floats at the node ids, 16-byte aligned
+            const float4* m4 = reinterpret_cast<const float4*>(A.Rl + (int64_t(f) * A.PR + g) * 3);
+            float main[12];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const float4 c = m4[q];
+                main[4 * q + 0] = c.x; main[4 * q + 1] = c.y; main[4 * q + 2] = c.z; main[4 * q + 3] = c.w;
+            }
+            staged_sums<4>(A, f, s0, s1, kb, active, sl, sp, ids, main, r);
             if (!active) continue;
         } else if (GATHER) {
 #pragma unroll
-            for (int n = 0; n < 4; ++n) gather_r(A, f, kb[n], kb[n + 1], &r[3 * n]);
+            for (int n = 0; n < 4; ++n) gather_r(A, f, g + n, kb[n], kb[n + 1], &r[3 * n]);
         } else {
             const float4* R4 = reinterpret_cast<const float4*>(A.R + o);
 #pragma unroll
@@ -1609,7 +1639,7 @@ __global__ void __launch_bounds__(256) k_scatter_slots(NodeArgs A) {
     const int f = blockIdx.y;
     if (n >= A.G4) return;
     float r[3];
-    gather_r(A, f, A.off[n], A.off[n + 1], r);
+    gather_r(A, f, n, A.off[n], A.off[n + 1], r);
     float* o = A.R + int64_t(f) * A.fstride + n * 3;
     o[0] = r[0];
     o[1] = r[1];
@@ -1623,14 +1653,14 @@ __global__ void __launch_bounds__(256) k_scatter_slots(NodeArgs A) {
 // and ATOMIC hold assembled R) and the additions follow the cell order, as
 // the element kernels' own source loop does.  GATHER: into Rl at the point.
 __global__ void __launch_bounds__(128)
-    k_inject(float* __restrict__ Rl, float* __restrict__ R, const uint32_t* __restrict__ ibool,
-             const int32_t* __restrict__ cells, const float* __restrict__ w, int n_cell, int dir, float stf,
-             int64_t P, int64_t fstride, int f, int gather) {
+    k_inject(float* __restrict__ Rl, const int32_t* __restrict__ rdst, float* __restrict__ R,
+             const uint32_t* __restrict__ ibool, const int32_t* __restrict__ cells, const float* __restrict__ w,
+             int n_cell, int dir, float stf, int64_t PR, int64_t fstride, int f, int gather) {
     const int p = threadIdx.x;
     for (int c = 0; c < n_cell; ++c) {
         if (p < NP) {
             const int64_t pt = int64_t(cells[c]) * NP + p;
-            float* o = gather ? Rl + (int64_t(f) * P + pt) * 3 + dir
+            float* o = gather ? Rl + (int64_t(f) * PR + rdst[pt]) * 3 + dir
                               : R + int64_t(f) * fstride + int64_t(ibool[pt] & ~FIRST_TOUCH) * 3 + dir;
             *o += stf * w[int64_t(c) * NP + p];
         }

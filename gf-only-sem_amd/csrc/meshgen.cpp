@@ -299,6 +299,34 @@ struct GeneralCells {
     double bnd[6];
 };
 
+// Boundary detection on general hexahedra (preprocess/boundary_detector.py:18-86):
+// a face whose centre (mean of its four corners) lies on z = zmin is free
+// surface; one on x = xmin/xmax, y = ymin/ymax or z = zmax (tolerance 1e-6 of
+// the largest box extent) is absorbing; a cell with an absorbing face is a PML
+// cell.  Faces by GMSH corner order: 0-3 and 4-7 are opposite quads.
+void face_flags(const GeneralCells& g, std::vector<char>& is_pml) {
+    static const int F[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 1, 5, 4},
+                                {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}};
+    const double* b = g.bnd;
+    const double dmax = std::max({b[1] - b[0], b[3] - b[2], b[5] - b[4]});
+    const double tol = dmax > 0 ? 1e-6 * dmax : 1e-9;
+    // np.isclose(a, v, atol=tol) with its default rtol = 1e-5
+    auto close = [&](double a, double v) { return std::abs(a - v) <= tol + 1e-5 * std::abs(v); };
+    const int64_t E = int64_t(g.cv.size() / 24);
+    for (int64_t e = 0; e < E; ++e) {
+        const double* cv = &g.cv[size_t(e) * 24];
+        for (int f = 0; f < 6 && !is_pml[e]; ++f) {
+            double c[3] = {0, 0, 0};
+            for (int a = 0; a < 4; ++a)
+                for (int q = 0; q < 3; ++q) c[q] += 0.25 * cv[F[f][a] * 3 + q];
+            if (close(c[2], b[4])) continue;  // free surface
+            if (close(c[0], b[0]) || close(c[0], b[1]) || close(c[1], b[2]) || close(c[1], b[3]) ||
+                close(c[2], b[5]))
+                is_pml[e] = 1;
+        }
+    }
+}
+
 void validate(const gfx_box_spec& s, bool general) {
     require(s.nx >= 1 && s.ny >= 1 && (general || s.nz >= 1), "element counts must be >= 1");
     require(general || (s.lx > 0 && s.ly > 0 && s.lz > 0), "box lengths must be > 0");
@@ -376,15 +404,17 @@ void build(MeshHolder& H, const GeneralCells* gen = nullptr) {
     const int nx = s.nx, ny = s.ny;
     int64_t E = 0;
     int nz = s.nz;
+    // C-PML flags: by cell index when the cells are an x-fastest nx * ny * nz
+    // lattice, else from the cells' faces on the domain bounds
+    // (preprocess/cli.py:239-264, boundary_detector.py:18-86)
+    bool index_flags = true;
     if (general) {
         E = int64_t(gen->cv.size() / 24);
         require(E >= 1, "no cells");
-        // the index-based C-PML flags need nx * ny * nz = n_cell (preprocess/cli.py:241-247)
-        require(E % (int64_t(nx) * ny) == 0,
-                "n_cell is not a multiple of nx_elements * ny_elements (x-fastest structured cell order "
-                "is required for the C-PML flags)");
-        nz = int(E / (int64_t(nx) * ny));
-        require(s.pml_zmax <= nz, "PML thicker than the box");
+        nz = int(E / (int64_t(nx) * ny));  // cli.py:462: n_cell // (nx * ny)
+        index_flags = int64_t(nx) * ny * nz == E;
+        if (index_flags) require(s.pml_zmax <= nz, "PML thicker than the box");
+        nz = std::max(nz, 1);  // element height for the C-PML widths (cli.py:466)
     } else {
         E = int64_t(nx) * ny * nz;
     }
@@ -703,11 +733,15 @@ void build(MeshHolder& H, const GeneralCells* gen = nullptr) {
 
     // PML flags (cli.py:245-264) and widths (cli.py:464-474)
     std::vector<char> is_pml(E, 0);
-    for (int64_t e = 0; e < E; ++e) {
-        const int i = int(e % nx), j = int((e / nx) % ny), k = int(e / (int64_t(nx) * ny));
-        if (i < s.pml_xmin || i >= nx - s.pml_xmax || j < s.pml_ymin || j >= ny - s.pml_ymax ||
-            k >= nz - s.pml_zmax)
-            is_pml[e] = 1;
+    if (index_flags) {
+        for (int64_t e = 0; e < E; ++e) {
+            const int i = int(e % nx), j = int((e / nx) % ny), k = int(e / (int64_t(nx) * ny));
+            if (i < s.pml_xmin || i >= nx - s.pml_xmax || j < s.pml_ymin || j >= ny - s.pml_ymax ||
+                k >= nz - s.pml_zmax)
+                is_pml[e] = 1;
+        }
+    } else {
+        face_flags(*gen, is_pml);
     }
     const double wid[6] = {s.pml_xmin * hx, s.pml_xmax * hx, s.pml_ymin * hy,
                            s.pml_ymax * hy, 0.0,             s.pml_zmax * hz};

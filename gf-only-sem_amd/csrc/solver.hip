@@ -72,10 +72,15 @@ struct gfx_solver {
     gfx::DevBuf<uint8_t> pmask;  // nodal C-PML: memory directions in use (bits x, y, z)
     gfx::DevBuf<int32_t> pml_slot, src_off;
     gfx::DevBuf<float> U, W, R, Aacc, Rl;
-    // GATHER assembly: node n owns slots [goff[n], goff[n+1]); slot q sums the
-    // residual of element point gpts[q] (increasing in solver order per node)
+    // GATHER assembly: node n owns slots [goff_host[n], goff_host[n+1]); slot q
+    // sums the entry gpts_host[q] of Rl (increasing in solver order per node;
+    // the first is the node's own index).  On the device goff / gpts list
+    // every slot but the first, which the node kernels read at the node id.
     gfx::DevBuf<int32_t> goff, gpts;
     std::vector<int32_t> goff_host, gpts_host;
+    // entry of each element point in Rl ([P]) and entries per field (G4 + P - G)
+    gfx::DevBuf<int32_t> rdst;
+    int64_t PR = 0;
     std::vector<gfx::FieldSource> sources;
     gfx::DevBuf<int4> src_ent;
     gfx::DevBuf<float> src_w, stf;
@@ -410,6 +415,8 @@ StepArgs make_step_args(gfx_solver& S) {
     ea.U = S.U.p;
     ea.R = S.R.p;
     ea.Rl = S.Rl.p;
+    ea.rdst = S.rdst.p;
+    ea.PR = S.PR;
     ea.W = S.W.p;
     ea.Aacc = S.Aacc.p;
     ea.astride = S.astride;
@@ -447,6 +454,7 @@ StepArgs make_step_args(gfx_solver& S) {
     na.astride = S.astride;
     na.gather = (S.assembly == GFX_ASSEMBLY_GATHER) ? 1 : 0;
     na.Rl = S.Rl.p;
+    na.PR = S.PR;
     na.off = S.goff.p;
     na.pts = S.gpts.p;
     na.minv = S.minv.p;
@@ -1032,11 +1040,31 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
         require(off[S.G4] == P, "internal: gather slots do not cover every element point");
         std::vector<int32_t> fill(off.begin(), off.end() - 1);
         for (int64_t q = 0; q < P; ++q) pts[fill[hib[q]]++] = int32_t(q);
+        // residual entry of each point: a node's first point (lowest solver order) at
+        // the node's own index, every other point at G4 + its rank among those points
+        // in solver order; a node's slots then list entries, first one = the node
+        std::vector<int32_t> rdst(P, -1);
+        for (int64_t n = 0; n < S.G4; ++n)
+            if (off[n + 1] > off[n]) rdst[pts[off[n]]] = int32_t(n);
+        int64_t k = S.G4;
+        for (int64_t q = 0; q < P; ++q)
+            if (rdst[q] < 0) rdst[q] = int32_t(k++);
+        require(k == S.G4 + P - S.G, "internal: residual entries do not match the point count");
+        S.PR = (k + 3) / 4 * 4;  // 16-byte aligned field stride
+        for (int64_t q = 0; q < P; ++q) pts[q] = rdst[pts[q]];
+        // device slot lists: every point but the node's first (read at the node id)
+        std::vector<int32_t> xoff(S.G4 + 1, 0), xpts;
+        xpts.reserve(size_t(P - S.G));
+        for (int64_t n = 0; n < S.G4; ++n) {
+            for (int32_t q = off[n] + 1; q < off[n + 1]; ++q) xpts.push_back(pts[q]);
+            xoff[n + 1] = int32_t(xpts.size());
+        }
         S.goff_host = off;
         S.gpts_host = pts;
-        dv.upload(S.goff, std::move(off));
-        dv.upload(S.gpts, std::move(pts));
-        dv.alloc(S.Rl, size_t(S.nf) * P * 3);
+        dv.upload(S.goff, std::move(xoff));
+        dv.upload(S.gpts, std::move(xpts));
+        dv.upload(S.rdst, std::move(rdst));
+        dv.alloc(S.Rl, size_t(S.nf) * S.PR * 3);
     }
     std::vector<uint32_t>().swap(hib);
 
@@ -1188,8 +1216,7 @@ extern "C" gfx_status gfx_solver_get_state(const gfx_solver* s, int field, doubl
         if (s->assembly == GFX_ASSEMBLY_GATHER && !s->r_in_R) {
             // assembled r of the last step: fixed-order sum of the node's slots, as the
             // node kernels form it
-            const int64_t P = int64_t(s->E) * NP;
-            const std::vector<float> hl = fetch(s, s->Rl.p + int64_t(field) * P * 3, size_t(P) * 3);
+            const std::vector<float> hl = fetch(s, s->Rl.p + int64_t(field) * s->PR * 3, size_t(s->PR) * 3);
             const std::vector<int32_t>& off = s->goff_host;
             const std::vector<int32_t>& pts = s->gpts_host;
             hR.assign(size_t(s->G4) * 3, 0.f);
@@ -1252,14 +1279,13 @@ extern "C" gfx_status gfx_solver_set_state(gfx_solver* s, int field, const doubl
         if (!hA.empty())
             GFX_HIP(hipMemcpy(s->Aacc.p + int64_t(field) * s->astride, hA.data(), hA.size() * 4,
                               hipMemcpyHostToDevice));
-        if (s->assembly == GFX_ASSEMBLY_GATHER) {  // r = m a at the first point of each node
-            const int64_t P = int64_t(s->E) * NP;
-            std::vector<float> hl(size_t(P) * 3, 0.f);
+        if (s->assembly == GFX_ASSEMBLY_GATHER) {  // r = m a at the first entry of each node
+            std::vector<float> hl(size_t(s->PR) * 3, 0.f);
             for (int64_t n = 0; n < s->G; ++n)
                 if (s->goff_host[n + 1] > s->goff_host[n])
                     for (int c = 0; c < 3; ++c)
                         hl[int64_t(s->gpts_host[s->goff_host[n]]) * 3 + c] = hR[n * 3 + c];
-            GFX_HIP(hipMemcpy(s->Rl.p + int64_t(field) * P * 3, hl.data(), hl.size() * 4,
+            GFX_HIP(hipMemcpy(s->Rl.p + int64_t(field) * s->PR * 3, hl.data(), hl.size() * 4,
                               hipMemcpyHostToDevice));
         }
     });
@@ -1398,9 +1424,9 @@ extern "C" gfx_status gfx_step_source_injection(gfx_solver* s, int field, int di
         if (s->inj_w.n < w.size()) s->inj_w.alloc(w.size());
         GFX_HIP(hipMemcpy(s->inj_cells.p, cells.data(), cells.size() * 4, hipMemcpyHostToDevice));
         GFX_HIP(hipMemcpy(s->inj_w.p, w.data(), w.size() * 4, hipMemcpyHostToDevice));
-        dev::k_inject<<<1, 128, 0, s->stream>>>(s->Rl.p, s->R.p, s->ibool.p, s->inj_cells.p, s->inj_w.p,
+        dev::k_inject<<<1, 128, 0, s->stream>>>(s->Rl.p, s->rdst.p, s->R.p, s->ibool.p, s->inj_cells.p, s->inj_w.p,
                                                int(cells.size()), direction, float(stf_val),
-                                               int64_t(s->E) * NP, s->fstride, field,
+                                               s->PR, s->fstride, field,
                                                s->assembly == GFX_ASSEMBLY_GATHER ? 1 : 0);
         GFX_HIP(hipGetLastError());
     });
@@ -1586,10 +1612,9 @@ extern "C" gfx_status gfx_solver_checkpoint_save(const gfx_solver* s, void* host
         if (s->assembly != GFX_ASSEMBLY_GATHER || s->r_in_R) {
             get(s->R.p, fl);
         } else {
-            const int64_t P = int64_t(s->E) * NP;
             std::fill(p, p + fl, 0.f);
             for (int f = 0; f < s->nf; ++f) {
-                const std::vector<float> hl = fetch(s, s->Rl.p + int64_t(f) * P * 3, size_t(P) * 3);
+                const std::vector<float> hl = fetch(s, s->Rl.p + int64_t(f) * s->PR * 3, size_t(s->PR) * 3);
                 float* r = p + int64_t(f) * s->fstride;
                 for (int64_t n = 0; n < s->G; ++n)
                     for (int q = s->goff_host[n]; q < s->goff_host[n + 1]; ++q)
@@ -1629,15 +1654,14 @@ extern "C" gfx_status gfx_solver_checkpoint_load(gfx_solver* s, const void* host
         } else {
             // the sum at the first point of each node, zeros elsewhere: get_state
             // then forms the same r
-            const int64_t P = int64_t(s->E) * NP;
-            std::vector<float> hl(size_t(P) * 3);
+            std::vector<float> hl(size_t(s->PR) * 3);
             for (int f = 0; f < s->nf; ++f) {
                 std::fill(hl.begin(), hl.end(), 0.f);
                 const float* r = p + int64_t(f) * s->fstride;
                 for (int64_t n = 0; n < s->G; ++n)
                     if (s->goff_host[n + 1] > s->goff_host[n])
                         for (int c = 0; c < 3; ++c) hl[int64_t(s->gpts_host[s->goff_host[n]]) * 3 + c] = r[n * 3 + c];
-                GFX_HIP(hipMemcpy(s->Rl.p + int64_t(f) * P * 3, hl.data(), hl.size() * 4, hipMemcpyHostToDevice));
+                GFX_HIP(hipMemcpy(s->Rl.p + int64_t(f) * s->PR * 3, hl.data(), hl.size() * 4, hipMemcpyHostToDevice));
             }
         }
         s->step = int(hdr[5]);

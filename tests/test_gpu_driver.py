@@ -40,11 +40,15 @@ def make_inputs(d, n_parts=1, precision="float64", restart_stride=0, duration=2.
 
 
 def read_records(d, direction, steps, sub=""):
+    """Record fields per snapshot step, cells in increasing global id: a run over
+    several partitions lists them in merged order, each record file naming its
+    cells' global ids (recording_cell_model_index)."""
     out = {k: [] for k in ("strain", "displacement", "velocity", "acceleration")}
     for s in steps:
         p = os.path.join(d, "wavefields", sub, direction, f"record_0_{s}.h5")
+        order = np.argsort(pygfx.h5_read(p, "/recording_cell_model_index"), kind="stable")
         for k in out:
-            out[k].append(pygfx.h5_read(p, "/" + k))
+            out[k].append(pygfx.h5_read(p, "/" + k)[:, order])
     return {k: np.stack(v) for k, v in out.items()}
 
 
@@ -54,6 +58,7 @@ def snapshot_steps(m, upto=None):
 
 
 def oracle_records(m, stf, xyz, direction):
+    assert np.all(np.diff(m.rec_cell) > 0)  # the oracle records cells in increasing global id
     cells, w = m.locate_source(*xyz)
     run = pyoracle.Run(m.arrays, m.n_elem, 5, m.n_node, m.dt, m.nsteps, m.snapshot_stride, direction,
                        cells, w, stf, m.rec_cell)

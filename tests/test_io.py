@@ -159,10 +159,17 @@ def test_inputs_roundtrip_through_the_merge(tmp_path, n_parts):
     stf = pygfx.ricker(1.0, 0.5, 1e20, m.dt, m.nsteps)
     pygfx.write_inputs(m, tmp_path, (3000.0, 2500.0, 0.0), stf, n_parts=n_parts)
     m2 = pygfx.read_inputs(tmp_path)
+    # the merge concatenates the partitions' cells (io.cpp:361-447): cell c of the
+    # merged mesh is global cell ids[c]
+    ids = np.concatenate([pygfx.h5_read(tmp_path / "partitions" / f"partition_{r}.h5", "/partition/local_cell_ids")
+                          for r in range(n_parts)]).astype(np.int64)
+    assert np.array_equal(np.sort(ids), np.arange(m.n_elem))
+    E = m.n_elem
     for k in ("coords", "jacobian", "dxi_dx", "mass", "density", "lambda_", "mu_", "damping", "ibool",
-              "pml_region", "coef_alpha", "coef_abar", "node_xyz"):
-        assert np.array_equal(m.arrays[k], m2.arrays[k]), k
-    assert np.array_equal(m.rec_cell, m2.rec_cell)
+              "pml_region", "coef_alpha", "coef_abar"):
+        assert np.array_equal(m.arrays[k].reshape(E, -1)[ids], m2.arrays[k].reshape(E, -1)), k
+    assert np.array_equal(m.arrays["node_xyz"], m2.arrays["node_xyz"])
+    assert np.array_equal(np.sort(ids[m2.rec_cell]), np.sort(m.rec_cell))
     assert (m2.n_node, m2.dt, m2.nsteps, m2.snapshot_stride) == (m.n_node, m.dt, m.nsteps, m.snapshot_stride)
     merged = pygfx.read_partition(str(tmp_path / "partitions"), merge=True)
     # n_rank_node = max global id + 1; ibool := local_cell2global_node (io.cpp:361-368,442-447)
@@ -177,6 +184,28 @@ def test_inputs_roundtrip_through_the_merge(tmp_path, n_parts):
         rib = pygfx.h5_read(tmp_path / "partitions" / f"partition_{r}.h5", "/field/cell/local_cell2rank_node")
         gib = pygfx.h5_read(tmp_path / "partitions" / f"partition_{r}.h5", "/field/cell/local_cell2global_node")
         assert d["n_rank_node"] == rib.max() + 1 == np.unique(gib).size
+
+
+def test_partitions_are_centroid_slabs_along_the_longest_axis(tmp_path):
+    """Without METIS the reference sorts cell centroids along the axis of
+    largest spread and cuts the order into chunks of ceil(E / n)
+    (preprocess/partition.py:75-110); each partition lists its cells in
+    increasing global order."""
+    m = small_box(jitter=0.1, jitter_seed=5)
+    E, n = m.n_elem, 4
+    stf = pygfx.ricker(1.0, 0.5, 1e20, m.dt, m.nsteps)
+    pygfx.write_inputs(m, tmp_path, (3000.0, 2500.0, 0.0), stf, n_parts=n)
+    cen = m.coords.reshape(E, 125, 3).mean(axis=1)
+    ax = int(np.argmax(np.ptp(cen, axis=0)))
+    chunk = -(-E // n)
+    prev_max = -np.inf
+    for r in range(n):
+        ids = pygfx.h5_read(tmp_path / "partitions" / f"partition_{r}.h5", "/partition/local_cell_ids").astype(np.int64)
+        assert np.all(np.diff(ids) > 0)
+        assert len(ids) == (chunk if r < n - 1 else E - chunk * (n - 1))
+        c = cen[ids, ax]
+        assert c.min() >= prev_max
+        prev_max = c.max()
 
 
 @pytest.mark.parametrize("precision", ["float32", "float64"])

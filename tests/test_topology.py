@@ -272,16 +272,35 @@ def test_general_mesh_round_trips_through_the_input_files(tmp_path):
     pygfx.write_inputs(m, d, (2500.0, 2000.0, 0.0), stf, n_parts=2)
     back = pygfx.read_inputs(d)
     assert back.n_elem == m.n_elem and back.n_node == m.n_node
+    ids = np.concatenate([pygfx.h5_read(os.path.join(d, "partitions", f"partition_{r}.h5"),
+                                        "/partition/local_cell_ids") for r in range(2)]).astype(np.int64)
+    assert np.array_equal(np.sort(ids), np.arange(m.n_elem))
     for k in ("jacobian", "mass", "lambda_", "mu_"):
-        np.testing.assert_allclose(getattr(back, k), getattr(m, k), rtol=1e-15, err_msg=k)
+        got = getattr(back, k).reshape(m.n_elem, -1)
+        np.testing.assert_allclose(got, getattr(m, k).reshape(m.n_elem, -1)[ids], rtol=1e-15, err_msg=k)
     cfg = pygfx.read_config(os.path.join(d, "config.h5"))
     assert cfg["xmax"] == 5000.0 and cfg["n_tilex"] == 1
 
 
-def test_cell_order_must_fill_the_pml_grid():
-    xyz, hexes = box_cells(3, 2, 2)
-    with pytest.raises(pygfx.GfxError, match="multiple of nx_elements"):
-        pygfx.build_hex(xyz, hexes, **_flat_spec(nx=5, ny=1))
+def test_cells_off_the_lattice_take_pml_flags_from_boundary_faces():
+    """A cell count that is not nx_elements * ny_elements * nz takes the
+    reference's face test (preprocess/cli.py:258-261, boundary_detector.py):
+    the PML cells are those with a face on x = xmin/xmax, y = ymin/ymax or
+    z = zmax; the free surface z = 0 does not count.  On a 4 x 3 x 3 jittered
+    lattice (36 cells, given with nx_elements 5, ny_elements 3) those are the
+    cells with i in {0, 3}, j in {0, 2} or k = 2."""
+    xyz, hexes = box_cells(4, 3, 3, h=(1000.0, 1000.0, 1000.0), jitter=0.15, seed=4)
+    spec = _flat_spec(nx=5, ny=3, order=4, output_dt=1e-3, duration=1e-2)
+    spec.update(pml_xmin=1, pml_xmax=1, pml_ymin=1, pml_ymax=1, pml_zmax=1)
+    m = pygfx.build_hex(xyz, hexes, **spec)
+    k, j, i = np.meshgrid(np.arange(3), np.arange(3), np.arange(4), indexing="ij")
+    want = ((i == 0) | (i == 3) | (j == 0) | (j == 2) | (k == 2)).ravel()
+    assert np.array_equal(m.pml_region != 0, want)
+    abar = np.abs(m.coef_abar.reshape(36, -1)).max(axis=1)
+    assert np.all(abar[~want] == 0.0) and np.all(abar[want] > 0.0)
+    # on the lattice count the index rule stays (here: the same cells)
+    lat = pygfx.build_hex(xyz, hexes, **dict(spec, nx=4, ny=3))
+    assert np.array_equal(lat.pml_region != 0, want)
 
 
 def test_cli_converts_gmsh(tmp_path):
