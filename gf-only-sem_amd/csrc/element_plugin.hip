@@ -9,6 +9,7 @@
 // instead of the 45 atomics per point of the CUDA kernel
 // (forward/elastic/src/element_cuda.cu:131-183).
 #include <cstring>
+#include <mutex>
 
 #include "common.hpp"
 
@@ -169,8 +170,109 @@ void element_residual(int n_elem, const double* dxi_dx, const double* jacobian,
     dr.get(r, P * 3);
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free variant: the device copies of the inputs and of r live in
+// one file-scope device array instead of hipMalloc'd buffers.  Each GPU holds
+// its own copy of the array (at its own address) from the first use until the
+// process ends.  Host code never passes the array's name to a kernel or a
+// copy: it passes the device address hipGetSymbolAddress returns for the
+// current device, fetched once per device and kept in scratch_ptr.
+constexpr size_t SCRATCH_BYTES = size_t(GFX_NOALLOC_SCRATCH_BYTES);
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;              // one call at a time in this process (all devices)
+void* scratch_ptr[MAX_DEVICES] = {}; // device address of `scratch` per device
+
+void* scratch_address(int dev) {
+    require(dev >= 0 && dev < MAX_DEVICES, "device index out of range");
+    if (!scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw Error(GFX_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        scratch_ptr[dev] = p;
+    }
+    return scratch_ptr[dev];
+}
+
+inline size_t round256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// Scratch bytes the allocation-free plugin needs for n_elem elements of
+// ngll^3 points (FP64: dxi_dx, J, lambda, mu, u, r per point; D and w).
+size_t noalloc_bytes(int64_t n_elem, int ngll) {
+    const size_t P = size_t(n_elem) * ngll * ngll * ngll, d = sizeof(double);
+    return round256(P * 9 * d) + 3 * round256(P * d) + round256(size_t(ngll) * ngll * d) +
+           round256(size_t(ngll) * d) + 2 * round256(P * 3 * d);
+}
+
+void element_residual_noalloc(int n_elem, const double* dxi_dx, const double* jacobian,
+                              const double* lambda_, const double* mu_, const double* D,
+                              const double* weights, int ngll, const double* u, double* r) {
+    require(n_elem >= 0, "n_elem must be >= 0");
+    require(ngll >= 2 && ngll <= 8, "NGLL must be in [2, 8]");
+    if (n_elem == 0) return;
+    require(dxi_dx && jacobian && lambda_ && mu_ && D && weights && u && r, "null array");
+    const size_t need = noalloc_bytes(n_elem, ngll);
+    if (need > SCRATCH_BYTES) {
+        const size_t per = noalloc_bytes(1, ngll);
+        throw Error(GFX_ERR_INVALID, "gfx_compute_element_residual_noalloc: " + std::to_string(n_elem) +
+                                         " elements of NGLL " + std::to_string(ngll) + " need " +
+                                         std::to_string(need) + " bytes of scratch; the maximum is " +
+                                         std::to_string(SCRATCH_BYTES) + " bytes (about " +
+                                         std::to_string(SCRATCH_BYTES / per) + " elements)");
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        throw Error(GFX_ERR_DEVICE, "no HIP device available");
+    }
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    int dev = 0;
+    GFX_HIP(hipGetDevice(&dev));
+    unsigned char* base = static_cast<unsigned char*>(scratch_address(dev));
+    const size_t nn = size_t(ngll) * ngll * ngll, P = size_t(n_elem) * nn, d = sizeof(double);
+    size_t off = 0;
+    auto put = [&](const double* h, size_t n) {
+        double* p = reinterpret_cast<double*>(base + off);
+        GFX_HIP(hipMemcpy(p, h, n * d, hipMemcpyHostToDevice));
+        off += round256(n * d);
+        return p;
+    };
+    double* ddd = put(dxi_dx, P * 9);
+    double* dj = put(jacobian, P);
+    double* dl = put(lambda_, P);
+    double* dm = put(mu_, P);
+    double* dD = put(D, size_t(ngll) * ngll);
+    double* dw = put(weights, ngll);
+    double* du = put(u, P * 3);
+    double* dr = put(r, P * 3);  // every byte the kernel reads is written by this call
+    switch (ngll) {
+        case 2: launch_plugin<double, 2>(n_elem, ddd, dj, dl, dm, dD, dw, du, dr); break;
+        case 3: launch_plugin<double, 3>(n_elem, ddd, dj, dl, dm, dD, dw, du, dr); break;
+        case 4: launch_plugin<double, 4>(n_elem, ddd, dj, dl, dm, dD, dw, du, dr); break;
+        case 5: launch_plugin<double, 5>(n_elem, ddd, dj, dl, dm, dD, dw, du, dr); break;
+        case 6: launch_plugin<double, 6>(n_elem, ddd, dj, dl, dm, dD, dw, du, dr); break;
+        case 7: launch_plugin<double, 7>(n_elem, ddd, dj, dl, dm, dD, dw, du, dr); break;
+        default: launch_plugin<double, 8>(n_elem, ddd, dj, dl, dm, dD, dw, du, dr); break;
+    }
+    GFX_HIP(hipGetLastError());
+    GFX_HIP(hipDeviceSynchronize());
+    GFX_HIP(hipMemcpy(r, dr, P * 3 * d, hipMemcpyDeviceToHost));
+}
+
 }  // namespace
 }  // namespace gfx
+
+extern "C" gfx_status gfx_compute_element_residual_noalloc(int n_elem, const double* dxi_dx,
+                                                           const double* jacobian,
+                                                           const double* lambda_, const double* mu_,
+                                                           const double* D, const double* weights,
+                                                           int ngll, const double* u, double* r) {
+    return gfx::guarded([&] {
+        gfx::element_residual_noalloc(n_elem, dxi_dx, jacobian, lambda_, mu_, D, weights, ngll, u, r);
+    });
+}
 
 extern "C" gfx_status gfx_compute_element_residual(int n_elem, const double* dxi_dx,
                                                    const double* jacobian, const double* lambda_,

@@ -59,6 +59,23 @@ gfx_status gfx_compute_element_residual_f32(int n_elem, const double* dxi_dx,
                                             const double* weights, int ngll, const double* u,
                                             double* r);
 
+/* Same contract and FP64 arithmetic as gfx_compute_element_residual, with no
+ * device allocation: no counterpart in the reference; it is for callers that
+ * cannot allocate device memory (graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  The device copies of the inputs and of r are staged in
+ * a fixed array of GFX_NOALLOC_SCRATCH_BYTES per GPU, which the library holds
+ * from its first use on that GPU until the process ends; each call overwrites
+ * every byte it reads, so no contents carry over from one call to the next.
+ * Calls needing more (about 3,500 elements at NGLL 5) fail with
+ * GFX_ERR_INVALID.  Calls in one process are serialised; calls that overlap on
+ * one GPU from several threads would otherwise race on the shared array. */
+#define GFX_NOALLOC_SCRATCH_BYTES (64u << 20)
+gfx_status gfx_compute_element_residual_noalloc(int n_elem, const double* dxi_dx,
+                                                const double* jacobian, const double* lambda_,
+                                                const double* mu_, const double* D,
+                                                const double* weights, int ngll,
+                                                const double* u, double* r);
+
 /* -------------------------------------------------------------------------
  * Device-resident solver state.
  * Replaces CudaDeviceState + cuda_allocate_state/cuda_free_state
