@@ -163,14 +163,11 @@ LDS_MEASURED_TBS = 63.9
 
 def element_lds(launch, n_elem, n_fields, ms_elem):
     """LDS read rate of the block element kernel's field loop (its binding limit)."""
-    if launch["element_kernel"] not in ("block", "vec2", "vec4") or ms_elem <= 0:
+    if launch["element_kernel"] != "block" or ms_elem <= 0:
         return None
-    # every form reads 90 neighbour values per point and field (45 displacement, 45 flux);
-    # block reads them as 32-bit words, vec2 / vec4 as 64- / 128-bit words serving 2 / 4 fields
     tbs = n_elem * 125 * n_fields * BLOCK_LDS_DWORDS * 4 / (ms_elem * 1e-3) / 1e12
     return {"dwords_per_point_field": BLOCK_LDS_DWORDS, "achieved_tbs": tbs, "measured_rate_tbs": LDS_MEASURED_TBS,
-            "frac_of_measured_rate": tbs / LDS_MEASURED_TBS,
-            "lds_word_bytes": {"block": 4, "vec2": 8, "vec4": 16}[launch["element_kernel"]]}
+            "frac_of_measured_rate": tbs / LDS_MEASURED_TBS}
 
 
 def uneven_specs(pygfx, n=16, seed=4):
@@ -333,8 +330,7 @@ def main():
     ap.add_argument("--assembly", choices=["auto", "colored", "gather", "atomic"], default="auto",
                     help="auto = the library autotuner picks assembly, element kernel and block size")
     ap.add_argument("--epb", type=int, default=0)
-    ap.add_argument("--kernel", choices=["auto", "block", "wave", "pair", "pencil", "vec2", "vec4"],
-                    default="auto")
+    ap.add_argument("--kernel", choices=["auto", "block", "wave", "pair", "pencil"], default="auto")
     ap.add_argument("--node-groups", type=int, default=0, help="node-kernel field groups (1, 2, 4, 8)")
     ap.add_argument("--small", action="store_true", help="40x40x20 mesh instead of 100x100x50 (smoke)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -377,8 +373,7 @@ def main():
     asm_of = {"colored": pygfx.GFX_ASSEMBLY_COLORED, "gather": pygfx.GFX_ASSEMBLY_GATHER,
               "atomic": pygfx.GFX_ASSEMBLY_ATOMIC}
     kern_of = {"auto": pygfx.GFX_ELEMENT_AUTO, "block": pygfx.GFX_ELEMENT_BLOCK,
-               "wave": pygfx.GFX_ELEMENT_WAVE, "pair": pygfx.GFX_ELEMENT_PAIR, "pencil": pygfx.GFX_ELEMENT_PENCIL,
-               "vec2": pygfx.GFX_ELEMENT_VEC2, "vec4": pygfx.GFX_ELEMENT_VEC4}
+               "wave": pygfx.GFX_ELEMENT_WAVE, "pair": pygfx.GFX_ELEMENT_PAIR, "pencil": pygfx.GFX_ELEMENT_PENCIL}
 
     def launch_config(n_fields):
         if args.assembly == "auto":
@@ -394,8 +389,7 @@ def main():
     names = {pygfx.GFX_ASSEMBLY_COLORED: "colored", pygfx.GFX_ASSEMBLY_GATHER: "gather",
              pygfx.GFX_ASSEMBLY_ATOMIC: "atomic"}
     knames = {pygfx.GFX_ELEMENT_BLOCK: "block", pygfx.GFX_ELEMENT_WAVE: "wave", pygfx.GFX_ELEMENT_PAIR: "pair",
-              pygfx.GFX_ELEMENT_PENCIL: "pencil", pygfx.GFX_ELEMENT_VEC2: "vec2",
-              pygfx.GFX_ELEMENT_VEC4: "vec4", 0: "auto"}
+              pygfx.GFX_ELEMENT_PENCIL: "pencil", 0: "auto"}
 
     t0 = time.perf_counter()
     dims = (40, 40, 20) if args.small else (100, 100, 50)
@@ -454,9 +448,8 @@ def main():
               "element_kernel": knames[int(sb.query("element_kernel"))],
               "elems_per_block": int(sb.query("elems_per_block")),
               "node_field_groups": int(sb.query("node_field_groups"))}
-    kern = {"element": ({"wave": "k_element_wave", "pair": "k_element_pair", "pencil": "k_element_pencil",
-                         "vec2": "k_element_vec", "vec4": "k_element_vec"}.get(launch["element_kernel"],
-                                                                               "k_element"), 1),
+    kern = {"element": ({"wave": "k_element_wave", "pair": "k_element_pair",
+                         "pencil": "k_element_pencil"}.get(launch["element_kernel"], "k_element"), 1),
             "node_special": ("k_node_special", 3), "node_regular": ("k_node_regular", 4)}
     dom = max(kern, key=lambda k: tb[k][0])
     kname, which = kern[dom]

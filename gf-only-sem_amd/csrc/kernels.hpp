@@ -791,181 +791,6 @@ __global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
     }
 }
 
-// Field-vector block kernel: as k_element, with FV fields per pass staged
-// together in LDS -- each LDS entry holds one value of FV consecutive fields
-// (8 or 16 bytes), so every neighbour read (45 for the gradient, 45 for the
-// divergence) is one wide read serving FV fields, and a pass needs two
-// barriers for FV fields.  The per-field operation sequence is flux_core's
-// and div_core's, so the residuals equal the other forms' bit for bit.  A
-// final pass with fewer than FV fields left computes the missing ones on
-// zeros and stores nothing for them.  Nodal C-PML only.
-template <int EPB, int MODE, bool DIAG, int FV>
-__global__ void __launch_bounds__(((EPB * NP + 63) / 64) * 64)
-    k_element_vec(ElemArgs A, int e0, int ne) {
-    using V = float __attribute__((ext_vector_type(FV)));
-    __shared__ float sD[25];
-    __shared__ V su[EPB][3][NP];
-    __shared__ V sf[EPB][9][NP];
-    const int tid = threadIdx.x;
-    if (tid < 25) sD[tid] = A.D[tid];
-    const int blk = xcd_block(blockIdx.x, gridDim.x);
-    const int le = tid / NP;
-    const int p = tid - le * NP;
-    const int eloc = blk * EPB + le;
-    const bool valid = (le < EPB) && (eloc < ne);
-    const int e = e0 + (valid ? eloc : 0);
-    const int i = p / 25, j = (p / 5) % 5, k = p % 5;
-    const int lel = valid ? le : 0;
-
-    float g[9];
-    float lam = 0.f, mu = 0.f;
-    uint32_t nodef = 0;
-    const int64_t pt = int64_t(e) * NP + p;
-    if (valid) {
-        load_geometry<DIAG>(A, pt, g);
-        lam = __builtin_nontemporal_load(&A.lam[pt]);
-        mu = __builtin_nontemporal_load(&A.mu[pt]);
-        nodef = __builtin_nontemporal_load(&A.ibool[pt]);
-    } else {
-#pragma unroll
-        for (int c = 0; c < 9; ++c) g[c] = 0.f;
-    }
-    const int64_t node = int64_t(nodef & ~FIRST_TOUCH);
-    const bool first = (nodef & FIRST_TOUCH) != 0;
-    int sb = 0, se = 0;
-    if (valid) {
-        sb = A.src_off[e];
-        se = A.src_off[e + 1];
-    }
-    const int step = A.ctr[0];
-    const int nf = A.nf;
-    float un[FV][3];  // the pass's fields, loaded a pass ahead
-#pragma unroll
-    for (int h = 0; h < FV; ++h) {
-        un[h][0] = un[h][1] = un[h][2] = 0.f;
-        if (valid && h < nf) {
-            const float* ut = A.U + int64_t(h) * A.fstride + node * 3;
-            un[h][0] = ut[0];
-            un[h][1] = ut[1];
-            un[h][2] = ut[2];
-        }
-    }
-    __syncthreads();
-    float Di[5], Dj[5], Dk[5], Dti[5], Dtj[5], Dtk[5];
-#pragma unroll
-    for (int q = 0; q < 5; ++q) {
-        Di[q] = sD[i * 5 + q];
-        Dj[q] = sD[j * 5 + q];
-        Dk[q] = sD[k * 5 + q];
-        Dti[q] = sD[q * 5 + i];
-        Dtj[q] = sD[q * 5 + j];
-        Dtk[q] = sD[q * 5 + k];
-    }
-    const int ri = j * 5 + k, rj = i * 25 + k, rk = i * 25 + j * 5;  // row bases along i, j, k
-    const int64_t rslot = residual_entry<MODE>(A, pt, valid);
-
-    for (int f = 0; f < nf; f += FV) {
-        if (valid) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                V t;
-#pragma unroll
-                for (int h = 0; h < FV; ++h) t[h] = un[h][c];
-                su[le][c][p] = t;
-            }
-#pragma unroll
-            for (int h = 0; h < FV; ++h) {
-                un[h][0] = un[h][1] = un[h][2] = 0.f;
-                if (f + FV + h < nf) {
-                    const float* ut = A.U + int64_t(f + FV + h) * A.fstride + node * 3;
-                    un[h][0] = ut[0];
-                    un[h][1] = ut[1];
-                    un[h][2] = ut[2];
-                }
-            }
-        }
-        float rold[FV][3];
-#pragma unroll
-        for (int h = 0; h < FV; ++h) {
-            rold[h][0] = rold[h][1] = rold[h][2] = 0.f;
-            if (MODE == ASM_COLORED && valid && !first && f + h < nf) {
-                const float* r = A.R + int64_t(f + h) * A.fstride + node * 3;
-                rold[h][0] = r[0];
-                rold[h][1] = r[1];
-                rold[h][2] = r[2];
-            }
-        }
-        __syncthreads();
-        if (valid) {
-            float du[FV][3][3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                float a0[FV], a1[FV], a2[FV];
-#pragma unroll
-                for (int h = 0; h < FV; ++h) a0[h] = a1[h] = a2[h] = 0.f;
-#pragma unroll
-                for (int s = 0; s < 5; ++s) {
-                    const V x = su[lel][c][s * 25 + ri];
-                    const V y = su[lel][c][rj + s * 5];
-                    const V z = su[lel][c][rk + s];
-#pragma unroll
-                    for (int h = 0; h < FV; ++h) {
-                        a0[h] += Di[s] * x[h];
-                        a1[h] += Dj[s] * y[h];
-                        a2[h] += Dk[s] * z[h];
-                    }
-                }
-#pragma unroll
-                for (int h = 0; h < FV; ++h) {
-                    du[h][c][0] = a0[h];
-                    du[h][c][1] = a1[h];
-                    du[h][c][2] = a2[h];
-                }
-            }
-            float F[FV][9];
-#pragma unroll
-            for (int h = 0; h < FV; ++h) flux_from_du<DIAG>(du[h], g, lam, mu, F[h]);
-#pragma unroll
-            for (int q = 0; q < 9; ++q) {
-                V t;
-#pragma unroll
-                for (int h = 0; h < FV; ++h) t[h] = F[h][q];
-                sf[le][q][p] = t;
-            }
-        }
-        __syncthreads();
-        if (valid) {
-            float rv[FV][3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                float acc[FV];
-#pragma unroll
-                for (int h = 0; h < FV; ++h) acc[h] = 0.f;
-#pragma unroll
-                for (int s = 0; s < 5; ++s) {
-                    const V x = sf[lel][c * 3 + 0][s * 25 + ri];
-                    const V y = sf[lel][c * 3 + 1][rj + s * 5];
-                    const V z = sf[lel][c * 3 + 2][rk + s];
-#pragma unroll
-                    for (int h = 0; h < FV; ++h) {
-                        acc[h] += Dti[s] * x[h];
-                        acc[h] += Dtj[s] * y[h];
-                        acc[h] += Dtk[s] * z[h];
-                    }
-                }
-#pragma unroll
-                for (int h = 0; h < FV; ++h) rv[h][c] = -acc[h];
-            }
-#pragma unroll
-            for (int h = 0; h < FV; ++h)
-                if (f + h < nf) {
-                    add_sources(A, step, f + h, sb, se, p, rv[h]);
-                    emit_residual<MODE>(A, f + h, node, rslot, rold[h], rv[h]);
-                }
-        }
-    }
-}
-
 // Pencil kernel: ten elements per 256-lane workgroup (250 lanes busy); lane
 // (element, i, j) owns the five points (i, j, 0..4) of its element, a pencil
 // along k.  The k-direction contractions of the gradient and of the

@@ -239,23 +239,6 @@ void launch_pair_t(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
     }
 }
 
-template <int EPB, int FV, bool DIAG>
-void launch_vec_t(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
-    constexpr int NT = ((EPB * NP + 63) / 64) * 64;
-    const int grid = (n + EPB - 1) / EPB;
-    if (grid == 0) return;
-    switch (S.assembly) {
-        case GFX_ASSEMBLY_ATOMIC:
-            dev::k_element_vec<EPB, dev::ASM_ATOMIC, DIAG, FV><<<grid, NT, 0, S.stream>>>(a, b, n);
-            break;
-        case GFX_ASSEMBLY_GATHER:
-            dev::k_element_vec<EPB, dev::ASM_GATHER, DIAG, FV><<<grid, NT, 0, S.stream>>>(a, b, n);
-            break;
-        default:
-            dev::k_element_vec<EPB, dev::ASM_COLORED, DIAG, FV><<<grid, NT, 0, S.stream>>>(a, b, n);
-    }
-}
-
 template <bool DIAG>
 void launch_pencil(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
     const int grid = (n + dev::PENCIL_EPB - 1) / dev::PENCIL_EPB;
@@ -275,9 +258,9 @@ void launch_pencil(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
 // Element kernel form for a request: AUTO = wave when the nodal C-PML form
 // applies, else block; WAVE, PAIR and PENCIL need the nodal form.
 int resolve_kernel(int requested, bool nodal_ok) {
-    require(requested >= GFX_ELEMENT_AUTO && requested <= GFX_ELEMENT_VEC4, "unknown element kernel");
+    require(requested >= GFX_ELEMENT_AUTO && requested <= GFX_ELEMENT_PENCIL, "unknown element kernel");
     if (requested != GFX_ELEMENT_AUTO && requested != GFX_ELEMENT_BLOCK)
-        require(nodal_ok, "the wave, pair, pencil and field-vector element kernels need the nodal C-PML form");
+        require(nodal_ok, "the wave, pair and pencil element kernels need the nodal C-PML form");
     if (requested == GFX_ELEMENT_AUTO) return nodal_ok ? GFX_ELEMENT_WAVE : GFX_ELEMENT_BLOCK;
     return requested;
 }
@@ -285,8 +268,6 @@ int resolve_kernel(int requested, bool nodal_ok) {
 void check_epb(int kernel, int epb) {
     require(epb == 0 || epb == 1 || epb == 2 || epb == 4 || epb == 8, "elems_per_block must be 1, 2, 4 or 8");
     require(!(kernel == GFX_ELEMENT_PAIR && epb == 8), "the pair element kernel takes 1, 2 or 4 elements per block");
-    require(!(kernel == GFX_ELEMENT_VEC2 && epb == 8), "the VEC2 element kernel takes 1, 2 or 4 elements per block");
-    require(!(kernel == GFX_ELEMENT_VEC4 && epb > 2), "the VEC4 element kernel takes 1 or 2 elements per block");
     require(!(kernel == GFX_ELEMENT_PENCIL && epb != 0),
             "the pencil element kernel has a fixed ten elements per block (elems_per_block must be 0)");
 }
@@ -301,17 +282,6 @@ template <bool DIAG>
 void launch_element_d(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
     if (S.kernel == GFX_ELEMENT_PENCIL) {
         launch_pencil<DIAG>(S, a, b, n);
-    } else if (S.kernel == GFX_ELEMENT_VEC2) {
-        switch (S.epb) {
-            case 1: launch_vec_t<1, 2, DIAG>(S, a, b, n); break;
-            case 4: launch_vec_t<4, 2, DIAG>(S, a, b, n); break;
-            default: launch_vec_t<2, 2, DIAG>(S, a, b, n); break;
-        }
-    } else if (S.kernel == GFX_ELEMENT_VEC4) {
-        switch (S.epb) {
-            case 1: launch_vec_t<1, 4, DIAG>(S, a, b, n); break;
-            default: launch_vec_t<2, 4, DIAG>(S, a, b, n); break;
-        }
     } else if (S.kernel == GFX_ELEMENT_PAIR) {
         switch (S.epb) {
             case 1: launch_pair_t<1, DIAG>(S, a, b, n); break;
@@ -659,7 +629,7 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
     require(d.assembly == GFX_ASSEMBLY_COLORED || d.assembly == GFX_ASSEMBLY_ATOMIC ||
                 d.assembly == GFX_ASSEMBLY_GATHER,
             "unknown assembly mode");
-    require(d.element_kernel >= GFX_ELEMENT_AUTO && d.element_kernel <= GFX_ELEMENT_VEC4,
+    require(d.element_kernel >= GFX_ELEMENT_AUTO && d.element_kernel <= GFX_ELEMENT_PENCIL,
             "unknown element kernel");
     int epb = d.elems_per_block;
     require(epb == 0 || epb == 1 || epb == 2 || epb == 4 || epb == 8,
@@ -1747,12 +1717,10 @@ extern "C" gfx_status gfx_autotune(const gfx_solver_desc* base, int trial_steps,
         double best_t = 1e300;
         gfx_solver_desc pick = *base;
         const int asms[2] = {GFX_ASSEMBLY_COLORED, GFX_ASSEMBLY_GATHER};
-        constexpr int NL = 13;
+        constexpr int NL = 9;
         const int launch[NL][2] = {{GFX_ELEMENT_BLOCK, 1}, {GFX_ELEMENT_BLOCK, 2}, {GFX_ELEMENT_BLOCK, 4},
                                    {GFX_ELEMENT_WAVE, 2},  {GFX_ELEMENT_WAVE, 4},  {GFX_ELEMENT_WAVE, 8},
-                                   {GFX_ELEMENT_PAIR, 1},  {GFX_ELEMENT_PAIR, 2},  {GFX_ELEMENT_PENCIL, 0},
-                                   {GFX_ELEMENT_VEC2, 1},  {GFX_ELEMENT_VEC2, 2},  {GFX_ELEMENT_VEC4, 1},
-                                   {GFX_ELEMENT_VEC4, 2}};
+                                   {GFX_ELEMENT_PAIR, 1},  {GFX_ELEMENT_PAIR, 2},  {GFX_ELEMENT_PENCIL, 0}};
         auto time_steps = [&](gfx_solver& S) {
             run_steps(S, 1);  // warm-up
             hipEvent_t e0, e1;
@@ -1786,8 +1754,7 @@ extern "C" gfx_status gfx_autotune(const gfx_solver_desc* base, int trial_steps,
             for (int li = 0; li < NL; ++li) {
                 const auto& lc = launch[li];
                 if (lc[0] != GFX_ELEMENT_BLOCK && S->has_cpml && !S->pml_nodal) continue;
-                if ((lc[0] == GFX_ELEMENT_PAIR || lc[0] == GFX_ELEMENT_VEC2) && S->nf < 2) continue;
-                if (lc[0] == GFX_ELEMENT_VEC4 && S->nf < 3) continue;
+                if (lc[0] == GFX_ELEMENT_PAIR && S->nf < 2) continue;
                 S->kernel = lc[0];
                 S->epb = resolve_epb(lc[0], lc[1]);
                 const double per = time_steps(*S);

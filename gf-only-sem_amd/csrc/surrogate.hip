@@ -10,9 +10,9 @@
 //
 // MLP: (x, y, z, t) scaled to [-1, 1] -> [sin, cos](2 pi B x) with B ~ N(0,
 // sigma^2) (n_freq x 4) -> depth x (Linear(width) + GELU) -> Linear(n_out).
-// Every product runs on v_mfma_f32_16x16x16bf16_1k: operands rounded to BF16
-// (round to nearest even), FP32 accumulation; master weights and Adam moments
-// stay FP32.
+// Every product runs on v_mfma_f32_16x16x32_bf16 (LDS-tiled): operands
+// rounded to BF16 (round to nearest even), FP32 accumulation; master weights
+// and Adam moments stay FP32.
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -29,7 +29,6 @@ namespace sur {
 constexpr int NP = 125;
 constexpr double EXACT_TOL = 1e-6;  // gll_interpolator.py:19
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __host__ __device__ inline uint16_t f2bf(float f) {
@@ -177,53 +176,97 @@ JudgeArgs judge_args(const gfx_green& G) {
 
 // -------------------------------- MLP -------------------------------------
 
-// C (+)= A . B with A(m,k) = TA ? A[k*lda+m] : A[m*lda+k] and
-// B(k,n) = TB ? B[n*ldb+k] : B[k*ldb+n]; one wavefront per 32x32 tile of C
-// (2 x 2 MFMA 16x16 tiles), four wavefronts per workgroup (64x64).  The K
-// range is split over gridDim.z; split z writes C + z*M*ldc (summed later in
-// a fixed order).  MFMA 16x16x16 lane layout: lane l feeds A row l%16 and B
-// column l%16 with k = 4*(l/16) .. +3, and receives D rows 4*(l/16) .. +3 of
-// column l%16.
+// C = op(A) op(B) with A(m,k) = TA ? A[k*lda+m] : A[m*lda+k] and
+// B(k,n) = TB ? B[n*ldb+k] : B[k*ldb+n], on gfx950's
+// v_mfma_f32_16x16x32_bf16, tiled through LDS: a workgroup of four wavefronts computes a 128 x 128 tile of C (each wavefront
+// 64 x 64 = 4 x 4 MFMA tiles, 16 accumulators of 4 floats), stepping K by 32.
+// Each step stages the A (128 x 32) and B (32 x 128) slices in LDS as BF16
+// (round to nearest even), both k-contiguous per row / column, so every MFMA
+// operand is one 16-byte LDS read; rows are padded to 40 BF16 (80 bytes), so
+// the 16 lanes of a read phase hit distinct banks.  Lane layout of the 16 x 16
+// x 32 BF16 MFMA: lane l feeds A row l % 16 and B column l % 16 with k =
+// 8 (l / 16) .. + 7, and receives D rows 4 (l / 16) .. + 3 of column l % 16.
+// The K range is split over gridDim.z (kc a multiple of 32); split z writes
+// C + z*M*ldc, summed later in a fixed order.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+constexpr int GT = 128, GK = 32, GKP = 40;
+
 template <bool TA, bool TB>
-__global__ void __launch_bounds__(256) k_gemm(const float* __restrict__ A, int lda, const float* __restrict__ B,
-                                              int ldb, float* __restrict__ C, int ldc, int M, int N, int K, int kc) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int m0 = blockIdx.y * 64 + (wave >> 1) * 32;
-    const int n0 = blockIdx.x * 64 + (wave & 1) * 32;
+__global__ void __launch_bounds__(256) k_gemm_lds(const float* __restrict__ A, int lda, const float* __restrict__ B,
+                                                  int ldb, float* __restrict__ C, int ldc, int M, int N, int K,
+                                                  int kc) {
+    __shared__ __attribute__((aligned(16))) uint16_t sA[GT * GKP];
+    __shared__ __attribute__((aligned(16))) uint16_t sB[GT * GKP];
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const int mb = blockIdx.y * GT, nb = blockIdx.x * GT;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
     const int kbeg = blockIdx.z * kc, kend = min(K, kbeg + kc);
     float* Cz = C + int64_t(blockIdx.z) * M * ldc;
-    const int r = lane & 15, kb = (lane >> 4) * 4;
-    f32x4 acc[2][2];
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = kbeg; k0 < kend; k0 += 16) {
-        bf16x4 a[2], b[2];
+    f32x4 acc[4][4];
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            const int m = m0 + t * 16 + r, n = n0 + t * 16 + r;
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int k = k0 + kb + q;
-                const bool kin = k < kend;
-                const float av = (kin && m < M) ? (TA ? A[int64_t(k) * lda + m] : A[int64_t(m) * lda + k]) : 0.f;
-                const float bv = (kin && n < N) ? (TB ? B[int64_t(n) * ldb + k] : B[int64_t(k) * ldb + n]) : 0.f;
-                a[t][q] = short(f2bf(av));
-                b[t][q] = short(f2bf(bv));
+        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int r16 = lane & 15, kq = (lane >> 4) * 8;
+    for (int k0 = kbeg; k0 < kend; k0 += GK) {
+        __syncthreads();  // the previous slices have been read
+        // stage: 16 values of A and 16 of B per lane
+        if (!TA) {  // A(m, k) = A[m * lda + k]: lane -> row t / 2, k half (t & 1)
+            const int m = t >> 1, kk = (t & 1) * 16;
+            const int gm = mb + m;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int k = k0 + kk + q;
+                const float v = (gm < M && k < kend) ? A[int64_t(gm) * lda + k] : 0.f;
+                sA[m * GKP + kk + q] = f2bf(v);
+            }
+        } else {  // A(m, k) = A[k * lda + m]: lane -> k = t / 8, 16 rows
+            const int kk = t >> 3, m0 = (t & 7) * 16;
+            const int k = k0 + kk;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const int gm = mb + m0 + q;
+                const float v = (gm < M && k < kend) ? A[int64_t(k) * lda + gm] : 0.f;
+                sA[(m0 + q) * GKP + kk] = f2bf(v);
             }
         }
+        if (TB) {  // B(k, n) = B[n * ldb + k]: lane -> column t / 2, k half (t & 1)
+            const int n = t >> 1, kk = (t & 1) * 16;
+            const int gn = nb + n;
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+            for (int q = 0; q < 16; ++q) {
+                const int k = k0 + kk + q;
+                const float v = (gn < N && k < kend) ? B[int64_t(gn) * ldb + k] : 0.f;
+                sB[n * GKP + kk + q] = f2bf(v);
+            }
+        } else {  // B(k, n) = B[k * ldb + n]: lane -> k = t / 8, 16 columns
+            const int kk = t >> 3, n0 = (t & 7) * 16;
+            const int k = k0 + kk;
 #pragma unroll
-            for (int j = 0; j < 2; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+            for (int q = 0; q < 16; ++q) {
+                const int gn = nb + n0 + q;
+                const float v = (gn < N && k < kend) ? B[int64_t(k) * ldb + gn] : 0.f;
+                sB[(n0 + q) * GKP + kk] = f2bf(v);
+            }
+        }
+        __syncthreads();
+        bf16x8 a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const bf16x8*>(&sA[(wm + i * 16 + r16) * GKP + kq]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const bf16x8*>(&sB[(wn + j * 16 + r16) * GKP + kq]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     }
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < 4; ++j)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int m = m0 + i * 16 + kb + q, n = n0 + j * 16 + r;
+                const int m = mb + wm + i * 16 + (lane >> 4) * 4 + q, n = nb + wn + j * 16 + r16;
                 if (m < M && n < N) Cz[int64_t(m) * ldc + n] = acc[i][j][q];
             }
 }
@@ -439,20 +482,21 @@ void ensure_rows(gfx_mlp& M, int rows) {
 // C[M][N] = op(A) op(B), split over K when the output grid is small
 template <bool TA, bool TB>
 void gemm(gfx_mlp& Mm, const float* A, int lda, const float* B, int ldb, float* C, int ldc, int M, int N, int K) {
-    const int gx = (N + 63) / 64, gy = (M + 63) / 64;
+    const int gx = (N + sur::GT - 1) / sur::GT, gy = (M + sur::GT - 1) / sur::GT;
     int splits = 1;
     while (gx * gy * splits < 256 && K / (splits * 2) >= 256) splits *= 2;
     if (splits == 1) {
-        sur::k_gemm<TA, TB><<<dim3(gx, gy, 1), 256, 0, Mm.stream>>>(A, lda, B, ldb, C, ldc, M, N, K, K);
+        sur::k_gemm_lds<TA, TB><<<dim3(gx, gy, 1), 256, 0, Mm.stream>>>(A, lda, B, ldb, C, ldc, M, N, K, K);
         return;
     }
-    const int kc = ((K + splits - 1) / splits + 15) / 16 * 16;
+    const int kc = ((K + splits - 1) / splits + sur::GK - 1) / sur::GK * sur::GK;
     const size_t need = size_t(splits) * M * ldc;
     if (Mm.work.n < need) {
         GFX_HIP(hipStreamSynchronize(Mm.stream));  // the old buffer may still be read
         Mm.work.alloc(need);
     }
-    sur::k_gemm<TA, TB><<<dim3(gx, gy, splits), 256, 0, Mm.stream>>>(A, lda, B, ldb, Mm.work.p, ldc, M, N, K, kc);
+    sur::k_gemm_lds<TA, TB><<<dim3(gx, gy, splits), 256, 0, Mm.stream>>>(A, lda, B, ldb, Mm.work.p, ldc, M, N, K,
+                                                                         kc);
     const int64_t n = int64_t(M) * ldc;
     sur::k_sum_splits<<<int((n + 255) / 256), 256, 0, Mm.stream>>>(Mm.work.p, splits, n, C);
 }
@@ -992,7 +1036,9 @@ extern "C" gfx_status gfx_surrogate_query(gfx_mlp* m, const gfx_green* g, const 
         t.alloc(n);
         c.alloc(n);
         o.alloc(size_t(n) * 18);
-        fb.upload(std::vector<uint8_t>(fallback, fallback + size_t(g->n_cell) * n_window));
+        std::vector<uint8_t> judge_only(size_t(g->n_cell) * n_window);
+        for (size_t r = 0; r < judge_only.size(); ++r) judge_only[r] = fallback[r] == 1 ? 1 : 0;
+        fb.upload(judge_only);
         GFX_HIP(hipMemcpy(q.p, xyz, size_t(n) * 24, hipMemcpyHostToDevice));
         GFX_HIP(hipMemcpy(t.p, tix, size_t(n) * 4, hipMemcpyHostToDevice));
         GFX_HIP(hipMemsetAsync(o.p, 0, size_t(n) * 72, g->stream));
@@ -1003,9 +1049,11 @@ extern "C" gfx_status gfx_surrogate_query(gfx_mlp* m, const gfx_green* g, const 
         GFX_HIP(hipMemcpy(jo.data(), o.p, jo.size() * 4, hipMemcpyDeviceToHost));
         GFX_HIP(hipMemcpy(cell.data(), c.p, size_t(n) * 4, hipMemcpyDeviceToHost));
         int nf = 0;
+        const float zero[18] = {};
         for (int i = 0; i < n; ++i) {
-            const bool use_judge = cell[i] < 0 || fallback[sur::region_of(cell[i], tix[i], g->n_time, n_window)];
-            const float* src = use_judge ? &jo[size_t(i) * 18] : &mo[size_t(i) * 18];
+            const uint8_t how = cell[i] < 0 ? 1 : fallback[sur::region_of(cell[i], tix[i], g->n_time, n_window)];
+            const bool use_judge = how == 1;
+            const float* src = use_judge ? &jo[size_t(i) * 18] : how == 2 ? zero : &mo[size_t(i) * 18];
             if (use_judge) {
                 ++nf;
                 if (cell[i] < 0)

@@ -24,8 +24,6 @@ GFX_ELEMENT_BLOCK = 1
 GFX_ELEMENT_WAVE = 2
 GFX_ELEMENT_PAIR = 3
 GFX_ELEMENT_PENCIL = 4
-GFX_ELEMENT_VEC2 = 5
-GFX_ELEMENT_VEC4 = 6
 MAX_LAYERS = 8
 
 _dp = C.POINTER(C.c_double)
@@ -1082,7 +1080,8 @@ class MLP:
         return loss.value
 
     def query(self, tile, fallback, t0, dt, xyz, tix):
-        """fallback: [n_cell] or [n_cell][n_window] flags (regions answered by the judge)."""
+        """fallback: [n_cell] or [n_cell][n_window] per region: 0 the network answers,
+        1 the judge, 2 zero."""
         xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
         tix = np.ascontiguousarray(np.broadcast_to(tix, (xyz.shape[0],)), np.int32)
         fb = np.ascontiguousarray(fallback, np.uint8)

@@ -224,6 +224,14 @@ def experiment(tile, dt_snap, source_xy, iters=4000, n_query=1_000_000, n_valid=
     served_zero = (reg >= 0) & serve_zero[np.maximum(reg, 0)]
     zero_out = np.where(served_zero[:, None], 0.0, jv)
     rel_zero = float(np.linalg.norm((zero_out - jv).astype(np.float64)) / nj)
+    # combined gate: per region the cheapest answer within the bound -- zero
+    # where the values themselves are, else the network where its error is,
+    # else the judge.  Without the network the same gate has zero or judge.
+    policy = np.where(serve_zero, 2, np.where(serve, 0, 1)).astype(np.uint8)
+    out_c, n_judge_c = net.query(tile, policy, 0.0, dt_snap, qp, qt)
+    rel_c = float(np.linalg.norm((out_c - jv).astype(np.float64)) / nj)
+    pol_q = np.where(reg >= 0, policy[np.maximum(reg, 0)], 1)
+    net_only = serve & ~serve_zero
     tb = tile.n_time * tile.n_node * 72
     res = dict(params=int(net.n_params), iterations=iters, train_s=t_train, final_loss=loss,
                sigma=sigma, freq_xy=freq[0], freq_z=freq[1], freq_t=freq[2], envelope=bool(envelope),
@@ -235,6 +243,11 @@ def experiment(tile, dt_snap, source_xy, iters=4000, n_query=1_000_000, n_valid=
                mlp_bytes_bf16=int(net.n_params) * 2, tile_bytes=tb,
                fallback_tile_bytes=int(fallback_bytes(tile, fallback, n_window)),
                zero_gate_tile_bytes=int(fallback_bytes(tile, ~serve_zero, n_window)),
+               combined_rel_l2=rel_c, combined_judge_fraction=n_judge_c / n_query,
+               combined_zero_fraction=float((pol_q == 2).mean()),
+               combined_network_fraction=float((pol_q == 0).mean()),
+               network_only_regions=int(net_only.sum()),
+               combined_tile_bytes=int(fallback_bytes(tile, policy == 1, n_window)),
                judge_query_s=t_judge, mlp_query_s=t_mlp)
     net.close()
     return res

@@ -100,13 +100,9 @@ typedef enum {
     GFX_ELEMENT_BLOCK = 1, /* one lane per GLL point, workgroup barriers between stages */
     GFX_ELEMENT_WAVE = 2,  /* one wavefront per element, two points per lane, no barriers */
     GFX_ELEMENT_PAIR = 3,  /* as BLOCK, two fields per pass (nodal C-PML only) */
-    GFX_ELEMENT_PENCIL = 4, /* lane per 5-point pencil along k, ten elements per workgroup
+    GFX_ELEMENT_PENCIL = 4 /* lane per 5-point pencil along k, ten elements per workgroup
                               (fixed; elems_per_block must be 0), k contractions in
                               registers (nodal C-PML only) */
-    GFX_ELEMENT_VEC2 = 5,  /* as BLOCK, two fields per pass sharing each LDS entry (one
-                              8-byte LDS read per neighbour for both; nodal C-PML only) */
-    GFX_ELEMENT_VEC4 = 6   /* as VEC2 with four fields per pass (16-byte LDS entries;
-                              1 or 2 elements per block) */
 } gfx_element_kernel;
 
 typedef struct {

@@ -121,11 +121,12 @@ gfx_status gfx_mlp_gradient(gfx_mlp* m, int n, const double* x, const float* y, 
  * *loss = mean loss of the last 10 % of iterations. */
 gfx_status gfx_mlp_train(gfx_mlp* m, const gfx_green* g, double t0, double dt, int iters, uint64_t seed,
                          double* loss);
-/* Query: the MLP, except in regions marked for fallback (and outside the
- * tile), where the judge answers; *n_fallback counts judge answers.  Regions
- * are (cell, time window): the tile's n_time samples split into n_window
- * windows, time index k in window k * n_window / n_time, and
- * fallback[cell * n_window + window] != 0 marks one. */
+/* Query: per region fallback[cell * n_window + window] = 0: the MLP answers;
+ * 1: the judge answers (also everywhere outside the tile); 2: the answer is
+ * zero (a region whose values are within the error bound of 0).  Regions are
+ * (cell, time window): the tile's n_time samples split into n_window windows,
+ * time index k in window k * n_window / n_time.  *n_fallback counts judge
+ * answers. */
 gfx_status gfx_surrogate_query(gfx_mlp* m, const gfx_green* g, const uint8_t* fallback, int n_window, double t0,
                                double dt, int n, const double* xyz, const int32_t* tix, float* out,
                                int* n_fallback);

@@ -48,7 +48,7 @@ def read_records(d, direction, steps, sub=""):
         p = os.path.join(d, "wavefields", sub, direction, f"record_0_{s}.h5")
         order = np.argsort(pygfx.h5_read(p, "/recording_cell_model_index"), kind="stable")
         for k in out:
-            out[k].append(pygfx.h5_read(p, "/" + k)[:, order])
+            out[k].append(pygfx.h5_read(p, "/" + k).reshape(len(order), -1)[order].ravel())
     return {k: np.stack(v) for k, v in out.items()}
 
 

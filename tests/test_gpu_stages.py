@@ -10,7 +10,8 @@ Gates:
     (state and every record snapshot), with and without C-PML, two fields;
   * COLORED / ATOMIC: the source is added after the element sum instead of
     inside it (one rounding apart; ATOMIC sums in no fixed order), so the two
-    runs agree to 1e-5 rel L2 in strain, u and v, 1e-4 in a;
+    runs agree to 1e-5 rel L2 in strain, u and v, 1e-3 in a (the parity bars'
+    ratio for the acceleration);
   * the stage-driven run against the FP64 oracle at the parity bar (1e-4);
   * exchange copies (u~ and r out to the host and back) leave the run
     unchanged, a residual changed between scatter and correct moves it, and
@@ -121,7 +122,7 @@ def test_colored_atomic_stages_match_fused_run(assembly):
         assert mc.rel_l2(rb[f][:, :n_strain], ra[f][:, :n_strain]) < 1e-5  # strain records
         for x, y in zip(sb[f][:2], sa[f][:2]):  # u, v
             assert mc.rel_l2(x, y) < 1e-5
-        assert mc.rel_l2(sb[f][2], sa[f][2]) < 1e-4  # a, the most cancellation-prone
+        assert mc.rel_l2(sb[f][2], sa[f][2]) < 1e-3  # a, the most cancellation-prone
 
 
 def test_stage_driven_run_matches_oracle():
