@@ -1011,14 +1011,23 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
         std::vector<int32_t> fill(off.begin(), off.end() - 1);
         for (int64_t q = 0; q < P; ++q) pts[fill[hib[q]]++] = int32_t(q);
         // residual entry of each point: a node's first point (lowest solver order) at
-        // the node's own index, every other point at G4 + its rank among those points
-        // in solver order; a node's slots then list entries, first one = the node
+        // the node's own index, the other points after the nodes, element by element,
+        // ordered by node id inside each element (the points a node-kernel workgroup
+        // gathers from one element then sit together); a node's slots list entries,
+        // the first one being the node itself.  Entry positions do not change the
+        // sums: each node adds its slots in increasing solver point order.
         std::vector<int32_t> rdst(P, -1);
         for (int64_t n = 0; n < S.G4; ++n)
             if (off[n + 1] > off[n]) rdst[pts[off[n]]] = int32_t(n);
         int64_t k = S.G4;
-        for (int64_t q = 0; q < P; ++q)
-            if (rdst[q] < 0) rdst[q] = int32_t(k++);
+        for (int64_t e = 0; e < E; ++e) {
+            int64_t rest[NP];
+            int nr = 0;
+            for (int p = 0; p < NP; ++p)
+                if (rdst[e * NP + p] < 0) rest[nr++] = e * NP + p;
+            std::sort(rest, rest + nr, [&](int64_t a, int64_t b) { return hib[a] < hib[b] || (hib[a] == hib[b] && a < b); });
+            for (int q = 0; q < nr; ++q) rdst[rest[q]] = int32_t(k++);
+        }
         require(k == S.G4 + P - S.G, "internal: residual entries do not match the point count");
         S.PR = (k + 3) / 4 * 4;  // 16-byte aligned field stride
         for (int64_t q = 0; q < P; ++q) pts[q] = rdst[pts[q]];
