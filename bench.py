@@ -496,6 +496,21 @@ def main():
                           for k, (_, w) in kern.items() if tb[k][0] > 0},
         "ms_record_per_snapshot": (tb["record"][0] / tb["record"][1]) if tb["record"][1] else None,
     }
+    # records leave the GPU once per snapshot: time copying one snapshot of every
+    # field to host FP64 buffers (the forward driver's drain), and the
+    # throughput with that copy added for every snapshot step (not overlapped)
+    if sb.n_snapshots > 0:
+        t0 = time.perf_counter()
+        for f in range(F):
+            sb.get_record(f, sb.n_snapshots - 1)
+        drain = time.perf_counter() - t0
+        snaps = args.steps / mesh.snapshot_stride
+        result["record_drain"] = {
+            "s_per_snapshot": drain, "bytes_per_snapshot": int(F * len(mesh.rec_cell) * 125 * 6 * 4),
+            "value_with_drain": E * 125 * F * args.steps / (tb["wall_s"] + snaps * drain),
+            "note": "one snapshot of every field (strain at the recording cells) copied to host "
+                    "FP64 arrays after the timed steps; value_with_drain adds that copy at every snapshot "
+                    "step without overlap (the forward driver overlaps it with the next segment)"}
     sb.close()
     del sb
     if not args.no_uneven and not args.small:

@@ -303,13 +303,14 @@ __global__ void k_gelu_back(float* __restrict__ dH, const float* __restrict__ Z,
 
 // column sums of X[rows][cols] in a fixed order: partial sums over chunks of
 // 256 rows, then summed in chunk order
+constexpr int COLSUM_ROWS = 32;  // rows per partial column sum
 __global__ void k_colsum_part(const float* __restrict__ X, int rows, int cols, float* __restrict__ part) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const int chunk = blockIdx.y;
     if (c >= cols) return;
     float s = 0.f;
-    const int r1 = min(rows, (chunk + 1) * 256);
-    for (int r = chunk * 256; r < r1; ++r) s += X[int64_t(r) * cols + c];
+    const int r1 = min(rows, (chunk + 1) * COLSUM_ROWS);
+    for (int r = chunk * COLSUM_ROWS; r < r1; ++r) s += X[int64_t(r) * cols + c];
     part[int64_t(chunk) * cols + c] = s;
 }
 
@@ -474,7 +475,7 @@ void ensure_rows(gfx_mlp& M, int rows) {
     M.out.alloc(size_t(cap) * M.cfg.n_out);
     M.dA.alloc(size_t(cap) * std::max(wdt, M.n_feat));
     M.dB.alloc(size_t(cap) * std::max(wdt, M.n_feat));
-    M.part.alloc(size_t(cap / 256 + 1) * std::max(wdt, 64));
+    M.part.alloc(size_t(cap / sur::COLSUM_ROWS + 1) * std::max(wdt, 64));
     M.lossp.alloc(size_t(cap) * M.cfg.n_out / 256 + 1);
     M.cap = cap;
 }
@@ -502,7 +503,7 @@ void gemm(gfx_mlp& Mm, const float* A, int lda, const float* B, int ldb, float* 
 }
 
 void colsum(gfx_mlp& M, const float* X, int rows, int cols, float* dst) {
-    const int chunks = (rows + 255) / 256;
+    const int chunks = (rows + sur::COLSUM_ROWS - 1) / sur::COLSUM_ROWS;
     sur::k_colsum_part<<<dim3((cols + 63) / 64, chunks), 64, 0, M.stream>>>(X, rows, cols, M.part.p);
     sur::k_sum_splits<<<(cols + 255) / 256, 256, 0, M.stream>>>(M.part.p, chunks, cols, dst);
 }

@@ -247,3 +247,31 @@ def test_tile_and_surrogate_pipeline_small():
     assert res["rel_l2_with_fallback"] <= res["rel_l2_mlp"]
     assert res["fallback_tile_bytes"] <= res["tile_bytes"]
     assert res["mlp_bytes_bf16"] < res["tile_bytes"]
+
+
+def test_network_serves_regions_the_zero_predictor_cannot():
+    """What the trained network adds over the method without it.  Without the
+    network, a query gate can answer a (cell, time window) region with 0 where
+    the region's values are within the error bound of 0, and must send every
+    other region to the judge (stored tile + GLL interpolation).  With it, the
+    gate also serves the regions where the network's validation error is
+    within the bound.  On a small halfspace tile (0.5 Hz source, the network
+    of the benchmark configuration: 64 frequencies, 6 x 256 GELU layers, 3000
+    Adam iterations) the network serves regions the zero predictor cannot, the
+    judge needs fewer tile bytes, and the combined answer stays within the
+    bound (x 2 for validation-set noise) on an independent test set."""
+    from pygfx import greens
+    mesh = mc.small_halfspace(nx=8, ny=8, nz=5, pml=2, duration=2.5)
+    stf = pygfx.ricker(0.5, 2.4, 1e20, mesh.dt, mesh.nsteps)
+    xyz = (0.5 * (mesh.bounds[0] + mesh.bounds[1]), 0.5 * (mesh.bounds[2] + mesh.bounds[3]), 0.0)
+    tile, info = greens.build_tile(mesh, xyz, stf, depth_max=1000.0, ring=4)
+    res = greens.experiment(tile, info["dt_snap"], xyz[:2], iters=3000, n_query=20000, n_valid=400000, n_window=4,
+                            bound=0.01, n_freq=64, width=256, depth=6, batch=4096)
+    print({k: res[k] for k in ("rel_l2_mlp", "regions", "regions_served", "regions_served_by_zero",
+                               "network_only_regions", "combined_network_fraction", "combined_zero_fraction",
+                               "combined_judge_fraction", "combined_rel_l2", "combined_tile_bytes",
+                               "zero_gate_tile_bytes", "tile_bytes")})
+    assert res["network_only_regions"] > 0 and res["combined_network_fraction"] > 0.0
+    assert res["combined_tile_bytes"] < res["zero_gate_tile_bytes"] <= res["tile_bytes"]
+    assert res["combined_judge_fraction"] < 1.0 - res["combined_zero_fraction"]
+    assert res["combined_rel_l2"] <= 0.02
