@@ -1503,17 +1503,21 @@ extern "C" gfx_status gfx_solver_get_record(const gfx_solver* s, int field, int 
         if (strain) {
             const std::vector<float> h =
                 fetch(s, s->rec_strain.p + (int64_t(slot) * s->nf + field) * nrp * 6, size_t(nrp) * 6);
-            for (size_t q = 0; q < h.size(); ++q) strain[q] = h[q];
+            // FP32 -> FP64 widening on the host threads, one recording point per index
+            parallel_for(nrp, [&](int64_t q) {
+                for (int c = 0; c < 6; ++c) strain[q * 6 + c] = h[q * 6 + c];
+            });
         }
         if (disp || vel || acc) {
             const std::vector<float> h =
                 fetch(s, s->rec_uva.p + (int64_t(slot) * s->nf + field) * nrp * 9, size_t(nrp) * 9);
-            for (int64_t q = 0; q < nrp; ++q)
+            parallel_for(nrp, [&](int64_t q) {
                 for (int c = 0; c < 3; ++c) {
                     if (disp) disp[q * 3 + c] = h[q * 9 + c];
                     if (vel) vel[q * 3 + c] = h[q * 9 + 3 + c];
                     if (acc) acc[q * 3 + c] = h[q * 9 + 6 + c];
                 }
+            });
         }
     });
 }
