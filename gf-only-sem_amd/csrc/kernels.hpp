@@ -33,6 +33,8 @@ namespace dev {
 constexpr int NG = 5;    // GLL points per axis (polynomial order 4)
 constexpr int NP = 125;  // points per element
 constexpr uint32_t FIRST_TOUCH = 0x80000000u;
+constexpr int CPML_NC = 28;  // complete C-PML coefficients per point
+constexpr int CPML_NM = 45;  // complete C-PML memories per point and field
 
 enum { ASM_COLORED = 0, ASM_ATOMIC = 1, ASM_GATHER = 2 };
 
@@ -61,6 +63,10 @@ struct ElemArgs {
     float* __restrict__ pmem;          // [nf][9][Ppml]
     float* __restrict__ pnew;          // [nf][3][Ppml]
     float c1, c2, dt;
+    // complete C-PML (PML elements; slot from pml_slot): coefficients
+    // [CPML_NC][Ppml] (axis q at 9q: K, d, alpha, a b c of alpha, a b c of
+    // beta; 27: rho J w) and memories pmem [nf][CPML_NM][Ppml]
+    const float* __restrict__ pcoef;
     // point sources
     const int32_t* __restrict__ src_off;  // [E+1]
     const int4* __restrict__ src_ent;     // (field, dir, weight offset, 0)
@@ -662,6 +668,187 @@ __global__ void __launch_bounds__(128) k_element_att(ElemArgs A, int e0, int ne)
         if (valid) {
             float rv[3];
             point_div(&sf[0][0], NP, Dti, Dtj, Dtk, i, j, k, rv);
+            add_sources(A, step, f, sb, se, p, rv);
+            emit_residual<MODE>(A, f, node, rslot, rold, rv);
+        }
+        __syncthreads();  // su and sf are rewritten by the next field
+    }
+}
+
+// One step of conv(g, p) (exact for g linear in time over the step): the
+// carry q holds a M + b g of the previous level; returns M = q + c g and
+// leaves a M + b g for the next step.
+__device__ __forceinline__ float conv_step(const float* abc, float& q, float g) {
+    const float m = q + abc[2] * g;
+    q = abc[0] * m + abc[1] * g;
+    return m;
+}
+
+// Complete C-PML element kernel (PML elements only): one element per
+// workgroup, one lane per GLL point, general geometry.  Stress column j of a
+// point sees the gradient G^(j)_kl = F^-1[s_x s_y s_z / (s_j s_l)] * du_k/dx_l,
+// built from chains of first-order filters of the stretches
+// s_q = K_q + d_q / (alpha_q + i w):
+//   diagonal j = l: s_m s_n / s_l (m < n the other axes), memories
+//     conv(g, alpha_m), conv(g, alpha_n), conv(conv(g, alpha_m), alpha_n),
+//     conv(s_m s_n g, beta_l);
+//   j != l: the stretch of the third axis, K g + d conv(g, alpha).
+// S_ij = lambda delta_ij sum_k G^(j)_kk + mu (G^(j)_ij + G^(j)_ji) replaces the
+// stress, and each point subtracts rho J w L, the inertia term of
+// (i w)^2 s_x s_y s_z beyond K_x K_y K_z u'', from its residual, with u the
+// predicted displacement and v = w_n = v_n + dt/2 a_n; the rest of the
+// velocity terms at v_{n+1} (dt/2 a_{n+1} times their coefficient) is in the
+// lumped mass, as is K_x K_y K_z.
+template <int MODE>
+__global__ void __launch_bounds__(128) k_element_cpml(ElemArgs A, int e0, int ne) {
+    __shared__ float sD[25];
+    __shared__ float su[3][NP];
+    __shared__ float sf[9][NP];
+    const int tid = threadIdx.x;
+    if (tid < 25) sD[tid] = A.D[tid];
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int p = tid;
+    const bool valid = p < NP && blk < ne;
+    const int e = e0 + (valid ? blk : 0);
+    const int pc = valid ? p : 0;
+    const int i = pc / 25, j = (pc / 5) % 5, k = pc % 5;
+    float g[9], cf[CPML_NC];
+    float lam = 0.f, mu = 0.f;
+    uint32_t nodef = 0;
+    const int64_t pt = int64_t(e) * NP + pc;
+    int64_t q = 0;
+    if (valid) {
+        load_geometry<false>(A, pt, g);
+        lam = A.lam[pt];
+        mu = A.mu[pt];
+        nodef = A.ibool[pt];
+        q = int64_t(A.pml_slot[e]) * NP + p;
+#pragma unroll
+        for (int c = 0; c < CPML_NC; ++c) cf[c] = A.pcoef[c * A.Ppml + q];
+    } else {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) g[c] = 0.f;
+#pragma unroll
+        for (int c = 0; c < CPML_NC; ++c) cf[c] = 0.f;
+    }
+    const int64_t node = int64_t(nodef & ~FIRST_TOUCH);
+    const bool first = (nodef & FIRST_TOUCH) != 0;
+    int sb = 0, se = 0;
+    if (valid) {
+        sb = A.src_off[e];
+        se = A.src_off[e + 1];
+    }
+    const int step = A.ctr[0];
+    const int64_t rslot = residual_entry<MODE>(A, pt, valid);
+    __syncthreads();
+    float Di[5], Dj[5], Dk[5], Dti[5], Dtj[5], Dtk[5];
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+        Di[s] = sD[i * 5 + s];
+        Dj[s] = sD[j * 5 + s];
+        Dk[s] = sD[k * 5 + s];
+        Dti[s] = sD[s * 5 + i];
+        Dtj[s] = sD[s * 5 + j];
+        Dtk[s] = sD[s * 5 + k];
+    }
+    for (int f = 0; f < A.nf; ++f) {
+        const int64_t fo = int64_t(f) * A.fstride;
+        if (valid) {
+            const float* ut = A.U + fo + node * 3;
+            su[0][p] = ut[0];
+            su[1][p] = ut[1];
+            su[2][p] = ut[2];
+        }
+        float rold[3] = {0.f, 0.f, 0.f};
+        if (MODE == ASM_COLORED && valid && !first) {
+            const float* r = A.R + fo + node * 3;
+            rold[0] = r[0];
+            rold[1] = r[1];
+            rold[2] = r[2];
+        }
+        __syncthreads();
+        float* mem = A.pmem + int64_t(f) * CPML_NM * A.Ppml + q;
+        if (valid) {
+            float du[3][3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+#pragma unroll
+                for (int s = 0; s < 5; ++s) {
+                    a0 += Di[s] * su[c][s * 25 + j * 5 + k];
+                    a1 += Dj[s] * su[c][i * 25 + s * 5 + k];
+                    a2 += Dk[s] * su[c][i * 25 + j * 5 + s];
+                }
+                du[c][0] = a0;
+                du[c][1] = a1;
+                du[c][2] = a2;
+            }
+            float G[3][9];  // G[j][k*3+l]: the (scaled) gradient stress column j sees
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk)
+#pragma unroll
+                for (int l = 0; l < 3; ++l) {
+                    const int m = l == 0 ? 1 : 0, n = l == 2 ? 1 : 2;
+                    const float* cm = cf + m * 9;
+                    const float* cn = cf + n * 9;
+                    const float* cl = cf + l * 9;
+                    const float gv = du[kk][0] * g[l * 3 + 0] + du[kk][1] * g[l * 3 + 1] + du[kk][2] * g[l * 3 + 2];
+                    float* Qp = mem + int64_t((kk * 3 + l) * 4) * A.Ppml;
+                    float q0 = Qp[0], q1 = Qp[A.Ppml], q2 = Qp[2 * A.Ppml], q3 = Qp[3 * A.Ppml];
+                    const float Mm = conv_step(cm + 3, q0, gv);
+                    const float Mn = conv_step(cn + 3, q1, gv);
+                    const float Mmn = conv_step(cn + 3, q2, Mm);
+                    const float y2 = cm[0] * cn[0] * gv + cn[0] * cm[1] * Mm + cn[1] * (cm[0] * Mn + cm[1] * Mmn);
+                    const float Mb = conv_step(cl + 6, q3, y2);
+                    Qp[0] = q0;
+                    Qp[A.Ppml] = q1;
+                    Qp[2 * A.Ppml] = q2;
+                    Qp[3 * A.Ppml] = q3;
+                    G[l][kk * 3 + l] = (y2 - cl[1] / cl[0] * Mb) / cl[0];
+                    G[n][kk * 3 + l] = cm[0] * gv + cm[1] * Mm;
+                    G[m][kk * 3 + l] = cn[0] * gv + cn[1] * Mn;
+                }
+            float S[3][3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) {
+                    S[a][b] = mu * (G[b][a * 3 + b] + G[b][b * 3 + a]);
+                    if (a == b) S[a][b] += lam * (G[b][0] + G[b][4] + G[b][8]);
+                }
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    sf[c * 3 + a][p] = S[c][0] * g[0 * 3 + a] + S[c][1] * g[1 * 3 + a] + S[c][2] * g[2 * 3 + a];
+        }
+        __syncthreads();
+        if (valid) {
+            float rv[3];
+            point_div(&sf[0][0], NP, Dti, Dtj, Dtk, i, j, k, rv);
+            const float* cx = cf;
+            const float* cy = cf + 9;
+            const float* cz = cf + 18;
+            float* im = mem + int64_t(36) * A.Ppml;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float u = su[c][p], v = A.W[fo + node * 3 + c];
+                float q1 = im[(c * 3 + 0) * A.Ppml], q2 = im[(c * 3 + 1) * A.Ppml], q3 = im[(c * 3 + 2) * A.Ppml];
+                const float C1 = conv_step(cx + 3, q1, u);
+                const float z1 = cx[0] * u + cx[1] * C1;
+                const float C2 = conv_step(cy + 3, q2, z1);
+                const float z2 = cy[0] * z1 + cy[1] * C2;
+                const float C3 = conv_step(cz + 3, q3, z2);
+                const float L1 = cx[1] * (v - cx[2] * u + cx[2] * cx[2] * C1);
+                const float Dz1 = cx[0] * v + cx[1] * (u - cx[2] * C1);
+                const float L2 = cy[0] * L1 + cy[1] * (Dz1 - cy[2] * z1 + cy[2] * cy[2] * C2);
+                const float Dz2 = cy[0] * Dz1 + cy[1] * (z1 - cy[2] * C2);
+                const float L3 = cz[0] * L2 + cz[1] * (Dz2 - cz[2] * z2 + cz[2] * cz[2] * C3);
+                im[(c * 3 + 0) * A.Ppml] = q1;
+                im[(c * 3 + 1) * A.Ppml] = q2;
+                im[(c * 3 + 2) * A.Ppml] = q3;
+                rv[c] -= cf[27] * L3;
+            }
             add_sources(A, step, f, sb, se, p, rv);
             emit_residual<MODE>(A, f, node, rslot, rold, rv);
         }

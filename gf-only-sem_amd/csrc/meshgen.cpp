@@ -343,6 +343,8 @@ void validate(const gfx_box_spec& s, bool general) {
     require(s.cfl_safety > 0 && s.output_dt > 0 && s.duration > 0,
             "cfl_safety, output_dt and duration must be > 0");
     require(general || (s.jitter >= 0 && s.jitter < 0.25), "jitter must be in [0, 0.25)");
+    require(s.pml_profile == 0 || s.pml_profile == 1, "pml_profile must be 0 or 1");
+    require(s.pml_reflection >= 0.0 && s.pml_reflection < 1.0, "pml_reflection must be in [0, 1)");
     if (!general) {
         const double n_nodes = double(s.nx * s.order + 1) * (s.ny * s.order + 1) * (s.nz * s.order + 1);
         require(n_nodes < 2.0e9, "too many global nodes for int32 ids");
@@ -750,9 +752,15 @@ void build(MeshHolder& H, const GeneralCells* gen = nullptr) {
     H.coef_abar.assign(P * 5, 0.0);
     if (s.want_beta) H.coef_beta.resize(P * 9);
     if (s.want_strain) H.coef_strain.assign(P * 18, 0.0);
+    if (s.want_profiles) {
+        H.pml_K.assign(P * 3, 1.0);
+        H.pml_d.assign(P * 3, 0.0);
+        H.pml_alpha.assign(P * 3, 0.0);
+    }
     H.damping.assign(P, 0.0);
     const double alpha_max[3] = {M_PI * s.f0_pml * 0.9, M_PI * s.f0_pml * 1.0,
                                  M_PI * s.f0_pml * 1.1};
+    const double rcoef = s.pml_reflection > 0.0 ? s.pml_reflection : 1e-5;  // R_COEF, pml_cpml.py:23
     const int64_t ncell_approx = std::max<int64_t>(1, (int64_t)std::ceil(std::pow(double(E), 1.0 / 3.0)));
     parallel_for(E, [&](int64_t e) {
         int region = 0;
@@ -785,7 +793,29 @@ void build(MeshHolder& H, const GeneralCells* gen = nullptr) {
             const double* X = &H.node_xyz[int64_t(H.ibool[p]) * 3];
             const double vp = nvp[H.ibool[p]];
             double K[3] = {1, 1, 1}, d[3] = {0, 0, 0}, al[3] = {0, 0, 0};
-            if (is_pml[e]) {  // compute_pml_profiles, pml_cpml.py:69-155 (later faces overwrite)
+            if (is_pml[e] && s.pml_profile == 1) {
+                // graded into the layer: depth from the interior interface of each
+                // face's own layer, the deeper of an axis's two faces
+                for (int ax = 0; ax < 3; ++ax) {
+                    if (!axis_active(region, ax)) continue;
+                    double depth = -1.0, w = 0.0;
+                    for (int side = 0; side < 2; ++side) {
+                        const int f = 2 * ax + side;
+                        const double wf = wid[f];
+                        if (wf <= 0) continue;
+                        const double dep = side == 0 ? (bnd[f] + wf - X[ax]) / wf : (X[ax] - (bnd[f] - wf)) / wf;
+                        if (dep > depth) {
+                            depth = dep;
+                            w = wf;
+                        }
+                    }
+                    if (depth < 0.0) continue;  // the point is in no layer of this axis
+                    depth = std::min(1.0, depth);
+                    K[ax] = 1.0;  // K_MIN_PML = K_MAX_PML = 1
+                    d[ax] = -((2.0 + 1.0) * vp * std::log(rcoef) / (2.0 * w)) * std::pow(depth, 1.2 * 2.0);
+                    al[ax] = alpha_max[ax] * (1.0 - depth);
+                }
+            } else if (is_pml[e]) {  // compute_pml_profiles, pml_cpml.py:69-155 (later faces overwrite)
                 for (int f = 0; f < 6; ++f) {
                     const int ax = f / 2;
                     const double w = wid[f];
@@ -793,7 +823,7 @@ void build(MeshHolder& H, const GeneralCells* gen = nullptr) {
                     double dist = std::abs(X[ax] - bnd[f]) / w;
                     dist = std::min(1.0, std::max(0.0, dist));
                     double Kv = 1.0 + (1.0 - 1.0) * dist;  // K_MIN_PML=K_MAX_PML=1
-                    double dv = -((2.0 + 1.0) * vp * std::log(1e-5) / (2.0 * w)) *
+                    double dv = -((2.0 + 1.0) * vp * std::log(rcoef) / (2.0 * w)) *
                                 std::pow(dist, 1.2 * 2.0);
                     double av = alpha_max[ax] * (1.0 - dist);
                     if (Kv < 1.0 || dv < 0.0) {
@@ -815,6 +845,12 @@ void build(MeshHolder& H, const GeneralCells* gen = nullptr) {
                     for (int q = 0; q < 3; ++q) H.coef_beta[p * 9 + ax * 3 + q] = c3[q];
                 }
             }
+            if (s.want_profiles)
+                for (int ax = 0; ax < 3; ++ax) {
+                    H.pml_K[p * 3 + ax] = K[ax];
+                    H.pml_d[p * 3 + ax] = d[ax];
+                    H.pml_alpha[p * 3 + ax] = al[ax];
+                }
             if (region != 0) abar(region, K, d, al, &H.coef_abar[p * 5]);
             if (region != 0 && s.want_strain) strain_coef(region, K, d, al, &H.coef_strain[p * 18]);
         }
@@ -1201,6 +1237,9 @@ void wire_mesh(MeshHolder& H) {
     m.rec_cell = H.rec_cell.empty() ? nullptr : H.rec_cell.data();
     m.tau_sigma = H.tau_sigma.empty() ? nullptr : H.tau_sigma.data();
     m.tau_epsilon = H.tau_epsilon.empty() ? nullptr : H.tau_epsilon.data();
+    m.pml_K = H.pml_K.empty() ? nullptr : H.pml_K.data();
+    m.pml_d = H.pml_d.empty() ? nullptr : H.pml_d.data();
+    m.pml_alpha = H.pml_alpha.empty() ? nullptr : H.pml_alpha.data();
 }
 
 }  // namespace gfx

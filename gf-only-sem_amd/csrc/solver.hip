@@ -8,6 +8,7 @@
 //   -> regular-node kernel -> (snapshot steps) record kernel.
 // Host-side setup (colouring, node renumbering, C-PML assembly, FP64 -> FP32
 // packing) happens once in gfx_solver_create.
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -26,10 +27,12 @@ namespace gfx {
 
 namespace {
 // Solver elements [b, e) launched together: one colour (COLORED) or all
-// elements, split into the axis-aligned part (diag) and the rest.
+// elements, split into the axis-aligned part (diag) and the rest, and with the
+// complete C-PML into PML elements (pml) and the others.
 struct ElemRange {
     int b, e;
     bool diag;
+    bool pml;
 };
 
 struct FieldSource {
@@ -56,6 +59,11 @@ struct gfx_solver {
     double dt = 0;
     int stride = 1;
     bool has_cpml = false, pml_nodal = true;
+    // complete C-PML: stretched stresses and inertia terms inside the PML
+    // elements' own kernel (coefficients pcoef [28][Ppml], memories in pmem
+    // [nf][45][Ppml]); the node kernels then apply no C-PML term
+    bool cpml_complete = false;
+    gfx::DevBuf<float> pcoef;
     hipStream_t stream = nullptr;
     std::vector<int32_t> int_of_elem;        // caller element -> solver element
     std::vector<int32_t> node_new_of_old;    // caller node -> solver node
@@ -156,6 +164,108 @@ void build_sources(gfx_solver& S) {
     S.src_w.upload(w);
     S.stf.upload(stf);
     S.nstf = nstf;
+}
+
+// conv(g, p)(t) = int_0^t exp(-p (t - s)) g(s) ds over one step, for g linear
+// in time between the step's two levels: M = Q + c g, then Q <- a M + b g, with
+// a = exp(-p dt), b = I0 - I1/dt, c = I1/dt, I0 = (1 - a)/p, I1 = (dt - I0)/p.
+void cpml_recursion(double p, double dt, double* abc) {
+    const double x = p * dt;
+    double i0, c;  // I0 / dt and I1 / dt^2
+    if (std::abs(x) < 1e-2) {  // sum_k (-x)^k / (k+1)! and sum_k (-x)^k / (k+2)!
+        double term = 1.0, f1 = 1.0, f2 = 2.0;
+        i0 = 0.0;
+        c = 0.0;
+        for (int k = 0; k < 8; ++k) {
+            i0 += term / f1;
+            c += term / f2;
+            term *= -x;
+            f1 *= k + 2;
+            f2 *= k + 3;
+        }
+    } else {
+        const double em = std::expm1(-x);
+        i0 = -em / x;
+        c = (x + em) / (x * x);
+    }
+    abc[0] = std::exp(-x);
+    abc[1] = dt * (i0 - c);
+    abc[2] = dt * c;
+}
+
+// The 27 complete C-PML coefficients of one point: per axis K, d, alpha, the
+// recursion of alpha and that of beta = alpha + d / K (both zero on an axis
+// with d = 0, whose memories then stay zero).  False on an invalid profile.
+bool cpml_point_coefficients(const double* K, const double* d, const double* al, double dt, double* c) {
+    bool ok = true;
+    for (int q = 0; q < 3; ++q) {
+        double* cq = c + q * 9;
+        const bool good = std::isfinite(K[q]) && K[q] > 0.0 && std::isfinite(d[q]) && d[q] >= 0.0 &&
+                          std::isfinite(al[q]) && al[q] >= 0.0;
+        ok = ok && good;
+        cq[0] = K[q];
+        cq[1] = d[q];
+        cq[2] = al[q];
+        for (int t = 3; t < 9; ++t) cq[t] = 0.0;
+        if (good && d[q] != 0.0) {
+            cpml_recursion(al[q], dt, cq + 3);
+            cpml_recursion(al[q] + d[q] / K[q], dt, cq + 6);
+        }
+    }
+    return ok;
+}
+
+// Nodes of the complete C-PML's outer boundary, held rigid: the faces of PML
+// elements that no other element shares and whose normal (the largest
+// component of grad xi_a at the face centre) is an axis the element's region
+// damps (region codes of preprocess/pml_cpml.py:27-34).  The free surface and
+// the PML's interior interface are therefore not included.
+std::vector<char> cpml_rigid_nodes(const gfx_solver_desc& d, int E, int G) {
+    auto damps = [](int region, int axis) {
+        static const int mask[8] = {0, 1, 2, 4, 3, 5, 6, 7};  // bit 0 x, 1 y, 2 z
+        return region >= 1 && region <= 7 && (mask[region] >> axis & 1);
+    };
+    auto pidx = [](int a, int side, int s, int t) {  // point of face (a, side) at in-face indices (s, t)
+        const int f = side ? 4 : 0;
+        return a == 0 ? (f * 5 + s) * 5 + t : a == 1 ? (s * 5 + f) * 5 + t : (s * 5 + t) * 5 + f;
+    };
+    struct Face {
+        std::array<int32_t, 4> key;
+        int32_t e;
+        int8_t f;
+    };
+    std::vector<Face> faces;
+    faces.reserve(size_t(E) * 6);
+    for (int e = 0; e < E; ++e)
+        for (int f = 0; f < 6; ++f) {
+            Face q{};
+            for (int c = 0; c < 4; ++c)
+                q.key[c] = d.ibool[int64_t(e) * NP + pidx(f / 2, f % 2, (c & 1) * 4, (c >> 1) * 4)];
+            std::sort(q.key.begin(), q.key.end());
+            q.e = e;
+            q.f = int8_t(f);
+            faces.push_back(q);
+        }
+    std::sort(faces.begin(), faces.end(), [](const Face& a, const Face& b) { return a.key < b.key; });
+    std::vector<char> rigid(G, 0);
+    for (size_t i = 0; i < faces.size();) {
+        size_t j = i + 1;
+        while (j < faces.size() && faces[j].key == faces[i].key) ++j;
+        const Face& q = faces[i];
+        const int region = d.pml_region[q.e];
+        if (j == i + 1 && region != 0) {
+            const int a = q.f / 2, side = q.f % 2;
+            const double* g = d.dxi_dx + (int64_t(q.e) * NP + pidx(a, side, 2, 2)) * 9;
+            int axis = 0;
+            for (int m = 1; m < 3; ++m)
+                if (std::abs(g[m * 3 + a]) > std::abs(g[axis * 3 + a])) axis = m;
+            if (damps(region, axis))
+                for (int s2 = 0; s2 < 5; ++s2)
+                    for (int t = 0; t < 5; ++t) rigid[d.ibool[int64_t(q.e) * NP + pidx(a, side, s2, t)]] = 1;
+        }
+        i = j;
+    }
+    return rigid;
 }
 
 void init_state(gfx_solver& S) {
@@ -300,6 +410,9 @@ void launch_element_d(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
     }
 }
 
+// The reference's C-PML in its element-local form (the nodal form does not apply).
+bool ref_elem_pml(const gfx_solver& S) { return S.has_cpml && !S.pml_nodal && !S.cpml_complete; }
+
 template <int MODE>
 void launch_att(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
     switch (S.n_sls) {
@@ -323,7 +436,15 @@ void launch_element(gfx_solver& S, const dev::ElemArgs& a, const ElemRange& rg) 
         }
         return;
     }
-    if (S.has_cpml && !S.pml_nodal) {
+    if (rg.pml) {  // complete C-PML: one element per workgroup, general form
+        if (n > 0) {
+            switch (S.assembly) {
+                case GFX_ASSEMBLY_GATHER: dev::k_element_cpml<dev::ASM_GATHER><<<n, 128, 0, S.stream>>>(a, b, n); break;
+                case GFX_ASSEMBLY_ATOMIC: dev::k_element_cpml<dev::ASM_ATOMIC><<<n, 128, 0, S.stream>>>(a, b, n); break;
+                default: dev::k_element_cpml<dev::ASM_COLORED><<<n, 128, 0, S.stream>>>(a, b, n); break;
+            }
+        }
+    } else if (ref_elem_pml(S)) {
         launch_element_p<true, false>(S, a, b, n);  // element-local C-PML: block kernel, general form
     } else if (rg.diag) {
         launch_element_d<true>(S, a, b, n);
@@ -396,6 +517,7 @@ StepArgs make_step_args(gfx_solver& S) {
     ea.pabar = S.pabar.p;
     ea.pmem = S.pmem.p;
     ea.pnew = S.pnew.p;
+    ea.pcoef = S.pcoef.p;
     ea.c1 = c1;
     ea.c2 = c2;
     ea.dt = float(S.dt);
@@ -650,9 +772,16 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
     S.dt = d.dt;
     S.stride = d.snapshot_stride;
     S.has_cpml = d.has_cpml != 0;
-    if (S.has_cpml)
+    S.cpml_complete = d.cpml_complete != 0;
+    if (S.cpml_complete) {
+        require(S.has_cpml, "cpml_complete needs has_cpml");
+        require(d.pml_region && d.pml_K && d.pml_d && d.pml_alpha,
+                "cpml_complete needs pml_region, pml_K, pml_d and pml_alpha");
+        require(d.n_sls == 0, "the complete C-PML does not combine with shear attenuation");
+    } else if (S.has_cpml) {
         require(d.pml_region && d.coef_alpha && d.coef_abar && d.density,
                 "has_cpml needs pml_region, coef_alpha, coef_abar and density");
+    }
 
     const GLL gll(4);
     for (int q = 0; q < 25; ++q) S.D[q] = float(gll.D[q]);
@@ -695,12 +824,17 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
             diag_el[e] = ok;
         }
     }
+    // complete C-PML: PML elements run their own kernel, after the others
+    std::vector<char> pml_el(E, 0);
+    if (S.cpml_complete)
+        for (int e = 0; e < E; ++e) pml_el[e] = d.pml_region[e] != 0;
     std::vector<int32_t> elem_of_int(E);
     std::iota(elem_of_int.begin(), elem_of_int.end(), 0);
     std::vector<int> col(E, 0);
     if (S.assembly == GFX_ASSEMBLY_COLORED) col = colour_elements(E, d.ibool, G);
     std::stable_sort(elem_of_int.begin(), elem_of_int.end(), [&](int a, int b) {
         if (col[a] != col[b]) return col[a] < col[b];
+        if (pml_el[a] != pml_el[b]) return pml_el[a] < pml_el[b];
         if (diag_el[a] != diag_el[b]) return diag_el[a] > diag_el[b];
         return key[a] < key[b];
     });
@@ -708,22 +842,46 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
         int b = 0;
         for (int i = 1; i <= E; ++i) {
             const int e0 = elem_of_int[b];
-            if (i == E || col[elem_of_int[i]] != col[e0] || diag_el[elem_of_int[i]] != diag_el[e0]) {
-                S.ranges.push_back({b, i, diag_el[e0] != 0});
+            if (i == E || col[elem_of_int[i]] != col[e0] || pml_el[elem_of_int[i]] != pml_el[e0] ||
+                diag_el[elem_of_int[i]] != diag_el[e0]) {
+                const bool pml = pml_el[e0] != 0;
+                S.ranges.push_back({b, i, diag_el[e0] != 0 && !pml, pml});
                 b = i;
             }
         }
         S.n_colors = S.assembly == GFX_ASSEMBLY_COLORED ? (*std::max_element(col.begin(), col.end()) + 1) : 1;
-        S.n_diag = std::count(diag_el.begin(), diag_el.end(), char(1));
+        S.n_diag = 0;
+        for (const ElemRange& rg : S.ranges)
+            if (rg.diag) S.n_diag += rg.e - rg.b;
     }
     S.int_of_elem.assign(E, 0);
     for (int i = 0; i < E; ++i) S.int_of_elem[elem_of_int[i]] = i;
 
     // ---- nodal mass and legacy damping (solver.cpp:193-202), caller numbering ----
+    // The complete C-PML weights the mass of a PML element point by
+    // K_x K_y K_z + dt/2 c_v: its inertia term L has velocity coefficient
+    // c_v = K_z (K_y d_x + K_x d_y) + K_x K_y d_z and takes the velocity at
+    // v_{n+1} = w_n + dt/2 a_{n+1} (trapezoidal, stable for any damping); the
+    // element kernel evaluates L with v = w_n and the dt/2 c_v a_{n+1} part
+    // moves into the mass.
     std::vector<double> mg(G, 0.0), dg(G, 0.0);
     for (int64_t p = 0; p < P; ++p) {
-        mg[d.ibool[p]] += d.mass[p];
+        double m = d.mass[p];
+        if (S.cpml_complete && d.pml_region[p / NP] != 0) {
+            const double* K = d.pml_K + p * 3;
+            const double* dd = d.pml_d + p * 3;
+            const double cv = K[2] * (K[1] * dd[0] + K[0] * dd[1]) + K[0] * K[1] * dd[2];
+            m *= K[0] * K[1] * K[2] + 0.5 * S.dt * cv;
+        }
+        mg[d.ibool[p]] += m;
         if (d.damping) dg[d.ibool[p]] = d.damping[p];  // last writer wins
+    }
+    // the complete C-PML ends in a rigid outer boundary: zero mass there, so
+    // the massless-node rule keeps those nodes' u and v (zero from rest)
+    if (S.cpml_complete) {
+        const std::vector<char> rigid = cpml_rigid_nodes(d, E, G);
+        for (int g = 0; g < G; ++g)
+            if (rigid[g]) mg[g] = 0.0;
     }
 
     // ---- node classes: 2 = C-PML node, 1 = other special, 0 = regular ----
@@ -747,9 +905,8 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
     }
     // C-PML memory directions used at each node: direction d when some PML
     // element point on the node has A_{3+d} != 0 (the others keep zero memory)
-    std::vector<uint8_t> pm_old;
-    if (S.has_cpml) {
-        pm_old.assign(G, 0);
+    std::vector<uint8_t> pm_old(G, 0);  // stays 0 with the complete C-PML, which has no such memories
+    if (S.has_cpml && !S.cpml_complete) {
         for (int64_t e = 0; e < E; ++e) {
             if (d.pml_region[e] == 0) continue;
             for (int p = 0; p < NP; ++p) {
@@ -892,7 +1049,33 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
     // ---- C-PML ----
     std::vector<int32_t> hslot(E, -1);
     S.pml_nodal = true;
-    if (S.has_cpml) {
+    if (S.cpml_complete) {
+        // per PML element point: K, d, alpha and the one-step recursions of
+        // alpha and beta = alpha + d / K for each axis, and rho J w (the mass)
+        S.pml_nodal = false;
+        int npml = 0;
+        for (int ie = 0; ie < E; ++ie)
+            if (d.pml_region[elem_of_int[ie]] != 0) hslot[ie] = npml++;
+        S.Ppml = int64_t(npml) * NP;
+        const int64_t Q = std::max<int64_t>(S.Ppml, 1);
+        std::vector<float> hc(size_t(dev::CPML_NC) * Q, 0.f);
+        std::atomic<bool> bad{false};
+        parallel_for(E, [&](int64_t ie) {
+            if (hslot[ie] < 0) return;
+            const int64_t e = elem_of_int[ie];
+            for (int p = 0; p < NP; ++p) {
+                const int64_t src = e * NP + p, q = int64_t(hslot[ie]) * NP + p;
+                double c[27];
+                if (!cpml_point_coefficients(d.pml_K + src * 3, d.pml_d + src * 3, d.pml_alpha + src * 3, S.dt, c))
+                    bad = true;
+                for (int t = 0; t < 27; ++t) hc[t * Q + q] = float(c[t]);
+                hc[27 * Q + q] = float(d.mass[src]);
+            }
+        });
+        require(!bad, "pml_K must be > 0, pml_d and pml_alpha >= 0 (all finite) at PML element points");
+        dv.upload(S.pcoef, std::move(hc));
+        dv.alloc(S.pmem, size_t(S.nf) * dev::CPML_NM * Q);
+    } else if (S.has_cpml) {
         // Nodal form: the acceleration term sum_e w J/rho (A1 v + A2 u + A3 mx + A4 my + A5 mz)
         // assembles to per-node coefficients, and every element's memory variable at a node
         // evolves identically when the coef_alpha of all elements that use it (A_{3+d} != 0)
@@ -981,12 +1164,12 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
     dv.upload(S.pml_slot, std::move(hslot));
 
     // element-local C-PML runs the block kernel's general form only
-    if (S.has_cpml && !S.pml_nodal) {
+    if (ref_elem_pml(S)) {
         for (auto& rg : S.ranges) rg.diag = false;
         S.n_diag = 0;
     }
     // element kernel: the wavefront-per-element form needs no element-local C-PML state
-    const bool wave_ok = !(S.has_cpml && !S.pml_nodal);
+    const bool wave_ok = !ref_elem_pml(S);
     S.kernel = resolve_kernel(d.element_kernel, wave_ok);
     check_epb(S.kernel, epb);
     S.epb = resolve_epb(S.kernel, epb);
@@ -1107,6 +1290,20 @@ extern "C" int gfx_device_count(void) {
         return 0;
     }
     return n;
+}
+
+extern "C" gfx_status gfx_cpml_rigid_nodes(const gfx_solver_desc* d, uint8_t* mask) {
+    return guarded([&] {
+        require(d && mask, "null argument");
+        require(d->ngll == 5, "NGLL = 5 (polynomial order 4) only");
+        require(d->n_elem > 0 && d->n_rank_node > 0 && d->ibool && d->dxi_dx && d->pml_region,
+                "needs n_elem, n_rank_node, ibool, dxi_dx and pml_region");
+        const int64_t P = int64_t(d->n_elem) * NP;
+        for (int64_t p = 0; p < P; ++p)
+            require(d->ibool[p] >= 0 && d->ibool[p] < d->n_rank_node, "ibool entry out of range");
+        const std::vector<char> r = cpml_rigid_nodes(*d, d->n_elem, d->n_rank_node);
+        for (int g = 0; g < d->n_rank_node; ++g) mask[g] = uint8_t(r[g]);
+    });
 }
 
 extern "C" gfx_status gfx_solver_create(const gfx_solver_desc* desc, gfx_solver** out) {
@@ -1538,7 +1735,7 @@ uint64_t layout_signature(const gfx_solver& S) {
     };
     mix(S.node_new_of_old.data(), S.node_new_of_old.size() * sizeof(int32_t));
     mix(S.int_of_elem.data(), S.int_of_elem.size() * sizeof(int32_t));
-    const int64_t k[6] = {S.E, S.G, S.nf, S.Ga, S.Gpml, int64_t(S.assembly)};
+    const int64_t k[7] = {S.E, S.G, S.nf, S.Ga, S.Gpml, int64_t(S.assembly), int64_t(S.cpml_complete)};
     mix(k, sizeof(k));
     return h;
 }
@@ -1694,7 +1891,7 @@ extern "C" gfx_status gfx_solver_kernel_time(const gfx_solver* s, const char* na
 extern "C" gfx_status gfx_solver_set_launch(gfx_solver* s, int element_kernel, int elems_per_block) {
     return guarded([&] {
         require(s != nullptr, "null solver");
-        const bool wave_ok = !(s->has_cpml && !s->pml_nodal);
+        const bool wave_ok = !ref_elem_pml(*s);
         const int kern = resolve_kernel(element_kernel, wave_ok);
         check_epb(kern, elems_per_block);
         s->kernel = kern;
@@ -1766,7 +1963,7 @@ extern "C" gfx_status gfx_autotune(const gfx_solver_desc* base, int trial_steps,
             int best_lc = -1;
             for (int li = 0; li < NL; ++li) {
                 const auto& lc = launch[li];
-                if (lc[0] != GFX_ELEMENT_BLOCK && S->has_cpml && !S->pml_nodal) continue;
+                if (lc[0] != GFX_ELEMENT_BLOCK && ref_elem_pml(*S)) continue;
                 if (lc[0] == GFX_ELEMENT_PAIR && S->nf < 2) continue;
                 S->kernel = lc[0];
                 S->epb = resolve_epb(lc[0], lc[1]);
@@ -1816,6 +2013,7 @@ extern "C" gfx_status gfx_solver_query(const gfx_solver* s, const char* key, dou
         else if (k == "n_sls") *value = s->n_sls;
         else if (k == "pml_nodal") *value = (s->has_cpml && s->pml_nodal) ? 1.0 : 0.0;
         else if (k == "pml_elements") *value = s->n_pml_elem;
+        else if (k == "cpml_complete") *value = s->cpml_complete ? 1.0 : 0.0;
         else if (k == "special_nodes") *value = double(s->Ga);
         else if (k == "pml_nodes") *value = double(s->Gpml);
         else if (k == "fields") *value = s->nf;
@@ -1835,7 +2033,10 @@ extern "C" double gfx_solver_algorithmic_bytes(const gfx_solver* s, int which) {
     // field the predicted displacement read and the residual written once per node.
     const double Pd = double(s->n_diag) * NP;
     double elem = 48.0 * (P - Pd) + 24.0 * Pd + F * 24.0 * G;
-    if (s->has_cpml && !s->pml_nodal) elem += 56.0 * Q + F * (96.0 * Q + 36.0 * Gp);
+    if (ref_elem_pml(*s)) elem += 56.0 * Q + F * (96.0 * Q + 36.0 * Gp);
+    // complete C-PML: 28 coefficients per PML point, and per field its 45
+    // memories read and written and w read
+    if (s->cpml_complete) elem += 112.0 * Q + F * 372.0 * Q;
     // shear attenuation: the general form (48 B) at every point, 3 step
     // coefficients per solid per point, and per field the memory (5 per solid)
     // and previous deviatoric strain (5) read and written at every point

@@ -62,6 +62,9 @@ class BoxSpec(C.Structure):
         ("n_sls", C.c_int),
         ("q_mu", C.c_double * MAX_LAYERS),
         ("att_fmin", C.c_double), ("att_fmax", C.c_double),
+        ("want_profiles", C.c_int),
+        ("pml_profile", C.c_int),
+        ("pml_reflection", C.c_double),
     ]
 
 
@@ -84,6 +87,7 @@ class _Mesh(C.Structure):
         ("n_sls", C.c_int),
         ("tau_sigma", _dp),
         ("tau_epsilon", _dp),
+        ("pml_K", _dp), ("pml_d", _dp), ("pml_alpha", _dp),
     ]
 
 
@@ -102,6 +106,7 @@ class SolverDesc(C.Structure):
         ("node_field_groups", C.c_int),
         ("general_geometry", C.c_int),
         ("n_sls", C.c_int), ("tau_sigma", _dp), ("tau_epsilon", _dp),
+        ("cpml_complete", C.c_int), ("pml_K", _dp), ("pml_d", _dp), ("pml_alpha", _dp),
     ]
 
 
@@ -142,6 +147,7 @@ def _load():
         "gfx_compute_element_residual": (st, [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp, _dp]),
         "gfx_compute_element_residual_f32": (st, [C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp, _dp]),
         "gfx_solver_create": (st, [C.POINTER(SolverDesc), C.POINTER(vp)]),
+        "gfx_cpml_rigid_nodes": (st, [C.POINTER(SolverDesc), C.POINTER(C.c_uint8)]),
         "gfx_solver_destroy": (st, [vp]),
         "gfx_solver_set_source": (st, [vp, C.c_int, C.c_int, C.c_int, _ip, _dp, C.c_int, _dp]),
         "gfx_solver_reset": (st, [vp]),
@@ -489,6 +495,9 @@ def _wrap_mesh(h, keep_handle, copy) -> Mesh:
         "coef_abar": arr(m.coef_abar, P * 5),
         "tau_sigma": arr(m.tau_sigma, P * m.n_sls),
         "tau_epsilon": arr(m.tau_epsilon, P * m.n_sls),
+        "pml_K": arr(m.pml_K, P * 3),
+        "pml_d": arr(m.pml_d, P * 3),
+        "pml_alpha": arr(m.pml_alpha, P * 3),
     }
     rec = arr(m.rec_cell, m.n_rec_cell, np.int32)
     rec = None if rec is None else rec.copy()
@@ -669,8 +678,9 @@ def h5_layout(path) -> dict:
 def make_desc(mesh: Mesh, keep: list, n_fields=1, assembly=GFX_ASSEMBLY_COLORED, elems_per_block=0,
               record_capacity=None, record_uva=False, has_cpml=True, dt=None,
               snapshot_stride=None, rec_cell=None, element_kernel=GFX_ELEMENT_AUTO, morton=True,
-              node_field_groups=0, general_geometry=False, attenuation=True):
-    """Solver descriptor over mesh arrays; arrays it points to are appended to keep."""
+              node_field_groups=0, general_geometry=False, attenuation=True, cpml_complete=False):
+    """Solver descriptor over mesh arrays; arrays it points to are appended to keep.
+    cpml_complete: the complete C-PML from the mesh's pml_K, pml_d, pml_alpha."""
     a = mesh.arrays
 
     def kp(x, dt_=np.float64):
@@ -712,7 +722,23 @@ def make_desc(mesh: Mesh, keep: list, n_fields=1, assembly=GFX_ASSEMBLY_COLORED,
         d.n_sls = mesh.n_sls
         d.tau_sigma = _ptr(kp(a["tau_sigma"]))
         d.tau_epsilon = _ptr(kp(a["tau_epsilon"]))
+    if cpml_complete:
+        if a.get("pml_K") is None:
+            raise ValueError("cpml_complete needs the mesh's pml_K, pml_d, pml_alpha (want_profiles=1)")
+        d.cpml_complete = 1
+        d.pml_K = _ptr(kp(a["pml_K"]))
+        d.pml_d = _ptr(kp(a["pml_d"]))
+        d.pml_alpha = _ptr(kp(a["pml_alpha"]))
     return d
+
+
+def cpml_rigid_nodes(mesh: Mesh):
+    """Node mask [n_node] of the complete C-PML's rigid outer boundary."""
+    keep = []
+    d = make_desc(mesh, keep)
+    mask = np.zeros(mesh.n_node, np.uint8)
+    _check(lib.gfx_cpml_rigid_nodes(C.byref(d), mask.ctypes.data_as(C.POINTER(C.c_uint8))))
+    return mask.astype(bool)
 
 
 def autotune(mesh: Mesh, n_fields=1, trial_steps=3, **kw):
@@ -733,7 +759,8 @@ class Solver:
     def __init__(self, mesh: Mesh, n_fields=1, assembly=GFX_ASSEMBLY_COLORED, elems_per_block=0,
                  record_capacity=None, record_uva=False, has_cpml=True, dt=None,
                  snapshot_stride=None, rec_cell=None, element_kernel=GFX_ELEMENT_AUTO,
-                 morton=True, node_field_groups=0, general_geometry=False, attenuation=True):
+                 morton=True, node_field_groups=0, general_geometry=False, attenuation=True,
+                 cpml_complete=False):
         self._keep = []
         d = make_desc(mesh, self._keep, n_fields=n_fields, assembly=assembly,
                       elems_per_block=elems_per_block, record_capacity=record_capacity,
@@ -741,7 +768,7 @@ class Solver:
                       snapshot_stride=snapshot_stride, rec_cell=rec_cell,
                       element_kernel=element_kernel, morton=morton,
                       node_field_groups=node_field_groups, general_geometry=general_geometry,
-                      attenuation=attenuation)
+                      attenuation=attenuation, cpml_complete=cpml_complete)
         rk_size = d.n_rec_cell
         h = C.c_void_p()
         _check(lib.gfx_solver_create(C.byref(d), C.byref(h)))

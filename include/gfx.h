@@ -157,9 +157,34 @@ typedef struct {
     int n_sls;
     const double* tau_sigma;
     const double* tau_epsilon;
+    /* Complete C-PML (1; needs has_cpml and n_sls == 0).  Not in the reference,
+       whose solver applies only the acceleration part of its C-PML
+       (forward/share/src/pml.cpp) and defers the strain part
+       (docs/deferred.md:46-80).  From the stretch profiles
+       s_q = K_q + d_q / (alpha_q + i w) of every element point
+       ([n_elem*ngll^3*3], the partition datasets pml_K, pml_d, pml_alpha), the
+       PML elements (pml_region != 0) take the stretched weak form: stress
+       column j sees the gradient filtered by s_x s_y s_z / (s_j s_l), the mass
+       gains K_x K_y K_z and the inertia terms of (i w)^2 s_x s_y s_z enter the
+       residual.  coef_alpha and coef_abar are then not used, and the node
+       kernels apply no C-PML term.  The memories (45 per PML point and field)
+       are advanced exactly for inputs linear in time over a step. */
+    int cpml_complete;
+    const double* pml_K;
+    const double* pml_d;
+    const double* pml_alpha;
 } gfx_solver_desc;
 
 gfx_status gfx_solver_create(const gfx_solver_desc* desc, gfx_solver** out);
+
+/* The nodes the complete C-PML holds rigid (mask[n_rank_node], 1 = rigid;
+ * host only): its outer boundary, the faces of PML elements that no other
+ * element shares and whose normal (largest component of grad xi at the face
+ * centre) is an axis the element's region damps.  They get zero mass, so the
+ * massless-node rule keeps their u and v (zero from rest).  Needs n_elem,
+ * ngll, n_rank_node, ibool, dxi_dx and pml_region of the descriptor.  No
+ * counterpart in the reference, whose C-PML has free outer faces. */
+gfx_status gfx_cpml_rigid_nodes(const gfx_solver_desc* desc, uint8_t* mask);
 gfx_status gfx_solver_destroy(gfx_solver* s);
 
 /* Point-force source of one field: the cells containing the point with their
@@ -385,6 +410,21 @@ typedef struct {
     int n_sls;
     double q_mu[GFX_MAX_LAYERS];
     double att_fmin, att_fmax;
+    /* C-PML stretch profiles K, d, alpha per axis at every point
+       (pml_K, pml_d, pml_alpha [E,n,3], the inputs of the complete C-PML) */
+    int want_profiles;
+    /* 0: the profiles as preprocess/pml_cpml.py:63-155 computes them (the
+       normalised distance measured from the outer boundary face, and for an
+       axis with PML on both faces the later face's values, so d is largest at
+       the interior interface); 1: graded into the layer, the distance measured
+       from the interior interface as pml_cpml.py:44-60 defines it (d = 0 and
+       alpha = alpha_max at the interface, d largest and alpha = 0 at the
+       boundary), each face inside its own layer only; same d0, exponent and
+       alpha_max */
+    int pml_profile;
+    /* target reflection R of the damping d0 = 3 vp ln(1/R) / (2 w) of both
+       profiles; 0 = the reference's 1e-5 (pml_cpml.py:23) */
+    double pml_reflection;
 } gfx_box_spec;
 
 typedef struct {
@@ -418,6 +458,9 @@ typedef struct {
     int n_sls;            /* shear attenuation mechanisms (0 = elastic) */
     double* tau_sigma;    /* [E,n,n_sls] or NULL */
     double* tau_epsilon;  /* [E,n,n_sls] or NULL */
+    double* pml_K;        /* [E,n,3] or NULL (want_profiles) */
+    double* pml_d;
+    double* pml_alpha;
 } gfx_mesh;
 
 gfx_status gfx_box_spec_default(gfx_box_spec* spec);

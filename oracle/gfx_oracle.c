@@ -327,6 +327,171 @@ void gfo_element_residual_att(int n_elem, const double* dxi_dx, const double* ja
 }
 
 /* ------------------------------------------------------------------ */
+/* Complete C-PML (stretched weak form); no reference counterpart: the */
+/* reference defers the strain part (docs/deferred.md:46-80).          */
+/* ------------------------------------------------------------------ */
+
+/* conv(g, p) over one step for g linear in time between the two levels. */
+void gfo_cpml_recursion(double p, double dt, double* abc) {
+    const double x = p * dt;
+    double i0, c;
+    if (fabs(x) < 1e-2) { /* I0/dt and I1/dt^2 as power series in x */
+        i0 = 1.0 - x / 2.0 + x * x / 6.0 - x * x * x / 24.0 + x * x * x * x / 120.0 - pow(x, 5) / 720.0 +
+             pow(x, 6) / 5040.0;
+        c = 0.5 - x / 6.0 + x * x / 24.0 - x * x * x / 120.0 + x * x * x * x / 720.0 - pow(x, 5) / 5040.0 +
+            pow(x, 6) / 40320.0;
+    } else {
+        const double e = exp(-x);
+        i0 = (1.0 - e) / x;
+        c = (1.0 - i0) / x;
+    }
+    abc[0] = exp(-x);
+    abc[1] = dt * (i0 - c);
+    abc[2] = dt * c;
+}
+
+void gfo_cpml_point_coefficients(const double* K, const double* d, const double* alpha, double dt, double* coef) {
+    for (int q = 0; q < 3; ++q) {
+        double* cq = coef + q * 9;
+        cq[0] = K[q];
+        cq[1] = d[q];
+        cq[2] = alpha[q];
+        for (int t = 3; t < 9; ++t) cq[t] = 0.0;
+        if (d[q] != 0.0) { /* an axis without damping keeps its memories at zero */
+            gfo_cpml_recursion(alpha[q], dt, cq + 3);
+            gfo_cpml_recursion(alpha[q] + d[q] / K[q], dt, cq + 6);
+        }
+    }
+}
+
+/* M^{n+1} = Q + c g^{n+1}; the carry becomes a M^{n+1} + b g^{n+1}. */
+static inline double conv_step(const double* abc, double* q, double g) {
+    const double m = *q + abc[2] * g;
+    *q = abc[0] * m + abc[1] * g;
+    return m;
+}
+
+void gfo_cpml_gradient(const double* coef, const double* g9, double* carry, double* G27) {
+    for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l) {
+            const int m = l == 0 ? 1 : 0, n = l == 2 ? 1 : 2; /* the other two axes, m < n */
+            const double* cm = coef + m * 9;
+            const double* cn = coef + n * 9;
+            const double* cl = coef + l * 9;
+            const double g = g9[k * 3 + l];
+            double* Q = carry + (k * 3 + l) * 4;
+            const double Mm = conv_step(cm + 3, Q + 0, g);  /* conv(g, alpha_m) */
+            const double Mn = conv_step(cn + 3, Q + 1, g);  /* conv(g, alpha_n) */
+            const double Mmn = conv_step(cn + 3, Q + 2, Mm); /* conv(conv(g, alpha_m), alpha_n) */
+            const double y2 = cm[0] * cn[0] * g + cn[0] * cm[1] * Mm + cn[1] * (cm[0] * Mn + cm[1] * Mmn);
+            const double Mb = conv_step(cl + 6, Q + 3, y2); /* conv(s_m s_n g, beta_l) */
+            G27[l * 9 + k * 3 + l] = (y2 - cl[1] / cl[0] * Mb) / cl[0]; /* s_m s_n / s_l */
+            G27[n * 9 + k * 3 + l] = cm[0] * g + cm[1] * Mm;              /* column n: s_m */
+            G27[m * 9 + k * 3 + l] = cn[0] * g + cn[1] * Mn;              /* column m: s_n */
+        }
+}
+
+void gfo_cpml_inertia(const double* coef, const double* u3, const double* v3, double* carry, double* L3) {
+    const double *cx = coef, *cy = coef + 9, *cz = coef + 18;
+    for (int c = 0; c < 3; ++c) {
+        const double u = u3[c], v = v3[c];
+        double* Q = carry + c * 3;
+        const double C1 = conv_step(cx + 3, Q + 0, u);
+        const double z1 = cx[0] * u + cx[1] * C1;
+        const double C2 = conv_step(cy + 3, Q + 1, z1);
+        const double z2 = cy[0] * z1 + cy[1] * C2;
+        const double C3 = conv_step(cz + 3, Q + 2, z2);
+        const double L1 = cx[1] * (v - cx[2] * u + cx[2] * cx[2] * C1);
+        const double Dz1 = cx[0] * v + cx[1] * (u - cx[2] * C1);
+        const double L2 = cy[0] * L1 + cy[1] * (Dz1 - cy[2] * z1 + cy[2] * cy[2] * C2);
+        const double Dz2 = cy[0] * Dz1 + cy[1] * (z1 - cy[2] * C2);
+        L3[c] = cz[0] * L2 + cz[1] * (Dz2 - cz[2] * z2 + cz[2] * cz[2] * C3);
+    }
+}
+
+void gfo_element_residual_cpml(int n_elem, const double* dxi_dx, const double* jacobian, const double* lambda_,
+                               const double* mu_, const int32_t* pml_region, const double* coef,
+                               const double* D, const double* weights, int ngll, const double* u, double* carry,
+                               double* r) {
+    const int n_node = ngll * ngll * ngll;
+#pragma omp parallel for schedule(static)
+    for (long e = 0; e < n_elem; ++e) {
+        const int pml = pml_region && pml_region[e] != 0;
+        const double* edd = dxi_dx + e * n_node * 9;
+        const double* eu = u + e * n_node * 3;
+        double* er = r + e * n_node * 3;
+        for (int i = 0; i < ngll; ++i)
+            for (int j = 0; j < ngll; ++j)
+                for (int k = 0; k < ngll; ++k) {
+                    const int n = nidx(i, j, k, ngll);
+                    const long pt = e * n_node + n;
+                    const double lam = lambda_[pt], mu = mu_[pt];
+                    if (mu <= 0.0) continue; /* element_cpu.cpp:42-43 */
+                    const double* dd = edd + 9 * n;
+                    double gx[3] = {0, 0, 0}, ge[3] = {0, 0, 0}, gz[3] = {0, 0, 0};
+                    for (int s = 0; s < ngll; ++s) {
+                        const double di = D[i * ngll + s], dj = D[j * ngll + s], dk = D[k * ngll + s];
+                        const int a = nidx(s, j, k, ngll), b = nidx(i, s, k, ngll), c = nidx(i, j, s, ngll);
+                        for (int q = 0; q < 3; ++q) {
+                            gx[q] += di * eu[3 * a + q];
+                            ge[q] += dj * eu[3 * b + q];
+                            gz[q] += dk * eu[3 * c + q];
+                        }
+                    }
+                    double g[9]; /* g[c*3+m] = d u_c / d x_m */
+                    for (int c = 0; c < 3; ++c) {
+                        g[c * 3 + 0] = gx[c] * dd[0] + ge[c] * dd[1] + gz[c] * dd[2];
+                        g[c * 3 + 1] = gx[c] * dd[3] + ge[c] * dd[4] + gz[c] * dd[5];
+                        g[c * 3 + 2] = gx[c] * dd[6] + ge[c] * dd[7] + gz[c] * dd[8];
+                    }
+                    double G[27]; /* G[j*9 + k*3 + l]: the gradient seen by stress column j */
+                    if (pml) {
+                        gfo_cpml_gradient(coef + pt * 27, g, carry + pt * 45, G);
+                    } else {
+                        for (int c = 0; c < 3; ++c)
+                            for (int q = 0; q < 9; ++q) G[c * 9 + q] = g[q];
+                    }
+                    double S[3][3];
+                    for (int a = 0; a < 3; ++a)
+                        for (int b = 0; b < 3; ++b) {
+                            const double* Gb = G + b * 9;
+                            S[a][b] = mu * (Gb[a * 3 + b] + Gb[b * 3 + a]);
+                            if (a == b) S[a][b] += lam * (Gb[0] + Gb[4] + Gb[8]);
+                        }
+                    const double fac = jacobian[pt] * weights[i] * weights[j] * weights[k];
+                    for (int dir = 0; dir < 3; ++dir) {
+                        for (int s = 0; s < ngll; ++s) {
+                            double Dq;
+                            int ns;
+                            if (dir == 0) {
+                                Dq = D[i * ngll + s];
+                                ns = nidx(s, j, k, ngll);
+                            } else if (dir == 1) {
+                                Dq = D[j * ngll + s];
+                                ns = nidx(i, s, k, ngll);
+                            } else {
+                                Dq = D[k * ngll + s];
+                                ns = nidx(i, j, s, ngll);
+                            }
+                            const double gn[3] = {Dq * dd[dir], Dq * dd[3 + dir], Dq * dd[6 + dir]};
+                            for (int c = 0; c < 3; ++c)
+                                er[3 * ns + c] -= (S[c][0] * gn[0] + S[c][1] * gn[1] + S[c][2] * gn[2]) * fac;
+                        }
+                    }
+                }
+    }
+}
+
+void gfo_cpml_filter_history(const double* coef, int n, const double* g, const double* u, const double* v,
+                             double* G27, double* L3) {
+    double cs[45] = {0};
+    for (int t = 0; t < n; ++t) {
+        if (g) gfo_cpml_gradient(coef, g + t * 9, cs, G27 + t * 27);
+        if (u) gfo_cpml_inertia(coef, u + t * 3, v + t * 3, cs + 36, L3 + t * 3);
+    }
+}
+
+/* ------------------------------------------------------------------ */
 /* Assembly: forward/share/src/assembly.cpp:10-41                      */
 /* ------------------------------------------------------------------ */
 
@@ -526,6 +691,7 @@ typedef struct {
     double *u, *v, *a, *r, *ut, *uloc, *rloc, *mass_g, *damp_g;
     double *pold, *pnew, *rmem;
     double *zeta, *devp;
+    double *ccoef, *ccarry; /* complete C-PML: [P][27] coefficients, [P][45] memories */
     double gll_x[16], gll_w[16], D[256];
     gfo_cpml pml;
 } run_ws;
@@ -535,6 +701,7 @@ static void ws_free(run_ws* w) {
     free(w->uloc); free(w->rloc); free(w->mass_g); free(w->damp_g);
     free(w->pold); free(w->pnew); free(w->rmem);
     free(w->zeta); free(w->devp);
+    free(w->ccoef); free(w->ccarry);
 }
 
 static int ws_init(run_ws* w, const gfo_problem* pb) {
@@ -559,10 +726,29 @@ static int ws_init(run_ws* w, const gfo_problem* pb) {
     if (!w->u || !w->v || !w->a || !w->r || !w->ut || !w->uloc || !w->rloc || !w->mass_g ||
         !w->damp_g)
         return -2;
-    /* global mass sums, damping "last writer wins" (solver.cpp:193-202) */
+    if (pb->cpml_complete) {
+        if (!pb->pml_region || !pb->pml_K || !pb->pml_d || !pb->pml_alpha || pb->n_sls > 0) return -1;
+        w->ccoef = calloc(P * 27, sizeof(double));
+        w->ccarry = calloc(P * 45, sizeof(double));
+        if (!w->ccoef || !w->ccarry) return -2;
+        for (long p = 0; p < P; ++p)
+            if (pb->pml_region[p / n_node] != 0)
+                gfo_cpml_point_coefficients(pb->pml_K + p * 3, pb->pml_d + p * 3, pb->pml_alpha + p * 3, pb->dt,
+                                            w->ccoef + p * 27);
+    }
+    /* global mass sums, damping "last writer wins" (solver.cpp:193-202); the
+     * complete C-PML weights the mass of PML element points by
+     * K_x K_y K_z + dt/2 c_v (the inertia's velocity terms at v_{n+1}) */
     for (long p = 0; p < P; ++p) {
         const long g = pb->ibool[p];
-        w->mass_g[g] += pb->mass[p];
+        double m = pb->mass[p];
+        if (pb->cpml_complete && pb->pml_region[p / n_node] != 0) {
+            const double* K = pb->pml_K + p * 3;
+            const double* d = pb->pml_d + p * 3;
+            const double cv = K[2] * (K[1] * d[0] + K[0] * d[1]) + K[0] * K[1] * d[2];
+            m *= K[0] * K[1] * K[2] + 0.5 * pb->dt * cv;
+        }
+        w->mass_g[g] += m;
         if (pb->damping) w->damp_g[g] = pb->damping[p];
     }
     if (pb->n_sls > 0) {
@@ -571,7 +757,7 @@ static int ws_init(run_ws* w, const gfo_problem* pb) {
         w->devp = calloc(P * 5, sizeof(double));
         if (!w->zeta || !w->devp) return -2;
     }
-    if (pb->has_cpml) { /* cpml_initialize, pml.cpp:41-53 */
+    if (pb->has_cpml && !pb->cpml_complete) { /* cpml_initialize, pml.cpp:41-53 */
         w->pold = calloc(P * 3, sizeof(double));
         w->pnew = calloc(P * 3, sizeof(double));
         w->rmem = calloc(P * 9, sizeof(double));
@@ -602,11 +788,28 @@ static void run_one_step(run_ws* w, const gfo_problem* pb, int step) {
     gfo_newmark_predict(n_dof, dt, beta, w->u, w->v, w->a, w->ut);
     /* 3. gather predicted displacement (the reference calls it twice; idempotent) */
     gfo_gather_from_rank(w->ut, pb->ibool, pb->n_elem, n_node, w->uloc);
+    const int ref_cpml = pb->has_cpml && !pb->cpml_complete;
     /* 3a. C-PML displacement fields from u_n, v_n, a_n */
-    if (pb->has_cpml) gfo_cpml_update_displ_fields(&w->pml, w->u, w->v, w->a, dt);
+    if (ref_cpml) gfo_cpml_update_displ_fields(&w->pml, w->u, w->v, w->a, dt);
     /* 3b. zero + element kernel */
     memset(w->rloc, 0, sizeof(double) * 3 * P);
-    if (pb->n_sls > 0)
+    if (pb->cpml_complete) {
+        /* stretched stresses in PML elements, then the inertia terms with the
+         * predicted u and v_{n+1/2} = v_n + dt/2 a_n */
+        gfo_element_residual_cpml(pb->n_elem, pb->dxi_dx, pb->jacobian, pb->lambda_, pb->mu_, pb->pml_region,
+                                  w->ccoef, w->D, w->gll_w, ngll, w->uloc, w->ccarry, w->rloc);
+#pragma omp parallel for schedule(static)
+        for (long e = 0; e < pb->n_elem; ++e) {
+            if (pb->pml_region[e] == 0) continue;
+            for (int n = 0; n < n_node; ++n) {
+                const long pt = e * n_node + n, g = pb->ibool[pt];
+                double vh[3], L[3];
+                for (int c = 0; c < 3; ++c) vh[c] = w->v[g * 3 + c] + 0.5 * dt * w->a[g * 3 + c];
+                gfo_cpml_inertia(w->ccoef + pt * 27, w->ut + g * 3, vh, w->ccarry + pt * 45 + 36, L);
+                for (int c = 0; c < 3; ++c) w->rloc[pt * 3 + c] -= pb->mass[pt] * L[c];
+            }
+        }
+    } else if (pb->n_sls > 0)
         gfo_element_residual_att(pb->n_elem, pb->dxi_dx, pb->jacobian, pb->lambda_, pb->mu_, pb->n_sls,
                                  pb->tau_sigma, pb->tau_epsilon, dt, w->D, w->gll_w, ngll, w->uloc, w->zeta,
                                  w->devp, w->rloc);
@@ -614,9 +817,9 @@ static void run_one_step(run_ws* w, const gfo_problem* pb, int step) {
         gfo_element_residual(pb->n_elem, pb->dxi_dx, pb->jacobian, pb->lambda_, pb->mu_, w->D,
                              w->gll_w, ngll, w->uloc, w->rloc);
     /* 4. C-PML acceleration contribution, else legacy damping on v (solver.cpp:557-574) */
-    if (pb->has_cpml) {
+    if (ref_cpml) {
         gfo_cpml_accel_contribution(&w->pml, w->u, w->v, w->gll_w, w->rloc);
-    } else {
+    } else if (!pb->cpml_complete) {
         gfo_apply_pml_damping(w->damp_g, w->v, (long)pb->n_rank_node * 3);
     }
     /* 5. source: stf_values[step] = f(t_n) (solver.cpp:576-605) */
@@ -643,7 +846,7 @@ static void run_one_step(run_ws* w, const gfo_problem* pb, int step) {
     /* 7. corrector */
     gfo_newmark_correct(n_dof, dt, beta, gamma, w->mass_g, w->u, w->v, w->a, w->r);
     /* 8. C-PML memory */
-    if (pb->has_cpml) gfo_cpml_update_displ_memory(&w->pml);
+    if (ref_cpml) gfo_cpml_update_displ_memory(&w->pml);
 }
 
 /* Steps [start_step, nsteps) of solver.cpp:389-688 from (u0, v0, a0) (zero when

@@ -38,6 +38,16 @@
  *                         jittered mesh, and the recording-cell subset
  *   gfo_run_from          resume from (u, v, a) equals the uninterrupted run
  *                         without C-PML
+ *   complete C-PML        (tests/test_cpml_complete.py) gfo_cpml_recursion
+ *                         exact on inputs linear in time (closed-form
+ *                         integrals); gfo_cpml_gradient / gfo_cpml_inertia
+ *                         reproduce s_x s_y s_z / (s_j s_l) and
+ *                         (i w)^2 (s_x s_y s_z - K_x K_y K_z) in the periodic
+ *                         steady state to second order in dt (coincident
+ *                         poles, undamped axes, alpha = 0, K != 1);
+ *                         gfo_element_residual_cpml with d = 0 equals the
+ *                         elastic residual of the stretched geometry; a
+ *                         d = 0, K = 1 run equals the elastic run bit for bit
  * No reference test covers C-PML; the hand-worked sequences pin the
  * restatement of forward/share/src/pml.cpp:55-216 and its call order.
  */
@@ -130,6 +140,12 @@ typedef struct {
     int n_sls;
     const double* tau_sigma;    /* [E*n_node*n_sls] stress relaxation times (s) */
     const double* tau_epsilon;  /* [E*n_node*n_sls] strain relaxation times (s) */
+    /* complete C-PML (1): replaces the reference's acceleration-only C-PML and
+     * the legacy ramp; PML elements are those with pml_region != 0 */
+    int cpml_complete;
+    const double* pml_K;        /* [E*n_node*3] stretch profiles per axis */
+    const double* pml_d;
+    const double* pml_alpha;
 } gfo_problem;
 
 /* ---- shear attenuation (standard linear solids) ----
@@ -155,6 +171,52 @@ void gfo_element_residual_att(int n_elem, const double* dxi_dx, const double* ja
  * after each of the n-1 steps, by the same update as the element residual. */
 void gfo_sls_point_history(int n_sls, const double* tau_sigma, const double* tau_epsilon, double mu_relaxed,
                            double dt, int n, const double* dev, double* sig);
+
+/* ---- complete C-PML ----
+ * Not in the reference: its solver applies only the acceleration term of the
+ * C-PML (pml.cpp) and defers the strain term (docs/deferred.md:46-80).
+ * Stretches s_q(w) = K_q + d_q / (alpha_q + i w), beta_q = alpha_q + d_q / K_q.
+ * In a PML element stress column j sees the gradient
+ *   G^(j)_kl = F^-1[s_x s_y s_z / (s_j s_l)] * du_k/dx_l,
+ *   S_ij = lambda delta_ij sum_k G^(j)_kk + mu (G^(j)_ij + G^(j)_ji),  r_i -= int S_ij dphi/dx_j,
+ * and the inertia F^-1[(i w)^2 s_x s_y s_z] * rho u = rho K_x K_y K_z u'' + rho L:
+ * r -= rho J w L per element point.  L is linear in v with the coefficient
+ * c_v = K_z (K_y d_x + K_x d_y) + K_x K_y d_z; its velocity terms are taken at
+ * v_{n+1} = v_n + dt/2 (a_n + a_{n+1}) (trapezoidal, unconditionally stable
+ * for the damping): L is evaluated with v = v_n + dt/2 a_n and the rest,
+ * dt/2 c_v a_{n+1}, moves to the left, so the lumped mass of a PML element
+ * point is rho J w (K_x K_y K_z + dt/2 c_v).
+ * The factors are chains of first-order filters (no partial fractions, so
+ * coincident poles need no special case):
+ *   s_q g = K_q g + d_q conv(g, alpha_q),  g / s_q = (g - (d_q / K_q) conv(g, beta_q)) / K_q,
+ *   conv(g, p)(t) = int_0^t exp(-p (t - s)) g(s) ds,
+ * advanced exactly for g linear in time over a step:
+ *   M^{n+1} = Q + c g^{n+1},  Q <- a M^{n+1} + b g^{n+1},
+ *   a = exp(-p dt), b = I0 - I1/dt, c = I1/dt, I0 = (1 - a)/p, I1 = (dt - I0)/p.
+ * Inertia: C1 = conv(u, alpha_x), z1 = s_x u, C2 = conv(z1, alpha_y), z2 = s_y z1,
+ * C3 = conv(z2, alpha_z); with D = d/dt and D conv(g, p) = g - p conv(g, p):
+ *   L = K_z L2 + d_z (D z2 - alpha_z z2 + alpha_z^2 C3),
+ *   L2 = K_y L1 + d_y (D z1 - alpha_y z1 + alpha_y^2 C2),  L1 = d_x (v - alpha_x u + alpha_x^2 C1).
+ * In the step u is the predicted displacement (u_{n+1}).
+ * Coefficients per point (27): axis q at [9q]: K, d, alpha, (a, b, c) of
+ * alpha, (a, b, c) of beta (zero where d == 0).  Memories per point (45):
+ * [(k*3+l)*4 + {conv(g, alpha_m), conv(g, alpha_n), conv(conv(g, alpha_m),
+ * alpha_n), conv(s_m s_n g, beta_l)}] with m < n the axes other than l, then
+ * [36 + c*3 + {Q1, Q2, Q3}] of the inertia. */
+void gfo_cpml_recursion(double p, double dt, double* abc);
+void gfo_cpml_point_coefficients(const double* K, const double* d, const double* alpha, double dt, double* coef27);
+/* g9[k*3+l] = du_k/dx_l -> G27[j*9 + k*3 + l]; carry36 updated */
+void gfo_cpml_gradient(const double* coef27, const double* g9, double* carry36, double* G27);
+/* u3, v3 -> L3; carry9 updated */
+void gfo_cpml_inertia(const double* coef27, const double* u3, const double* v3, double* carry9, double* L3);
+void gfo_element_residual_cpml(int n_elem, const double* dxi_dx, const double* jacobian, const double* lambda_,
+                               const double* mu_, const int32_t* pml_region, const double* coef27,
+                               const double* D, const double* weights, int ngll, const double* u,
+                               double* carry45, double* r);
+/* Histories of one point from zero memories: g[n][9] -> G27[n][27] and
+ * (u, v)[n][3] -> L3[n][3] (either pair may be NULL). */
+void gfo_cpml_filter_history(const double* coef27, int n, const double* g, const double* u, const double* v,
+                             double* G27, double* L3);
 
 /* Runs nsteps steps.  Outputs (any may be NULL):
  *   rec_strain [n_snap][n_rec_cell][n_node][6], rec_disp/vel/acc [n_snap][n_rec_cell][n_node][3]

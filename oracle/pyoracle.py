@@ -28,6 +28,7 @@ class Problem(C.Structure):
         ("n_stf", C.c_int), ("stf", _dp), ("snapshot_stride", C.c_int),
         ("n_rec_cell", C.c_int), ("rec_cell", _ip),
         ("n_sls", C.c_int), ("tau_sigma", _dp), ("tau_epsilon", _dp),
+        ("cpml_complete", C.c_int), ("pml_K", _dp), ("pml_d", _dp), ("pml_alpha", _dp),
     ]
 
 
@@ -64,6 +65,13 @@ def _load():
         "gfo_run_state": (C.c_int, [C.POINTER(Problem), _dp, _dp, _dp]),
         "gfo_sls_coefficients": (None, [C.c_double, C.c_double, C.c_double, C.c_double, _dp]),
         "gfo_sls_point_history": (None, [C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_int, _dp, _dp]),
+        "gfo_cpml_recursion": (None, [C.c_double, C.c_double, _dp]),
+        "gfo_cpml_point_coefficients": (None, [_dp, _dp, _dp, C.c_double, _dp]),
+        "gfo_cpml_gradient": (None, [_dp, _dp, _dp, _dp]),
+        "gfo_cpml_inertia": (None, [_dp, _dp, _dp, _dp, _dp]),
+        "gfo_element_residual_cpml": (None, [C.c_int, _dp, _dp, _dp, _dp, _ip, _dp, _dp, _dp, C.c_int, _dp, _dp,
+                                             _dp]),
+        "gfo_cpml_filter_history": (None, [_dp, C.c_int, _dp, _dp, _dp, _dp, _dp]),
         "gfo_random_fill": (None, [_dp, C.c_int, C.c_uint]),
         "gfo_set_threads": (C.c_int, [C.c_int]),
     }
@@ -126,7 +134,8 @@ class Run:
     """One forward run of the oracle on mesh arrays (reference layout)."""
 
     def __init__(self, arrays, n_elem, ngll, n_node, dt, nsteps, snapshot_stride, direction,
-                 src_cells, src_weights, stf, rec_cell, has_cpml=True, tau_sigma=None, tau_epsilon=None):
+                 src_cells, src_weights, stf, rec_cell, has_cpml=True, tau_sigma=None, tau_epsilon=None,
+                 cpml_complete=False):
         k = self._k = {}
 
         def keep(name, x, dt_=np.float64):
@@ -163,6 +172,10 @@ class Run:
             pb.n_sls = ts.size // (n_elem * ngll ** 3)
             pb.tau_sigma = P(ts)
             pb.tau_epsilon = P(keep("tau_epsilon", tau_epsilon))
+        if cpml_complete:
+            pb.cpml_complete = 1
+            for name in ("pml_K", "pml_d", "pml_alpha"):
+                setattr(pb, name, P(keep(name, arrays[name])))
         self.pb = pb
         self.nn = ngll ** 3
         self.n_node = n_node
@@ -229,3 +242,37 @@ def sls_point_history(tau_sigma, tau_epsilon, mu_relaxed, dt, dev):
     sig = np.zeros((len(dev) - 1, 5))
     lib.gfo_sls_point_history(ts.size, P(ts), P(te), mu_relaxed, dt, len(dev), P(dev), P(sig))
     return sig
+
+
+def cpml_recursion(p, dt):
+    """(a, b, c) of conv(g, p) over one step for g linear in time."""
+    out = np.zeros(3)
+    lib.gfo_cpml_recursion(float(p), float(dt), P(out))
+    return out
+
+
+def cpml_point_coefficients(K, d, alpha, dt):
+    """The 27 complete C-PML coefficients of one point."""
+    K, d, alpha = (np.ascontiguousarray(x, np.float64) for x in (K, d, alpha))
+    out = np.zeros(27)
+    lib.gfo_cpml_point_coefficients(P(K), P(d), P(alpha), float(dt), P(out))
+    return out
+
+
+def cpml_filter_history(coef, g=None, u=None, v=None):
+    """One point from zero memories: gradient history g [n,9] -> G [n,3,9]
+    (column j, entry k*3+l) and/or (u, v) [n,3] -> inertia L [n,3]."""
+    coef = np.ascontiguousarray(coef, np.float64)
+    n = len(g) if g is not None else len(u)
+    G = np.zeros((n, 27)) if g is not None else None
+    L = np.zeros((n, 3)) if u is not None else None
+    g = None if g is None else np.ascontiguousarray(g, np.float64)
+    u = None if u is None else np.ascontiguousarray(u, np.float64)
+    v = None if v is None else np.ascontiguousarray(v, np.float64)
+    lib.gfo_cpml_filter_history(P(coef), n, P(g), P(u), P(v), P(G), P(L))
+    return (None if G is None else G.reshape(n, 3, 9)), L
+
+
+def element_residual_cpml(n_elem, dxi_dx, jac, lam, mu, pml_region, coef, D, w, ngll, u, carry, r):
+    lib.gfo_element_residual_cpml(n_elem, P(dxi_dx), P(jac), P(lam), P(mu), P(pml_region, C.c_int32), P(coef), P(D),
+                                  P(w), ngll, P(u), P(carry), P(r))

@@ -48,6 +48,31 @@ def buried_source(mesh, frac=(0.47, 0.53, 0.0), depth=None, f0=1.5, t0=1.0, amp=
     return cells, w, stf
 
 
+def rigid_outer_nodes(mesh, faces=("xmin", "xmax", "ymin", "ymax", "zmax")):
+    """Nodes on the named box faces (those that carry a C-PML layer), found
+    from the coordinates."""
+    xyz = mesh.node_xyz.reshape(-1, 3)
+    b = mesh.bounds
+    tol = 1e-6 * max(b[1] - b[0], b[3] - b[2], b[5] - b[4])
+    where = {"xmin": (0, b[0]), "xmax": (0, b[1]), "ymin": (1, b[2]), "ymax": (1, b[3]), "zmax": (2, b[5])}
+    on = np.zeros(len(xyz), bool)
+    for name in faces:
+        ax, v = where[name]
+        on |= np.abs(xyz[:, ax] - v) <= tol
+    return on
+
+
+def cpml_complete_arrays(mesh):
+    """Oracle inputs of the complete C-PML: the mesh arrays with zero mass at
+    the element points on the rigid outer boundary (the oracle's massless-node
+    rule then holds them, as the solver's rigid nodes)."""
+    arr = dict(mesh.arrays)
+    mass = np.array(mesh.mass, copy=True)
+    mass[rigid_outer_nodes(mesh)[mesh.ibool]] = 0.0
+    arr["mass"] = mass
+    return arr
+
+
 def rel_l2(a, b):
     a = np.asarray(a, np.float64).ravel()
     b = np.asarray(b, np.float64).ravel()

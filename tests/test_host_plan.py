@@ -52,3 +52,22 @@ def test_create_still_rejects_bad_arguments_first():
             pygfx.Solver(mesh)
     finally:
         mesh.arrays["ibool"] = good
+
+
+@pytest.mark.parametrize("assembly", [pygfx.GFX_ASSEMBLY_COLORED, pygfx.GFX_ASSEMBLY_GATHER])
+def test_complete_cpml_plans_and_rejects_bad_inputs(assembly):
+    mesh = mc.small_layered(nx=5, ny=4, nz=4, pml=1, want_profiles=1, pml_profile=1)
+    with pytest.raises(pygfx.GfxError) as ei:
+        pygfx.Solver(mesh, n_fields=2, assembly=assembly, cpml_complete=True, record_capacity=2)
+    assert ei.value.status == GFX_ERR_DEVICE, str(ei.value)
+    with pytest.raises(pygfx.GfxError, match="needs has_cpml"):
+        pygfx.Solver(mesh, cpml_complete=True, has_cpml=False)
+    bad = mesh.derive(pml_K=np.where(np.arange(mesh.pml_K.size) == 3 * 125 * 2 + 1, 0.0, mesh.pml_K))
+    with pytest.raises(pygfx.GfxError, match="pml_K must be > 0"):
+        pygfx.Solver(bad, cpml_complete=True)
+    bad = mesh.derive(pml_alpha=-mesh.pml_alpha)
+    with pytest.raises(pygfx.GfxError, match="pml_alpha >= 0"):
+        pygfx.Solver(bad, cpml_complete=True)
+    no_profiles = mc.small_layered(nx=5, ny=4, nz=4, pml=1)
+    with pytest.raises(ValueError, match="want_profiles"):
+        pygfx.Solver(no_profiles, cpml_complete=True)

@@ -16,7 +16,8 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SAN_LIB = os.path.join(ROOT, "gf-only-sem_amd", "libgfx_san.so")
-FILES = ["tests/test_host_plan.py", "tests/test_meshgen.py", "tests/test_topology.py", "tests/test_io.py"]
+FILES = ["tests/test_host_plan.py", "tests/test_meshgen.py", "tests/test_topology.py", "tests/test_io.py",
+         "tests/test_cpml_complete.py"]
 
 
 def _asan_runtime():
