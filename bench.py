@@ -37,14 +37,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cfg2_spec(pygfx, nx=100, ny=100, nz=50, h=600.0, duration=30.0, n_sls=0):
+def cfg2_spec(pygfx, nx=100, ny=100, nz=50, h=600.0, duration=30.0, n_sls=0, profiles=False):
     from pygfx import workloads
-    return workloads.layered_spec(nx, ny, nz, h=h, duration=duration, n_sls=n_sls)
+    spec = workloads.layered_spec(nx, ny, nz, h=h, duration=duration, n_sls=n_sls)
+    spec.want_profiles = 1 if profiles else 0  # pml_K, pml_d, pml_alpha for the complete C-PML leg
+    return spec
 
 
 def stations(mesh, n=8, seed=3):
+    """Seeded surface stations in the central 40 x 40 km (80 % of the extent on a
+    smaller mesh)."""
     from pygfx import workloads
-    return workloads.stations(mesh, n=n, seed=seed)
+    ext = min(mesh.bounds[1] - mesh.bounds[0], mesh.bounds[3] - mesh.bounds[2])
+    return workloads.stations(mesh, n=n, seed=seed, half=min(20000.0, 0.4 * ext))
 
 
 def time_run(pygfx, solver, steps, warmup):
@@ -55,7 +60,7 @@ def time_run(pygfx, solver, steps, warmup):
     t1 = time.perf_counter()
     solver.set_timing(False)
     out = dict(wall_s=t1 - t0)
-    for name in ("element", "node", "node_special", "node_regular", "record", "total"):
+    for name in ("element", "element_pml", "node", "node_special", "node_regular", "record", "total"):
         ms, n = solver.kernel_time(name)
         out[name] = (ms, n)
     return out
@@ -146,6 +151,7 @@ def traffic_from_profiles(kernel_prefix, launch, fields, dims):
             if p[1] == kernel_prefix and p[2] == tag:
                 same = (float(p[5]), p[2])
             elif p[1].startswith("k_element") and kernel_prefix.startswith("k_element") and \
+                    "k_element_cpml" not in (p[1], kernel_prefix) and \
                     p[2].split("/")[0] == launch["assembly"] and other is None:
                 other = (float(p[5]), f"{p[1]} {p[2]}")
     return same or other or (None, None)
@@ -321,6 +327,43 @@ def cpu_baseline(seconds=15.0):
             "sample_1_core": f"same, 1 thread, {one[1]} steps, {one[2]:.1f} s"}
 
 
+def cpml_complete_leg(pygfx, mesh, F, lc, st, stf, args, value_ref, launch, dims):
+    """The batched workload again with the complete C-PML (stretched stresses
+    and inertia terms in the PML elements' own kernel, the reference's profiles
+    of this mesh): whole-step throughput and the PML kernel's time, bytes and
+    HBM rate (events around its launch)."""
+    t0 = time.perf_counter()
+    s = pygfx.Solver(mesh, n_fields=F, record_capacity=2, cpml_complete=True, **lc)
+    for f in range(F):
+        x, y = st[f // 3]
+        cells, w = mesh.locate_source(x, y, 0.0)
+        s.set_source(f, f % 3, cells, w, stf)
+    log(f"complete C-PML solver ready ({time.perf_counter() - t0:.1f} s)")
+    t = time_run(pygfx, s, args.steps, args.warmup)
+    value = mesh.n_elem * 125 * F * args.steps / t["wall_s"]
+    ms_pml, n_pml = t["element_pml"]
+    Q = s.query("cpml_points")
+    mem = s.query("cpml_memories")
+    # per PML point: scaled geometry (9), lambda, mu, node id, residual entry and
+    # 28 coefficients; per field and point u and w read, the residual written,
+    # and the memories in use read and written
+    pml_bytes = (Q * 4.0 * (9 + 4 + 28) + F * (Q * 36.0 + 8.0 * mem)) if n_pml else None
+    out = {"value": value, "unit": "GLL-point updates/s", "ms_per_step": 1e3 * t["wall_s"] / args.steps,
+           "vs_reference_cpml": value / value_ref,
+           "ms_element_per_step": t["element"][0] / args.steps,
+           "ms_pml_kernel_per_step": (ms_pml / args.steps) if n_pml else None,
+           "pml_elements": mesh.n_pml_elem, "pml_points": int(Q),
+           "memories_in_use_per_point": mem / Q if Q else 0.0,
+           "pml_kernel_bytes_per_step": pml_bytes,
+           "pml_kernel_gbs": (pml_bytes / (ms_pml / args.steps * 1e-3) / 1e9) if n_pml and ms_pml > 0 else None,
+           "pml_kernel_traffic": traffic_from_profiles("k_element_cpml", launch, F, dims)[0],
+           "note": "the reference's own profiles of this mesh (pml_profile 0); PML kernel bytes: per PML "
+                   "point the geometry, moduli, node id, residual entry and 28 coefficients, and per field "
+                   "u and w read, the residual written and the memories in use read and written"}
+    s.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -346,6 +389,8 @@ def main():
     ap.add_argument("--surrogate-iters", type=int, default=16000)
     ap.add_argument("--surrogate-configs", default="",
                     help='JSON list of experiment overrides, e.g. \'[{"sigma": 1.0}, {"envelope": false}]\'')
+    ap.add_argument("--no-cpml-complete", action="store_true",
+                    help="skip the complete C-PML line (the batched fields again with the stretched-stress C-PML)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run 3 device-to-device copies of 1 GiB (PMC counter calibration)")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -393,7 +438,8 @@ def main():
 
     t0 = time.perf_counter()
     dims = (40, 40, 20) if args.small else (100, 100, 50)
-    mesh = pygfx.build_box(cfg2_spec(pygfx, *dims, n_sls=args.attenuation), copy=False)
+    want_cpml = not args.no_cpml_complete and args.attenuation == 0
+    mesh = pygfx.build_box(cfg2_spec(pygfx, *dims, n_sls=args.attenuation, profiles=want_cpml), copy=False)
     log(f"mesh: {mesh.n_elem} elements, {mesh.n_node} nodes, {mesh.n_pml_elem} PML elements, "
         f"{len(mesh.rec_cell)} recording cells, dt={mesh.dt:.5g} s, stride={mesh.snapshot_stride}, "
         f"nsteps={mesh.nsteps} ({time.perf_counter() - t0:.1f} s)")
@@ -513,6 +559,9 @@ def main():
                     "step without overlap (the forward driver overlaps it with the next segment)"}
     sb.close()
     del sb
+    if want_cpml:
+        result["cpml_complete"] = cpml_complete_leg(pygfx, mesh, F, lcb, st, stf, args, value, launch, dims)
+        log("complete C-PML:", json.dumps(result["cpml_complete"]))
     if not args.no_uneven and not args.small:
         result["uneven_batch"] = uneven_workload(pygfx, args.steps, args.warmup)
         log("uneven:", json.dumps(result["uneven_batch"]))

@@ -193,7 +193,7 @@ struct RunInputs {
     size_t P = 0;
 };
 
-void load_inputs(const fs::path& wd, Logger& log, RunInputs& in) {
+void load_inputs(const fs::path& wd, Logger& log, RunInputs& in, bool profiles = false) {
     const auto t_io = std::chrono::steady_clock::now();
     in.cfg = io::read_config((wd / "config.h5").string());
     in.ngll = in.cfg.polynomial_order + 1;
@@ -201,6 +201,7 @@ void load_inputs(const fs::path& wd, Logger& log, RunInputs& in) {
                               std::to_string(in.ngll));
     io::ReadOpts ro;
     ro.full = false;
+    ro.profiles = profiles;
     in.part = io::read_partition_all((wd / "partitions").string(), ro);
     io::Partition& part = in.part;
     in.E = part.n_local_cell;
@@ -241,7 +242,7 @@ void load_inputs(const fs::path& wd, Logger& log, RunInputs& in) {
 
 // Solver description over the loaded arrays; assembly < 0 autotunes the launch.
 gfx_solver_desc solver_desc(RunInputs& in, int F, int record_capacity, bool record_uva, int assembly,
-                            int element_kernel, int elems_per_block, Logger& log) {
+                            int element_kernel, int elems_per_block, Logger& log, bool cpml_complete = false) {
     io::Partition& part = in.part;
     const bool recording = part.recording.has_recording && !part.recording.gll_node_ids.empty();
     gfx_solver_desc d{};
@@ -277,6 +278,17 @@ gfx_solver_desc solver_desc(RunInputs& in, int F, int record_capacity, bool reco
         d.n_sls = part.n_sls;
         d.tau_sigma = part.tau_sigma.data();
         d.tau_epsilon = part.tau_epsilon.data();
+    }
+    if (cpml_complete) {
+        require(part.has_cpml && part.pml_K.size() == in.P * 3 && part.pml_d.size() == in.P * 3 &&
+                    part.pml_alpha.size() == in.P * 3,
+                "the complete C-PML needs /field/cell/pml_K, pml_d and pml_alpha [n_cell, ngll^3, 3] in the "
+                "partitions");
+        d.cpml_complete = 1;
+        d.pml_K = part.pml_K.data();
+        d.pml_d = part.pml_d.data();
+        d.pml_alpha = part.pml_alpha.data();
+        log.info("  complete C-PML (stretched stresses, rigid outer boundary)");
     }
     if (assembly < 0) {
         gfx_solver_desc best{};
@@ -327,7 +339,7 @@ int run_forward_impl(const gfx_forward_options& o) {
     try {
         log.info("single process, direction=" + tag + (o.stations ? " stations=" + std::string(o.stations) : ""));
         RunInputs in;
-        load_inputs(wd, log, in);
+        load_inputs(wd, log, in, o.cpml_complete != 0);
         const io::Config& cfg = in.cfg;
         io::Partition& part = in.part;
         const int ngll = in.ngll, NP = in.NP;
@@ -350,7 +362,7 @@ int run_forward_impl(const gfx_forward_options& o) {
         const int n_snap_total = cfg.snapshot_stride > 0 ? (cfg.nsteps + cfg.snapshot_stride - 1) / cfg.snapshot_stride : 0;
         const int ring = std::max(1, o.ring > 0 ? o.ring : 8);
         gfx_solver_desc d = solver_desc(in, F, n_snap_total > 0 ? std::min(ring, n_snap_total) : 0, o.record_uva != 0,
-                                        o.assembly, o.element_kernel, o.elems_per_block, log);
+                                        o.assembly, o.element_kernel, o.elems_per_block, log, o.cpml_complete != 0);
         gfx_solver* S = nullptr;
         check(gfx_solver_create(&d, &S), "solver create");
         std::unique_ptr<gfx_solver, gfx_status (*)(gfx_solver*)> guard(S, gfx_solver_destroy);
@@ -1164,6 +1176,11 @@ extern "C" gfx_status gfx_mesh_read_inputs(const char* dir, gfx_mesh** out) {
             H->coef_beta = std::move(p.coef_beta);
             H->coef_strain = std::move(p.coef_strain);
             H->coef_abar = std::move(p.coef_abar);
+        }
+        if (p.pml_K.size() == P * 3 && p.pml_d.size() == P * 3 && p.pml_alpha.size() == P * 3) {
+            H->pml_K = std::move(p.pml_K);
+            H->pml_d = std::move(p.pml_d);
+            H->pml_alpha = std::move(p.pml_alpha);
         }
         H->rec_cell = std::move(p.recording.rec_cell_local);
         H->tau_sigma = std::move(p.tau_sigma);

@@ -8,7 +8,7 @@
 //       batched: one field per (station, direction), advanced together;
 //       records under wavefields/s{k}/{dir}/ (or wavefields/{dir}/ without stations).
 //   Options: --assembly {auto,colored,gather,atomic} --ring N --stop-after STEP
-//            --restart-stride N --quiet --workdir DIR
+//            --restart-stride N --quiet --workdir DIR --cpml-complete
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -22,7 +22,7 @@ static void usage(const char* prog) {
     std::cerr << "Usage: " << prog << " --direction {x,y,z} [--resume]\n"
               << "       " << prog << " --directions x,y,z [--stations FILE] [--resume]\n"
               << "  options: --assembly {auto,colored,gather,atomic} --ring N --stop-after STEP\n"
-              << "           --restart-stride N --workdir DIR --quiet\n"
+              << "           --restart-stride N --workdir DIR --quiet --cpml-complete\n"
               << "  All I/O paths are relative to the working directory:\n"
               << "    Input:  config.h5, partitions/partition_{r}.h5\n"
               << "    Output: wavefields/{direction}/record_0_{step}.h5\n"
@@ -70,6 +70,8 @@ int main(int argc, char** argv) {
             workdir = next(a);
         } else if (!std::strcmp(a, "--quiet")) {
             o.verbose = 0;
+        } else if (!std::strcmp(a, "--cpml-complete")) {
+            o.cpml_complete = 1;
         } else if (!std::strcmp(a, "--help") || !std::strcmp(a, "-h")) {
             usage(argv[0]);
             return 0;

@@ -57,6 +57,11 @@ Partition read_partition(const std::string& path, const ReadOpts& o) {
         p.coef_strain = h5::try_read<double>(id, "/field/cell/pml_coef_strain");
     }
     p.coef_abar = h5::try_read<double>(id, "/field/cell/pml_coef_abar");
+    if (o.full || o.profiles) {
+        p.pml_K = h5::try_read<double>(id, "/field/cell/pml_K");
+        p.pml_d = h5::try_read<double>(id, "/field/cell/pml_d");
+        p.pml_alpha = h5::try_read<double>(id, "/field/cell/pml_alpha");
+    }
     p.has_cpml = !p.coef_abar.empty();  // io.cpp:237
     if (h5::exists(id, "/field/cell/tau_sigma")) {
         std::vector<h5::hsize_t> dims;
@@ -130,6 +135,9 @@ Partition read_partition_all(const std::string& dir, const ReadOpts& o) {
         cat(m.coef_beta, part.coef_beta);
         cat(m.coef_abar, part.coef_abar);
         cat(m.coef_strain, part.coef_strain);
+        cat(m.pml_K, part.pml_K);
+        cat(m.pml_d, part.pml_d);
+        cat(m.pml_alpha, part.pml_alpha);
         if (part.n_sls != m.n_sls) throw Error(GFX_ERR_IO, "partitions differ in their number of attenuation solids");
         cat(m.tau_sigma, part.tau_sigma);
         cat(m.tau_epsilon, part.tau_epsilon);
@@ -444,6 +452,11 @@ void write_inputs(const gfx_mesh& m, const std::string& dir, int n_parts, const 
                 h5::write_vec(id, "/field/cell/pml_coef_strain", slice(m.coef_strain, 18), {hn, hsize_t(NP), 18},
                               gz);
             h5::write_vec(id, "/field/cell/pml_coef_abar", slice(m.coef_abar, 5), {hn, hsize_t(NP), 5}, gz);
+        }
+        if (m.pml_K && m.pml_d && m.pml_alpha) {  // pml_cpml.py compute_pml_profiles output
+            h5::write_vec(id, "/field/cell/pml_K", slice(m.pml_K, 3), {hn, hsize_t(NP), 3}, gz);
+            h5::write_vec(id, "/field/cell/pml_d", slice(m.pml_d, 3), {hn, hsize_t(NP), 3}, gz);
+            h5::write_vec(id, "/field/cell/pml_alpha", slice(m.pml_alpha, 3), {hn, hsize_t(NP), 3}, gz);
         }
         if (m.n_sls > 0 && m.tau_sigma && m.tau_epsilon) {
             const hsize_t L = hsize_t(m.n_sls);

@@ -33,6 +33,8 @@ struct Partition {
     std::vector<double> coords, jacobian, dxi_dx, mass, vp, vs, density, lambda_, mu_, damping;
     std::vector<int32_t> pml_region;
     std::vector<double> coef_alpha, coef_beta, coef_abar, coef_strain;
+    // C-PML stretch profiles /field/cell/pml_K, pml_d, pml_alpha [n_cell, n, 3]
+    std::vector<double> pml_K, pml_d, pml_alpha;
     // shear attenuation (planned in the reference, not read by its solver):
     // /field/cell/tau_sigma, tau_epsilon [n_cell, ngll, ngll, ngll, n_sls]
     int n_sls = 0;
@@ -51,6 +53,7 @@ struct Partition {
 // array the reference reads.
 struct ReadOpts {
     bool full = true;
+    bool profiles = false;  // the C-PML stretch profiles even when !full
 };
 
 Partition read_partition(const std::string& path, const ReadOpts& o = {});

@@ -675,6 +675,26 @@ __global__ void __launch_bounds__(128) k_element_att(ElemArgs A, int e0, int ne)
     }
 }
 
+// Whether complete C-PML memory t of a point is in use, from its damped axes
+// on[] (d != 0): gradient memory (k*3+l)*4 + r uses axis m, n, both, or l for
+// r = 0..3 (m < n the axes other than l); inertia memory 36 + c*3 + r uses axis r.
+__device__ __forceinline__ bool cpml_mem_on(const bool (&on)[3], int t) {
+    if (t >= 36) return on[(t - 36) % 3];
+    const int l = (t / 4) % 3, r = t % 4;
+    const int m = l == 0 ? 1 : 0, n = l == 2 ? 1 : 2;
+    return r == 0 ? on[m] : r == 1 ? on[n] : r == 2 ? (on[m] && on[n]) : on[l];
+}
+
+// The complete C-PML coefficients of one point, held in LDS (column p of a
+// [CPML_NC][128] block; c[i] is coefficient o + i).  Read from LDS where they
+// are used instead of kept in registers over the field loop, they leave the
+// kernel room for three waves per SIMD.
+struct CpmlCoef {
+    const float (*b)[128];
+    int p, o;
+    __device__ __forceinline__ float operator[](int i) const { return b[o + i][p]; }
+};
+
 // One step of conv(g, p) (exact for g linear in time over the step): the
 // carry q holds a M + b g of the previous level; returns M = q + c g and
 // leaves a M + b g for the next step.
@@ -700,10 +720,12 @@ __device__ __forceinline__ float conv_step(const float* abc, float& q, float g) 
 // velocity terms at v_{n+1} (dt/2 a_{n+1} times their coefficient) is in the
 // lumped mass, as is K_x K_y K_z.
 template <int MODE>
-__global__ void __launch_bounds__(128) k_element_cpml(ElemArgs A, int e0, int ne) {
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3, 3)))
+k_element_cpml(ElemArgs A, int e0, int ne) {
     __shared__ float sD[25];
     __shared__ float su[3][NP];
     __shared__ float sf[9][NP];
+    __shared__ float scf[CPML_NC][128];  // each lane's own coefficients
     const int tid = threadIdx.x;
     if (tid < 25) sD[tid] = A.D[tid];
     const int blk = xcd_block(blockIdx.x, gridDim.x);
@@ -712,7 +734,8 @@ __global__ void __launch_bounds__(128) k_element_cpml(ElemArgs A, int e0, int ne
     const int e = e0 + (valid ? blk : 0);
     const int pc = valid ? p : 0;
     const int i = pc / 25, j = (pc / 5) % 5, k = pc % 5;
-    float g[9], cf[CPML_NC];
+    float g[9];
+    const CpmlCoef cf{scf, p, 0};
     float lam = 0.f, mu = 0.f;
     uint32_t nodef = 0;
     const int64_t pt = int64_t(e) * NP + pc;
@@ -724,12 +747,12 @@ __global__ void __launch_bounds__(128) k_element_cpml(ElemArgs A, int e0, int ne
         nodef = A.ibool[pt];
         q = int64_t(A.pml_slot[e]) * NP + p;
 #pragma unroll
-        for (int c = 0; c < CPML_NC; ++c) cf[c] = A.pcoef[c * A.Ppml + q];
+        for (int c = 0; c < CPML_NC; ++c) scf[c][p] = A.pcoef[c * A.Ppml + q];
     } else {
 #pragma unroll
         for (int c = 0; c < 9; ++c) g[c] = 0.f;
 #pragma unroll
-        for (int c = 0; c < CPML_NC; ++c) cf[c] = 0.f;
+        for (int c = 0; c < CPML_NC; ++c) scf[c][p] = 0.f;
     }
     const int64_t node = int64_t(nodef & ~FIRST_TOUCH);
     const bool first = (nodef & FIRST_TOUCH) != 0;
@@ -740,6 +763,9 @@ __global__ void __launch_bounds__(128) k_element_cpml(ElemArgs A, int e0, int ne
     }
     const int step = A.ctr[0];
     const int64_t rslot = residual_entry<MODE>(A, pt, valid);
+    // a memory whose axis has d = 0 has zero recursion coefficients and stays
+    // zero: it is neither read nor written (single-axis layers use 12 of 45)
+    const bool on[3] = {cf[1] != 0.f, cf[10] != 0.f, cf[19] != 0.f};
     __syncthreads();
     float Di[5], Dj[5], Dk[5], Dti[5], Dtj[5], Dtk[5];
 #pragma unroll
@@ -766,8 +792,12 @@ __global__ void __launch_bounds__(128) k_element_cpml(ElemArgs A, int e0, int ne
             rold[1] = r[1];
             rold[2] = r[2];
         }
-        __syncthreads();
+        // this field's memories in use, loaded together before the barrier
         float* mem = A.pmem + int64_t(f) * CPML_NM * A.Ppml + q;
+        float Qm[CPML_NM];
+#pragma unroll
+        for (int t = 0; t < CPML_NM; ++t) Qm[t] = (valid && cpml_mem_on(on, t)) ? mem[int64_t(t) * A.Ppml] : 0.f;
+        __syncthreads();
         if (valid) {
             float du[3][3];
 #pragma unroll
@@ -789,21 +819,16 @@ __global__ void __launch_bounds__(128) k_element_cpml(ElemArgs A, int e0, int ne
 #pragma unroll
                 for (int l = 0; l < 3; ++l) {
                     const int m = l == 0 ? 1 : 0, n = l == 2 ? 1 : 2;
-                    const float* cm = cf + m * 9;
-                    const float* cn = cf + n * 9;
-                    const float* cl = cf + l * 9;
+                    const CpmlCoef cm{scf, p, m * 9}, cn{scf, p, n * 9}, cl{scf, p, l * 9};
                     const float gv = du[kk][0] * g[l * 3 + 0] + du[kk][1] * g[l * 3 + 1] + du[kk][2] * g[l * 3 + 2];
-                    float* Qp = mem + int64_t((kk * 3 + l) * 4) * A.Ppml;
-                    float q0 = Qp[0], q1 = Qp[A.Ppml], q2 = Qp[2 * A.Ppml], q3 = Qp[3 * A.Ppml];
-                    const float Mm = conv_step(cm + 3, q0, gv);
-                    const float Mn = conv_step(cn + 3, q1, gv);
-                    const float Mmn = conv_step(cn + 3, q2, Mm);
+                    float* Qk = Qm + (kk * 3 + l) * 4;
+                    const float am[3] = {cm[3], cm[4], cm[5]}, an[3] = {cn[3], cn[4], cn[5]},
+                                bl[3] = {cl[6], cl[7], cl[8]};
+                    const float Mm = conv_step(am, Qk[0], gv);
+                    const float Mn = conv_step(an, Qk[1], gv);
+                    const float Mmn = conv_step(an, Qk[2], Mm);
                     const float y2 = cm[0] * cn[0] * gv + cn[0] * cm[1] * Mm + cn[1] * (cm[0] * Mn + cm[1] * Mmn);
-                    const float Mb = conv_step(cl + 6, q3, y2);
-                    Qp[0] = q0;
-                    Qp[A.Ppml] = q1;
-                    Qp[2 * A.Ppml] = q2;
-                    Qp[3 * A.Ppml] = q3;
+                    const float Mb = conv_step(bl, Qk[3], y2);
                     G[l][kk * 3 + l] = (y2 - cl[1] / cl[0] * Mb) / cl[0];
                     G[n][kk * 3 + l] = cm[0] * gv + cm[1] * Mm;
                     G[m][kk * 3 + l] = cn[0] * gv + cn[1] * Mn;
@@ -826,29 +851,28 @@ __global__ void __launch_bounds__(128) k_element_cpml(ElemArgs A, int e0, int ne
         if (valid) {
             float rv[3];
             point_div(&sf[0][0], NP, Dti, Dtj, Dtk, i, j, k, rv);
-            const float* cx = cf;
-            const float* cy = cf + 9;
-            const float* cz = cf + 18;
-            float* im = mem + int64_t(36) * A.Ppml;
+            const CpmlCoef cx{scf, p, 0}, cy{scf, p, 9}, cz{scf, p, 18};
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const float u = su[c][p], v = A.W[fo + node * 3 + c];
-                float q1 = im[(c * 3 + 0) * A.Ppml], q2 = im[(c * 3 + 1) * A.Ppml], q3 = im[(c * 3 + 2) * A.Ppml];
-                const float C1 = conv_step(cx + 3, q1, u);
+                float* Qc = Qm + 36 + c * 3;
+                const float ax[3] = {cx[3], cx[4], cx[5]}, ay[3] = {cy[3], cy[4], cy[5]},
+                            az[3] = {cz[3], cz[4], cz[5]};
+                const float C1 = conv_step(ax, Qc[0], u);
                 const float z1 = cx[0] * u + cx[1] * C1;
-                const float C2 = conv_step(cy + 3, q2, z1);
+                const float C2 = conv_step(ay, Qc[1], z1);
                 const float z2 = cy[0] * z1 + cy[1] * C2;
-                const float C3 = conv_step(cz + 3, q3, z2);
+                const float C3 = conv_step(az, Qc[2], z2);
                 const float L1 = cx[1] * (v - cx[2] * u + cx[2] * cx[2] * C1);
                 const float Dz1 = cx[0] * v + cx[1] * (u - cx[2] * C1);
                 const float L2 = cy[0] * L1 + cy[1] * (Dz1 - cy[2] * z1 + cy[2] * cy[2] * C2);
                 const float Dz2 = cy[0] * Dz1 + cy[1] * (z1 - cy[2] * C2);
                 const float L3 = cz[0] * L2 + cz[1] * (Dz2 - cz[2] * z2 + cz[2] * cz[2] * C3);
-                im[(c * 3 + 0) * A.Ppml] = q1;
-                im[(c * 3 + 1) * A.Ppml] = q2;
-                im[(c * 3 + 2) * A.Ppml] = q3;
                 rv[c] -= cf[27] * L3;
             }
+#pragma unroll
+            for (int t = 0; t < CPML_NM; ++t)
+                if (cpml_mem_on(on, t)) mem[int64_t(t) * A.Ppml] = Qm[t];
             add_sources(A, step, f, sb, se, p, rv);
             emit_residual<MODE>(A, f, node, rslot, rold, rv);
         }

@@ -798,7 +798,7 @@ void build(MeshHolder& H, const GeneralCells* gen = nullptr) {
                 // face's own layer, the deeper of an axis's two faces
                 for (int ax = 0; ax < 3; ++ax) {
                     if (!axis_active(region, ax)) continue;
-                    double depth = -1.0, w = 0.0;
+                    double depth = -1e300, w = 0.0;
                     for (int side = 0; side < 2; ++side) {
                         const int f = 2 * ax + side;
                         const double wf = wid[f];
@@ -809,8 +809,8 @@ void build(MeshHolder& H, const GeneralCells* gen = nullptr) {
                             w = wf;
                         }
                     }
-                    if (depth < 0.0) continue;  // the point is in no layer of this axis
-                    depth = std::min(1.0, depth);
+                    if (w <= 0.0) continue;  // no layer on this axis
+                    depth = std::min(1.0, std::max(0.0, depth));  // points on the interface: depth 0
                     K[ax] = 1.0;  // K_MIN_PML = K_MAX_PML = 1
                     d[ax] = -((2.0 + 1.0) * vp * std::log(rcoef) / (2.0 * w)) * std::pow(depth, 1.2 * 2.0);
                     al[ax] = alpha_max[ax] * (1.0 - depth);

@@ -64,6 +64,7 @@ struct gfx_solver {
     // [nf][45][Ppml]); the node kernels then apply no C-PML term
     bool cpml_complete = false;
     gfx::DevBuf<float> pcoef;
+    int64_t cpml_mem_active = 0;  // memories the kernel reads and writes, over all PML points
     hipStream_t stream = nullptr;
     std::vector<int32_t> int_of_elem;        // caller element -> solver element
     std::vector<int32_t> node_new_of_old;    // caller node -> solver node
@@ -114,8 +115,12 @@ struct gfx_solver {
     gfx::DevBuf<float> inj_w;
     // timing
     bool timing = false;
-    double t_elem = 0, t_spec = 0, t_reg = 0, t_rec = 0, t_total = 0;
-    int n_elem_launch = 0, n_spec_launch = 0, n_reg_launch = 0, n_rec_launch = 0, n_total_launch = 0;
+    double t_elem = 0, t_elem_pml = 0, t_spec = 0, t_reg = 0, t_rec = 0, t_total = 0;
+    int n_elem_launch = 0, n_elem_pml_launch = 0, n_spec_launch = 0, n_reg_launch = 0, n_rec_launch = 0,
+        n_total_launch = 0;
+    // complete C-PML: index of the first range when the PML ranges close the
+    // element launches (one colour), else -1 (their time is then not split out)
+    int pml_tail = -1;
 
     ~gfx_solver() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -596,24 +601,32 @@ StepArgs make_step_args(gfx_solver& S) {
 // Enqueues one whole step on S.stream.  The step index and snapshot counter
 // live on the device (S.ctr), so the same enqueue can be captured once into a
 // graph and replayed.  `record` launches the record kernel (which itself does
-// nothing off snapshot steps); ev, when given, receives 4 events: after the
-// elements, the special nodes, the regular nodes and the end of the step.
+// nothing off snapshot steps); ev, when given, receives 5 events: before the
+// complete C-PML element ranges when they close the element launches (else
+// with the next), after the elements, the special nodes, the regular nodes
+// and the end of the step.
 void enqueue_step(gfx_solver& S, const StepArgs& a, bool record, hipEvent_t* ev) {
     if (S.assembly == GFX_ASSEMBLY_ATOMIC)
         dev::k_zero<<<a.zero_grid, 256, 0, S.stream>>>(reinterpret_cast<float4*>(S.R.p), a.r4);
-    for (const auto& rg : S.ranges) launch_element(S, a.ea, rg);
+    const int nr = int(S.ranges.size());
+    const int pml_tail = S.pml_tail >= 0 ? S.pml_tail : nr;
+    for (int i = 0; i < nr; ++i) {
+        if (ev && i == pml_tail) GFX_HIP(hipEventRecord(ev[0], S.stream));
+        launch_element(S, a.ea, S.ranges[i]);
+    }
+    if (ev && pml_tail == nr) GFX_HIP(hipEventRecord(ev[0], S.stream));
     GFX_HIP(hipGetLastError());
-    if (ev) GFX_HIP(hipEventRecord(ev[0], S.stream));
-    if (a.spec_grid > 0) launch_node_special(S, a.na, a.spec_grid);
     if (ev) GFX_HIP(hipEventRecord(ev[1], S.stream));
+    if (a.spec_grid > 0) launch_node_special(S, a.na, a.spec_grid);
+    if (ev) GFX_HIP(hipEventRecord(ev[2], S.stream));
     if (a.reg_grid > 0) launch_node_regular(S, a.na, a.reg_grid);
     GFX_HIP(hipGetLastError());
-    if (ev) GFX_HIP(hipEventRecord(ev[2], S.stream));
+    if (ev) GFX_HIP(hipEventRecord(ev[3], S.stream));
     if (record && a.recording && a.rec_grid > 0)
         dev::k_record<REPB><<<a.rec_grid, a.rec_threads, 0, S.stream>>>(a.ra);
     dev::k_advance<<<1, 64, 0, S.stream>>>(S.ctr.p, S.stride);
     GFX_HIP(hipGetLastError());
-    if (ev) GFX_HIP(hipEventRecord(ev[3], S.stream));
+    if (ev) GFX_HIP(hipEventRecord(ev[4], S.stream));
 }
 
 // Host mirror of k_advance.
@@ -654,11 +667,12 @@ void run_steps(gfx_solver& S, int nsteps) {
     }
     const StepArgs a = make_step_args(S);
     std::vector<hipEvent_t> ev;
-    constexpr size_t EV = 4;
+    constexpr size_t EV = 5;
     if (S.timing) {
-        // ev[0] = start; step t: ev[4t+1] after elements, ev[4t+2] after the
-        // special-node kernel, ev[4t+3] after the regular-node kernel, ev[4t+4]
-        // after recording (= start of step t+1)
+        // ev[0] = start; step t: ev[5t+1] before the complete C-PML ranges (when
+        // they close the element launches), ev[5t+2] after elements, ev[5t+3]
+        // after the special-node kernel, ev[5t+4] after the regular-node kernel,
+        // ev[5t+5] after recording (= start of step t+1)
         ev.resize(size_t(nsteps) * EV + 1);
         for (auto& e : ev) GFX_HIP(hipEventCreate(&e));
         GFX_HIP(hipEventRecord(ev[0], S.stream));
@@ -672,20 +686,24 @@ void run_steps(gfx_solver& S, int nsteps) {
     }
     GFX_HIP(hipStreamSynchronize(S.stream));
     if (S.timing) {
-        S.t_elem = S.t_spec = S.t_reg = S.t_rec = S.t_total = 0;
-        S.n_elem_launch = S.n_spec_launch = S.n_reg_launch = S.n_rec_launch = 0;
+        S.t_elem = S.t_elem_pml = S.t_spec = S.t_reg = S.t_rec = S.t_total = 0;
+        S.n_elem_launch = S.n_elem_pml_launch = S.n_spec_launch = S.n_reg_launch = S.n_rec_launch = 0;
+        const int n_pml_ranges = S.pml_tail >= 0 ? int(S.ranges.size()) - S.pml_tail : 0;
         for (int t = 0; t < nsteps; ++t) {
-            float e = 0, b = 0, c = 0, d = 0;
+            float e = 0, ep = 0, b = 0, c = 0, d = 0;
             const size_t o = size_t(t) * EV;
-            GFX_HIP(hipEventElapsedTime(&e, ev[o], ev[o + 1]));
-            GFX_HIP(hipEventElapsedTime(&b, ev[o + 1], ev[o + 2]));
-            GFX_HIP(hipEventElapsedTime(&c, ev[o + 2], ev[o + 3]));
-            GFX_HIP(hipEventElapsedTime(&d, ev[o + 3], ev[o + 4]));
+            GFX_HIP(hipEventElapsedTime(&e, ev[o], ev[o + 2]));
+            GFX_HIP(hipEventElapsedTime(&ep, ev[o + 1], ev[o + 2]));
+            GFX_HIP(hipEventElapsedTime(&b, ev[o + 2], ev[o + 3]));
+            GFX_HIP(hipEventElapsedTime(&c, ev[o + 3], ev[o + 4]));
+            GFX_HIP(hipEventElapsedTime(&d, ev[o + 4], ev[o + 5]));
             S.t_elem += e;
+            S.t_elem_pml += ep;
             S.t_spec += b;
             S.t_reg += c;
             S.t_rec += d;
             S.n_elem_launch += int(S.ranges.size());
+            S.n_elem_pml_launch += n_pml_ranges;
             S.n_spec_launch += a.spec_grid > 0;
             S.n_reg_launch += a.reg_grid > 0;
             S.n_rec_launch += did_rec[t];
@@ -853,6 +871,13 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
         S.n_diag = 0;
         for (const ElemRange& rg : S.ranges)
             if (rg.diag) S.n_diag += rg.e - rg.b;
+        if (S.cpml_complete && S.assembly != GFX_ASSEMBLY_COLORED) {
+            for (int i = 0; i < int(S.ranges.size()); ++i)
+                if (S.ranges[i].pml) {
+                    S.pml_tail = i;
+                    break;
+                }
+        }
     }
     S.int_of_elem.assign(E, 0);
     for (int i = 0; i < E; ++i) S.int_of_elem[elem_of_int[i]] = i;
@@ -1073,6 +1098,18 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
             }
         });
         require(!bad, "pml_K must be > 0, pml_d and pml_alpha >= 0 (all finite) at PML element points");
+        // memories in use per point: per derivative axis l (x3 components) the
+        // ones of axes m, n with d != 0, their product, and beta_l's; inertia 3 per damped axis
+        S.cpml_mem_active = 0;
+        for (int64_t q = 0; q < S.Ppml; ++q) {
+            bool on[3];
+            for (int ax = 0; ax < 3; ++ax) on[ax] = hc[size_t(ax * 9 + 1) * Q + q] != 0.f;
+            for (int l = 0; l < 3; ++l) {
+                const int m = l == 0 ? 1 : 0, n = l == 2 ? 1 : 2;
+                S.cpml_mem_active += 3 * (int(on[m]) + int(on[n]) + int(on[m] && on[n]) + int(on[l]));
+            }
+            S.cpml_mem_active += 3 * (int(on[0]) + int(on[1]) + int(on[2]));
+        }
         dv.upload(S.pcoef, std::move(hc));
         dv.alloc(S.pmem, size_t(S.nf) * dev::CPML_NM * Q);
     } else if (S.has_cpml) {
@@ -1867,6 +1904,9 @@ extern "C" gfx_status gfx_solver_kernel_time(const gfx_solver* s, const char* na
         if (n == "element") {
             *ms = s->t_elem;
             *launches = s->n_elem_launch;
+        } else if (n == "element_pml") {  // the complete C-PML's element launches (within "element")
+            *ms = s->t_elem_pml;
+            *launches = s->n_elem_pml_launch;
         } else if (n == "node") {
             *ms = s->t_spec + s->t_reg;
             *launches = s->n_spec_launch + s->n_reg_launch;
@@ -2014,6 +2054,8 @@ extern "C" gfx_status gfx_solver_query(const gfx_solver* s, const char* key, dou
         else if (k == "pml_nodal") *value = (s->has_cpml && s->pml_nodal) ? 1.0 : 0.0;
         else if (k == "pml_elements") *value = s->n_pml_elem;
         else if (k == "cpml_complete") *value = s->cpml_complete ? 1.0 : 0.0;
+        else if (k == "cpml_points") *value = double(s->Ppml);
+        else if (k == "cpml_memories") *value = double(s->cpml_mem_active);
         else if (k == "special_nodes") *value = double(s->Ga);
         else if (k == "pml_nodes") *value = double(s->Gpml);
         else if (k == "fields") *value = s->nf;
@@ -2034,9 +2076,9 @@ extern "C" double gfx_solver_algorithmic_bytes(const gfx_solver* s, int which) {
     const double Pd = double(s->n_diag) * NP;
     double elem = 48.0 * (P - Pd) + 24.0 * Pd + F * 24.0 * G;
     if (ref_elem_pml(*s)) elem += 56.0 * Q + F * (96.0 * Q + 36.0 * Gp);
-    // complete C-PML: 28 coefficients per PML point, and per field its 45
-    // memories read and written and w read
-    if (s->cpml_complete) elem += 112.0 * Q + F * 372.0 * Q;
+    // complete C-PML: 28 coefficients per PML point, and per field the
+    // memories in use (of 45) read and written and w read
+    if (s->cpml_complete) elem += 112.0 * Q + F * (8.0 * double(s->cpml_mem_active) + 12.0 * Q);
     // shear attenuation: the general form (48 B) at every point, 3 step
     // coefficients per solid per point, and per field the memory (5 per solid)
     // and previous deviatoric strain (5) read and written at every point

@@ -116,6 +116,7 @@ class ForwardOptions(C.Structure):
         ("resume", C.c_int), ("record_uva", C.c_int), ("assembly", C.c_int),
         ("element_kernel", C.c_int), ("elems_per_block", C.c_int), ("ring", C.c_int),
         ("stop_after_step", C.c_int), ("restart_stride", C.c_int), ("verbose", C.c_int),
+        ("cpml_complete", C.c_int),
     ]
 
 
@@ -536,7 +537,7 @@ def write_record(inputs_dir, out_dir, direction, step, strain, disp=None, vel=No
 
 def run_forward(workdir=None, directions="z", stations=None, resume=False, record_uva=True, assembly=-1,
                 element_kernel=GFX_ELEMENT_AUTO, elems_per_block=0, ring=8, stop_after_step=-1,
-                restart_stride=-1, verbose=False):
+                restart_stride=-1, verbose=False, cpml_complete=False):
     """The forward driver over the reference's files (gfx_run_forward_ex)."""
     o = ForwardOptions()
     _check(lib.gfx_forward_options_default(C.byref(o)))
@@ -552,6 +553,7 @@ def run_forward(workdir=None, directions="z", stations=None, resume=False, recor
     o.stop_after_step = stop_after_step
     o.restart_stride = restart_stride
     o.verbose = 1 if verbose else 0
+    o.cpml_complete = 1 if cpml_complete else 0
     _check(lib.gfx_run_forward_ex(C.byref(o)))
 
 

@@ -41,6 +41,8 @@ typedef struct {
     int stop_after_step;    /* < 0: run to nsteps; else stop after this step index */
     int restart_stride;     /* < 0: from config.h5; 0: no restart files; > 0: this stride */
     int verbose;            /* echo log lines to stdout */
+    int cpml_complete;      /* the complete C-PML (gfx_solver_desc.cpml_complete) from the
+                               partitions' /field/cell/pml_K, pml_d, pml_alpha (--cpml-complete) */
 } gfx_forward_options;
 
 gfx_status gfx_forward_options_default(gfx_forward_options* o);

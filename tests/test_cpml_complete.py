@@ -219,3 +219,42 @@ def test_rigid_boundary_is_the_layered_faces():
         got = pygfx.cpml_rigid_nodes(mesh)
         want = mc.rigid_outer_nodes(mesh) if faces is None else mc.rigid_outer_nodes(mesh, faces)
         assert np.array_equal(got, want)
+
+
+def test_generator_profiles():
+    """pml_profile 0 is what preprocess/pml_cpml.py:63-155 computes: per damped
+    axis the normalised distance from a boundary face, the later face (max)
+    overwriting the earlier (min), so the min-side layer gets dist = 1
+    (d = d0, alpha = 0) throughout and the max-side layer d = d0 at its
+    interior interface, 0 at the boundary.  pml_profile 1 measures the depth
+    from the interior interface of each face's own layer.  Halfspace, vp 6000,
+    d0 = 3 vp ln(1/R) / (2 w), exponent 2.4, alpha_max = pi f0 (0.9, 1, 1.1)."""
+    kw = dict(nx=8, ny=7, nz=5, pml=2, want_profiles=1, want_coords=1, f0_pml=1.2)
+    ref = mc.small_halfspace(pml_profile=0, **kw)
+    grad = mc.small_halfspace(pml_profile=1, pml_reflection=1e-3, **kw)
+    P = ref.n_elem * 125
+    xyz = ref.coords.reshape(P, 3)
+    reg = np.repeat(ref.pml_region, 125)
+    b = ref.bounds
+    w = 2000.0
+    amax = np.pi * 1.2 * np.array([0.9, 1.0, 1.1])
+    damps = {1: (0,), 2: (1,), 3: (2,), 4: (0, 1), 5: (0, 2), 6: (1, 2), 7: (0, 1, 2)}
+    for m, R, graded in ((ref, 1e-5, False), (grad, 1e-3, True)):
+        d0 = 3.0 * 6000.0 * np.log(1.0 / R) / (2.0 * w)
+        K, d, al = (m.arrays[k].reshape(P, 3) for k in ("pml_K", "pml_d", "pml_alpha"))
+        assert np.all(K == 1.0)
+        for ax, (lo, hi) in enumerate(((b[0], b[1]), (b[2], b[3]), (b[4], b[5]))):
+            on = np.array([r in damps and ax in damps[r] for r in reg])
+            assert np.all(d[~on, ax] == 0.0) and np.all(al[~on, ax] == 0.0)
+            x = xyz[on, ax]
+            if graded:
+                depth = (x - (hi - w)) / w
+                if ax < 2:
+                    depth = np.maximum(depth, (lo + w - x) / w)
+                dist = np.clip(depth, 0.0, 1.0)
+            else:
+                dist = np.clip(np.abs(x - hi) / w, 0.0, 1.0)  # the max face written last
+            np.testing.assert_allclose(d[on, ax], d0 * dist ** 2.4, rtol=1e-12, atol=1e-9)
+            np.testing.assert_allclose(al[on, ax], amax[ax] * (1.0 - dist), rtol=1e-12, atol=1e-12)
+    # the reference's acceleration coefficients follow the profiles it is given
+    assert not np.array_equal(ref.coef_abar, grad.coef_abar)

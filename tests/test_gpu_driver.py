@@ -192,3 +192,41 @@ def test_cli_runs_the_reference_command_line(tmp_path):
     miss = subprocess.run([CLI, "--direction", "x", "--quiet"], cwd=empty, capture_output=True, text=True,
                           timeout=60)
     assert miss.returncode == 1 and "config.h5" in miss.stderr
+
+
+def test_forward_run_with_complete_cpml(tmp_path):
+    """--cpml-complete: the driver reads /field/cell/pml_K, pml_d, pml_alpha
+    from the partitions and runs the complete C-PML; records against the
+    oracle's complete C-PML (rigid outer boundary as zero mass), and the CLI
+    flag gives the same files bit for bit."""
+    m = mc.small_halfspace(nx=8, ny=8, nz=5, duration=2.0, want_profiles=1, pml_profile=1)
+    stf = pygfx.ricker(1.0, 1.5, 1e20, m.dt, m.nsteps)
+    xyz = (0.5 * (m.bounds[0] + m.bounds[1]), 0.5 * (m.bounds[2] + m.bounds[3]), 0.0)
+    pygfx.write_inputs(m, tmp_path, xyz, stf, n_parts=2)
+    print(f"forward driver, complete C-PML: {m.n_elem} elements, {m.nsteps} steps", flush=True)
+    pygfx.run_forward(str(tmp_path), directions="z", cpml_complete=True)
+    steps = snapshot_steps(m)
+    rec = read_records(tmp_path, "z", steps)
+    cells, w = m.locate_source(*xyz)
+    ref = pyoracle.Run(mc.cpml_complete_arrays(m), m.n_elem, 5, m.n_node, m.dt, m.nsteps, m.snapshot_stride, 2,
+                       cells, w, stf, m.rec_cell, cpml_complete=True).run(uva=True)
+    old = oracle_records(m, stf, xyz, 2)
+    n = len(steps)
+    for k, r in (("strain", "strain"), ("displacement", "disp"), ("velocity", "vel")):
+        e = mc.rel_l2(rec[k], ref[r][:n])
+        print(f"driver {k} rel L2 vs the complete C-PML oracle = {e:.3e}, vs the reference C-PML's "
+              f"{mc.rel_l2(rec[k], old[r][:n]):.3e}")
+        assert e <= TOL
+    cli_dir = tmp_path / "cli"
+    pygfx.write_inputs(m, cli_dir, xyz, stf, n_parts=1)
+    r = subprocess.run([CLI, "--direction", "z", "--quiet", "--cpml-complete"], cwd=cli_dir, capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    rec_cli = read_records(cli_dir, "z", steps)
+    for k in rec:
+        assert np.array_equal(rec_cli[k], rec[k]), k
+    # inputs without the profiles are refused
+    bare = tmp_path / "bare"
+    pygfx.write_inputs(mc.small_halfspace(nx=8, ny=8, nz=5, duration=0.2), bare, xyz, stf[:16], n_parts=1)
+    with pytest.raises(pygfx.GfxError, match="pml_K"):
+        pygfx.run_forward(str(bare), directions="z", cpml_complete=True)

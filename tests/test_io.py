@@ -186,6 +186,24 @@ def test_inputs_roundtrip_through_the_merge(tmp_path, n_parts):
         assert d["n_rank_node"] == rib.max() + 1 == np.unique(gib).size
 
 
+def test_stretch_profiles_roundtrip(tmp_path):
+    """pml_K, pml_d, pml_alpha [n_cell, 125, 3] (the datasets pml_cpml.py
+    computes) are written when generated and merged back by cell id."""
+    m = small_box(want_profiles=1, pml_profile=1)
+    stf = pygfx.ricker(1.0, 0.5, 1e20, m.dt, m.nsteps)
+    pygfx.write_inputs(m, tmp_path, (3000.0, 2500.0, 0.0), stf, n_parts=2)
+    assert "/field/cell/pml_K f64 [" in pygfx.h5_describe(tmp_path / "partitions" / "partition_0.h5")
+    m2 = pygfx.read_inputs(tmp_path)
+    ids = np.concatenate([pygfx.h5_read(tmp_path / "partitions" / f"partition_{r}.h5", "/partition/local_cell_ids")
+                          for r in range(2)]).astype(np.int64)
+    for k in ("pml_K", "pml_d", "pml_alpha"):
+        assert np.array_equal(m.arrays[k].reshape(m.n_elem, -1)[ids], m2.arrays[k].reshape(m.n_elem, -1)), k
+    bare = tmp_path / "bare"
+    pygfx.write_inputs(small_box(), bare, (3000.0, 2500.0, 0.0), stf, n_parts=1)
+    assert "pml_K" not in pygfx.h5_describe(bare / "partitions" / "partition_0.h5")
+    assert pygfx.read_inputs(bare).arrays["pml_K"] is None
+
+
 def test_partitions_are_centroid_slabs_along_the_longest_axis(tmp_path):
     """Without METIS the reference sorts cell centroids along the axis of
     largest spread and cuts the order into chunks of ceil(E / n)
