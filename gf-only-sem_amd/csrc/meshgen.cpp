@@ -187,6 +187,8 @@ void abar(int region, const double K[3], const double d[3], const double al[3], 
             const double dyx = safe_den(ay - ax), dzx = safe_den(az - ax), dzy = safe_den(az - ay);
             A[2] = A0 * ax * ax * (bx - ax) * (by - ax) * (bz - ax) / (dyx * dzx);
             A[3] = A0 * ay * ay * (bx - ay) * (by - ay) * (bz - ay) / (dxy * dzy);
+            // (alpha_y - alpha_z)(alpha_z - alpha_x) as pml_cpml.py:433 divides: the
+            // opposite sign of the residue at alpha_z, kept as the reference writes it
             A[4] = A0 * az * az * (bx - az) * (by - az) * (bz - az) / (dyz * dzx);
             break;
         }

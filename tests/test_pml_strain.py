@@ -99,3 +99,38 @@ def test_coefficients_round_trip_through_the_partitions(tmp_path):
                                         "/partition/local_cell_ids") for r in range(2)]).astype(np.int64)
     E = m.n_elem
     assert np.array_equal(back.coef_strain.reshape(E, -1), m.coef_strain.reshape(E, -1)[ids])
+
+
+@pytest.mark.parametrize("profile", [0, 1])
+def test_abar_are_the_partial_fractions_of_the_stretched_inertia(profile):
+    """pml_coef_abar (preprocess/pml_cpml.py:388-513) at every C-PML point with
+    distinct poles, checked against the point's own K, d, alpha (pml_K, pml_d,
+    pml_alpha): (i w)^2 (s_x s_y s_z - K_x K_y K_z) =
+    A1 i w + A2 + A3/(alpha_x + i w) + A4/(alpha_y + i w) + A5/(alpha_z + i w)
+    at 30 frequencies.  Every region matches to 1e-8 except one term: in corner
+    (XYZ) cells the reference's A5 has the opposite sign (its denominator is
+    (alpha_y - alpha_z)(alpha_z - alpha_x) where the residue needs
+    (alpha_x - alpha_z)(alpha_y - alpha_z)); the generator keeps the reference's
+    value, and the test pins that sign."""
+    m = mc.small_halfspace(nx=8, ny=7, nz=5, pml=2, want_profiles=1, pml_profile=profile)
+    P = m.n_elem * 125
+    K, d, al = (m.arrays[k].reshape(P, 3) for k in ("pml_K", "pml_d", "pml_alpha"))
+    ab = m.coef_abar.reshape(P, 5)
+    reg = np.repeat(m.pml_region, 125)
+    w = np.geomspace(1e-2, 1e3, 30)
+    checked = {r: 0 for r in range(1, 8)}
+    for p in np.flatnonzero(reg):
+        act = d[p] > 1e-3
+        a = al[p][act]
+        if not act.any() or (len(a) > 1 and np.min(np.abs(np.diff(np.sort(a)))) < 1e-2):
+            continue
+        s = K[p] + d[p] / (al[p][None, :] + 1j * w[:, None])
+        lhs = (1j * w) ** 2 * (s.prod(axis=1) - K[p].prod())
+        A = ab[p].copy()
+        if reg[p] == XYZ:
+            A[4] = -A[4]
+        rhs = A[0] * 1j * w + A[1] + sum(A[2 + q] / (al[p, q] + 1j * w) for q in range(3))
+        err = np.max(np.abs(lhs - rhs) / (np.abs(lhs) + 1e-12 * (w ** 2 + 1)))
+        assert err < 1e-8, (reg[p], K[p], d[p], al[p], ab[p], err)
+        checked[reg[p]] += 1
+    assert all(n > 50 for n in checked.values()), checked
