@@ -910,8 +910,10 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
     }
 
     // ---- node classes: 2 = C-PML node, 1 = other special, 0 = regular ----
+    // (the complete C-PML keeps its terms in the element kernel: its PML nodes
+    // need no node-kernel memory and stay regular unless rigid or recorded)
     std::vector<char> cls(G, 0);
-    if (S.has_cpml) {
+    if (S.has_cpml && !S.cpml_complete) {
         for (int e = 0; e < E; ++e)
             if (d.pml_region[e] != 0)
                 for (int p = 0; p < NP; ++p) cls[d.ibool[int64_t(e) * NP + p]] = 2;
