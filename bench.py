@@ -176,20 +176,20 @@ def element_lds(launch, n_elem, n_fields, ms_elem):
             "frac_of_measured_rate": tbs / LDS_MEASURED_TBS}
 
 
-def uneven_specs(pygfx, n=16, seed=4):
+def uneven_specs(pygfx, n=16, seed=4, lo=8e3, hi=2e5, pml=(3, 6)):
     from pygfx import workloads
-    return workloads.uneven_specs(n=n, seed=seed)
+    return workloads.uneven_specs(n=n, seed=seed, lo=lo, hi=hi, pml=pml)
 
 
-def uneven_workload(pygfx, steps, warmup, group_counts=(0, 1, 2, 4)):
-    """x, y and z forces at the surface centre of each of 16 uneven meshes:
+def uneven_workload(pygfx, steps, warmup, group_counts=(0, 1, 2, 4), n=16, lo=8e3, hi=2e5, pml=(3, 6)):
+    """x, y and z forces at the surface centre of each of n uneven meshes:
     the meshes advanced together as one captured graph per step, for each
     number of launch groups in group_counts (0: one branch of launches per
     mesh; k: k branches, each sharing one launch per stage among its meshes),
     against the same solvers advanced one after another, kernel by kernel.
     value is the fastest batched form."""
     t0 = time.perf_counter()
-    meshes = [pygfx.build_box(sp, copy=False) for sp in uneven_specs(pygfx)]
+    meshes = [pygfx.build_box(sp, copy=False) for sp in uneven_specs(pygfx, n=n, lo=lo, hi=hi, pml=pml)]
     solvers = []
     for m in meshes:
         s = pygfx.Solver(m, n_fields=3, assembly=pygfx.GFX_ASSEMBLY_GATHER, record_capacity=2)
@@ -565,6 +565,10 @@ def main():
     if not args.no_uneven and not args.small:
         result["uneven_batch"] = uneven_workload(pygfx, args.steps, args.warmup)
         log("uneven:", json.dumps(result["uneven_batch"]))
+        # many small meshes: per-launch cost dominates each mesh's step
+        result["uneven_batch_small"] = uneven_workload(pygfx, args.steps, args.warmup, n=64, lo=500.0, hi=4000.0,
+                                                       pml=(1, 2))
+        log("uneven small:", json.dumps(result["uneven_batch_small"]))
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline()

@@ -46,11 +46,11 @@ def stations(mesh, n=8, seed=3, half=20000.0):
     return [(cx + rng.uniform(-half, half), cy + rng.uniform(-half, half)) for _ in range(n)]
 
 
-def uneven_specs(n=16, seed=4, h=600.0, lo=8e3, hi=2e5):
+def uneven_specs(n=16, seed=4, h=600.0, lo=8e3, hi=2e5, pml=(3, 6)):
     """n independent boxes (seeded): n_target log-uniform in [lo, hi] elements,
     nx = ny = round((2n)^(1/3)), nz = round(nx/2), 0.6 km hexes; 2-4 layers with
     Vs ~ U(1500, 4500) increasing with depth, Vp = 1.73 Vs, rho ~ U(2300, 3300);
-    C-PML U{3..6}; run length U(5, 30) s; Ricker f0 = 1.5 Hz."""
+    C-PML U{pml[0]..pml[1]}; run length U(5, 30) s; Ricker f0 = 1.5 Hz."""
     rng = np.random.default_rng(seed)
     specs = []
     for _ in range(n):
@@ -64,10 +64,10 @@ def uneven_specs(n=16, seed=4, h=600.0, lo=8e3, hi=2e5):
         tops = np.sort(rng.uniform(0.1, 0.7, nl - 1)) * lz
         layers = [(float(tops[i]), 1.73 * vs[i], vs[i], rho[i]) for i in range(nl - 1)]
         layers.append((0.0, 1.73 * vs[-1], vs[-1], rho[-1]))
-        pml = int(rng.integers(3, 7))
+        npml = int(rng.integers(pml[0], pml[1] + 1))
         dur = float(rng.uniform(5.0, 30.0))
         specs.append(make_spec(nx=nx, ny=nx, nz=nz, lx=h * nx, ly=h * nx, lz=lz, layers=layers,
-                               pml_xmin=pml, pml_xmax=pml, pml_ymin=pml, pml_ymax=pml, pml_zmax=pml,
+                               pml_xmin=npml, pml_xmax=npml, pml_ymin=npml, pml_ymax=npml, pml_zmax=npml,
                                f0_pml=1.5, cfl_safety=0.5, output_dt=0.05, duration=dur,
                                record_depth_max=2000.0, want_coords=0, want_beta=0))
     return specs

@@ -16,7 +16,8 @@ gfo_cpml_inertia, pinned in tests/test_cpml_complete.py).  Gates:
     motion, so the final state is not the gate);
   * several fields in one launch equal the single-field runs bit for bit, and a
     checkpoint taken mid-run resumes bit for bit (the 45 memories per PML point
-    and field are part of the state);
+    and field are part of the state); graph replays and the stage-by-stage
+    device loop equal the fused run bit for bit;
   * the oracle is given the rigid outer boundary as zero mass at the nodes on
     the box faces that carry a layer (the same set as the solver's, which
     tests/test_cpml_complete.py checks);
@@ -137,6 +138,44 @@ def test_fields_and_checkpoint_are_bitwise():
     for f in range(2):
         for x, y in zip(a.get_state(f), b.get_state(f)):
             assert np.array_equal(x, y)
+
+
+def test_graph_replays_and_stage_driven_steps_equal_the_run():
+    """A captured step replayed in chunks, with state reads between the
+    replays, and the reference's stage-by-stage device loop
+    (solver.cpp:390-444: predict, gather, zero, element residual, pml damping,
+    source, scatter, correct) both equal gfx_solver_run bit for bit."""
+    mesh = _halfspace(duration=1.5)
+    c, w, stf = mc.surface_centre_source(mesh)
+    srcs = [(c, w, stf, 2)]
+    base = _gpu(mesh, srcs)
+    graph = _gpu(mesh, srcs)
+    graph.set_graph(True)
+    done = 0
+    for chunk in (1, 5, 3, 17, mesh.nsteps):
+        chunk = min(chunk, mesh.nsteps - done)
+        if chunk <= 0:
+            break
+        base.run(chunk)
+        graph.run(chunk)
+        done += chunk
+        for x, y in zip(graph.get_state(0), base.get_state(0)):
+            assert np.array_equal(x, y)
+    staged = _gpu(mesh, srcs)
+    dt = mesh.dt
+    for step in range(mesh.nsteps):
+        staged.stage("newmark_predict", dt, 0.0)
+        staged.stage("gather_predicted")
+        staged.stage("zero_residual")
+        staged.stage("element_residual", 5, mesh.n_elem)
+        staged.stage("pml_damping")
+        val = float(stf[step]) if step < len(stf) else 0.0
+        if val != 0.0:
+            staged.stage("source_injection", 0, 2, val, np.asarray(w, np.float64), len(c))
+        staged.stage("scatter_to_rank")
+        staged.stage("newmark_correct", dt, 0.0, 0.5)
+    for x, y in zip(staged.get_state(0), base.get_state(0)):
+        assert np.array_equal(x, y)
 
 
 def _interior_energy(mesh, s, field=0):
