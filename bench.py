@@ -438,7 +438,7 @@ def main():
 
     t0 = time.perf_counter()
     dims = (40, 40, 20) if args.small else (100, 100, 50)
-    want_cpml = not args.no_cpml_complete and args.attenuation == 0
+    want_cpml = not args.no_cpml_complete
     mesh = pygfx.build_box(cfg2_spec(pygfx, *dims, n_sls=args.attenuation, profiles=want_cpml), copy=False)
     log(f"mesh: {mesh.n_elem} elements, {mesh.n_node} nodes, {mesh.n_pml_elem} PML elements, "
         f"{len(mesh.rec_cell)} recording cells, dt={mesh.dt:.5g} s, stride={mesh.snapshot_stride}, "

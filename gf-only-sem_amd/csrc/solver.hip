@@ -431,7 +431,7 @@ void launch_att(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
 
 void launch_element(gfx_solver& S, const dev::ElemArgs& a, const ElemRange& rg) {
     const int b = rg.b, n = rg.e - rg.b;
-    if (S.n_sls > 0) {  // shear attenuation: one kernel form for every range
+    if (S.n_sls > 0 && !rg.pml) {  // shear attenuation: one kernel form for every range
         if (n > 0) {
             switch (S.assembly) {
                 case GFX_ASSEMBLY_GATHER: launch_att<dev::ASM_GATHER>(S, a, b, n); break;
@@ -441,7 +441,7 @@ void launch_element(gfx_solver& S, const dev::ElemArgs& a, const ElemRange& rg) 
         }
         return;
     }
-    if (rg.pml) {  // complete C-PML: one element per workgroup, general form
+    if (rg.pml) {  // complete C-PML: one element per workgroup, general form (elastic)
         if (n > 0) {
             switch (S.assembly) {
                 case GFX_ASSEMBLY_GATHER: dev::k_element_cpml<dev::ASM_GATHER><<<n, 128, 0, S.stream>>>(a, b, n); break;
@@ -795,7 +795,6 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
         require(S.has_cpml, "cpml_complete needs has_cpml");
         require(d.pml_region && d.pml_K && d.pml_d && d.pml_alpha,
                 "cpml_complete needs pml_region, pml_K, pml_d and pml_alpha");
-        require(d.n_sls == 0, "the complete C-PML does not combine with shear attenuation");
     } else if (S.has_cpml) {
         require(d.pml_region && d.coef_alpha && d.coef_abar && d.density,
                 "has_cpml needs pml_region, coef_alpha, coef_abar and density");
@@ -1043,7 +1042,9 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
                     hb[size_t(l) * P + dst] = solid ? float(Y * (q - a)) : 0.f;
                     hc[size_t(l) * P + dst] = solid ? float(Y * (1.0 - q)) : 0.f;
                 }
-                if (mu_r > 0.0) {
+                // the complete C-PML's elements stay elastic with the relaxed moduli
+                const bool pml_elastic = S.cpml_complete && d.pml_region[e] != 0;
+                if (mu_r > 0.0 && !pml_elastic) {
                     const double mu_u = mu_r * s_u;
                     hmu[dst] = float(mu_u);
                     hlam[dst] = float(d.lambda_[src] + 2.0 * (mu_r - mu_u) / 3.0);

@@ -157,7 +157,7 @@ typedef struct {
     int n_sls;
     const double* tau_sigma;
     const double* tau_epsilon;
-    /* Complete C-PML (1; needs has_cpml and n_sls == 0).  Not in the reference,
+    /* Complete C-PML (1; needs has_cpml).  Not in the reference,
        whose solver applies only the acceleration part of its C-PML
        (forward/share/src/pml.cpp) and defers the strain part
        (docs/deferred.md:46-80).  From the stretch profiles
@@ -168,7 +168,9 @@ typedef struct {
        gains K_x K_y K_z and the inertia terms of (i w)^2 s_x s_y s_z enter the
        residual.  coef_alpha and coef_abar are then not used, and the node
        kernels apply no C-PML term.  The memories (45 per PML point and field)
-       are advanced exactly for inputs linear in time over a step. */
+       are advanced exactly for inputs linear in time over a step.  With
+       n_sls > 0 the interior elements attenuate and the PML elements stay
+       elastic with the relaxed moduli. */
     int cpml_complete;
     const double* pml_K;
     const double* pml_d;

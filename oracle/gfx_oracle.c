@@ -692,6 +692,7 @@ typedef struct {
     double *pold, *pnew, *rmem;
     double *zeta, *devp;
     double *ccoef, *ccarry; /* complete C-PML: [P][27] coefficients, [P][45] memories */
+    double *mu_in, *mu_pml;  /* complete C-PML with attenuation: mu zeroed in the PML / interior elements */
     double gll_x[16], gll_w[16], D[256];
     gfo_cpml pml;
 } run_ws;
@@ -702,6 +703,7 @@ static void ws_free(run_ws* w) {
     free(w->pold); free(w->pnew); free(w->rmem);
     free(w->zeta); free(w->devp);
     free(w->ccoef); free(w->ccarry);
+    free(w->mu_in); free(w->mu_pml);
 }
 
 static int ws_init(run_ws* w, const gfo_problem* pb) {
@@ -727,7 +729,7 @@ static int ws_init(run_ws* w, const gfo_problem* pb) {
         !w->damp_g)
         return -2;
     if (pb->cpml_complete) {
-        if (!pb->pml_region || !pb->pml_K || !pb->pml_d || !pb->pml_alpha || pb->n_sls > 0) return -1;
+        if (!pb->pml_region || !pb->pml_K || !pb->pml_d || !pb->pml_alpha) return -1;
         w->ccoef = calloc(P * 27, sizeof(double));
         w->ccarry = calloc(P * 45, sizeof(double));
         if (!w->ccoef || !w->ccarry) return -2;
@@ -735,6 +737,19 @@ static int ws_init(run_ws* w, const gfo_problem* pb) {
             if (pb->pml_region[p / n_node] != 0)
                 gfo_cpml_point_coefficients(pb->pml_K + p * 3, pb->pml_d + p * 3, pb->pml_alpha + p * 3, pb->dt,
                                             w->ccoef + p * 27);
+        if (pb->n_sls > 0) {
+            /* attenuation in the interior elements, elastic (relaxed moduli) in the
+             * PML elements: each residual runs with mu zeroed where the other applies
+             * (a point with mu <= 0 contributes nothing) */
+            w->mu_in = malloc(sizeof(double) * P);
+            w->mu_pml = malloc(sizeof(double) * P);
+            if (!w->mu_in || !w->mu_pml) return -2;
+            for (long p = 0; p < P; ++p) {
+                const int pml = pb->pml_region[p / n_node] != 0;
+                w->mu_in[p] = pml ? 0.0 : pb->mu_[p];
+                w->mu_pml[p] = pml ? pb->mu_[p] : 0.0;
+            }
+        }
     }
     /* global mass sums, damping "last writer wins" (solver.cpp:193-202); the
      * complete C-PML weights the mass of PML element points by
@@ -796,8 +811,13 @@ static void run_one_step(run_ws* w, const gfo_problem* pb, int step) {
     if (pb->cpml_complete) {
         /* stretched stresses in PML elements, then the inertia terms with the
          * predicted u and v_{n+1/2} = v_n + dt/2 a_n */
-        gfo_element_residual_cpml(pb->n_elem, pb->dxi_dx, pb->jacobian, pb->lambda_, pb->mu_, pb->pml_region,
-                                  w->ccoef, w->D, w->gll_w, ngll, w->uloc, w->ccarry, w->rloc);
+        if (pb->n_sls > 0)
+            gfo_element_residual_att(pb->n_elem, pb->dxi_dx, pb->jacobian, pb->lambda_, w->mu_in, pb->n_sls,
+                                     pb->tau_sigma, pb->tau_epsilon, dt, w->D, w->gll_w, ngll, w->uloc, w->zeta,
+                                     w->devp, w->rloc);
+        gfo_element_residual_cpml(pb->n_elem, pb->dxi_dx, pb->jacobian, pb->lambda_,
+                                  pb->n_sls > 0 ? w->mu_pml : pb->mu_, pb->pml_region, w->ccoef, w->D, w->gll_w,
+                                  ngll, w->uloc, w->ccarry, w->rloc);
 #pragma omp parallel for schedule(static)
         for (long e = 0; e < pb->n_elem; ++e) {
             if (pb->pml_region[e] == 0) continue;

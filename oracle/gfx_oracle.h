@@ -141,7 +141,8 @@ typedef struct {
     const double* tau_sigma;    /* [E*n_node*n_sls] stress relaxation times (s) */
     const double* tau_epsilon;  /* [E*n_node*n_sls] strain relaxation times (s) */
     /* complete C-PML (1): replaces the reference's acceleration-only C-PML and
-     * the legacy ramp; PML elements are those with pml_region != 0 */
+     * the legacy ramp; PML elements are those with pml_region != 0 (elastic,
+     * relaxed moduli, when n_sls > 0) */
     int cpml_complete;
     const double* pml_K;        /* [E*n_node*3] stretch profiles per axis */
     const double* pml_d;
@@ -197,7 +198,9 @@ void gfo_sls_point_history(int n_sls, const double* tau_sigma, const double* tau
  * C3 = conv(z2, alpha_z); with D = d/dt and D conv(g, p) = g - p conv(g, p):
  *   L = K_z L2 + d_z (D z2 - alpha_z z2 + alpha_z^2 C3),
  *   L2 = K_y L1 + d_y (D z1 - alpha_y z1 + alpha_y^2 C2),  L1 = d_x (v - alpha_x u + alpha_x^2 C1).
- * In the step u is the predicted displacement (u_{n+1}).
+ * In the step u is the predicted displacement (u_{n+1}).  With n_sls > 0 the
+ * interior elements take the attenuating residual and the PML elements stay
+ * elastic with the relaxed moduli.
  * Coefficients per point (27): axis q at [9q]: K, d, alpha, (a, b, c) of
  * alpha, (a, b, c) of beta (zero where d == 0).  Memories per point (45):
  * [(k*3+l)*4 + {conv(g, alpha_m), conv(g, alpha_n), conv(conv(g, alpha_m),

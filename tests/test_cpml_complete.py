@@ -258,3 +258,23 @@ def test_generator_profiles():
             np.testing.assert_allclose(al[on, ax], amax[ax] * (1.0 - dist), rtol=1e-12, atol=1e-12)
     # the reference's acceleration coefficients follow the profiles it is given
     assert not np.array_equal(ref.coef_abar, grad.coef_abar)
+
+
+def test_attenuation_with_no_pml_elements_is_the_attenuating_run():
+    """Complete C-PML with attenuation: the interior elements take the
+    attenuating residual (mu zeroed in the PML) and the PML elements the
+    stretched elastic one (mu zeroed inside).  With no PML element the run is
+    the plain attenuating run bit for bit."""
+    mesh = mc.small_layered(nx=5, ny=5, nz=4, pml=1, duration=0.4, n_sls=3, q_mu=[20.0, 20.0, 20.0],
+                            att_fmin=0.1, att_fmax=4.0, want_profiles=1)
+    c, w, stf = mc.buried_source(mesh)
+    arr = dict(mesh.arrays)
+    arr["pml_region"] = np.zeros_like(arr["pml_region"])
+    arr["damping"] = np.zeros_like(arr["damping"])
+    args = (arr, mesh.n_elem, 5, mesh.n_node, mesh.dt, mesh.nsteps, mesh.snapshot_stride, 2, c, w, stf,
+            mesh.rec_cell)
+    att = dict(tau_sigma=mesh.tau_sigma, tau_epsilon=mesh.tau_epsilon)
+    a = po.Run(*args, has_cpml=False, **att).run()
+    b = po.Run(*args, has_cpml=True, cpml_complete=True, **att).run()
+    for k in ("strain", "u", "v", "a"):
+        assert np.array_equal(a[k], b[k]), k

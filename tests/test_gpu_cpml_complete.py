@@ -116,6 +116,35 @@ def test_jittered_layered_buried_source_matches_oracle():
     _check(mesh, c, w, stf, 0, pygfx.GFX_ASSEMBLY_GATHER, mesh.nsteps // 2)
 
 
+@pytest.mark.parametrize("assembly", [pygfx.GFX_ASSEMBLY_GATHER, pygfx.GFX_ASSEMBLY_COLORED])
+def test_with_attenuation_matches_oracle(assembly):
+    """Shear attenuation (3 solids, Q 20) in the interior elements, the PML
+    elements elastic with the relaxed moduli: the GPU against the oracle's
+    same combination, and far from the elastic complete run."""
+    mesh = mc.small_layered(nx=7, ny=6, nz=5, pml=2, duration=2.0, n_sls=3, q_mu=[20.0, 20.0, 20.0],
+                            att_fmin=0.1, att_fmax=4.0, want_profiles=1, pml_profile=1)
+    c, w, stf = mc.buried_source(mesh)
+    print(f"case: complete C-PML + attenuation, assembly={assembly}, {mesh.n_elem} elements", flush=True)
+    s = _gpu(mesh, [(c, w, stf, 2)], assembly, record_uva=True)
+    assert int(s.query("n_sls")) == 3 and int(s.query("cpml_complete")) == 1
+    n_mid = mesh.nsteps // 2
+    s.run(n_mid)
+    arr = mc.cpml_complete_arrays(mesh)
+    run = lambda n, **kw: pyoracle.Run(arr, mesh.n_elem, 5, mesh.n_node, mesh.dt, n, mesh.snapshot_stride, 2, c, w,
+                                       stf, mesh.rec_cell, cpml_complete=True, **kw).run()
+    att = dict(tau_sigma=mesh.tau_sigma, tau_epsilon=mesh.tau_epsilon)
+    mid = run(n_mid, **att)
+    ela = run(n_mid)
+    errs = [mc.rel_l2(x, mid[k]) for x, k in zip(s.get_state(0), ("u", "v", "a"))]
+    gap = mc.rel_l2(ela["u"], mid["u"])
+    s.run(mesh.nsteps - n_mid)
+    full = run(mesh.nsteps, **att)
+    errs.append(mc.rel_l2(_records(s), full["strain"]))
+    print("rel L2 vs oracle: u, v, a %.3e %.3e %.3e, strain records %.3e; elastic vs attenuated u %.3e"
+          % (*errs, gap), flush=True)
+    assert max(errs) <= TOL and gap > 100 * TOL
+
+
 def test_fields_and_checkpoint_are_bitwise():
     mesh = _halfspace(duration=2.0)
     zc, zw, zstf = mc.surface_centre_source(mesh)
