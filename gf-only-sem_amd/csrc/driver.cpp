@@ -836,7 +836,7 @@ int run_green_tiles_impl(const gfx_green_tile_options& o, int* n_tiles_out) {
     try {
         log.info("single process, Green's function tiles, directions x,y,z -> " + out.string());
         RunInputs in;
-        load_inputs(wd, log, in);
+        load_inputs(wd, log, in, o.cpml_complete != 0);
         const io::Config& cfg = in.cfg;
         const io::RecordingMap& rm = in.part.recording;
         require(rm.has_recording && !rm.gll_node_ids.empty(), "the partitions carry no recording map");
@@ -907,7 +907,7 @@ int run_green_tiles_impl(const gfx_green_tile_options& o, int* n_tiles_out) {
         const int ring = std::max(1, o.ring > 0 ? o.ring : 8);
         const bool uva = o.with_uva != 0;
         gfx_solver_desc d = solver_desc(in, 3, std::min(ring, nt), uva, o.assembly, o.element_kernel,
-                                        o.elems_per_block, log);
+                                        o.elems_per_block, log, o.cpml_complete != 0);
         gfx_solver* S = nullptr;
         check(gfx_solver_create(&d, &S), "solver create");
         std::unique_ptr<gfx_solver, gfx_status (*)(gfx_solver*)> guard(S, gfx_solver_destroy);

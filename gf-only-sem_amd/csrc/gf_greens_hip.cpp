@@ -1,7 +1,7 @@
 // gf_greens_hip -- Green's function tiles and GMSH conversion from the command line.
 //
 //   gf_greens_hip [model.h5] [-o DIR] [--assembly {auto,colored,gather,atomic}]
-//                 [--no-uva] [--ring N] [--workdir DIR] [--quiet]
+//                 [--no-uva] [--ring N] [--workdir DIR] [--quiet] [--cpml-complete]
 //       the reference's three forward runs plus its post-processor
 //       (postprocess/cpp/main.cpp:39-89 command line: model.h5, output
 //       directory -o, default greenfun) as one device run over config.h5 +
@@ -20,7 +20,7 @@
 
 static void usage(const char* prog) {
     std::cerr << "Usage: " << prog << " [model.h5] [-o DIR] [--assembly {auto,colored,gather,atomic}] [--no-uva]\n"
-              << "                 [--ring N] [--workdir DIR] [--quiet]\n"
+              << "                 [--ring N] [--workdir DIR] [--quiet] [--cpml-complete]\n"
               << "       " << prog << " --gmsh input.msh [-o model.h5] [--aux aux.h5]\n"
               << "  Input:  config.h5, partitions/partition_{r}.h5 (working directory)\n"
               << "  Output: DIR/tile_x{tx:03d}_y{ty:03d}.h5 (default DIR: greenfun)\n";
@@ -67,6 +67,8 @@ int main(int argc, char** argv) {
             workdir = next(a);
         } else if (!std::strcmp(a, "--quiet")) {
             o.verbose = 0;
+        } else if (!std::strcmp(a, "--cpml-complete")) {
+            o.cpml_complete = 1;
         } else if (!std::strcmp(a, "-h") || !std::strcmp(a, "--help")) {
             usage(argv[0]);
             return 0;

@@ -125,6 +125,7 @@ class GreenTileOptions(C.Structure):
         ("workdir", C.c_char_p), ("output_dir", C.c_char_p), ("model_path", C.c_char_p),
         ("with_uva", C.c_int), ("assembly", C.c_int), ("element_kernel", C.c_int),
         ("elems_per_block", C.c_int), ("ring", C.c_int), ("verbose", C.c_int),
+        ("cpml_complete", C.c_int),
     ]
 
 
@@ -558,7 +559,8 @@ def run_forward(workdir=None, directions="z", stations=None, resume=False, recor
 
 
 def run_green_tiles(workdir=None, output_dir=None, model_path=None, with_uva=True, assembly=-1,
-                    element_kernel=GFX_ELEMENT_AUTO, elems_per_block=0, ring=8, verbose=False) -> int:
+                    element_kernel=GFX_ELEMENT_AUTO, elems_per_block=0, ring=8, verbose=False,
+                    cpml_complete=False) -> int:
     """Green's function tiles straight from the device (gfx_run_green_tiles); returns the tile count."""
     o = GreenTileOptions()
     _check(lib.gfx_green_tile_options_default(C.byref(o)))
@@ -570,6 +572,7 @@ def run_green_tiles(workdir=None, output_dir=None, model_path=None, with_uva=Tru
     o.elems_per_block = elems_per_block
     o.ring = ring
     o.verbose = 1 if verbose else 0
+    o.cpml_complete = 1 if cpml_complete else 0
     n = C.c_int(0)
     _check(lib.gfx_run_green_tiles(C.byref(o), C.byref(n)))
     return n.value

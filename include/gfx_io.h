@@ -73,6 +73,7 @@ typedef struct {
     int elems_per_block;    /* 0 = default (ignored when autotuning) */
     int ring;               /* snapshots the device holds between drains (0 = 8) */
     int verbose;            /* echo log lines (log/forward_greens_0.log) to stdout */
+    int cpml_complete;      /* the complete C-PML, as gfx_forward_options.cpml_complete */
 } gfx_green_tile_options;
 
 gfx_status gfx_green_tile_options_default(gfx_green_tile_options* o);
