@@ -719,7 +719,13 @@ __device__ __forceinline__ float conv_step(const float* abc, float& q, float g) 
 // predicted displacement and v = w_n = v_n + dt/2 a_n; the rest of the
 // velocity terms at v_{n+1} (dt/2 a_{n+1} times their coefficient) is in the
 // lumped mass, as is K_x K_y K_z.
-template <int MODE>
+// AX: the axes (bit 0 x, 1 y, 2 z) whose d is nonzero at some point of the
+// launch's elements.  The memories of the other axes are known to stay zero
+// at compile time, so they hold no registers (a single-axis layer keeps 12 of
+// the 45).  The arithmetic is the same for every AX: the terms of an axis
+// with d = 0 are exact zeros either way, so the residuals do not depend on
+// how elements are grouped.
+template <int MODE, int AX>
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3, 3)))
 k_element_cpml(ElemArgs A, int e0, int ne) {
     __shared__ float sD[25];
@@ -765,18 +771,12 @@ k_element_cpml(ElemArgs A, int e0, int ne) {
     const int64_t rslot = residual_entry<MODE>(A, pt, valid);
     // a memory whose axis has d = 0 has zero recursion coefficients and stays
     // zero: it is neither read nor written (single-axis layers use 12 of 45)
-    const bool on[3] = {cf[1] != 0.f, cf[10] != 0.f, cf[19] != 0.f};
+    const bool on[3] = {(AX & 1) && cf[1] != 0.f, (AX & 2) && cf[10] != 0.f, (AX & 4) && cf[19] != 0.f};
     __syncthreads();
-    float Di[5], Dj[5], Dk[5], Dti[5], Dtj[5], Dtk[5];
-#pragma unroll
-    for (int s = 0; s < 5; ++s) {
-        Di[s] = sD[i * 5 + s];
-        Dj[s] = sD[j * 5 + s];
-        Dk[s] = sD[k * 5 + s];
-        Dti[s] = sD[s * 5 + i];
-        Dtj[s] = sD[s * 5 + j];
-        Dtk[s] = sD[s * 5 + k];
-    }
+    // rows of D read from LDS where they are used (registers go to the memories)
+    const float* Di = &sD[i * 5];
+    const float* Dj = &sD[j * 5];
+    const float* Dk = &sD[k * 5];
     for (int f = 0; f < A.nf; ++f) {
         const int64_t fo = int64_t(f) * A.fstride;
         if (valid) {
@@ -850,7 +850,16 @@ k_element_cpml(ElemArgs A, int e0, int ne) {
         __syncthreads();
         if (valid) {
             float rv[3];
-            point_div(&sf[0][0], NP, Dti, Dtj, Dtk, i, j, k, rv);
+            {
+                float Dti[5], Dtj[5], Dtk[5];
+#pragma unroll
+                for (int s = 0; s < 5; ++s) {
+                    Dti[s] = sD[s * 5 + i];
+                    Dtj[s] = sD[s * 5 + j];
+                    Dtk[s] = sD[s * 5 + k];
+                }
+                point_div(&sf[0][0], NP, Dti, Dtj, Dtk, i, j, k, rv);
+            }
             const CpmlCoef cx{scf, p, 0}, cy{scf, p, 9}, cz{scf, p, 18};
 #pragma unroll
             for (int c = 0; c < 3; ++c) {

@@ -33,6 +33,7 @@ struct ElemRange {
     int b, e;
     bool diag;
     bool pml;
+    int ax;  // complete C-PML: the axes (bit 0 x, 1 y, 2 z) with d != 0 at some point of the range's elements
 };
 
 struct FieldSource {
@@ -429,6 +430,20 @@ void launch_att(gfx_solver& S, const dev::ElemArgs& a, int b, int n) {
     }
 }
 
+template <int MODE>
+void launch_cpml(gfx_solver& S, const dev::ElemArgs& a, int ax, int b, int n) {
+    switch (ax) {
+        case 0: dev::k_element_cpml<MODE, 0><<<n, 128, 0, S.stream>>>(a, b, n); break;
+        case 1: dev::k_element_cpml<MODE, 1><<<n, 128, 0, S.stream>>>(a, b, n); break;
+        case 2: dev::k_element_cpml<MODE, 2><<<n, 128, 0, S.stream>>>(a, b, n); break;
+        case 3: dev::k_element_cpml<MODE, 3><<<n, 128, 0, S.stream>>>(a, b, n); break;
+        case 4: dev::k_element_cpml<MODE, 4><<<n, 128, 0, S.stream>>>(a, b, n); break;
+        case 5: dev::k_element_cpml<MODE, 5><<<n, 128, 0, S.stream>>>(a, b, n); break;
+        case 6: dev::k_element_cpml<MODE, 6><<<n, 128, 0, S.stream>>>(a, b, n); break;
+        default: dev::k_element_cpml<MODE, 7><<<n, 128, 0, S.stream>>>(a, b, n); break;
+    }
+}
+
 void launch_element(gfx_solver& S, const dev::ElemArgs& a, const ElemRange& rg) {
     const int b = rg.b, n = rg.e - rg.b;
     if (S.n_sls > 0 && !rg.pml) {  // shear attenuation: one kernel form for every range
@@ -444,9 +459,9 @@ void launch_element(gfx_solver& S, const dev::ElemArgs& a, const ElemRange& rg) 
     if (rg.pml) {  // complete C-PML: one element per workgroup, general form (elastic)
         if (n > 0) {
             switch (S.assembly) {
-                case GFX_ASSEMBLY_GATHER: dev::k_element_cpml<dev::ASM_GATHER><<<n, 128, 0, S.stream>>>(a, b, n); break;
-                case GFX_ASSEMBLY_ATOMIC: dev::k_element_cpml<dev::ASM_ATOMIC><<<n, 128, 0, S.stream>>>(a, b, n); break;
-                default: dev::k_element_cpml<dev::ASM_COLORED><<<n, 128, 0, S.stream>>>(a, b, n); break;
+                case GFX_ASSEMBLY_GATHER: launch_cpml<dev::ASM_GATHER>(S, a, rg.ax, b, n); break;
+                case GFX_ASSEMBLY_ATOMIC: launch_cpml<dev::ASM_ATOMIC>(S, a, rg.ax, b, n); break;
+                default: launch_cpml<dev::ASM_COLORED>(S, a, rg.ax, b, n); break;
             }
         }
     } else if (ref_elem_pml(S)) {
@@ -842,9 +857,19 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
         }
     }
     // complete C-PML: PML elements run their own kernel, after the others
-    std::vector<char> pml_el(E, 0);
+    // and are grouped by the axes their stretch profiles damp (d != 0 at some
+    // point), so that each launch carries only the memories those axes use
+    std::vector<char> pml_el(E, 0), pml_ax(E, 0);
     if (S.cpml_complete)
-        for (int e = 0; e < E; ++e) pml_el[e] = d.pml_region[e] != 0;
+        for (int e = 0; e < E; ++e) {
+            pml_el[e] = d.pml_region[e] != 0;
+            if (!pml_el[e] || !d.pml_d) continue;
+            int ax = 0;
+            for (int p = 0; p < NP; ++p)
+                for (int q = 0; q < 3; ++q)
+                    if (float(d.pml_d[(int64_t(e) * NP + p) * 3 + q]) != 0.f) ax |= 1 << q;
+            pml_ax[e] = char(ax);
+        }
     std::vector<int32_t> elem_of_int(E);
     std::iota(elem_of_int.begin(), elem_of_int.end(), 0);
     std::vector<int> col(E, 0);
@@ -852,6 +877,7 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
     std::stable_sort(elem_of_int.begin(), elem_of_int.end(), [&](int a, int b) {
         if (col[a] != col[b]) return col[a] < col[b];
         if (pml_el[a] != pml_el[b]) return pml_el[a] < pml_el[b];
+        if (pml_ax[a] != pml_ax[b]) return pml_ax[a] < pml_ax[b];
         if (diag_el[a] != diag_el[b]) return diag_el[a] > diag_el[b];
         return key[a] < key[b];
     });
@@ -860,9 +886,9 @@ void create(const gfx_solver_desc& d, gfx_solver& S) {
         for (int i = 1; i <= E; ++i) {
             const int e0 = elem_of_int[b];
             if (i == E || col[elem_of_int[i]] != col[e0] || pml_el[elem_of_int[i]] != pml_el[e0] ||
-                diag_el[elem_of_int[i]] != diag_el[e0]) {
+                pml_ax[elem_of_int[i]] != pml_ax[e0] || diag_el[elem_of_int[i]] != diag_el[e0]) {
                 const bool pml = pml_el[e0] != 0;
-                S.ranges.push_back({b, i, diag_el[e0] != 0 && !pml, pml});
+                S.ranges.push_back({b, i, diag_el[e0] != 0 && !pml, pml, int(pml_ax[e0])});
                 b = i;
             }
         }
