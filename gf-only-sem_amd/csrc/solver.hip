@@ -2085,6 +2085,8 @@ extern "C" gfx_status gfx_solver_query(const gfx_solver* s, const char* key, dou
         else if (k == "cpml_complete") *value = s->cpml_complete ? 1.0 : 0.0;
         else if (k == "cpml_points") *value = double(s->Ppml);
         else if (k == "cpml_memories") *value = double(s->cpml_mem_active);
+        else if (k == "cpml_groups")
+            *value = double(std::count_if(s->ranges.begin(), s->ranges.end(), [](const ElemRange& r) { return r.pml; }));
         else if (k == "special_nodes") *value = double(s->Ga);
         else if (k == "pml_nodes") *value = double(s->Gpml);
         else if (k == "fields") *value = s->nf;

@@ -337,7 +337,13 @@ gfx_status gfx_autotune(const gfx_solver_desc* base, int trial_steps, gfx_solver
 
 /* Introspection of the solver's setup.  key: "colors", "pml_nodal" (1 when the
  * C-PML memory is kept per node, 0 when element-local), "pml_elements",
- * "special_nodes", "pml_nodes", "fields", "nodes", "elements". */
+ * "special_nodes", "pml_nodes", "fields", "nodes", "elements", "assembly",
+ * "element_kernel", "elems_per_block", "node_field_groups", "diag_elements"
+ * (elements on the diagonal-geometry form), "n_sls"; with the complete C-PML
+ * "cpml_complete", "cpml_points" (PML element points), "cpml_memories"
+ * (memories in use per field, summed over those points) and "cpml_groups"
+ * (launches of the PML element kernel per step: one per set of damped axes,
+ * and per colour under COLORED assembly). */
 gfx_status gfx_solver_query(const gfx_solver* s, const char* key, double* value);
 
 /* Algorithmic HBM bytes per step for the current problem (each array counted

@@ -75,6 +75,14 @@ def _check(mesh, c, w, stf, direction, assembly, n_mid, sensitivity=False):
     the whole run against the oracle."""
     s = _gpu(mesh, [(c, w, stf, direction)], assembly, record_uva=True)
     assert s.query("cpml_complete") == 1 and s.query("pml_nodal") == 0
+    # PML elements launch in groups by the axes their profiles damp (d != 0 at some point)
+    reg = np.asarray(mesh.arrays["pml_region"]).ravel()
+    dmask = (np.asarray(mesh.arrays["pml_d"], np.float32).reshape(-1, 125, 3) != 0).any(axis=1)
+    n_groups = len({int(m[0]) | int(m[1]) << 1 | int(m[2]) << 2 for m in dmask[reg != 0]})
+    if assembly == pygfx.GFX_ASSEMBLY_COLORED:  # split by colour as well
+        assert s.query("cpml_groups") >= n_groups
+    else:
+        assert s.query("cpml_groups") == n_groups
     s.run(n_mid)
     mid = _oracle(mesh, c, w, stf, direction, n_mid)
     errs = [mc.rel_l2(x, mid[k]) for x, k in zip(s.get_state(0), ("u", "v", "a"))]
