@@ -1357,6 +1357,11 @@ __device__ __forceinline__ void staged_sums(const NodeArgs& A, int f, int64_t s0
     }
 }
 
+// Three floats moved as one 12-byte word (one dwordx3 access per lane).
+struct Vec3 {
+    float v[3];
+};
+
 // Special nodes [0, Ga): C-PML terms, legacy damping, massless rule, a_n kept.
 // FG field groups per workgroup: the 256 lanes cover 256/FG nodes, and lane
 // group fy handles fields fy, fy + FG, ... (each wavefront stays on one field
@@ -1422,8 +1427,8 @@ __device__ __forceinline__ void node_special_body(const NodeArgs& A, int blk) {
 #pragma unroll
         for (int q = 0; q < 9; ++q) m[q] = (pml && use[q % 3]) ? pmem[q * A.Gpml + g] : 0.f;
         if (staged) {
-            const float* rm = A.Rl + (int64_t(f) * A.PR + g) * 3;
-            const float main[3] = {rm[0], rm[1], rm[2]};
+            const Vec3 rm = *reinterpret_cast<const Vec3*>(A.Rl + (int64_t(f) * A.PR + g) * 3);
+            const float main[3] = {rm.v[0], rm.v[1], rm.v[2]};
             staged_sums<1>(A, f, s0, s1, kb, active, sl, sp, ids, main, R);
             if (!active) continue;
         } else if (A.gather) {
@@ -1452,6 +1457,7 @@ __device__ __forceinline__ void node_special_body(const NodeArgs& A, int blk) {
                 pnew[c * A.Gpml + g] = newv;
             }
         }
+        Vec3 nu, nw, na;  // stored as 12-byte words
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             // legacy ramp v -= d v before the corrector (solver.cpp:565-573)
@@ -1467,10 +1473,13 @@ __device__ __forceinline__ void node_special_body(const NodeArgs& A, int blk) {
                 v1 = vp;
             }
             const float w1 = v1 + hdt * a1;
-            A.U[o + c] = u1 + dt * w1;
-            A.W[o + c] = w1;
-            A.Aacc[oa + c] = a1;
+            nu.v[c] = u1 + dt * w1;
+            nw.v[c] = w1;
+            na.v[c] = a1;
         }
+        *reinterpret_cast<Vec3*>(A.U + o) = nu;
+        *reinterpret_cast<Vec3*>(A.W + o) = nw;
+        *reinterpret_cast<Vec3*>(A.Aacc + oa) = na;
     }
 }
 
