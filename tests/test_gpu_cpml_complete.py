@@ -117,6 +117,19 @@ def test_halfspace_matches_oracle(assembly):
     _check(mesh, c, w, stf, 2, assembly, 160, sensitivity=assembly == pygfx.GFX_ASSEMBLY_GATHER)
 
 
+def test_undamped_axis_matches_oracle():
+    """Profiles with d = 0 along x everywhere: the x-layer elements form a
+    launch group with no damped axis (no memories), the edge and corner
+    elements lose their x memories, and the run is still the oracle's."""
+    mesh = _halfspace()
+    d = np.array(mesh.pml_d, copy=True).reshape(-1, 125, 3)
+    d[:, :, 0] = 0.0
+    mesh = mesh.derive(pml_d=d.ravel())
+    c, w, stf = mc.surface_centre_source(mesh)
+    print(f"case: complete C-PML halfspace without x damping, {mesh.n_elem} elements", flush=True)
+    _check(mesh, c, w, stf, 2, pygfx.GFX_ASSEMBLY_GATHER, 160)
+
+
 def test_jittered_layered_buried_source_matches_oracle():
     mesh = _layered()
     c, w, stf = mc.buried_source(mesh)
